@@ -1,0 +1,276 @@
+"""Benchmark: FIR's projector + OSEM hot path on one MI355X.
+
+Workload: the clinical-scale plain path (24 rings x 448 crystals, radius
+330 mm, span 1, 47 segments, 325 tangential bins = 41 932 800 LORs;
+160x160x47 image at 2x2x3.27 mm), a seeded 12-ellipsoid phantom in a 200 mm
+water cylinder, attenuation-corrected Poisson data at 5e8 counts, OSEM with
+16 subsets and a 4 mm FWHM Gaussian every sub-iteration.
+
+One step = one full 16-subset OSEM iteration (every LOR forward- and
+back-projected once, fused per sub-iteration, plus the per-sub-iteration
+update and Gaussian). Inputs are resident in HBM before timing starts.
+
+    value = LORs per step / step time  [M LORs/s, fwd+backproj]
+
+The JSON line also carries the OSEM iteration and sub-iteration times, the
+separately timed forward and backprojection passes, the roofline of the
+dominant kernel (the fused OSEM projector) and the reference CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """Measured HBM bytes per launch of the dominant kernel, if a profile
+    summary with them is committed (profiles/osem_project_c2_traffic.json)."""
+    p = os.path.join(ROOT, "profiles", "osem_project_c2_traffic.json")
+    if os.path.exists(p):
+        try:
+            return json.load(open(p)).get("bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def cpu_baseline(w, n_subsets):
+    """Reference FIR (compiled from its own sources into oracle/_ref) timed
+    in-process on this host: one 16-subset OSEM iteration at the workload's
+    geometry with a uniform image and sinogram. Falls back to the oracle
+    restatement if the reference build is absent."""
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    if os.path.exists(drv):
+        with tempfile.TemporaryDirectory() as d:
+            w.scanner.write(os.path.join(d, "s.scan"))
+            w.proj.write(os.path.join(d, "p.hs"))
+            w.vol.write(os.path.join(d, "v.h33"))
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            r = subprocess.run([drv, "time", "subiter", "v.h33", "s.scan",
+                                "p.hs", str(n_subsets), "1"], cwd=d, env=env,
+                               capture_output=True, text=True, timeout=900)
+            if r.returncode != 0:
+                log("reference driver failed:", r.stderr[-500:])
+                return None
+            res = json.loads(r.stdout.strip().splitlines()[-1])
+        t = res["median_s"]
+        return {"value": w.n_lors / t / 1e6, "unit": "M LORs/s",
+                "cores": res["threads"], "kind": "reference",
+                "osem_iter_s": t,
+                "sample": (f"one {n_subsets}-subset OSEM iteration of the "
+                           "reference FIR built from source (oracle/_ref), "
+                           "same geometry, uniform image and sinogram, "
+                           "1 warm-up + 1 timed run")}
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    xy, z = O.scanner_tables(w.scanner)
+    sens = np.ones(w.vol.n_voxels * n_subsets, np.float32)
+    y = np.ones(w.n_lors, np.float32)
+    x0 = np.ones(w.vol.n_voxels, np.float32)
+    t0 = time.perf_counter()
+    O.osem(w.vol, w.proj, xy, z, y, sens, x0, 1, n_subsets,
+           nthreads=threads)
+    t = time.perf_counter() - t0
+    return {"value": w.n_lors / t / 1e6, "unit": "M LORs/s",
+            "cores": threads, "kind": "port", "osem_iter_s": t,
+            "sample": f"one {n_subsets}-subset OSEM iteration of the CPU "
+                      "oracle, uniform image and sinogram"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not "
+              "supported (multi-GPU sharding is a design note only)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from fir_amd import lib as firlib
+    from fir_amd import synth
+
+    w = synth.WORKLOADS[args.workload]()
+    n_sub = w.n_subsets
+    fwhm = w.fwhm
+    log("workload", w.name, "LORs", w.n_lors)
+    ctx = firlib.context_for(w)
+    v = firlib.VolDesc.from_spec(w.vol)
+    nvox = v.n_voxels
+    nb = ctx.n_bins
+
+    # --- synthetic data, resident in HBM ---------------------------------
+    act, hu = synth.phantom(w)
+    act_d = torch.from_numpy(act.ravel()).cuda()
+    mu = torch.from_numpy(hu.ravel()).cuda()
+    y0 = torch.zeros(nb, device="cuda")
+    ctx.forward(v, act_d, y0)
+    ctx.hu_to_mu(mu)
+    acf = torch.zeros(nb, device="cuda")
+    ctx.forward(v, mu, acf)
+    ctx.exp_clamped(acf)
+    mean = y0.double() / acf.double()
+    mean = mean * (w.counts / mean.sum())
+    g = torch.Generator(device="cuda").manual_seed(w.seed)
+    y = torch.poisson(mean.float(), generator=g)
+    ctx.mul_clamped(y, acf)  # FIR_OSEM: inputProj *= ACF (OSEM.cc:336-337)
+    sens = torch.zeros(n_sub * nvox, device="cuda")
+    ctx.sensitivity(v, sens, n_sub)
+    x = torch.ones(nvox, device="cuda")
+    bp = torch.zeros(nvox, device="cuda")
+    torch.cuda.synchronize()
+    del mean, act_d, y0
+
+    stream = torch.cuda.current_stream()
+
+    def osem_iteration(evs=None):
+        for s in range(n_sub):
+            if evs is not None:
+                evs[s][0].record(stream)
+            ctx.osem_project(v, x, y, None, bp, s, n_sub)
+            if evs is not None:
+                evs[s][1].record(stream)
+            ctx.osem_update(v, x, bp, sens[s * nvox:(s + 1) * nvox], 0.0,
+                            apply_cut=False, reset=False)
+            ctx.convolve(v, x, fwhm, 0.0)
+            ctx.fill(bp, 0.0)
+
+    # --- warm-up and timed OSEM iterations -------------------------------
+    for _ in range(args.warmup):
+        osem_iteration()
+    torch.cuda.synchronize()
+    evs = [[(torch.cuda.Event(enable_timing=True),
+             torch.cuda.Event(enable_timing=True)) for _ in range(n_sub)]
+           for _ in range(args.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t_start.record(stream)
+    for k in range(args.steps):
+        osem_iteration(evs[k])
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = t_start.elapsed_time(t_end)
+    ms_step = ms_total / args.steps
+    proj_ms = [a.elapsed_time(b) for row in evs for (a, b) in row]
+    proj_avg_ms = float(np.mean(proj_ms))
+    log(f"OSEM iteration {ms_step:.3f} ms (wall {wall / args.steps * 1e3:.3f}"
+        f" ms), projector launch avg {proj_avg_ms:.3f} ms")
+
+    # --- separately timed full forward and backprojection passes ---------
+    sino = torch.zeros(nb, device="cuda")
+    img = torch.zeros(nvox, device="cuda")
+    reps = 3
+    ctx.forward(v, x, sino)
+    e0, e1 = (torch.cuda.Event(enable_timing=True) for _ in range(2))
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        ctx.forward(v, x, sino)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / reps
+    ctx.backward_subset(v, y, img, 0, 1)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        ctx.backward_subset(v, y, img, 0, 1)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    bp_ms = e0.elapsed_time(e1) / reps
+    log(f"forward {fwd_ms:.3f} ms, backprojection {bp_ms:.3f} ms")
+
+    # --- roofline of the dominant kernel (fused OSEM projector) ----------
+    # Algorithmic bytes per launch (one sub-iteration's LORs): forward +
+    # backprojection of the cached-path format, 12 B per LOR and 6 B per
+    # path element each way, plus 4 B (x read) + 8 B (bp read-modify-write)
+    # per voxel.
+    n_e_total = 0
+    for s in range(n_sub):
+        n_e_total += ctx.path_count(v, s, n_sub)
+    lors_per_sub = w.n_lors / n_sub
+    n_e_per_sub = n_e_total / n_sub
+    bytes_launch = 2 * (12 * lors_per_sub + 6 * n_e_per_sub) + 12 * nvox
+    achieved = bytes_launch / (proj_avg_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles()
+
+    cpu = None
+    if not args.no_cpu:
+        log("timing the reference CPU path (one OSEM iteration)...")
+        cpu = cpu_baseline(w, n_sub)
+
+    value = w.n_lors / (ms_step * 1e-3) / 1e6
+    out = {
+        "metric": "M LORs/s fwd+backproj & OSEM iter ms: plain vs batched "
+                  "GPU at rel err<=1e-4",
+        "value": round(value, 3),
+        "unit": "M LORs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantom, Poisson counts)",
+        "config": {
+            "workload": w.name,
+            "scanner": "24 rings x 448 crystals, r=330 mm, span 1, 47 seg",
+            "n_lors": w.n_lors,
+            "image": "160x160x47 @ 2x2x3.27 mm",
+            "osem": f"{n_sub} subsets, 4 mm FWHM Gaussian every subiter",
+            "path": "plain (on-the-fly FP64 Siddon walk, FP32 accumulate)",
+        },
+        "osem_iter_ms": round(ms_step, 4),
+        "osem_subiter_ms": round(ms_step / n_sub, 4),
+        "fwd_mlors_s": round(w.n_lors / (fwd_ms * 1e-3) / 1e6, 3),
+        "bp_mlors_s": round(w.n_lors / (bp_ms * 1e-3) / 1e6, 3),
+        "fwd_ms": round(fwd_ms, 4),
+        "bp_ms": round(bp_ms, 4),
+        "path_elements_per_iter": int(n_e_total),
+        "roofline": {
+            "kernel": "k_osem_project",
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "avg_launch_ms": round(proj_avg_ms, 4),
+            "algorithmic_bytes_per_launch": int(bytes_launch),
+        },
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu_iter"] = round(cpu["osem_iter_s"] * 1e3 / ms_step, 2)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1210 @@
+// fir_hip.hip -- gfx950 kernels and the C-ABI of include/fir_hip.h.
+//
+// Plain (on-the-fly) path: every projector kernel maps one LOR to one lane.
+// Consecutive lanes take consecutive tangential bins of one
+// (segment, view, axial) row, i.e. neighbouring, nearly parallel LORs of one
+// fan (reference ProjData.cc:310-319): their walks touch neighbouring voxels
+// and their sinogram reads/writes are contiguous.
+//   - forward: gather, FP64 walk, FP32 accumulator updated as
+//     float(double(acc) + len * double(v)) -- the reference's
+//     VolData::computeLineIntegral (VolData.cc:357-371) -- so bins are
+//     bit-identical to the reference's.
+//   - backprojection: FP32 atomic scatter (global_atomic_add_f32) of
+//     float(len * double(v)) (VolData::projectLineIntegral, VolData.cc:373-385).
+//   - OSEM: one fused kernel per sub-iteration: walk, line integral, +bias,
+//     ratio, replay of the same walk (saved set-up state) for the scatter.
+// Element-wise and Gaussian kernels are memory-bound; they keep the
+// reference's FP32 operation order (no contraction).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fir_hip.h"
+#include "siddon.h"
+
+using fir::Grid;
+using fir::kEpsF;
+using fir::Lor;
+using fir::LorGeom;
+using fir::Walk;
+
+// ---------------------------------------------------------------------------
+// Context and error handling
+// ---------------------------------------------------------------------------
+
+struct fir_ctx
+{
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+
+  // scanner tables
+  double2* d_xy = nullptr;
+  double* d_z = nullptr;
+  int n_cpr = 0;
+  int n_slices = 0;
+
+  // sinogram geometry
+  bool proj_set = false;
+  int n_rings = 0, span = 1, n_seg = 0, n_tang = 0;
+  int seg_off = 0, tang_off = 0, n_views = 0, half_span = 0;
+  int64_t n_bins = 0;
+  std::vector<int> n_axial;
+  std::vector<int> ax_prefix;
+  std::vector<long long> seg_base;
+  int2* d_ax_slices = nullptr;
+  int* d_ax_prefix = nullptr;
+  long long* d_seg_base = nullptr;
+
+  int walk_mode = FIR_WALK_REFERENCE;
+
+  // 64-bit device counter of fir_path_count
+  unsigned long long* d_counter = nullptr;
+
+  // scratch images of the Gaussian (library-allocated, grown on demand)
+  float* d_scratch = nullptr;
+  size_t scratch_floats = 0;
+  // Gaussian taps on the device: [x: 64][y: 64][z: 64]
+  float* d_taps = nullptr;
+  float taps_key[3] = {-1.f, -1.f, -1.f};
+  double taps_vox[3] = {0, 0, 0};
+  int taps_h[3] = {0, 0, 0};
+};
+
+namespace
+{
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define FIR_HIP_CHECK(expr)                                                  \
+  do                                                                         \
+  {                                                                          \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(FIR_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+#define FIR_CHECK_LAUNCH()                                                   \
+  do                                                                         \
+  {                                                                          \
+    hipError_t e_ = hipGetLastError();                                       \
+    if (e_ != hipSuccess)                                                    \
+      return fail(FIR_E_HIP, "kernel launch: %s", hipGetErrorString(e_));    \
+  } while (0)
+
+constexpr int kBlock = 256;
+
+unsigned blocks_for(long long n, int block = kBlock)
+{
+  return (unsigned)((n + block - 1) / block);
+}
+
+// Grid of element-wise kernels: grid-stride loops over at most this many
+// blocks (8 per CU on 256 CUs).
+unsigned ew_blocks(long long n)
+{
+  const long long b = (n + kBlock - 1) / kBlock;
+  return (unsigned)std::max(1LL, std::min(b, 2048LL));
+}
+
+int make_grid(const fir_ctx* ctx, const fir_vol_desc* v, Grid* g)
+{
+  if (!v) return fail(FIR_E_ARG, "volume descriptor is NULL");
+  if (v->nx <= 0 || v->ny <= 0 || v->nz <= 0)
+    return fail(FIR_E_ARG,
+                "Number of voxels must be greater than zero in each "
+                "dimension");
+  if (!(v->dx > 0 && v->dy > 0 && v->dz > 0))
+    return fail(FIR_E_ARG,
+                "Voxel extent must be greater than zero in each dimension");
+  if ((long long)v->nx * v->ny * v->nz > 0x7fffffffLL)
+    return fail(FIR_E_ARG, "volume has more than 2^31-1 voxels");
+  const double ext[3] = {v->dx, v->dy, v->dz};
+  const int n[3] = {v->nx, v->ny, v->nz};
+  g->low[0] = v->ox - v->dx / 2;
+  g->low[1] = v->oy - v->dy / 2;
+  g->low[2] = v->oz - (v->nz * v->dz) / 2;
+  for (int d = 0; d < 3; ++d)
+  {
+    g->high[d] = g->low[d] + n[d] * ext[d];
+    if (ctx->walk_mode == FIR_WALK_REFERENCE)
+    {
+      const int t = (int)ext[d];
+      if (t <= 0)
+        return fail(FIR_E_ARG,
+                    "voxel extent %.7f mm truncates to 0 in the reference "
+                    "walk (which would not terminate); use the geometric "
+                    "walk mode",
+                    ext[d]);
+      g->step[d] = (double)t;
+    }
+    else
+      g->step[d] = ext[d];
+    g->nm1[d] = n[d] - 1;
+  }
+  g->row = v->nx;
+  g->slice = v->nx * v->ny;
+  g->max_iter = v->nx + v->ny + v->nz + 8;
+  return FIR_OK;
+}
+
+int need_lors(const fir_ctx* ctx)
+{
+  if (!ctx) return fail(FIR_E_ARG, "context is NULL");
+  if (!ctx->d_xy) return fail(FIR_E_STATE, "scanner tables not set");
+  if (!ctx->proj_set) return fail(FIR_E_STATE, "sinogram geometry not set");
+  if (ctx->n_cpr != 2 * ctx->n_views)
+    return fail(FIR_E_ARG,
+                "Projection must have the same number of crystals per ring "
+                "(%d) as the scanner",
+                ctx->n_cpr);
+  if (ctx->n_slices != 2 * ctx->n_rings - 1)
+    return fail(FIR_E_ARG,
+                "Projection must have the same number of rings (%d) as the "
+                "scanner",
+                (ctx->n_slices + 1) / 2);
+  return FIR_OK;
+}
+
+int check_subsets(const fir_ctx* ctx, int n_subsets)
+{
+  if (n_subsets <= 0 || ctx->n_views % n_subsets != 0)
+    return fail(FIR_E_ARG, "Number of subsets must be a divisor of %d",
+                ctx->n_views);
+  return FIR_OK;
+}
+
+LorGeom lor_geom(const fir_ctx* ctx)
+{
+  LorGeom g;
+  g.xy = ctx->d_xy;
+  g.z = ctx->d_z;
+  g.ax_slices = ctx->d_ax_slices;
+  g.ax_prefix = ctx->d_ax_prefix;
+  g.seg_base = ctx->d_seg_base;
+  g.n_seg = ctx->n_seg;
+  g.n_views = ctx->n_views;
+  g.n_tang = ctx->n_tang;
+  g.tang_off = ctx->tang_off;
+  g.n_cpr = ctx->n_cpr;
+  return g;
+}
+
+long long n_work(const fir_ctx* ctx, int n_subsets)
+{
+  const long long vpr = n_subsets > 0 ? ctx->n_views / n_subsets : ctx->n_views;
+  return vpr * ctx->ax_prefix.back() * (long long)ctx->n_tang;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+namespace
+{
+
+// Forward projection: one LOR per lane (all bins, or one subset).
+__global__ __launch_bounds__(kBlock) void k_forward(Grid g, LorGeom lg,
+                                                    const float* __restrict__ img,
+                                                    float* __restrict__ sino,
+                                                    long long n, int n_subsets,
+                                                    int subset)
+{
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  Lor L;
+  fir::lor_from_index(lg, t, n_subsets, subset, L);
+  Walk w;
+  float line = 0.0f;
+  if (fir::walk_setup(g, L.e1, L.e2, w))
+  {
+    fir::walk_run(g, w, [&](int idx, double len) {
+      line = (float)((double)line + len * (double)img[idx]);
+    });
+  }
+  sino[L.bin] = line;
+}
+
+// Backprojection scatter of one subset (or all LORs); sino == nullptr
+// backprojects ones.
+__global__ __launch_bounds__(kBlock) void k_backward(Grid g, LorGeom lg,
+                                                     const float* __restrict__ sino,
+                                                     float* __restrict__ img,
+                                                     long long n, int n_subsets,
+                                                     int subset)
+{
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  Lor L;
+  fir::lor_from_index(lg, t, n_subsets, subset, L);
+  const float val = sino ? sino[L.bin] : 1.0f;
+  if (val == 0.0f) return; // adding float(len * 0) changes nothing
+  Walk w;
+  if (!fir::walk_setup(g, L.e1, L.e2, w)) return;
+  const double dv = (double)val;
+  fir::walk_run(g, w, [&](int idx, double len) {
+    unsafeAtomicAdd(img + idx, (float)(len * dv));
+  });
+}
+
+// Fused OSEM projector pass (reconAlgos.cc:126-158).
+__global__ __launch_bounds__(kBlock) void k_osem_project(
+  Grid g, LorGeom lg, const float* __restrict__ x, const float* __restrict__ y,
+  const float* __restrict__ bias, float* __restrict__ bp, long long n,
+  int n_subsets, int subset)
+{
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  Lor L;
+  fir::lor_from_index(lg, t, n_subsets, subset, L);
+  Walk w;
+  if (!fir::walk_setup(g, L.e1, L.e2, w)) return; // empty path: nothing to scatter
+  float line = 0.0f;
+  fir::walk_run(g, w, [&](int idx, double len) {
+    line = (float)((double)line + len * (double)x[idx]);
+  });
+  if (bias) line += bias[L.bin];
+  if (!(line > kEpsF)) return;
+  const float ratio = y[L.bin] / line;
+  if (ratio == 0.0f) return;
+  const double dr = (double)ratio;
+  fir::walk_run(g, w, [&](int idx, double len) {
+    unsafeAtomicAdd(bp + idx, (float)(len * dr));
+  });
+}
+
+// Path dump for parity checks.
+__global__ __launch_bounds__(kBlock) void k_path_dump(Grid g, LorGeom lg,
+                                                      long long bin0,
+                                                      long long n, int maxp,
+                                                      int* __restrict__ counts,
+                                                      int* __restrict__ idx,
+                                                      double* __restrict__ len)
+{
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  Lor L;
+  fir::lor_from_index(lg, bin0 + t, 0, 0, L);
+  Walk w;
+  if (!fir::walk_setup(g, L.e1, L.e2, w))
+  {
+    counts[t] = -1;
+    return;
+  }
+  int c = 0;
+  int* oi = idx + t * (long long)maxp;
+  double* ol = len + t * (long long)maxp;
+  fir::walk_run(g, w, [&](int i, double l) {
+    if (c < maxp)
+    {
+      oi[c] = i;
+      ol[c] = l;
+    }
+    ++c;
+  });
+  counts[t] = c;
+}
+
+// Element count of each LOR's path, reduced per block into one 64-bit
+// counter.
+__global__ __launch_bounds__(kBlock) void k_path_count(
+  Grid g, LorGeom lg, long long n, int n_subsets, int subset,
+  unsigned long long* __restrict__ total)
+{
+  __shared__ unsigned long long part[kBlock];
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  unsigned long long c = 0;
+  if (t < n)
+  {
+    Lor L;
+    fir::lor_from_index(lg, t, n_subsets, subset, L);
+    Walk w;
+    if (fir::walk_setup(g, L.e1, L.e2, w))
+      fir::walk_run(g, w, [&](int, double) { ++c; });
+  }
+  part[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1)
+  {
+    if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(total, part[0]);
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ULL; }
+
+__global__ void k_fill(float* __restrict__ a, long long n, float v)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
+__global__ void k_mul_clamped(float* __restrict__ a, const float* __restrict__ b,
+                              long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+  {
+    const float x = a[i], y = b[i];
+    a[i] = (x > kEpsF && y > kEpsF) ? x * y : 0.0f;
+  }
+}
+
+__global__ void k_div_clamped(float* __restrict__ a, const float* __restrict__ b,
+                              long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+  {
+    const float x = a[i], y = b[i];
+    a[i] = (x > kEpsF && y > kEpsF) ? x / y : 0.0f;
+  }
+}
+
+__global__ void k_exp_clamped(float* __restrict__ a, long long n)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+  {
+    const float x = a[i];
+    a[i] = x > kEpsF ? expf(x) : 1.0f;
+  }
+}
+
+__global__ void k_hu_to_mu(float* __restrict__ a, long long n)
+{
+  const float water = 0.0096f;
+  const float s1 = water / 1000.0f;
+  const float s2 = (0.015f - water) / 1000.0f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+  {
+    float x = a[i];
+    if (x <= -1000.0f)
+      x = 0.0f;
+    else if (x <= 0.0f)
+    {
+      x *= s1;
+      x += water;
+    }
+    else
+    {
+      x *= s2;
+      x += water;
+    }
+    a[i] = x;
+  }
+}
+
+// Radial distance test of operations::cutCircle (operations.cc:206-214),
+// computed in double as the reference does.
+struct CutGeom
+{
+  double dx, dy, half_w, half_h;
+  int nx, ny;
+};
+
+__device__ __forceinline__ double radius_of(const CutGeom& c, int i, int j)
+{
+  const double px = i * c.dx + c.dx / 2.0 - c.half_w;
+  const double py = j * c.dy + c.dy / 2.0 - c.half_h;
+  return sqrt(px * px + py * py);
+}
+
+CutGeom cut_geom(const fir_vol_desc* v)
+{
+  CutGeom c;
+  c.dx = v->dx;
+  c.dy = v->dy;
+  c.half_w = (v->nx * v->dx) / 2.0;
+  c.half_h = (v->ny * v->dy) / 2.0;
+  c.nx = v->nx;
+  c.ny = v->ny;
+  return c;
+}
+
+__global__ void k_cut_circle(float* __restrict__ img, long long n, CutGeom c,
+                             float cut)
+{
+  const long long plane = (long long)c.nx * c.ny;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+  {
+    const long long r = i % plane;
+    const int y = (int)(r / c.nx);
+    const int x = (int)(r - (long long)y * c.nx);
+    if (radius_of(c, x, y) > cut) img[i] = 0.0f;
+  }
+}
+
+// OSEM update: bp /= sens (clamped), x *= bp (clamped), optional cut,
+// optional bp reset (VolData.cc:305-343, reconAlgos.cc:161-186).
+__global__ void k_osem_update(float* __restrict__ x, float* __restrict__ bp,
+                              const float* __restrict__ sens, long long n,
+                              CutGeom c, float cut, int apply_cut, int reset)
+{
+  const long long plane = (long long)c.nx * c.ny;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+  {
+    float b = bp[i];
+    const float s = sens[i];
+    b = (b > kEpsF && s > kEpsF) ? b / s : 0.0f;
+    float v = x[i];
+    v = (v > kEpsF && b > kEpsF) ? v * b : 0.0f;
+    if (apply_cut)
+    {
+      const long long r = i % plane;
+      const int yy = (int)(r / c.nx);
+      const int xx = (int)(r - (long long)yy * c.nx);
+      if (radius_of(c, xx, yy) > cut) v = 0.0f;
+    }
+    x[i] = v;
+    bp[i] = reset ? 0.0f : b;
+  }
+}
+
+// One axis of the separable Gaussian (operations.cc:83-156): out = sum(k*v) /
+// sum(k) over in-bounds taps, accumulated in tap order in FP32.
+// axis 0: x, 1: y, 2: z. For axis 2 with `restore`, voxels at radius >=
+// restore_r take keep[] (edge restore, operations.cc:158-185); with
+// cut > 0 the circular cut follows (operations.cc:194-220).
+template <int AXIS, bool EPILOGUE>
+__global__ __launch_bounds__(kBlock) void k_gauss(const float* __restrict__ in,
+                                                  float* __restrict__ out,
+                                                  const float* __restrict__ taps,
+                                                  int h, int nx, int ny, int nz,
+                                                  CutGeom c, int restore,
+                                                  double restore_r, float cut)
+{
+  const long long n = (long long)nx * ny * nz;
+  const long long plane = (long long)nx * ny;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+  {
+    const int k = (int)(i / plane);
+    const long long r = i - (long long)k * plane;
+    const int j = (int)(r / nx);
+    const int ii = (int)(r - (long long)j * nx);
+    const int coord = AXIS == 0 ? ii : (AXIS == 1 ? j : k);
+    const int lim = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
+    const long long stride = AXIS == 0 ? 1 : (AXIS == 1 ? nx : plane);
+    float s = 0.0f, w = 0.0f;
+    for (int q = -h; q <= h; ++q)
+    {
+      const int cq = coord + q;
+      if (cq >= 0 && cq < lim)
+      {
+        const float kv = taps[h + q];
+        s += kv * in[i + q * stride];
+        w += kv;
+      }
+    }
+    float o;
+    if (w > 0.0f)
+      o = s / w;
+    else
+      o = out[i];
+    if (EPILOGUE)
+    {
+      const double rad = radius_of(c, ii, j);
+      if (restore && rad >= restore_r) o = out[i];
+      if (cut > 0.0f && rad > cut) o = 0.0f;
+    }
+    out[i] = o;
+  }
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char* fir_last_error(void) { return g_err.c_str(); }
+
+int fir_abi_version(void) { return 1; }
+
+int fir_ctx_create(fir_ctx** out, int device)
+{
+  if (!out) return fail(FIR_E_ARG, "output pointer is NULL");
+  *out = nullptr;
+  int n = 0;
+  FIR_HIP_CHECK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return fail(FIR_E_ARG, "device %d out of range (%d devices)", device, n);
+  FIR_HIP_CHECK(hipSetDevice(device));
+  fir_ctx* c = new fir_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess)
+  {
+    delete c;
+    return fail(FIR_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  c->stream = c->own_stream;
+  *out = c;
+  return FIR_OK;
+}
+
+int fir_ctx_destroy(fir_ctx* c)
+{
+  if (!c) return FIR_OK;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  (void)hipFree(c->d_xy);
+  (void)hipFree(c->d_z);
+  (void)hipFree(c->d_ax_slices);
+  (void)hipFree(c->d_ax_prefix);
+  (void)hipFree(c->d_seg_base);
+  (void)hipFree(c->d_scratch);
+  (void)hipFree(c->d_taps);
+  (void)hipFree(c->d_counter);
+  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  delete c;
+  return FIR_OK;
+}
+
+int fir_set_stream(fir_ctx* c, void* s)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  c->stream = (hipStream_t)s;
+  return FIR_OK;
+}
+
+int fir_synchronize(fir_ctx* c)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return FIR_OK;
+}
+
+int fir_set_scanner(fir_ctx* c, const double* xy, int n_cpr, const double* z,
+                    int n_slices)
+{
+  if (!c || !xy || !z) return fail(FIR_E_ARG, "NULL argument");
+  if (n_cpr <= 0 || n_slices <= 0)
+    return fail(FIR_E_ARG, "empty scanner tables");
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  (void)hipFree(c->d_xy);
+  (void)hipFree(c->d_z);
+  c->d_xy = nullptr;
+  c->d_z = nullptr;
+  FIR_HIP_CHECK(hipMalloc(&c->d_xy, sizeof(double2) * n_cpr));
+  FIR_HIP_CHECK(hipMalloc(&c->d_z, sizeof(double) * n_slices));
+  FIR_HIP_CHECK(hipMemcpy(c->d_xy, xy, sizeof(double2) * n_cpr,
+                          hipMemcpyHostToDevice));
+  FIR_HIP_CHECK(
+    hipMemcpy(c->d_z, z, sizeof(double) * n_slices, hipMemcpyHostToDevice));
+  c->n_cpr = n_cpr;
+  c->n_slices = n_slices;
+  return FIR_OK;
+}
+
+// ProjHeader::check + ProjGeometry::fill + getCrystalAxialCoord tables.
+int fir_set_proj(fir_ctx* c, int n_rings, int n_cpr, int span, int n_seg,
+                 int n_tang)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (n_rings <= 0)
+    return fail(FIR_E_ARG, "\"number of rings\" must be provided and > 0");
+  if (n_cpr <= 0)
+    return fail(FIR_E_ARG,
+                "\"number of crystals per ring\" must be provided and > 0");
+  if (n_cpr % 4 != 0)
+    return fail(FIR_E_ARG,
+                "\"number of crystals per ring\" must be a multiple of four");
+  if (span <= 0) return fail(FIR_E_ARG, "\"segment span\" must be > 0");
+  if (span % 2 != 1)
+    return fail(FIR_E_ARG, "\"segment span\" must be an odd number");
+  const int max_span = 2 * n_rings - 1;
+  if (span > max_span)
+    return fail(FIR_E_ARG,
+                "For %d ring(s), \"segment span\" must not be > %d", n_rings,
+                max_span);
+  if (n_seg < 0)
+    return fail(FIR_E_ARG, "\"number of segments\" must not be negative");
+  if (n_seg % 2 != 1)
+    return fail(FIR_E_ARG, "\"number of segments\" must be an odd number");
+  int max_seg = max_span / span;
+  max_seg -= 1 - max_seg % 2;
+  if (n_seg > max_seg)
+    return fail(FIR_E_ARG,
+                "For %d ring(s) and a segment span of %d, \"number of "
+                "segments\" must not be > %d",
+                n_rings, span, max_seg);
+  if (n_tang < 0)
+    return fail(FIR_E_ARG,
+                "\"number of tangential coordinates\" must not be negative");
+  if (n_tang == 0)
+    n_tang = n_cpr - 1;
+  else if (n_tang > n_cpr - 1)
+    return fail(FIR_E_ARG,
+                "For %d crystals per ring, \"number of tangential "
+                "coordinates\" must not be > %d",
+                n_cpr, n_cpr - 1);
+
+  const int seg_off = (n_seg - 1) / 2;
+  const int n_views = n_cpr / 2;
+  const int half = (span - 1) / 2;
+  std::vector<int> na(n_seg);
+  for (int s = -seg_off; s <= seg_off; ++s)
+  {
+    const int a = s < 0 ? -s : s;
+    int len;
+    if (span == 1)
+      len = n_rings - a;
+    else
+    {
+      len = 2 * n_rings - 1;
+      if (a >= 1) len -= span + 1;
+      if (a >= 2) len -= 2 * span * (a - 1);
+    }
+    na[s + seg_off] = len;
+  }
+  std::vector<int> pre(n_seg + 1, 0);
+  std::vector<long long> base(n_seg, 0);
+  for (int k = 0; k < n_seg; ++k)
+  {
+    pre[k + 1] = pre[k] + na[k];
+    base[k] = (long long)pre[k] * n_views * n_tang;
+  }
+  const long long nbins = (long long)pre[n_seg] * n_views * n_tang;
+
+  // slice pair of every (seg, axial): ProjData::getCrystalAxialCoord
+  // (ProjData.cc:198-299)
+  std::vector<int2> sl(pre[n_seg]);
+  for (int s = -seg_off; s <= seg_off; ++s)
+  {
+    const int k = s + seg_off;
+    for (int ax = 0; ax < na[k]; ++ax)
+    {
+      int s1, s2;
+      if (span == 1)
+      {
+        if (s == 0)
+          s1 = s2 = 2 * ax;
+        else if (s > 0)
+        {
+          s1 = 2 * (ax + s);
+          s2 = 2 * ax;
+        }
+        else
+        {
+          s1 = 2 * ax;
+          s2 = 2 * (ax - s);
+        }
+      }
+      else if (s == 0)
+        s1 = s2 = ax;
+      else
+      {
+        const int mid = (s < 0 ? -s : s) * span;
+        if (ax < half)
+        {
+          const int rd = mid - half + ax;
+          s1 = s > 0 ? 2 * rd : 0;
+          s2 = s > 0 ? 0 : 2 * rd;
+        }
+        else
+        {
+          const int inv = na[k] - ax - 1;
+          if (inv < half)
+          {
+            const int rd = mid - half + inv;
+            const int smax = 2 * n_rings - 2;
+            s1 = s > 0 ? smax : smax - 2 * rd;
+            s2 = s > 0 ? smax - 2 * rd : smax;
+          }
+          else
+          {
+            const int adj = ax - half;
+            s1 = s > 0 ? 2 * mid + adj : adj;
+            s2 = s > 0 ? adj : 2 * mid + adj;
+          }
+        }
+      }
+      if (s1 < 0 || s2 < 0 || s1 > 2 * n_rings - 2 || s2 > 2 * n_rings - 2)
+        return fail(FIR_E_ARG, "slice index out of range for segment %d", s);
+      sl[pre[k] + ax] = make_int2(s1, s2);
+    }
+  }
+
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  (void)hipFree(c->d_ax_slices);
+  (void)hipFree(c->d_ax_prefix);
+  (void)hipFree(c->d_seg_base);
+  c->d_ax_slices = nullptr;
+  c->d_ax_prefix = nullptr;
+  c->d_seg_base = nullptr;
+  FIR_HIP_CHECK(hipMalloc(&c->d_ax_slices, sizeof(int2) * sl.size()));
+  FIR_HIP_CHECK(hipMalloc(&c->d_ax_prefix, sizeof(int) * pre.size()));
+  FIR_HIP_CHECK(hipMalloc(&c->d_seg_base, sizeof(long long) * base.size()));
+  FIR_HIP_CHECK(hipMemcpy(c->d_ax_slices, sl.data(), sizeof(int2) * sl.size(),
+                          hipMemcpyHostToDevice));
+  FIR_HIP_CHECK(hipMemcpy(c->d_ax_prefix, pre.data(), sizeof(int) * pre.size(),
+                          hipMemcpyHostToDevice));
+  FIR_HIP_CHECK(hipMemcpy(c->d_seg_base, base.data(),
+                          sizeof(long long) * base.size(),
+                          hipMemcpyHostToDevice));
+  c->proj_set = true;
+  c->n_rings = n_rings;
+  c->span = span;
+  c->n_seg = n_seg;
+  c->n_tang = n_tang;
+  c->seg_off = seg_off;
+  c->tang_off = n_tang / 2;
+  c->n_views = n_views;
+  c->half_span = half;
+  c->n_bins = nbins;
+  c->n_axial = na;
+  c->ax_prefix = pre;
+  c->seg_base = base;
+  return FIR_OK;
+}
+
+int64_t fir_proj_n_bins(const fir_ctx* c)
+{
+  return (c && c->proj_set) ? c->n_bins : 0;
+}
+
+int fir_set_walk_mode(fir_ctx* c, int mode)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (mode != FIR_WALK_REFERENCE && mode != FIR_WALK_GEOMETRIC)
+    return fail(FIR_E_ARG, "unknown walk mode %d", mode);
+  c->walk_mode = mode;
+  return FIR_OK;
+}
+
+int fir_malloc(fir_ctx* c, void** p, size_t bytes)
+{
+  if (!c || !p) return fail(FIR_E_ARG, "NULL argument");
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  if (hipMalloc(p, bytes) != hipSuccess)
+  {
+    *p = nullptr;
+    return fail(FIR_E_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+  }
+  return FIR_OK;
+}
+
+int fir_free(fir_ctx* c, void* p)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  FIR_HIP_CHECK(hipFree(p));
+  return FIR_OK;
+}
+
+int fir_memcpy_h2d(fir_ctx* c, void* dst, const void* src, size_t bytes)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  FIR_HIP_CHECK(
+    hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return FIR_OK;
+}
+
+int fir_memcpy_d2h(fir_ctx* c, void* dst, const void* src, size_t bytes)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  FIR_HIP_CHECK(
+    hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return FIR_OK;
+}
+
+int fir_fill(fir_ctx* c, float* dst, int64_t n, float value)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (n <= 0) return FIR_OK;
+  if (!dst) return fail(FIR_E_ARG, "NULL buffer");
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_fill, dim3(ew_blocks(n)), dim3(kBlock), 0, c->stream,
+                     dst, (long long)n, value);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+int fir_forward_subset(fir_ctx* c, const fir_vol_desc* v, const float* img,
+                       float* sino, int subset, int n_subsets)
+{
+  int rc = need_lors(c);
+  if (rc) return rc;
+  if (!img || !sino) return fail(FIR_E_ARG, "NULL buffer");
+  Grid g;
+  if ((rc = make_grid(c, v, &g))) return rc;
+  if (n_subsets > 0)
+  {
+    if ((rc = check_subsets(c, n_subsets))) return rc;
+    if (subset < 0 || subset >= n_subsets)
+      return fail(FIR_E_ARG, "subset %d out of range", subset);
+  }
+  const long long n = n_work(c, n_subsets);
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_forward, dim3(blocks_for(n)), dim3(kBlock), 0,
+                     c->stream, g, lor_geom(c), img, sino, n, n_subsets,
+                     subset);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+int fir_forward(fir_ctx* c, const fir_vol_desc* v, const float* img,
+                float* sino)
+{
+  return fir_forward_subset(c, v, img, sino, 0, 0);
+}
+
+int fir_backward_subset(fir_ctx* c, const fir_vol_desc* v, const float* sino,
+                        float* img, int subset, int n_subsets)
+{
+  int rc = need_lors(c);
+  if (rc) return rc;
+  if (!img) return fail(FIR_E_ARG, "NULL image");
+  Grid g;
+  if ((rc = make_grid(c, v, &g))) return rc;
+  if ((rc = check_subsets(c, n_subsets))) return rc;
+  if (subset < 0 || subset >= n_subsets)
+    return fail(FIR_E_ARG, "subset %d out of range", subset);
+  const long long n = n_work(c, n_subsets);
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_backward, dim3(blocks_for(n)), dim3(kBlock), 0,
+                     c->stream, g, lor_geom(c), sino, img, n, n_subsets,
+                     subset);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+int fir_backward(fir_ctx* c, const fir_vol_desc* v, const float* sino,
+                 float* img, int n_subsets)
+{
+  int rc = need_lors(c);
+  if (rc) return rc;
+  if ((rc = check_subsets(c, n_subsets))) return rc;
+  Grid g;
+  if ((rc = make_grid(c, v, &g))) return rc;
+  const long long nvox = (long long)v->nx * v->ny * v->nz;
+  if ((rc = fir_fill(c, img, nvox * n_subsets, 0.0f))) return rc;
+  for (int s = 0; s < n_subsets; ++s)
+    if ((rc = fir_backward_subset(c, v, sino, img + nvox * s, s, n_subsets)))
+      return rc;
+  return FIR_OK;
+}
+
+int fir_sensitivity(fir_ctx* c, const fir_vol_desc* v, float* sens,
+                    int n_subsets)
+{
+  return fir_backward(c, v, nullptr, sens, n_subsets);
+}
+
+int fir_osem_project(fir_ctx* c, const fir_vol_desc* v, const float* x,
+                     const float* y, const float* bias, float* bp, int subset,
+                     int n_subsets)
+{
+  int rc = need_lors(c);
+  if (rc) return rc;
+  if (!x || !y || !bp) return fail(FIR_E_ARG, "NULL buffer");
+  Grid g;
+  if ((rc = make_grid(c, v, &g))) return rc;
+  if ((rc = check_subsets(c, n_subsets))) return rc;
+  if (subset < 0 || subset >= n_subsets)
+    return fail(FIR_E_ARG, "subset %d out of range", subset);
+  const long long n = n_work(c, n_subsets);
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_osem_project, dim3(blocks_for(n)), dim3(kBlock), 0,
+                     c->stream, g, lor_geom(c), x, y, bias, bp, n, n_subsets,
+                     subset);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+int fir_osem_update(fir_ctx* c, const fir_vol_desc* v, float* x, float* bp,
+                    const float* sens_s, float cut, int apply_cut, int reset)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (!x || !bp || !sens_s) return fail(FIR_E_ARG, "NULL buffer");
+  Grid g;
+  int rc;
+  if ((rc = make_grid(c, v, &g))) return rc;
+  const long long n = (long long)v->nx * v->ny * v->nz;
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_osem_update, dim3(ew_blocks(n)), dim3(kBlock), 0,
+                     c->stream, x, bp, sens_s, n, cut_geom(v), cut,
+                     (apply_cut && cut > 0.0f) ? 1 : 0, reset ? 1 : 0);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+namespace
+{
+// Gaussian taps of operations::convolve (operations.cc:31-75), uploaded to
+// the context when the FWHM or voxel size changes. X and Y use a float
+// sigma, Z a double sigma; taps are computed in double, stored as float.
+int prepare_taps(fir_ctx* c, const fir_vol_desc* v, const float* fwhm)
+{
+  if (c->d_taps && c->taps_key[0] == fwhm[0] && c->taps_key[1] == fwhm[1] &&
+      c->taps_key[2] == fwhm[2] && c->taps_vox[0] == v->dx &&
+      c->taps_vox[1] == v->dy && c->taps_vox[2] == v->dz)
+    return FIR_OK;
+  const float sx = fwhm[0] / 2.3548f;
+  const float sy = fwhm[1] / 2.3548f;
+  const double sz = fwhm[2] / 2.3548;
+  int h[3];
+  h[0] = ((int)(6.0 * sx / v->dx) + 1) / 2;
+  h[1] = ((int)(6.0 * sy / v->dy) + 1) / 2;
+  h[2] = ((int)(6.0 * sz / v->dz) + 1) / 2;
+  for (int d = 0; d < 3; ++d)
+    if (h[d] > 31 || h[d] < 0)
+      return fail(FIR_E_ARG, "Gaussian kernel wider than 63 taps");
+  float taps[192];
+  std::memset(taps, 0, sizeof(taps));
+  for (int k = -h[0]; k <= h[0]; ++k)
+    taps[h[0] + k] = (float)std::exp(-(k * k * v->dx * v->dx) / (2 * sx * sx));
+  for (int k = -h[1]; k <= h[1]; ++k)
+    taps[64 + h[1] + k] =
+      (float)std::exp(-(k * k * v->dy * v->dy) / (2 * sy * sy));
+  for (int k = -h[2]; k <= h[2]; ++k)
+    taps[128 + h[2] + k] =
+      (float)std::exp(-(k * k * v->dz * v->dz) / (2 * sz * sz));
+  if (!c->d_taps) FIR_HIP_CHECK(hipMalloc(&c->d_taps, sizeof(taps)));
+  FIR_HIP_CHECK(hipMemcpyAsync(c->d_taps, taps, sizeof(taps),
+                               hipMemcpyHostToDevice, c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (int d = 0; d < 3; ++d)
+  {
+    c->taps_key[d] = fwhm[d];
+    c->taps_h[d] = h[d];
+  }
+  c->taps_vox[0] = v->dx;
+  c->taps_vox[1] = v->dy;
+  c->taps_vox[2] = v->dz;
+  return FIR_OK;
+}
+
+int ensure_scratch(fir_ctx* c, size_t floats)
+{
+  if (c->scratch_floats >= floats) return FIR_OK;
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  (void)hipFree(c->d_scratch);
+  c->d_scratch = nullptr;
+  c->scratch_floats = 0;
+  if (hipMalloc(&c->d_scratch, floats * sizeof(float)) != hipSuccess)
+    return fail(FIR_E_NOMEM, "scratch allocation of %zu floats failed",
+                floats);
+  c->scratch_floats = floats;
+  return FIR_OK;
+}
+
+// Gaussian of one frame plus edge restore and cut (fused in the Z pass).
+int convolve_frame(fir_ctx* c, const fir_vol_desc* v, float* img,
+                   const float* fwhm, float restore_cut, float final_cut)
+{
+  const long long n = (long long)v->nx * v->ny * v->nz;
+  float* t1 = c->d_scratch;
+  float* t2 = c->d_scratch + n;
+  const CutGeom cg = cut_geom(v);
+  const float fw = fwhm[0] > fwhm[1] ? fwhm[0] : fwhm[1];
+  const int restore = restore_cut > 0.0f ? 1 : 0;
+  const double rr = (double)restore_cut - 5.0 * fw;
+  const unsigned nb = ew_blocks(n);
+  hipLaunchKernelGGL((k_gauss<0, false>), dim3(nb), dim3(kBlock), 0,
+                     c->stream, img, t1, c->d_taps, c->taps_h[0], v->nx,
+                     v->ny, v->nz, cg, 0, 0.0, 0.0f);
+  hipLaunchKernelGGL((k_gauss<1, false>), dim3(nb), dim3(kBlock), 0,
+                     c->stream, t1, t2, c->d_taps + 64, c->taps_h[1], v->nx,
+                     v->ny, v->nz, cg, 0, 0.0, 0.0f);
+  // Z pass writes img in place: each voxel reads its own img value (the
+  // pre-convolution copy) only for the edge restore.
+  hipLaunchKernelGGL((k_gauss<2, true>), dim3(nb), dim3(kBlock), 0, c->stream,
+                     t2, img, c->d_taps + 128, c->taps_h[2], v->nx, v->ny,
+                     v->nz, cg, restore, rr, final_cut);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+} // namespace
+
+int fir_convolve(fir_ctx* c, const fir_vol_desc* v, float* img, int n_frames,
+                 const float* fwhm, float cut)
+{
+  if (!c || !fwhm) return fail(FIR_E_ARG, "NULL argument");
+  if (!img) return fail(FIR_E_ARG, "NULL image");
+  Grid g;
+  int rc;
+  if ((rc = make_grid(c, v, &g))) return rc;
+  if (!(fwhm[0] > 0.0 && fwhm[1] > 0.0 && fwhm[2] > 0.0)) return FIR_OK;
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  const long long n = (long long)v->nx * v->ny * v->nz;
+  if ((rc = prepare_taps(c, v, fwhm))) return rc;
+  if ((rc = ensure_scratch(c, 2 * (size_t)n))) return rc;
+  for (int f = 0; f < n_frames; ++f)
+    if ((rc = convolve_frame(c, v, img + n * f, fwhm, cut, 0.0f))) return rc;
+  return FIR_OK;
+}
+
+int fir_cut_circle(fir_ctx* c, const fir_vol_desc* v, float* img,
+                   int n_frames, float cut)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (!img) return fail(FIR_E_ARG, "NULL image");
+  Grid g;
+  int rc;
+  if ((rc = make_grid(c, v, &g))) return rc;
+  if (!(cut > 0.0f)) return FIR_OK;
+  const long long n = (long long)v->nx * v->ny * v->nz * n_frames;
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_cut_circle, dim3(ew_blocks(n)), dim3(kBlock), 0,
+                     c->stream, img, n, cut_geom(v), cut);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+int fir_osem_subiter(fir_ctx* c, const fir_vol_desc* v, float* x,
+                     const float* y, const float* bias, const float* sens_s,
+                     float* bp, int subset, int n_subsets, int do_convolve,
+                     const float* fwhm, float cut, int reset_bp)
+{
+  int rc = fir_osem_project(c, v, x, y, bias, bp, subset, n_subsets);
+  if (rc) return rc;
+  const bool conv = do_convolve && fwhm && fwhm[0] > 0.0 && fwhm[1] > 0.0 &&
+    fwhm[2] > 0.0;
+  if ((rc = fir_osem_update(c, v, x, bp, sens_s, cut, conv ? 0 : 1,
+                            reset_bp)))
+    return rc;
+  if (conv)
+  {
+    const long long n = (long long)v->nx * v->ny * v->nz;
+    if ((rc = prepare_taps(c, v, fwhm))) return rc;
+    if ((rc = ensure_scratch(c, 2 * (size_t)n))) return rc;
+    if ((rc = convolve_frame(c, v, x, fwhm, cut, cut))) return rc;
+  }
+  return FIR_OK;
+}
+
+int fir_mul_clamped(fir_ctx* c, float* a, const float* b, int64_t n)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (n <= 0) return FIR_OK;
+  if (!a || !b) return fail(FIR_E_ARG, "NULL buffer");
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_mul_clamped, dim3(ew_blocks(n)), dim3(kBlock), 0,
+                     c->stream, a, b, (long long)n);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+int fir_div_clamped(fir_ctx* c, float* a, const float* b, int64_t n)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (n <= 0) return FIR_OK;
+  if (!a || !b) return fail(FIR_E_ARG, "NULL buffer");
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_div_clamped, dim3(ew_blocks(n)), dim3(kBlock), 0,
+                     c->stream, a, b, (long long)n);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+int fir_exp_clamped(fir_ctx* c, float* a, int64_t n)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (n <= 0) return FIR_OK;
+  if (!a) return fail(FIR_E_ARG, "NULL buffer");
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_exp_clamped, dim3(ew_blocks(n)), dim3(kBlock), 0,
+                     c->stream, a, (long long)n);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+int fir_hu_to_mu(fir_ctx* c, float* a, int64_t n)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (n <= 0) return FIR_OK;
+  if (!a) return fail(FIR_E_ARG, "NULL buffer");
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_hu_to_mu, dim3(ew_blocks(n)), dim3(kBlock), 0,
+                     c->stream, a, (long long)n);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+int fir_path_dump(fir_ctx* c, const fir_vol_desc* v, int64_t bin0,
+                  int64_t nb, int maxp, int* counts, int* idx, double* len)
+{
+  int rc = need_lors(c);
+  if (rc) return rc;
+  Grid g;
+  if ((rc = make_grid(c, v, &g))) return rc;
+  if (bin0 < 0 || nb < 0 || bin0 + nb > c->n_bins)
+    return fail(FIR_E_ARG, "bin range out of bounds");
+  if (maxp <= 0) return fail(FIR_E_ARG, "max_per_lor must be > 0");
+  if (!counts || !idx || !len) return fail(FIR_E_ARG, "NULL buffer");
+  if (nb == 0) return FIR_OK;
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_path_dump, dim3(blocks_for(nb)), dim3(kBlock), 0,
+                     c->stream, g, lor_geom(c), (long long)bin0,
+                     (long long)nb, maxp, counts, idx, len);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+int fir_path_count(fir_ctx* c, const fir_vol_desc* v, int subset,
+                   int n_subsets, int64_t* total)
+{
+  int rc = need_lors(c);
+  if (rc) return rc;
+  if (!total) return fail(FIR_E_ARG, "NULL output");
+  Grid g;
+  if ((rc = make_grid(c, v, &g))) return rc;
+  if (n_subsets > 0)
+  {
+    if ((rc = check_subsets(c, n_subsets))) return rc;
+    if (subset < 0 || subset >= n_subsets)
+      return fail(FIR_E_ARG, "subset %d out of range", subset);
+  }
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  if (!c->d_counter)
+    FIR_HIP_CHECK(hipMalloc(&c->d_counter, sizeof(unsigned long long)));
+  const long long n = n_work(c, n_subsets);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, c->stream, c->d_counter);
+  hipLaunchKernelGGL(k_path_count, dim3(blocks_for(n)), dim3(kBlock), 0,
+                     c->stream, g, lor_geom(c), n, n_subsets, subset,
+                     c->d_counter);
+  FIR_CHECK_LAUNCH();
+  unsigned long long h = 0;
+  FIR_HIP_CHECK(hipMemcpyAsync(&h, c->d_counter, sizeof(h),
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  *total = (int64_t)h;
+  return FIR_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,237 @@
+// siddon.h -- device-side LOR geometry and Siddon walk for gfx950.
+//
+// The walk reproduces the reference's FP64 parametric Siddon walk
+// (Siddon.cc:112-412) operation for operation, so that the voxel-index
+// sequence of every LOR is bit-identical to the reference's and the lengths
+// are the same doubles:
+//   - per-dimension setup with the |diff| <= 1e-5 "cancelled dimension"
+//     rule (Siddon.cc:260-317),
+//   - entry/exit alphas as max/min over dimensions and [0, 1]
+//     (Siddon.cc:319-334),
+//   - start voxel by truncation and clamp (Siddon.cc:336-350),
+//   - first crossing per dimension (Siddon.cc:352-377),
+//   - the loop: next = min(alphaMax, ax, ay, az); emit when in bounds;
+//     advance every dimension whose crossing is within 1e-5 of next
+//     (Siddon.cc:205-252, 379-392).
+// The translation unit is compiled with -ffp-contract=off: every multiply
+// and add rounds separately, as on the reference's x86-64 build.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fir
+{
+
+constexpr double kEps = 0.00001;
+// (double)x > 1e-5 holds exactly when the float x > 1e-5f: float(1e-5) is
+// the float just below 1e-5 and the next float is above it.
+constexpr float kEpsF = 0.00001f;
+
+// Walk constants of one volume (reference Siddon ctor, Siddon.cc:20-52).
+struct Grid
+{
+  double low[3];
+  double high[3];
+  double step[3]; // int-truncated extents (reference) or true extents
+  int nm1[3];
+  int row;        // nx
+  int slice;      // nx * ny
+  int max_iter;   // nx + ny + nz + 8: bound on the walk loop
+};
+
+// Sinogram / scanner geometry needed to turn a work index into a LOR.
+struct LorGeom
+{
+  const double2* xy;      // [n_cpr] crystal (x, y)
+  const double* z;        // [n_slices] slice z
+  const int2* ax_slices;  // [sum n_axial] slice pair of (seg, axial)
+  const int* ax_prefix;   // [n_seg + 1] prefix sums of n_axial
+  const long long* seg_base; // [n_seg] first bin of each segment
+  int n_seg;
+  int n_views;
+  int n_tang;
+  int tang_off;
+  int n_cpr;
+};
+
+struct Lor
+{
+  double e1[3];
+  double e2[3];
+  long long bin;
+};
+
+// Work index t -> LOR. In all-bins mode (n_subsets == 0) t is the bin
+// index itself. In subset mode the rows of each segment are the views
+// subview * n_subsets + subset (reference LORCache.cc:49-51, 118-133).
+// Returns the segment's rank k via the prefix search.
+__device__ __forceinline__ void lor_from_index(const LorGeom& g, long long t,
+                                               int n_subsets, int subset,
+                                               Lor& L)
+{
+  const int vpr = n_subsets > 0 ? g.n_views / n_subsets : g.n_views;
+  const long long row = t / g.n_tang;
+  const int tt = (int)(t - row * g.n_tang);
+  // largest k with vpr * ax_prefix[k] <= row
+  int lo = 0, hi = g.n_seg;
+  while (hi - lo > 1)
+  {
+    const int mid = (lo + hi) >> 1;
+    if ((long long)vpr * g.ax_prefix[mid] <= row)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int k = lo;
+  const int a0 = g.ax_prefix[k];
+  const int na = g.ax_prefix[k + 1] - a0;
+  const long long r = row - (long long)vpr * a0;
+  const int sv = (int)(r / na);
+  const int axial = (int)(r - (long long)sv * na);
+  const int view = n_subsets > 0 ? sv * n_subsets + subset : sv;
+  L.bin = g.seg_base[k] + ((long long)view * na + axial) * g.n_tang + tt;
+
+  // (view, tang) -> angular crystals (ProjData.cc:301-340); C++ integer
+  // division truncates toward zero.
+  const int tang = tt - g.tang_off;
+  const int shift = tang >= 0 ? tang / 2 : (tang - 1) / 2;
+  const int odd = (tang < 0 ? -tang : tang) & 1;
+  int c1 = view + shift + odd;
+  int c2 = view + g.n_views - shift;
+  if (c1 < 0)
+    c1 += g.n_cpr;
+  else if (c1 >= g.n_cpr)
+    c1 -= g.n_cpr;
+  if (c2 < 0)
+    c2 += g.n_cpr;
+  else if (c2 >= g.n_cpr)
+    c2 -= g.n_cpr;
+  const int2 s = g.ax_slices[a0 + axial];
+  const double2 p1 = g.xy[c1];
+  const double2 p2 = g.xy[c2];
+  L.e1[0] = p1.x;
+  L.e1[1] = p1.y;
+  L.e1[2] = g.z[s.x];
+  L.e2[0] = p2.x;
+  L.e2[1] = p2.y;
+  L.e2[2] = g.z[s.y];
+}
+
+// Walk state after set-up; copying it replays the identical walk.
+struct Walk
+{
+  double a[3];
+  double da[3];
+  double prev;
+  double a_hi;
+  double d12;
+  int pos[3];
+  int dir[3];
+};
+
+__device__ __forceinline__ double dmin(double a, double b)
+{
+  return a < b ? a : b;
+}
+
+__device__ __forceinline__ double dmax(double a, double b)
+{
+  return a > b ? a : b;
+}
+
+__device__ __forceinline__ double dabs(double a) { return a < 0 ? -a : a; }
+
+// Set-up of the walk between e1 and e2. Returns false if the line misses the
+// volume (the reference's computePath returns false).
+__device__ __forceinline__ bool walk_setup(const Grid& g, const double* c1,
+                                           const double* c2, Walk& w)
+{
+  double diff[3], amin[3], amax[3], nxt[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    double df = c2[d] - c1[d];
+    if (dabs(df) > kEps)
+    {
+      if (df > 0)
+      {
+        w.dir[d] = 1;
+        amin[d] = (g.low[d] - c1[d]) / df;
+        amax[d] = (g.high[d] - c1[d]) / df;
+      }
+      else
+      {
+        w.dir[d] = -1;
+        amin[d] = (g.high[d] - c1[d]) / df;
+        amax[d] = (g.low[d] - c1[d]) / df;
+      }
+      nxt[d] = 0.0;
+    }
+    else
+    {
+      if (c1[d] < g.low[d] || c1[d] > g.high[d]) return false;
+      df = kEps;
+      w.dir[d] = -1;
+      amin[d] = 0.0;
+      amax[d] = 1.0;
+      nxt[d] = 1.0;
+    }
+    diff[d] = df;
+  }
+  const double a_lo = dmax(amin[0], dmax(amin[1], dmax(amin[2], 0.0)));
+  const double a_hi = dmin(amax[0], dmin(amax[1], dmin(amax[2], 1.0)));
+  if (a_lo >= a_hi) return false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) w.da[d] = g.step[d] / dabs(diff[d]);
+  w.d12 = sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    int i = (int)((c1[d] + diff[d] * a_lo - g.low[d]) / g.step[d]);
+    i = i < g.nm1[d] ? i : g.nm1[d];
+    i = 0 > i ? 0 : i;
+    w.pos[d] = i;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    double al = nxt[d];
+    if (al < 1.0) al = (g.low[d] + g.step[d] * w.pos[d] - c1[d]) / diff[d];
+    if (w.dir[d] > 0) al += w.da[d];
+    w.a[d] = al;
+  }
+  w.prev = a_lo;
+  w.a_hi = a_hi;
+  return true;
+}
+
+// Run the walk, calling emit(linear_index, length) for every in-volume
+// element in path order.
+template <class Emit>
+__device__ __forceinline__ void walk_run(const Grid& g, Walk w, Emit&& emit)
+{
+  double prev = w.prev;
+  for (int it = 0; prev < w.a_hi && it < g.max_iter; ++it)
+  {
+    const double next = dmin(w.a_hi, dmin(w.a[0], dmin(w.a[1], w.a[2])));
+    if ((unsigned)w.pos[0] <= (unsigned)g.nm1[0] &&
+        (unsigned)w.pos[1] <= (unsigned)g.nm1[1] &&
+        (unsigned)w.pos[2] <= (unsigned)g.nm1[2])
+    {
+      emit(w.pos[0] + w.pos[1] * g.row + w.pos[2] * g.slice,
+           (next - prev) * w.d12);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      if (dabs(w.a[d] - next) < kEps)
+      {
+        w.a[d] += w.da[d];
+        w.pos[d] += w.dir[d];
+      }
+    }
+    prev = next;
+  }
+}
+
+} // namespace fir

@@ -1,0 +1,210 @@
+/*
+ * fir_hip.h -- C-ABI of the MI355X (gfx950) implementation of FIR's
+ * projector and OSEM hot path.
+ *
+ * Conventions
+ *   - Every function returns 0 (FIR_OK) or a nonzero FIR_E_* code; the
+ *     message of the last failure on the calling thread is returned by
+ *     fir_last_error(). Nothing throws across this boundary. This mirrors the
+ *     reference's error() exceptions (console.inl:74-82), which the C++ host
+ *     layer re-raises from these codes.
+ *   - Image, sinogram, sensitivity and scratch pointers are DEVICE pointers
+ *     (hipMalloc'd, or fir_malloc'd). Host pointers appear only where named
+ *     (geometry tables, *_h2d/*_d2h copies).
+ *   - Work is enqueued asynchronously on the context's stream
+ *     (fir_set_stream); fir_synchronize() waits for it. Calls on one
+ *     context must not overlap; different contexts may run concurrently.
+ *   - Images are FP32, x fastest, then y, then z (reference VolData.inl:53-56),
+ *     frames contiguous. Sinograms are FP32 in the concatenated segment order
+ *     of the reference .s file: segment -S..S, each [view][axial][tang]
+ *     (reference ProjData.h:9-21, ProjInterfileReader.cc:79-93).
+ */
+#ifndef FIR_HIP_H
+#define FIR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum
+{
+  FIR_OK = 0,
+  FIR_E_ARG = 1,   /* invalid argument or geometry */
+  FIR_E_HIP = 2,   /* HIP runtime failure */
+  FIR_E_STATE = 3, /* call made before the geometry it needs was set */
+  FIR_E_NOMEM = 4  /* device allocation failed */
+};
+
+/* Siddon step extents: the reference truncates voxel extents to int for the
+ * walk (Siddon.h:15-16,113); FIR_WALK_GEOMETRIC uses the true extents. */
+enum
+{
+  FIR_WALK_REFERENCE = 0,
+  FIR_WALK_GEOMETRIC = 1
+};
+
+typedef struct fir_ctx fir_ctx;
+
+/* Volume geometry: the reference VolHeader (VolHeader.h:6-34). */
+typedef struct
+{
+  int nx, ny, nz;    /* voxels per axis */
+  double dx, dy, dz; /* voxel extent, mm */
+  double ox, oy, oz; /* x,y: centre of the first voxel; z: volume-centre offset */
+} fir_vol_desc;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fir_last_error(void);
+
+/* ABI version of this library (incremented on incompatible changes). */
+int fir_abi_version(void);
+
+/* ---- context -------------------------------------------------------- */
+
+/* Create a context on HIP device `device` with its own stream. */
+int fir_ctx_create(fir_ctx** ctx, int device);
+int fir_ctx_destroy(fir_ctx* ctx);
+/* Enqueue on an external hipStream_t (NULL: the device's null stream).
+ * A new context uses a non-blocking stream of its own. */
+int fir_set_stream(fir_ctx* ctx, void* hip_stream);
+int fir_synchronize(fir_ctx* ctx);
+
+/* ---- geometry ------------------------------------------------------- */
+
+/* Scanner position tables, computed by the host as the reference
+ * ScannerData does (ScannerData.cc:248-377): crystal_xy[2*n_cpr] holds (x,y)
+ * of each angular crystal index, slice_z[n_slices] the z of each slice
+ * (n_slices = 2*n_rings - 1). Host pointers; copied to the device. */
+int fir_set_scanner(fir_ctx* ctx, const double* crystal_xy, int n_cpr,
+                    const double* slice_z, int n_slices);
+
+/* Sinogram geometry from the reference ProjHeader keys (ProjHeader.h:15-40);
+ * validated as ProjHeader::check (ProjHeader.cc:6-106) and derived as
+ * ProjGeometry::fill (ProjHeader.cc:108-186). n_tang = 0 selects n_cpr - 1. */
+int fir_set_proj(fir_ctx* ctx, int n_rings, int n_cpr, int span,
+                 int n_segments, int n_tang);
+
+/* Number of bins of the current sinogram geometry (0 if unset). */
+int64_t fir_proj_n_bins(const fir_ctx* ctx);
+
+/* FIR_WALK_REFERENCE (default) or FIR_WALK_GEOMETRIC. */
+int fir_set_walk_mode(fir_ctx* ctx, int mode);
+
+/* ---- device memory -------------------------------------------------- */
+
+int fir_malloc(fir_ctx* ctx, void** dptr, size_t bytes);
+int fir_free(fir_ctx* ctx, void* dptr);
+int fir_memcpy_h2d(fir_ctx* ctx, void* dst, const void* src, size_t bytes);
+int fir_memcpy_d2h(fir_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Fill n floats with value (a kernel, safe under graph capture). */
+int fir_fill(fir_ctx* ctx, float* dst, int64_t n, float value);
+
+/* ---- projectors ----------------------------------------------------- */
+
+/* Forward projection of every bin: replaces projections::forward
+ * (projections.cc:15-92). sino gets fir_proj_n_bins() floats; bins whose
+ * LOR misses the volume get 0. */
+int fir_forward(fir_ctx* ctx, const fir_vol_desc* vol, const float* img,
+                float* sino);
+
+/* Forward projection of the bins of one subset (views v with
+ * v % n_subsets == subset, reference LORCache.cc:49-51); other bins are
+ * left untouched. */
+int fir_forward_subset(fir_ctx* ctx, const fir_vol_desc* vol,
+                       const float* img, float* sino, int subset,
+                       int n_subsets);
+
+/* Backprojection: replaces projections::backward (projections.cc:94-184).
+ * img has n_subsets frames; all are zeroed, then frame s receives the
+ * scatter of subset s. */
+int fir_backward(fir_ctx* ctx, const fir_vol_desc* vol, const float* sino,
+                 float* img, int n_subsets);
+
+/* Backprojection of one subset, accumulated into img (not zeroed). A NULL
+ * sino backprojects ones. */
+int fir_backward_subset(fir_ctx* ctx, const fir_vol_desc* vol,
+                        const float* sino, float* img, int subset,
+                        int n_subsets);
+
+/* Sensitivity: replaces projections::computeSensitivityVol
+ * (projections.cc:186-197): backprojection of ones, one frame per subset. */
+int fir_sensitivity(fir_ctx* ctx, const fir_vol_desc* vol, float* sens,
+                    int n_subsets);
+
+/* ---- OSEM ----------------------------------------------------------- */
+
+/* Projector pass of one OSEM sub-iteration (reconAlgos.cc:119-159): for
+ * every LOR of the subset, l = A x (+ bias); if l > 1e-5,
+ * bp += A^T (y / l). bias may be NULL. */
+int fir_osem_project(fir_ctx* ctx, const fir_vol_desc* vol, const float* x,
+                     const float* y, const float* bias, float* bp, int subset,
+                     int n_subsets);
+
+/* Update of one sub-iteration (reconAlgos.cc:161-186, VolData.cc:305-343):
+ * bp = clamp(bp / sens_s), x = clamp(x * bp), then the circular cut of x
+ * when cut_radius > 0 and no convolution follows, and bp = 0 when
+ * reset_bp != 0. */
+int fir_osem_update(fir_ctx* ctx, const fir_vol_desc* vol, float* x,
+                    float* bp, const float* sens_s, float cut_radius,
+                    int apply_cut, int reset_bp);
+
+/* One complete sub-iteration: project, update, optional Gaussian
+ * (do_convolve != 0) with edge restore, circular cut, bp reset. bp must be
+ * zero on entry. */
+int fir_osem_subiter(fir_ctx* ctx, const fir_vol_desc* vol, float* x,
+                     const float* y, const float* bias, const float* sens_s,
+                     float* bp, int subset, int n_subsets, int do_convolve,
+                     const float* fwhm_xyz, float cut_radius, int reset_bp);
+
+/* ---- image / sinogram operations ------------------------------------ */
+
+/* Separable Gaussian with border renormalisation and edge restore:
+ * replaces operations::convolve (operations.cc:11-192) on n_frames frames.
+ * fwhm_xyz is a host pointer to 3 floats. */
+int fir_convolve(fir_ctx* ctx, const fir_vol_desc* vol, float* img,
+                 int n_frames, const float* fwhm_xyz, float cut_radius);
+
+/* operations::cutCircle (operations.cc:194-220) on n_frames frames. */
+int fir_cut_circle(fir_ctx* ctx, const fir_vol_desc* vol, float* img,
+                   int n_frames, float cut_radius);
+
+/* a = (a > 1e-5 && b > 1e-5) ? a*b : 0 -- VolData::operator*=
+ * (VolData.cc:305-323) and ProjData::operator*= (ProjData.cc:481-512). */
+int fir_mul_clamped(fir_ctx* ctx, float* a, const float* b, int64_t n);
+
+/* a = (a > 1e-5 && b > 1e-5) ? a/b : 0 -- VolData::operator/=
+ * (VolData.cc:325-343). */
+int fir_div_clamped(fir_ctx* ctx, float* a, const float* b, int64_t n);
+
+/* a = a > 1e-5 ? exp(a) : 1 -- ProjData::exponential (ProjData.cc:514-539). */
+int fir_exp_clamped(fir_ctx* ctx, float* a, int64_t n);
+
+/* Hounsfield units to mu (1/mm) -- operations::HounsfieldToMuMap
+ * (operations.cc:243-284). */
+int fir_hu_to_mu(fir_ctx* ctx, float* img, int64_t n);
+
+/* ---- parity helper -------------------------------------------------- */
+
+/* Siddon paths of bins [bin0, bin0 + n_bins) (all-bins order): counts[b]
+ * = number of elements (-1 if the LOR misses the volume); idx/len receive
+ * the elements of bin b at [b * max_per_lor, ...). Replaces the path
+ * computed by Siddon::computePath (Siddon.cc:112-258). */
+int fir_path_dump(fir_ctx* ctx, const fir_vol_desc* vol, int64_t bin0,
+                  int64_t n_bins, int max_per_lor, int* counts, int* idx,
+                  double* len);
+
+/* Total number of Siddon path elements (reference semantics) over the LORs
+ * of one subset (n_subsets = 0: every bin); *total is a host pointer. Used
+ * for the algorithmic-bytes count of the projector kernels. Synchronizes
+ * the context's stream. */
+int fir_path_count(fir_ctx* ctx, const fir_vol_desc* vol, int subset,
+                   int n_subsets, int64_t* total);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FIR_HIP_H */

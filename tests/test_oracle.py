@@ -1,0 +1,415 @@
+"""Pins the CPU oracle (oracle/liboracle.so) before it is trusted as the
+checker of the HIP path:
+
+* the reference's own known answers (SiddonUnitTest.cc:80-423,
+  ProjHeaderUnitTest.cc:22-232),
+* golden outputs of the compiled reference (tests/golden/, made by
+  tests/golden/make_golden.py),
+* closed forms and invariants chosen so that a dropped term, a wrong sign
+  or index, or a transposed operand fails one of them,
+* when oracle/_ref is present (the development container), fresh random
+  and near-tie LORs against the compiled reference.
+"""
+import hashlib
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from conftest import REF_DIR, have_ref, rel_l2, rel_max
+from fir_amd import interfile as io
+from fir_amd import synth
+
+
+def _vol(d):
+    return io.VolSpec(d["size"], d["voxel"], d["offset"])
+
+
+# --------------------------------------------------------------------------
+# reference known answers
+# --------------------------------------------------------------------------
+
+def test_siddon_known_answers(siddon_cases):
+    assert len(siddon_cases) == 28
+    for i, c in enumerate(siddon_cases):
+        r = O.path(_vol(c["vol"]), c["p1"], c["p2"])
+        if not c["idx"]:
+            assert r is None, f"case {i} should miss"
+            assert c["ref_idx"] == []
+            continue
+        idx, ln = r
+        assert list(idx) == c["idx"], f"case {i}"
+        np.testing.assert_allclose(ln, c["len"][:len(idx)], atol=1e-7,
+                                   rtol=0, err_msg=f"case {i}")
+        # identical to the compiled reference's doubles
+        assert list(idx) == c["ref_idx"] and list(ln) == c["ref_len"]
+
+
+def test_proj_fill_known_answers():
+    rc, p = O.proj_fill(5, 32, 1, 3, 8)
+    assert rc == 0
+    assert (p.n_bins, p.seg_off, p.tang_off, p.n_views, p.half_span,
+            p.max_ring_diff) == (1664, 1, 4, 16, 0, 1)
+    assert list(p.n_axial[:3]) == [4, 5, 4]
+    rc, p = O.proj_fill(5, 32, 3, 3, 8)
+    assert rc == 0
+    assert (p.n_bins, p.seg_off, p.tang_off, p.n_views, p.half_span,
+            p.max_ring_diff) == (2432, 1, 4, 16, 1, 4)
+    assert list(p.n_axial[:3]) == [5, 9, 5]
+
+
+@pytest.mark.parametrize("args", [
+    (0, 4, 1, 1, 0), (-1, 4, 1, 1, 0), (1, 0, 1, 1, 0), (1, 6, 1, 1, 0),
+    (2, 8, 2, 1, 0), (2, 8, 5, 1, 0), (2, 8, 1, 2, 0), (2, 8, 1, 5, 0),
+    (2, 8, 1, 1, -1), (2, 8, 1, 1, 8),
+])
+def test_proj_check_rejects(args):
+    rc, _ = O.proj_fill(*args)
+    assert rc != 0
+
+
+def test_proj_default_tang():
+    rc, p = O.proj_fill(1, 4, 1, 1, 0)
+    assert rc == 0 and p.n_tang == 3
+
+
+# --------------------------------------------------------------------------
+# golden outputs of the compiled reference
+# --------------------------------------------------------------------------
+
+def test_scanner_tables_golden(tables_golden):
+    for name, sc in (("c1", synth.c1().scanner), ("c2", synth.c2().scanner)):
+        xy, z = O.scanner_tables(sc)
+        assert np.array_equal(xy, tables_golden[f"{name}_xy"])
+        assert np.array_equal(z, tables_golden[f"{name}_z"])
+    hdr = tables_golden["modular_hdr"]
+    sc = io.ScannerSpec(
+        rsector_repeat_number=int(hdr[17]), rsector_inner_radius=hdr[18],
+        crystal_dimensions=hdr[0:3], crystal_repeat_numbers=hdr[3:5].astype(int),
+        inter_crystal_distance=hdr[5:7], module_dimensions=hdr[7:10],
+        module_repeat_numbers=hdr[10:12].astype(int),
+        inter_module_distance=hdr[12:14], rsector_dimensions=hdr[14:17])
+    xy, z = O.scanner_tables(sc)
+    assert np.array_equal(xy, tables_golden["modular_xy"])
+    assert np.array_equal(z, tables_golden["modular_z"])
+
+
+def test_scanner_closed_form():
+    """One crystal per rSector: crystal i sits at R(cos 2pi i/N, sin 2pi i/N);
+    even slices are rings at the axial pitch, odd slices their midpoints."""
+    w = synth.c2()
+    xy, z = O.scanner_tables(w.scanner)
+    n = w.scanner.n_crystals_per_ring
+    ang = 2 * np.pi * np.arange(n) / n
+    np.testing.assert_allclose(xy[0::2], 330 * np.cos(ang), atol=1e-9)
+    np.testing.assert_allclose(xy[1::2], 330 * np.sin(ang), atol=1e-9)
+    rings = 4.0 * (np.arange(24) - 11.5)
+    np.testing.assert_allclose(z[0::2], rings, atol=1e-12)
+    np.testing.assert_allclose(z[1::2], (rings[:-1] + rings[1:]) / 2,
+                               atol=1e-12)
+
+
+def test_crystal_mapping_closed_form():
+    """c2 - c1 = nViews - t and c1 + c2 = 2 view + nViews + (t odd),
+    modulo nCrystals; span-1 slices are 2*axial and 2*(axial+|seg|)."""
+    p = O.proj_desc(synth.c2().proj)
+    nv, nc = p.n_views, p.n_cpr
+    for view in (0, 1, 57, nv - 1):
+        for t in range(-p.tang_off, -p.tang_off + p.n_tang):
+            c1, c2 = O.crystal_ang(p, view, t)
+            assert 0 <= c1 < nc and 0 <= c2 < nc
+            assert (c2 - c1) % nc == (nv - t) % nc
+            assert (c1 + c2) % nc == (2 * view + nv + (t & 1)) % nc
+    for seg in range(-p.seg_off, p.seg_off + 1):
+        for ax in range(p.n_axial[seg + p.seg_off]):
+            s1, s2 = O.crystal_axial(p, seg, ax)
+            lo, hi = (s2, s1) if seg > 0 else (s1, s2)
+            assert lo == 2 * ax and hi == 2 * (ax + abs(seg))
+
+
+def test_forward_golden_bitexact(c1_golden):
+    w = synth.c1()
+    xy, z = O.scanner_tables(w.scanner)
+    sino = O.forward(w.vol, w.proj, xy, z, c1_golden["act"])
+    assert np.array_equal(sino, c1_golden["y0"])
+
+
+def test_acf_golden_bitexact(c1_golden):
+    w = synth.c1()
+    xy, z = O.scanner_tables(w.scanner)
+    mu = O.hu_to_mu(c1_golden["hu"])
+    acf = O.exp_clamped(O.forward(w.vol, w.proj, xy, z, mu))
+    assert np.array_equal(acf, c1_golden["acf"])
+
+
+@pytest.mark.parametrize("tag", ["c1", "t1"])
+def test_paths_golden(paths_golden, tag):
+    meta, arr = paths_golden
+    m = meta[tag]
+    w = synth.c1()
+    xy, z = O.scanner_tables(w.scanner)
+    counts, idx, ln = O.path_dump(_vol(m["vol"]), w.proj, xy, z)
+    assert np.array_equal(counts, arr[f"{tag}_counts"].astype(np.int32))
+    assert idx.size == m["total"]
+    assert hashlib.sha256(idx.tobytes()).hexdigest() == m["idx_sha256"]
+    assert hashlib.sha256(ln.tobytes()).hexdigest() == m["len_sha256"]
+
+
+def test_backward_golden(c1_golden):
+    w = synth.c1()
+    xy, z = O.scanner_tables(w.scanner)
+    bp = O.backward(w.vol, w.proj, xy, z, c1_golden["y"])
+    # reference backprojection is itself non-deterministic at ~1e-7
+    assert rel_max(bp, c1_golden["bp"]) < 1e-6
+
+
+def test_sensitivity_golden(c1_golden):
+    w = synth.c1()
+    xy, z = O.scanner_tables(w.scanner)
+    sens = O.sensitivity(w.vol, w.proj, xy, z, 4)
+    assert rel_max(sens, c1_golden["sens"]) < 1e-6
+
+
+def test_osem_golden(c1_golden):
+    w = synth.c1()
+    g = c1_golden
+    xy, z = O.scanner_tables(w.scanner)
+    sens = O.sensitivity(w.vol, w.proj, xy, z, 4)
+    x0 = np.ones(w.vol.n_voxels, np.float32)
+    xa = O.osem(w.vol, w.proj, xy, z, g["y"], sens, x0, 2, 4)
+    assert rel_l2(xa, g["xa"]) < 1e-5
+    # ACF from the cut mu map, Gaussian every sub-iteration, cut 55 mm
+    mu = O.cut_circle(w.vol, O.hu_to_mu(g["hu"]), 55.0)
+    acf = O.exp_clamped(O.forward(w.vol, w.proj, xy, z, mu))
+    yb = O.mul_clamped(g["y"], acf)
+    xb = O.osem(w.vol, w.proj, xy, z, yb, sens, x0, 2, 4, cut=55.0,
+                conv_interval=1, fwhm=(4, 4, 4))
+    assert rel_l2(xb, g["xb"]) < 1e-5
+    bias = np.full_like(g["y"], float(g["bias_value"]))
+    xc = O.osem(w.vol, w.proj, xy, z, g["y"], sens, x0, 1, 4, bias=bias)
+    assert rel_l2(xc, g["xc"]) < 1e-5
+
+
+# --------------------------------------------------------------------------
+# closed forms and invariants
+# --------------------------------------------------------------------------
+
+def _slab_clip(p1, p2, lo, hi):
+    """Independent chord clipping: alpha range of the segment inside the
+    box [lo, hi] (Liang-Barsky)."""
+    a0, a1 = 0.0, 1.0
+    for d in range(3):
+        df = p2[d] - p1[d]
+        if abs(df) < 1e-12:
+            if p1[d] < lo[d] or p1[d] > hi[d]:
+                return None
+            continue
+        t0, t1 = (lo[d] - p1[d]) / df, (hi[d] - p1[d]) / df
+        a0, a1 = max(a0, min(t0, t1)), min(a1, max(t0, t1))
+    return (a0, a1) if a0 < a1 else None
+
+
+def _box(vol):
+    lo = np.array([vol.offset[0] - vol.voxel[0] / 2,
+                   vol.offset[1] - vol.voxel[1] / 2,
+                   vol.offset[2] - vol.size[2] * vol.voxel[2] / 2])
+    hi = lo + np.array(vol.size) * np.array(vol.voxel)
+    return lo, hi
+
+
+def test_chord_length_and_voxels_brute_force():
+    """Geometric walk on a tiny non-integer grid: the sum of lengths equals
+    the chord inside the box, and the voxel sequence equals the one found
+    by enumerating every plane crossing and sorting."""
+    rng = np.random.default_rng(7)
+    vol = io.VolSpec((5, 4, 3), (1.7, 2.3, 3.1), (-3.0, -2.5, 0.4))
+    lo, hi = _box(vol)
+    n_checked = 0
+    for _ in range(300):
+        p1 = rng.uniform(lo - 6, hi + 6)
+        p2 = rng.uniform(lo - 6, hi + 6)
+        r = O.path(vol, p1, p2, geometric=True)
+        clip = _slab_clip(p1, p2, lo, hi)
+        if r is None:
+            assert clip is None or clip[1] - clip[0] < 1e-9
+            continue
+        idx, ln = r
+        d12 = np.linalg.norm(p2 - p1)
+        np.testing.assert_allclose(ln.sum(), (clip[1] - clip[0]) * d12,
+                                   rtol=1e-9, atol=1e-9)
+        # brute force: all crossings in (a0, a1), sorted
+        al = [clip[0], clip[1]]
+        for d in range(3):
+            df = p2[d] - p1[d]
+            if abs(df) < 1e-12:
+                continue
+            for k in range(vol.size[d] + 1):
+                a = (lo[d] + k * vol.voxel[d] - p1[d]) / df
+                if clip[0] < a < clip[1]:
+                    al.append(a)
+        al = np.sort(np.asarray(al))
+        if np.min(np.diff(al)) < 1e-4:  # near-tie: merge rule applies
+            continue
+        mids = (al[:-1] + al[1:]) / 2
+        pts = p1[None, :] + mids[:, None] * (p2 - p1)[None, :]
+        ijk = np.floor((pts - lo) / np.array(vol.voxel)).astype(int)
+        lin = ijk[:, 0] + ijk[:, 1] * vol.size[0] + \
+            ijk[:, 2] * vol.size[0] * vol.size[1]
+        assert list(idx) == list(lin)
+        np.testing.assert_allclose(ln, np.diff(al) * d12, rtol=1e-9,
+                                   atol=1e-9)
+        n_checked += 1
+    assert n_checked > 30
+
+
+def test_reference_mode_truncates_steps():
+    """Reference walk on 3.27 mm slices steps by 3 mm (int-truncated
+    extent): an axial line through a 1x1x10 column yields elements of
+    3 mm until z leaves the 3 mm grid's in-bounds indices."""
+    vol = io.VolSpec((1, 1, 10), (2.0, 2.0, 3.27), (0.0, 0.0, 0.0))
+    r = O.path(vol, [0.1, 0.2, -40.0], [0.1, 0.2, 40.0])
+    idx, ln = r
+    np.testing.assert_allclose(ln, 3.0, atol=1e-9)
+    assert list(idx) == list(range(10))
+    rg = O.path(vol, [0.1, 0.2, -40.0], [0.1, 0.2, 40.0], geometric=True)
+    np.testing.assert_allclose(rg[1], 3.27, atol=1e-9)
+
+
+def test_adjoint_fp64():
+    """<A x, y> = <x, A^T y> with the all-FP64 oracle."""
+    w = synth.c1()
+    xy, z = O.scanner_tables(w.scanner)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0, 1, w.vol.n_voxels)
+    y = rng.uniform(0, 1, w.n_lors)
+    ax = O.forward(w.vol, w.proj, xy, z, x)
+    aty = O.backward(w.vol, w.proj, xy, z, y)
+    lhs, rhs = float(ax @ y), float(x @ aty)
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+
+
+def test_uniform_image_line_integral():
+    """Forward of a uniform image equals value x chord-in-box."""
+    w = synth.c1()
+    xy, z = O.scanner_tables(w.scanner)
+    sino = O.forward(w.vol, w.proj, xy, z,
+                     np.full(w.vol.n_voxels, 2.0, np.float64))
+    counts, idx, ln = O.path_dump(w.vol, w.proj, xy, z)
+    owner = np.repeat(np.arange(counts.size), np.maximum(counts, 0))
+    sums = np.bincount(owner, weights=ln, minlength=counts.size)
+    np.testing.assert_allclose(sino, 2.0 * sums, rtol=1e-12)
+
+
+def test_convolve_delta_closed_form():
+    w = synth.c2()
+    vol = io.VolSpec((21, 19, 11), w.vol.voxel, (0.0, 0.0, 0.0))
+    img = np.zeros(vol.n_voxels, np.float32)
+    c = (10, 9, 5)
+    img[c[0] + 21 * c[1] + 21 * 19 * c[2]] = 1.0
+    kx, ky, kz = O.gauss_taps(vol, (4.0, 4.0, 4.0))
+    assert (kx.size, ky.size, kz.size) == (7, 7, 5)
+    sx = np.float32(4.0) / np.float32(2.3548)
+    h = 3
+    ref = np.exp(-(np.arange(-h, h + 1) ** 2 * 4.0) / (2 * sx * sx))
+    np.testing.assert_allclose(kx, ref.astype(np.float32), rtol=1e-7)
+    out = O.convolve(vol, img, (4.0, 4.0, 4.0)).reshape(11, 19, 21)
+    expect = np.einsum("k,j,i->kji", kz / kz.sum(), ky / ky.sum(),
+                       kx / kx.sum())
+    got = out[c[2] - 2:c[2] + 3, c[1] - 3:c[1] + 4, c[0] - 3:c[0] + 4]
+    np.testing.assert_allclose(got, expect, rtol=2e-6, atol=1e-9)
+    assert abs(out.sum() - 1.0) < 1e-5
+
+
+def test_convolve_border_renormalised():
+    """A constant image stays constant (border taps renormalised)."""
+    vol = io.VolSpec((9, 8, 7), (2.0, 2.0, 3.27), (0.0, 0.0, 0.0))
+    img = np.full(vol.n_voxels, 3.5, np.float32)
+    out = O.convolve(vol, img, (4.0, 4.0, 4.0))
+    np.testing.assert_allclose(out, 3.5, rtol=1e-6)
+
+
+def test_cut_circle_closed_form():
+    vol = io.VolSpec((16, 12, 2), (2.0, 3.0, 1.0), (0.0, 0.0, 0.0))
+    img = np.arange(vol.n_voxels, dtype=np.float32) + 1
+    out = O.cut_circle(vol, img, 10.0).reshape(2, 12, 16)
+    px = np.arange(16) * 2.0 + 1.0 - 16.0
+    py = np.arange(12) * 3.0 + 1.5 - 18.0
+    keep = np.sqrt(px[None, :] ** 2 + py[:, None] ** 2) <= 10.0
+    ref = img.reshape(2, 12, 16) * keep[None]
+    assert np.array_equal(out, ref)
+
+
+def test_clamps_closed_form():
+    e = np.float32(1e-5)
+    nxt = np.nextafter(e, np.float32(1))
+    a = np.array([2, 2, e, nxt, -1, 3], np.float32)
+    b = np.array([3, e, 2, 2, 2, nxt], np.float32)
+    assert np.array_equal(O.mul_clamped(a, b),
+                          np.array([6, 0, 0, nxt * 2, 0, 3 * nxt], np.float32))
+    assert np.array_equal(O.div_clamped(a, b),
+                          np.array([np.float32(2) / 3, 0, 0, nxt / 2, 0,
+                                    np.float32(3) / nxt], np.float32))
+    x = np.array([0, e, nxt, 1, -2], np.float32)
+    assert np.array_equal(O.exp_clamped(x),
+                          np.array([1, 1, np.exp(np.float64(nxt)), np.exp(1.0),
+                                    1]).astype(np.float32))
+
+
+def test_hu_to_mu_closed_form():
+    hu = np.array([-2000, -1000, -500, 0, 500, 1000, 3000], np.float32)
+    mu = O.hu_to_mu(hu)
+    np.testing.assert_allclose(
+        mu, [0, 0, 0.0048, 0.0096, 0.0123, 0.015, 0.0258], rtol=1e-6,
+        atol=1e-9)
+
+
+def test_fp64_oracle_close_to_fp32(c1_golden):
+    """The all-FP64 oracle agrees with the reference-precision oracle to
+    FP32 rounding (a transposed or dropped term would not)."""
+    w = synth.c1()
+    xy, z = O.scanner_tables(w.scanner)
+    s32 = O.forward(w.vol, w.proj, xy, z, c1_golden["act"])
+    s64 = O.forward(w.vol, w.proj, xy, z, c1_golden["act"].astype(np.float64))
+    assert rel_max(s32, s64) < 1e-6
+
+
+# --------------------------------------------------------------------------
+# live comparison with the compiled reference (development container only)
+# --------------------------------------------------------------------------
+
+@pytest.mark.skipif(not have_ref(), reason="oracle/_ref not built here")
+def test_random_and_near_tie_lors_vs_reference():
+    rng = np.random.default_rng(11)
+    vols = [io.VolSpec((7, 6, 5), (2.0, 2.0, 3.27), (-6.0, -5.0, 0.3)),
+            io.VolSpec((6, 6, 4), (1.0, 3.0, 2.0), (-2.5, -7.5, -1.0))]
+    drv = os.path.join(REF_DIR, "ref_driver")
+    n = 0
+    for vol in vols:
+        lo, hi = _box(vol)
+        for k in range(60):
+            p1 = rng.uniform(lo - 5, hi + 5)
+            if k % 3 == 0:
+                # through a voxel corner (near-simultaneous crossings)
+                corner = lo + np.array(vol.voxel) * rng.integers(1, 3, 3)
+                corner[2] = lo[2] + vol.voxel[2] * 1  # exact plane
+                p2 = p1 + 2.3 * (corner + rng.normal(0, 1e-7, 3) - p1)
+            else:
+                p2 = rng.uniform(lo - 5, hi + 5)
+            args = [drv, "segpath"] + [str(s) for s in vol.size]
+            args += [repr(float(v)) for v in list(vol.voxel) +
+                     list(vol.offset) + list(p1) + list(p2)]
+            out = subprocess.run(args, capture_output=True, text=True,
+                                 check=True).stdout.split()
+            m = int(out[0])
+            r = O.path(vol, p1, p2)
+            if m < 0:
+                assert r is None
+                continue
+            ridx = [int(out[1 + 2 * i]) for i in range(m)]
+            rlen = [float(out[2 + 2 * i]) for i in range(m)]
+            assert r is not None
+            assert list(r[0]) == ridx and list(r[1]) == rlen
+            n += 1
+    assert n > 20
