@@ -1,0 +1,35 @@
+// reconAlgos.h -- OSEM of the FIR host API (reference reconAlgos.h:9-45),
+// run on the GPU with every image and sinogram resident in HBM.
+#pragma once
+
+#include <ProjData.h>
+#include <ScannerData.h>
+#include <VolData.h>
+
+#include <optional>
+#include <string>
+#include <vector>
+
+struct OSEMCoreParams
+{
+  int nIterations{1};
+  int nSubsets{1};
+  int saveInterval{0};
+  float cutRadius{0.0};
+  int convolutionInterval{0};
+  std::vector<float> fwhmXYZ{0.0, 0.0, 0.0};
+};
+
+namespace reconAlgos
+{
+// OSEM on the active frame of outputVol (the initial estimate). Per
+// sub-iteration: projector pass over the subset's LORs (bias added to the
+// forward projection when given), bp /= sensitivity frame, x *= bp,
+// Gaussian every convolutionInterval sub-iterations, circular cut.
+// Intermediate images go to <outputVolFileName>_subiter_<n> every
+// saveInterval sub-iterations.
+void OSEM(const ProjData& inputProj, const ScannerData& scanner,
+          VolData& outputVol, const std::string& outputVolFileName,
+          const OSEMCoreParams& params, const VolData& sensitivityMap,
+          const std::optional<ProjData>& biasProj);
+}

@@ -1,0 +1,169 @@
+#include <ProjHeader.h>
+#include <ScannerData.h>
+#include <console.h>
+#include <device.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace device
+{
+namespace
+{
+struct State
+{
+  fir_ctx* ctx = nullptr;
+  // geometry currently loaded in ctx
+  std::vector<double> xy, z;
+  ProjHeader proj{};
+  bool projSet = false;
+
+  ~State()
+  {
+    if (ctx) fir_ctx_destroy(ctx);
+  }
+};
+
+State& state()
+{
+  static State s;
+  return s;
+}
+}
+
+void check(int rc)
+{
+  if (rc != FIR_OK) error(fir_last_error());
+}
+
+fir_ctx* context()
+{
+  State& s = state();
+  if (!s.ctx)
+  {
+    const char* env = std::getenv("FIR_DEVICE");
+    const int dev = env ? std::atoi(env) : 0;
+    check(fir_ctx_create(&s.ctx, dev));
+  }
+  return s.ctx;
+}
+
+Buffer::Buffer(size_t bytes) : mBytes(bytes)
+{
+  if (bytes) check(fir_malloc(context(), &mPtr, bytes));
+}
+
+Buffer::~Buffer()
+{
+  if (mPtr) fir_free(context(), mPtr);
+}
+
+Buffer::Buffer(Buffer&& o) noexcept : mPtr(o.mPtr), mBytes(o.mBytes)
+{
+  o.mPtr = nullptr;
+  o.mBytes = 0;
+}
+
+Buffer& Buffer::operator=(Buffer&& o) noexcept
+{
+  if (this != &o)
+  {
+    if (mPtr) fir_free(context(), mPtr);
+    mPtr = o.mPtr;
+    mBytes = o.mBytes;
+    o.mPtr = nullptr;
+    o.mBytes = 0;
+  }
+  return *this;
+}
+
+void Buffer::upload(const void* host, size_t bytes)
+{
+  if (bytes > mBytes) error("device buffer too small for upload");
+  if (bytes) check(fir_memcpy_h2d(context(), mPtr, host, bytes));
+}
+
+void Buffer::download(void* host, size_t bytes) const
+{
+  if (bytes > mBytes) error("device buffer too small for download");
+  if (bytes) check(fir_memcpy_d2h(context(), host, mPtr, bytes));
+}
+
+fir_vol_desc describe(const VolHeader& h)
+{
+  fir_vol_desc d;
+  d.nx = h.volSize.nPixelsX;
+  d.ny = h.volSize.nPixelsY;
+  d.nz = h.volSize.nSlices;
+  d.dx = h.voxelExtent.pixelWidth;
+  d.dy = h.voxelExtent.pixelHeight;
+  d.dz = h.voxelExtent.sliceThickness;
+  d.ox = h.volOffset.x;
+  d.oy = h.volOffset.y;
+  d.oz = h.volOffset.z;
+  return d;
+}
+
+void useGeometry(const ScannerData& scanner, const ProjHeader& proj)
+{
+  State& s = state();
+  fir_ctx* c = context();
+  const int n = scanner.getGeometry().nCrystalsPerRing;
+  const int ns = scanner.getGeometry().nSlices;
+  std::vector<double> xy(2 * static_cast<size_t>(n));
+  const auto* p = scanner.getCrystalXYPositionVector();
+  for (int i = 0; i < n; ++i)
+  {
+    xy[2 * i] = p[i].x;
+    xy[2 * i + 1] = p[i].y;
+  }
+  std::vector<double> z(scanner.getSliceZPositionVector(),
+                        scanner.getSliceZPositionVector() + ns);
+  if (xy != s.xy || z != s.z)
+  {
+    check(fir_set_scanner(c, xy.data(), n, z.data(), ns));
+    s.xy = xy;
+    s.z = z;
+  }
+  if (!s.projSet || s.proj != proj)
+  {
+    check(fir_set_proj(c, proj.nRings, proj.nCrystalsPerRing,
+                       proj.segmentSpan, proj.nSegments, proj.nTangCoords));
+    s.proj = proj;
+    s.projSet = true;
+  }
+}
+
+void clampedOp(ClampedOp op, float* a, const float* b, long long n)
+{
+  if (n <= 0) return;
+  const size_t bytes = sizeof(float) * static_cast<size_t>(n);
+  Buffer da(bytes), db(bytes);
+  da.upload(a, bytes);
+  db.upload(b, bytes);
+  check(op == ClampedOp::Mul ? fir_mul_clamped(context(), da.f(), db.f(), n)
+                             : fir_div_clamped(context(), da.f(), db.f(), n));
+  da.download(a, bytes);
+}
+
+void expClamped(float* a, long long n)
+{
+  if (n <= 0) return;
+  const size_t bytes = sizeof(float) * static_cast<size_t>(n);
+  Buffer da(bytes);
+  da.upload(a, bytes);
+  check(fir_exp_clamped(context(), da.f(), n));
+  da.download(a, bytes);
+}
+
+void huToMu(float* a, long long n)
+{
+  if (n <= 0) return;
+  const size_t bytes = sizeof(float) * static_cast<size_t>(n);
+  Buffer da(bytes);
+  da.upload(a, bytes);
+  check(fir_hu_to_mu(context(), da.f(), n));
+  da.download(a, bytes);
+}
+}

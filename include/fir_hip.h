@@ -10,7 +10,7 @@
  *     layer re-raises from these codes.
  *   - Image, sinogram, sensitivity and scratch pointers are DEVICE pointers
  *     (hipMalloc'd, or fir_malloc'd). Host pointers appear only where named
- *     (geometry tables, *_h2d/*_d2h copies).
+ *     (geometry tables and the h2d/d2h copies).
  *   - Work is enqueued asynchronously on the context's stream
  *     (fir_set_stream); fir_synchronize() waits for it. Calls on one
  *     context must not overlap; different contexts may run concurrently.

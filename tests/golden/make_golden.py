@@ -20,6 +20,8 @@ Outputs (all data: inputs and the reference's outputs):
   reference's int-truncated step extents.
 * ``tables_golden.npz`` -- crystal/slice position tables for three
   scanners (c1, c2 and a multi-module scanner with gaps).
+* ``ref_headers.json`` -- the text of .h33/.hs headers written by the
+  reference executables (byte-compatibility of the writers).
 """
 from __future__ import annotations
 
@@ -226,6 +228,13 @@ def main():
                     bias_projection="bias.hs")
         run([os.path.join(REF, "FIR_OSEM"), "c.params"], d)
         xc = io.read_vol(os.path.join(d, "xc.h33"))
+
+        headers = {}
+        for name in ("bp.h33", "y0.hs", "sens.h33", "acf.hs", "xa.h33"):
+            with open(os.path.join(d, name)) as f:
+                headers[name] = f.read()
+        with open(os.path.join(HERE, "ref_headers.json"), "w") as f:
+            json.dump(headers, f, indent=1)
 
         np.savez_compressed(
             os.path.join(HERE, "c1_golden.npz"), act=act, hu=hu, y0=y0,
