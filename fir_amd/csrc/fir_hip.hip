@@ -291,6 +291,63 @@ __global__ __launch_bounds__(kBlock) void k_osem_project(
   });
 }
 
+// Poisson negative log-likelihood L = sum_b (l_b - y_b log l_b), with
+// l = A x + r, and its gradient A^T (1 - y / l), fused per LOR: walk,
+// line integral, weight, replay of the walk for the scatter. Bins with
+// l <= 0 contribute l to L and weight 1 (their y log l term is dropped).
+// L is reduced per block in double and added to *loss.
+__global__ __launch_bounds__(kBlock) void k_poisson_grad(
+  Grid g, LorGeom lg, const float* __restrict__ x, const float* __restrict__ y,
+  const float* __restrict__ r, float* __restrict__ grad,
+  double* __restrict__ loss, long long n)
+{
+  __shared__ double part[kBlock];
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  double term = 0.0;
+  if (t < n)
+  {
+    Lor L;
+    fir::lor_from_index(lg, t, 0, 0, L);
+    Walk w;
+    const bool hit = fir::walk_setup(g, L.e1, L.e2, w);
+    float line = 0.0f;
+    if (hit)
+      fir::walk_run(g, w, [&](int idx, double len) {
+        line = (float)((double)line + len * (double)x[idx]);
+      });
+    const float l = r ? line + r[L.bin] : line;
+    const float yb = y[L.bin];
+    float wgt;
+    if (l > 0.0f)
+    {
+      term = (double)l - (double)yb * log((double)l);
+      wgt = 1.0f - yb / l;
+    }
+    else
+    {
+      term = (double)l;
+      wgt = 1.0f;
+    }
+    if (hit && wgt != 0.0f)
+    {
+      const double dw = (double)wgt;
+      fir::walk_run(g, w, [&](int idx, double len) {
+        unsafeAtomicAdd(grad + idx, (float)(len * dw));
+      });
+    }
+  }
+  part[threadIdx.x] = term;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1)
+  {
+    if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) unsafeAtomicAdd(loss, part[0]);
+}
+
+__global__ void k_zero_f64(double* p) { *p = 0.0; }
+
 // Path dump for parity checks.
 __global__ __launch_bounds__(kBlock) void k_path_dump(Grid g, LorGeom lg,
                                                       long long bin0,
@@ -1204,6 +1261,27 @@ int fir_path_count(fir_ctx* c, const fir_vol_desc* v, int subset,
                                hipMemcpyDeviceToHost, c->stream));
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
   *total = (int64_t)h;
+  return FIR_OK;
+}
+
+int fir_poisson_nll_grad(fir_ctx* c, const fir_vol_desc* v, const float* x,
+                         const float* y, const float* r, float* grad,
+                         double* loss)
+{
+  int rc = need_lors(c);
+  if (rc) return rc;
+  if (!x || !y || !grad || !loss) return fail(FIR_E_ARG, "NULL buffer");
+  Grid g;
+  if ((rc = make_grid(c, v, &g))) return rc;
+  const long long n = c->n_bins;
+  const long long nvox = (long long)v->nx * v->ny * v->nz;
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_fill, dim3(ew_blocks(nvox)), dim3(kBlock), 0,
+                     c->stream, grad, nvox, 0.0f);
+  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, c->stream, loss);
+  hipLaunchKernelGGL(k_poisson_grad, dim3(blocks_for(n)), dim3(kBlock), 0,
+                     c->stream, g, lor_geom(c), x, y, r, grad, loss, n);
+  FIR_CHECK_LAUNCH();
   return FIR_OK;
 }
 

@@ -29,7 +29,7 @@ EXPORTS = (
     "fir_sensitivity", "fir_osem_project", "fir_osem_update",
     "fir_osem_subiter", "fir_convolve", "fir_cut_circle", "fir_mul_clamped",
     "fir_div_clamped", "fir_exp_clamped", "fir_hu_to_mu", "fir_path_dump",
-    "fir_path_count",
+    "fir_path_count", "fir_poisson_nll_grad",
 )
 
 
@@ -97,6 +97,7 @@ def load() -> C.CDLL:
         "fir_hu_to_mu": [vp, vp, i64],
         "fir_path_dump": [vp, P, i64, i64, i, vp, vp, vp],
         "fir_path_count": [vp, P, i, i, C.POINTER(C.c_int64)],
+        "fir_poisson_nll_grad": [vp, P, vp, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -244,6 +245,12 @@ class Context:
 
     def hu_to_mu(self, a):
         _check(self.lib.fir_hu_to_mu(self.h, _ptr(a), a.numel()))
+
+    def poisson_nll_grad(self, vol, x, y, r, grad, loss):
+        """loss[0] = sum(l - y log l), grad = A^T(1 - y/l), l = A x + r."""
+        _check(self.lib.fir_poisson_nll_grad(self.h, C.byref(vol), _ptr(x),
+                                             _ptr(y), _ptr(r), _ptr(grad),
+                                             _ptr(loss)))
 
     def path_count(self, vol, subset=0, n_subsets=0) -> int:
         """Total path elements over one subset (n_subsets=0: all bins)."""

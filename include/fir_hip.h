@@ -159,6 +159,19 @@ int fir_osem_subiter(fir_ctx* ctx, const fir_vol_desc* vol, float* x,
                      float* bp, int subset, int n_subsets, int do_convolve,
                      const float* fwhm_xyz, float cut_radius, int reset_bp);
 
+/* ---- differentiable projector --------------------------------------- */
+
+/* Poisson negative log-likelihood of x for data y with additive term r:
+ * *loss = sum_b (l_b - y_b log l_b), l = A x + r (r may be NULL), and
+ * grad = A^T (1 - y / l), the exact adjoint of fir_forward applied to the
+ * weights. Bins with l <= 0 contribute l and weight 1. grad (N_vox floats)
+ * and loss (one double) are device pointers, both overwritten. No
+ * counterpart in the reference (its OSEM needs no gradient); it backs the
+ * PyTorch autograd function of the projector. */
+int fir_poisson_nll_grad(fir_ctx* ctx, const fir_vol_desc* vol,
+                         const float* x, const float* y, const float* r,
+                         float* grad, double* loss);
+
 /* ---- image / sinogram operations ------------------------------------ */
 
 /* Separable Gaussian with border renormalisation and edge restore:
