@@ -348,6 +348,57 @@ __global__ __launch_bounds__(kBlock) void k_poisson_grad(
 
 __global__ void k_zero_f64(double* p) { *p = 0.0; }
 
+// Batched OSEM projector pass: NB reconstructions that share the image
+// geometry and subset schedule. One walk per LOR serves all NB images: the
+// walk (FP64 ALU work) is paid once, each element gathers NB voxels and
+// scatters NB ratios.
+constexpr int kMaxBatch = 8;
+
+struct BatchPtrs
+{
+  const float* x[kMaxBatch];
+  const float* y[kMaxBatch];
+  const float* bias[kMaxBatch];
+  float* bp[kMaxBatch];
+};
+
+template <int NB>
+__global__ __launch_bounds__(kBlock) void k_osem_project_batch(
+  Grid g, LorGeom lg, BatchPtrs p, long long n, int n_subsets, int subset)
+{
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  Lor L;
+  fir::lor_from_index(lg, t, n_subsets, subset, L);
+  Walk w;
+  if (!fir::walk_setup(g, L.e1, L.e2, w)) return;
+  float line[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) line[b] = 0.0f;
+  fir::walk_run(g, w, [&](int idx, double len) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      line[b] = (float)((double)line[b] + len * (double)p.x[b][idx]);
+  });
+  double ratio[NB];
+  bool any = false;
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+  {
+    float l = line[b];
+    if (p.bias[b]) l += p.bias[b][L.bin];
+    const float r = l > kEpsF ? p.y[b][L.bin] / l : 0.0f;
+    ratio[b] = (double)r;
+    any = any || r != 0.0f;
+  }
+  if (!any) return;
+  fir::walk_run(g, w, [&](int idx, double len) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if (ratio[b] != 0.0) unsafeAtomicAdd(p.bp[b] + idx, (float)(len * ratio[b]));
+  });
+}
+
 // Path dump for parity checks.
 __global__ __launch_bounds__(kBlock) void k_path_dump(Grid g, LorGeom lg,
                                                       long long bin0,
@@ -629,8 +680,8 @@ int fir_ctx_create(fir_ctx** out, int device)
 int fir_ctx_destroy(fir_ctx* c)
 {
   if (!c) return FIR_OK;
-  hipSetDevice(c->device);
-  hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->d_xy);
   (void)hipFree(c->d_z);
   (void)hipFree(c->d_ax_slices);
@@ -639,7 +690,7 @@ int fir_ctx_destroy(fir_ctx* c)
   (void)hipFree(c->d_scratch);
   (void)hipFree(c->d_taps);
   (void)hipFree(c->d_counter);
-  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return FIR_OK;
 }
@@ -1282,6 +1333,54 @@ int fir_poisson_nll_grad(fir_ctx* c, const fir_vol_desc* v, const float* x,
   hipLaunchKernelGGL(k_poisson_grad, dim3(blocks_for(n)), dim3(kBlock), 0,
                      c->stream, g, lor_geom(c), x, y, r, grad, loss, n);
   FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+int fir_osem_project_batch(fir_ctx* c, const fir_vol_desc* v, int n_images,
+                           const float* const* x, const float* const* y,
+                           const float* const* bias, float* const* bp,
+                           int subset, int n_subsets)
+{
+  int rc = need_lors(c);
+  if (rc) return rc;
+  if (n_images <= 0) return fail(FIR_E_ARG, "n_images must be > 0");
+  if (!x || !y || !bp) return fail(FIR_E_ARG, "NULL pointer array");
+  for (int i = 0; i < n_images; ++i)
+    if (!x[i] || !y[i] || !bp[i]) return fail(FIR_E_ARG, "NULL buffer");
+  Grid g;
+  if ((rc = make_grid(c, v, &g))) return rc;
+  if ((rc = check_subsets(c, n_subsets))) return rc;
+  if (subset < 0 || subset >= n_subsets)
+    return fail(FIR_E_ARG, "subset %d out of range", subset);
+  const long long n = n_work(c, n_subsets);
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  for (int i0 = 0; i0 < n_images; i0 += kMaxBatch)
+  {
+    const int nb = std::min(kMaxBatch, n_images - i0);
+    BatchPtrs p;
+    std::memset(&p, 0, sizeof(p));
+    for (int b = 0; b < nb; ++b)
+    {
+      p.x[b] = x[i0 + b];
+      p.y[b] = y[i0 + b];
+      p.bias[b] = bias ? bias[i0 + b] : nullptr;
+      p.bp[b] = bp[i0 + b];
+    }
+    const dim3 grid(blocks_for(n)), block(kBlock);
+    const LorGeom lg = lor_geom(c);
+    switch (nb)
+    {
+    case 1: hipLaunchKernelGGL(k_osem_project_batch<1>, grid, block, 0, c->stream, g, lg, p, n, n_subsets, subset); break;
+    case 2: hipLaunchKernelGGL(k_osem_project_batch<2>, grid, block, 0, c->stream, g, lg, p, n, n_subsets, subset); break;
+    case 3: hipLaunchKernelGGL(k_osem_project_batch<3>, grid, block, 0, c->stream, g, lg, p, n, n_subsets, subset); break;
+    case 4: hipLaunchKernelGGL(k_osem_project_batch<4>, grid, block, 0, c->stream, g, lg, p, n, n_subsets, subset); break;
+    case 5: hipLaunchKernelGGL(k_osem_project_batch<5>, grid, block, 0, c->stream, g, lg, p, n, n_subsets, subset); break;
+    case 6: hipLaunchKernelGGL(k_osem_project_batch<6>, grid, block, 0, c->stream, g, lg, p, n, n_subsets, subset); break;
+    case 7: hipLaunchKernelGGL(k_osem_project_batch<7>, grid, block, 0, c->stream, g, lg, p, n, n_subsets, subset); break;
+    default: hipLaunchKernelGGL(k_osem_project_batch<8>, grid, block, 0, c->stream, g, lg, p, n, n_subsets, subset); break;
+    }
+    FIR_CHECK_LAUNCH();
+  }
   return FIR_OK;
 }
 

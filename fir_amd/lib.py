@@ -29,7 +29,7 @@ EXPORTS = (
     "fir_sensitivity", "fir_osem_project", "fir_osem_update",
     "fir_osem_subiter", "fir_convolve", "fir_cut_circle", "fir_mul_clamped",
     "fir_div_clamped", "fir_exp_clamped", "fir_hu_to_mu", "fir_path_dump",
-    "fir_path_count", "fir_poisson_nll_grad",
+    "fir_path_count", "fir_poisson_nll_grad", "fir_osem_project_batch",
 )
 
 
@@ -98,6 +98,7 @@ def load() -> C.CDLL:
         "fir_path_dump": [vp, P, i64, i64, i, vp, vp, vp],
         "fir_path_count": [vp, P, i, i, C.POINTER(C.c_int64)],
         "fir_poisson_nll_grad": [vp, P, vp, vp, vp, vp, vp],
+        "fir_osem_project_batch": [vp, P, i, vp, vp, vp, vp, i, i],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -207,6 +208,18 @@ class Context:
         _check(self.lib.fir_osem_project(self.h, C.byref(vol), _ptr(x),
                                          _ptr(y), _ptr(bias), _ptr(bp),
                                          subset, n_subsets))
+
+    def osem_project_batch(self, vol, xs, ys, biases, bps, subset,
+                           n_subsets):
+        """One projector pass for several reconstructions of one geometry."""
+        n = len(xs)
+        arr = C.c_void_p * n
+        px = arr(*[_ptr(t) for t in xs])
+        py = arr(*[_ptr(t) for t in ys])
+        pb = None if biases is None else arr(*[_ptr(t) for t in biases])
+        pp = arr(*[_ptr(t) for t in bps])
+        _check(self.lib.fir_osem_project_batch(self.h, C.byref(vol), n, px,
+                                               py, pb, pp, subset, n_subsets))
 
     def osem_update(self, vol, x, bp, sens_s, cut=0.0, apply_cut=True,
                     reset=True):

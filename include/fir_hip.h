@@ -143,6 +143,16 @@ int fir_osem_project(fir_ctx* ctx, const fir_vol_desc* vol, const float* x,
                      const float* y, const float* bias, float* bp, int subset,
                      int n_subsets);
 
+/* Projector pass for n_images reconstructions that share the image
+ * geometry and the subset schedule (noise realisations, dynamic frames):
+ * x, y, bias, bp are HOST arrays of n_images DEVICE pointers (bias may be
+ * NULL, or hold NULL entries). Each LOR is walked once for up to 8 images;
+ * each image gets exactly what fir_osem_project would give it. */
+int fir_osem_project_batch(fir_ctx* ctx, const fir_vol_desc* vol,
+                           int n_images, const float* const* x,
+                           const float* const* y, const float* const* bias,
+                           float* const* bp, int subset, int n_subsets);
+
 /* Update of one sub-iteration (reconAlgos.cc:161-186, VolData.cc:305-343):
  * bp = clamp(bp / sens_s), x = clamp(x * bp), then the circular cut of x
  * when cut_radius > 0 and no convolution follows, and bp = 0 when

@@ -292,3 +292,24 @@ def test_errors_are_reported():
     with pytest.raises(firlib.FirError, match="subset 4 out of range"):
         ctx.backward_subset(v, y, bp, 4, 4)
     ctx.close()
+
+
+def test_batched_projector_matches_single(c1ctx, c1_golden):
+    """fir_osem_project_batch on 11 images (two launches: 8 + 3) gives each
+    image what fir_osem_project gives it."""
+    w, ctx = c1ctx
+    v = firlib.VolDesc.from_spec(w.vol)
+    rng = np.random.default_rng(12)
+    n = 11
+    xs = [dev(rng.uniform(0.2, 2, v.n_voxels).astype(np.float32))
+          for _ in range(n)]
+    ys = [dev(rng.poisson(c1_golden["y0"] * (i + 1) / 4).astype(np.float32))
+          for i in range(n)]
+    bias = [None if i % 3 else torch.full((ctx.n_bins,), 0.1, device="cuda")
+            for i in range(n)]
+    bps = [torch.zeros(v.n_voxels, device="cuda") for _ in range(n)]
+    ctx.osem_project_batch(v, xs, ys, bias, bps, 1, 4)
+    for i in range(n):
+        ref = torch.zeros(v.n_voxels, device="cuda")
+        ctx.osem_project(v, xs[i], ys[i], bias[i], ref, 1, 4)
+        assert rel_max(host(bps[i]), host(ref)) < 1e-5, i
