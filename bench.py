@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=8,
+                    help="reconstructions in the batched measurement (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not "
@@ -136,6 +138,7 @@ def main():
     g = torch.Generator(device="cuda").manual_seed(w.seed)
     y = torch.poisson(mean.float(), generator=g)
     ctx.mul_clamped(y, acf)  # FIR_OSEM: inputProj *= ACF (OSEM.cc:336-337)
+    mean_f = mean.float()
     sens = torch.zeros(n_sub * nvox, device="cuda")
     ctx.sensitivity(v, sens, n_sub)
     x = torch.ones(nvox, device="cuda")
@@ -204,6 +207,42 @@ def main():
     bp_ms = e0.elapsed_time(e1) / reps
     log(f"forward {fwd_ms:.3f} ms, backprojection {bp_ms:.3f} ms")
 
+    # --- batched: noise realisations sharing one walk per LOR ------------
+    batched = None
+    if args.batch > 0:
+        from fir_amd import recon
+        g2 = torch.Generator(device="cuda").manual_seed(w.seed + 1)
+        scales = np.logspace(-2, 1, args.batch)
+        ys = []
+        for sc in scales:
+            yi = torch.poisson((mean_f * float(sc)), generator=g2)
+            ctx.mul_clamped(yi, acf)
+            ys.append(yi)
+        grp = recon.OSEMGroup(ctx, v, n_sub, ys, sens=sens, fwhm=fwhm,
+                              conv_interval=1)
+        grp.capture()
+        grp.iteration()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            grp.iteration()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        bms = e0.elapsed_time(e1) / args.steps
+        batched = {
+            "n_images": args.batch,
+            "osem_iter_ms": round(bms, 4),
+            "osem_iter_ms_per_recon": round(bms / args.batch, 4),
+            "mlors_s_per_recon_aggregate": round(
+                args.batch * w.n_lors / (bms * 1e-3) / 1e6, 3),
+            "speedup_per_recon_vs_plain": round(ms_step * args.batch / bms, 3),
+            "graph": "one HIP graph per iteration (16 sub-iterations)",
+        }
+        log(f"batched x{args.batch}: {bms:.3f} ms per iteration "
+            f"({bms / args.batch:.3f} ms per reconstruction)")
+        del grp, ys
+        ctx.set_stream(stream)
+
     # --- roofline of the dominant kernel (fused OSEM projector) ----------
     # Algorithmic bytes per launch (one sub-iteration's LORs): forward +
     # backprojection of the cached-path format, 12 B per LOR and 6 B per
@@ -264,6 +303,7 @@ def main():
             "avg_launch_ms": round(proj_avg_ms, 4),
             "algorithmic_bytes_per_launch": int(bytes_launch),
         },
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     if cpu:

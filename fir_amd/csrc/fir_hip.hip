@@ -63,6 +63,8 @@ struct fir_ctx
   long long* d_seg_base = nullptr;
 
   int walk_mode = FIR_WALK_REFERENCE;
+  // allow subset counts that do not divide the number of views
+  bool uneven_subsets = false;
 
   // 64-bit device counter of fir_path_count
   unsigned long long* d_counter = nullptr;
@@ -184,6 +186,8 @@ int need_lors(const fir_ctx* ctx)
 
 int check_subsets(const fir_ctx* ctx, int n_subsets)
 {
+  if (ctx->uneven_subsets && n_subsets > 0 && n_subsets <= ctx->n_views)
+    return FIR_OK;
   if (n_subsets <= 0 || ctx->n_views % n_subsets != 0)
     return fail(FIR_E_ARG, "Number of subsets must be a divisor of %d",
                 ctx->n_views);
@@ -206,9 +210,9 @@ LorGeom lor_geom(const fir_ctx* ctx)
   return g;
 }
 
-long long n_work(const fir_ctx* ctx, int n_subsets)
+long long n_work(const fir_ctx* ctx, int n_subsets, int subset)
 {
-  const long long vpr = n_subsets > 0 ? ctx->n_views / n_subsets : ctx->n_views;
+  const long long vpr = fir::views_in_subset(ctx->n_views, n_subsets, subset);
   return vpr * ctx->ax_prefix.back() * (long long)ctx->n_tang;
 }
 
@@ -236,9 +240,12 @@ __global__ __launch_bounds__(kBlock) void k_forward(Grid g, LorGeom lg,
   float line = 0.0f;
   if (fir::walk_setup(g, L.e1, L.e2, w))
   {
-    fir::walk_run(g, w, [&](int idx, double len) {
-      line = (float)((double)line + len * (double)img[idx]);
-    });
+    fir::walk_gather(
+      g, w, [&](int idx) { return img[idx]; },
+      [&](float v, double len, bool ok) {
+        const float nl = (float)((double)line + len * (double)v);
+        line = ok ? nl : line;
+      });
   }
   sino[L.bin] = line;
 }
@@ -278,9 +285,12 @@ __global__ __launch_bounds__(kBlock) void k_osem_project(
   Walk w;
   if (!fir::walk_setup(g, L.e1, L.e2, w)) return; // empty path: nothing to scatter
   float line = 0.0f;
-  fir::walk_run(g, w, [&](int idx, double len) {
-    line = (float)((double)line + len * (double)x[idx]);
-  });
+  fir::walk_gather(
+    g, w, [&](int idx) { return x[idx]; },
+    [&](float v, double len, bool ok) {
+      const float nl = (float)((double)line + len * (double)v);
+      line = ok ? nl : line;
+    });
   if (bias) line += bias[L.bin];
   if (!(line > kEpsF)) return;
   const float ratio = y[L.bin] / line;
@@ -312,9 +322,12 @@ __global__ __launch_bounds__(kBlock) void k_poisson_grad(
     const bool hit = fir::walk_setup(g, L.e1, L.e2, w);
     float line = 0.0f;
     if (hit)
-      fir::walk_run(g, w, [&](int idx, double len) {
-        line = (float)((double)line + len * (double)x[idx]);
-      });
+      fir::walk_gather(
+        g, w, [&](int idx) { return x[idx]; },
+        [&](float v, double len, bool ok) {
+          const float nl = (float)((double)line + len * (double)v);
+          line = ok ? nl : line;
+        });
     const float l = r ? line + r[L.bin] : line;
     const float yb = y[L.bin];
     float wgt;
@@ -375,11 +388,26 @@ __global__ __launch_bounds__(kBlock) void k_osem_project_batch(
   float line[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) line[b] = 0.0f;
-  fir::walk_run(g, w, [&](int idx, double len) {
+  struct Vals
+  {
+    float v[NB];
+  };
+  fir::walk_gather(
+    g, w,
+    [&](int idx) {
+      Vals r;
 #pragma unroll
-    for (int b = 0; b < NB; ++b)
-      line[b] = (float)((double)line[b] + len * (double)p.x[b][idx]);
-  });
+      for (int b = 0; b < NB; ++b) r.v[b] = p.x[b][idx];
+      return r;
+    },
+    [&](const Vals& r, double len, bool ok) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+      {
+        const float nl = (float)((double)line[b] + len * (double)r.v[b]);
+        line[b] = ok ? nl : line[b];
+      }
+    });
   double ratio[NB];
   bool any = false;
 #pragma unroll
@@ -976,7 +1004,7 @@ int fir_forward_subset(fir_ctx* c, const fir_vol_desc* v, const float* img,
     if (subset < 0 || subset >= n_subsets)
       return fail(FIR_E_ARG, "subset %d out of range", subset);
   }
-  const long long n = n_work(c, n_subsets);
+  const long long n = n_work(c, n_subsets, subset);
   FIR_HIP_CHECK(hipSetDevice(c->device));
   hipLaunchKernelGGL(k_forward, dim3(blocks_for(n)), dim3(kBlock), 0,
                      c->stream, g, lor_geom(c), img, sino, n, n_subsets,
@@ -1002,7 +1030,7 @@ int fir_backward_subset(fir_ctx* c, const fir_vol_desc* v, const float* sino,
   if ((rc = check_subsets(c, n_subsets))) return rc;
   if (subset < 0 || subset >= n_subsets)
     return fail(FIR_E_ARG, "subset %d out of range", subset);
-  const long long n = n_work(c, n_subsets);
+  const long long n = n_work(c, n_subsets, subset);
   FIR_HIP_CHECK(hipSetDevice(c->device));
   hipLaunchKernelGGL(k_backward, dim3(blocks_for(n)), dim3(kBlock), 0,
                      c->stream, g, lor_geom(c), sino, img, n, n_subsets,
@@ -1045,7 +1073,7 @@ int fir_osem_project(fir_ctx* c, const fir_vol_desc* v, const float* x,
   if ((rc = check_subsets(c, n_subsets))) return rc;
   if (subset < 0 || subset >= n_subsets)
     return fail(FIR_E_ARG, "subset %d out of range", subset);
-  const long long n = n_work(c, n_subsets);
+  const long long n = n_work(c, n_subsets, subset);
   FIR_HIP_CHECK(hipSetDevice(c->device));
   hipLaunchKernelGGL(k_osem_project, dim3(blocks_for(n)), dim3(kBlock), 0,
                      c->stream, g, lor_geom(c), x, y, bias, bp, n, n_subsets,
@@ -1301,7 +1329,7 @@ int fir_path_count(fir_ctx* c, const fir_vol_desc* v, int subset,
   FIR_HIP_CHECK(hipSetDevice(c->device));
   if (!c->d_counter)
     FIR_HIP_CHECK(hipMalloc(&c->d_counter, sizeof(unsigned long long)));
-  const long long n = n_work(c, n_subsets);
+  const long long n = n_work(c, n_subsets, subset);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, c->stream, c->d_counter);
   hipLaunchKernelGGL(k_path_count, dim3(blocks_for(n)), dim3(kBlock), 0,
                      c->stream, g, lor_geom(c), n, n_subsets, subset,
@@ -1352,7 +1380,7 @@ int fir_osem_project_batch(fir_ctx* c, const fir_vol_desc* v, int n_images,
   if ((rc = check_subsets(c, n_subsets))) return rc;
   if (subset < 0 || subset >= n_subsets)
     return fail(FIR_E_ARG, "subset %d out of range", subset);
-  const long long n = n_work(c, n_subsets);
+  const long long n = n_work(c, n_subsets, subset);
   FIR_HIP_CHECK(hipSetDevice(c->device));
   for (int i0 = 0; i0 < n_images; i0 += kMaxBatch)
   {
@@ -1381,6 +1409,13 @@ int fir_osem_project_batch(fir_ctx* c, const fir_vol_desc* v, int n_images,
     }
     FIR_CHECK_LAUNCH();
   }
+  return FIR_OK;
+}
+
+int fir_set_uneven_subsets(fir_ctx* c, int allow)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  c->uneven_subsets = allow != 0;
   return FIR_OK;
 }
 

@@ -61,15 +61,26 @@ struct Lor
   long long bin;
 };
 
+// Views of one subset: v = subview * n_subsets + subset, v < n_views
+// (reference LORCache.cc:49-51). With a divisor n_subsets this is
+// n_views / n_subsets for every subset; otherwise the first subsets get one
+// view more.
+__host__ __device__ __forceinline__ int views_in_subset(int n_views,
+                                                        int n_subsets,
+                                                        int subset)
+{
+  return n_subsets > 0 ? (n_views - subset + n_subsets - 1) / n_subsets
+                       : n_views;
+}
+
 // Work index t -> LOR. In all-bins mode (n_subsets == 0) t is the bin
-// index itself. In subset mode the rows of each segment are the views
-// subview * n_subsets + subset (reference LORCache.cc:49-51, 118-133).
-// Returns the segment's rank k via the prefix search.
+// index itself. In subset mode the rows of each segment are the views of
+// the subset (reference LORCache.cc:118-133).
 __device__ __forceinline__ void lor_from_index(const LorGeom& g, long long t,
                                                int n_subsets, int subset,
                                                Lor& L)
 {
-  const int vpr = n_subsets > 0 ? g.n_views / n_subsets : g.n_views;
+  const int vpr = views_in_subset(g.n_views, n_subsets, subset);
   const long long row = t / g.n_tang;
   const int tt = (int)(t - row * g.n_tang);
   // largest k with vpr * ax_prefix[k] <= row
@@ -232,6 +243,48 @@ __device__ __forceinline__ void walk_run(const Grid& g, Walk w, Emit&& emit)
     }
     prev = next;
   }
+}
+
+// Gather walk, software-pipelined by one element: at step k the voxel
+// load of element k is issued (fetch) before element k-1 is accumulated
+// (consume), so each load's latency overlaps one step of walk arithmetic.
+// Elements are consumed in path order, so accumulations are unchanged.
+// fetch(idx) must accept idx = 0 (used for out-of-volume steps, whose value
+// is ignored); consume(value, length, valid) must ignore calls with
+// valid == false.
+template <class Fetch, class Consume>
+__device__ __forceinline__ void walk_gather(const Grid& g, Walk w,
+                                            Fetch&& fetch, Consume&& consume)
+{
+  double prev = w.prev;
+  bool have = false;
+  auto pv = fetch(0);
+  double plen = 0.0;
+  for (int it = 0; prev < w.a_hi && it < g.max_iter; ++it)
+  {
+    const double next = dmin(w.a_hi, dmin(w.a[0], dmin(w.a[1], w.a[2])));
+    const bool inb = (unsigned)w.pos[0] <= (unsigned)g.nm1[0] &&
+      (unsigned)w.pos[1] <= (unsigned)g.nm1[1] &&
+      (unsigned)w.pos[2] <= (unsigned)g.nm1[2];
+    const int idx = inb ? w.pos[0] + w.pos[1] * g.row + w.pos[2] * g.slice : 0;
+    const auto v = fetch(idx);
+    const double len = (next - prev) * w.d12;
+    consume(pv, plen, have);
+    pv = v;
+    plen = len;
+    have = inb;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      if (dabs(w.a[d] - next) < kEps)
+      {
+        w.a[d] += w.da[d];
+        w.pos[d] += w.dir[d];
+      }
+    }
+    prev = next;
+  }
+  consume(pv, plen, have);
 }
 
 } // namespace fir

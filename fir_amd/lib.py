@@ -30,6 +30,7 @@ EXPORTS = (
     "fir_osem_subiter", "fir_convolve", "fir_cut_circle", "fir_mul_clamped",
     "fir_div_clamped", "fir_exp_clamped", "fir_hu_to_mu", "fir_path_dump",
     "fir_path_count", "fir_poisson_nll_grad", "fir_osem_project_batch",
+    "fir_set_uneven_subsets",
 )
 
 
@@ -80,6 +81,7 @@ def load() -> C.CDLL:
         "fir_set_scanner": [vp, vp, i, vp, i],
         "fir_set_proj": [vp, i, i, i, i, i],
         "fir_set_walk_mode": [vp, i],
+        "fir_set_uneven_subsets": [vp, i],
         "fir_fill": [vp, vp, i64, f],
         "fir_forward": [vp, P, vp, vp],
         "fir_forward_subset": [vp, P, vp, vp, i, i],
@@ -178,6 +180,9 @@ class Context:
 
     def set_walk_mode(self, mode: int):
         _check(self.lib.fir_set_walk_mode(self.h, mode))
+
+    def set_uneven_subsets(self, allow: bool):
+        _check(self.lib.fir_set_uneven_subsets(self.h, int(allow)))
 
     @property
     def n_bins(self) -> int:

@@ -1,0 +1,150 @@
+"""Device-resident OSEM drivers over the C-ABI.
+
+``OSEMGroup`` runs several OSEM reconstructions that share the scanner,
+the sinogram geometry, the image geometry and the subset count (noise
+realisations or dynamic frames of one scan): every sub-iteration is one
+batched projector launch (fir_osem_project_batch: one walk per LOR serves
+up to 8 images) followed by the per-image update, Gaussian and cut. After a
+first eager iteration, one whole iteration is captured in a HIP graph and
+replayed.
+
+``BatchedOSEM`` spreads reconstructions of uneven size (different image
+grids, subset counts and count levels) over such groups.
+
+Per sub-iteration each image follows reconAlgos::OSEM
+(reconAlgos.cc:119-186): projector pass, bp /= sens, x *= bp, Gaussian every
+``conv_interval`` sub-iterations (with edge restore), circular cut.
+"""
+from __future__ import annotations
+
+from typing import List, Optional, Sequence
+
+import torch
+
+from . import lib as firlib
+
+
+class OSEMGroup:
+    def __init__(self, ctx: firlib.Context, vol: firlib.VolDesc,
+                 n_subsets: int, ys: Sequence[torch.Tensor],
+                 biases: Optional[Sequence[Optional[torch.Tensor]]] = None,
+                 sens: Optional[torch.Tensor] = None,
+                 fwhm=(0.0, 0.0, 0.0), conv_interval: int = 0,
+                 cut: float = 0.0, x0: Optional[Sequence[torch.Tensor]] = None):
+        self.ctx, self.vol, self.n_subsets = ctx, vol, n_subsets
+        nv = vol.n_voxels
+        self.ys = [y.contiguous() for y in ys]
+        self.biases = list(biases) if biases is not None else None
+        self.fwhm = tuple(float(f) for f in fwhm)
+        self.conv_interval = conv_interval
+        self.cut = float(cut)
+        self.convolve = conv_interval > 0 and all(f > 0 for f in self.fwhm)
+        if self.convolve and n_subsets % conv_interval != 0:
+            raise ValueError("the convolution schedule must repeat every "
+                             "iteration (conv_interval must divide the "
+                             "number of subsets)")
+        dev = self.ys[0].device
+        if sens is None:
+            sens = torch.zeros(n_subsets * nv, device=dev)
+            ctx.sensitivity(vol, sens, n_subsets)
+        self.sens = sens
+        self.x = ([t.clone() for t in x0] if x0 is not None else
+                  [torch.ones(nv, device=dev) for _ in self.ys])
+        self.bp = [torch.zeros(nv, device=dev) for _ in self.ys]
+        for x in self.x:
+            ctx.cut_circle(vol, x, self.cut)
+        self.graph = None
+        self.iterations_done = 0
+
+    @property
+    def n_images(self) -> int:
+        return len(self.ys)
+
+    def _subiteration(self, subset: int):
+        nv = self.vol.n_voxels
+        ctx, vol = self.ctx, self.vol
+        ctx.osem_project_batch(vol, self.x, self.ys, self.biases, self.bp,
+                               subset, self.n_subsets)
+        sens_s = self.sens[subset * nv:(subset + 1) * nv]
+        conv = self.convolve and (subset + 1) % self.conv_interval == 0
+        for x, bp in zip(self.x, self.bp):
+            ctx.osem_update(vol, x, bp, sens_s, self.cut,
+                            apply_cut=not conv, reset=True)
+            if conv:
+                ctx.convolve(vol, x, self.fwhm, self.cut)
+                ctx.cut_circle(vol, x, self.cut)
+
+    def iteration_eager(self):
+        for s in range(self.n_subsets):
+            self._subiteration(s)
+        self.iterations_done += 1
+
+    def capture(self):
+        """Capture one iteration in a HIP graph (after one eager iteration,
+        which also sizes the library's Gaussian scratch and taps, so that
+        nothing allocates or synchronizes during capture)."""
+        if self.iterations_done == 0:
+            self.iteration_eager()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.ctx.set_stream(torch.cuda.current_stream())
+            for s in range(self.n_subsets):
+                self._subiteration(s)
+        self.ctx.set_stream(torch.cuda.current_stream())
+        self.graph = g
+
+    def iteration(self):
+        if self.graph is None:
+            self.iteration_eager()
+        else:
+            self.graph.replay()
+            self.iterations_done += 1
+
+
+class BatchedOSEM:
+    """Reconstructions of uneven size grouped by (image grid, subset count).
+
+    ``solves`` is a list of dicts with keys ``vol`` (VolDesc), ``n_subsets``
+    and ``y`` (device sinogram); every solve shares one scanner/sinogram
+    geometry, given by ``workload``. Contexts are made per group.
+    """
+
+    def __init__(self, workload, solves: List[dict], fwhm=(0.0, 0.0, 0.0),
+                 conv_interval: int = 0, cut: float = 0.0, device: int = 0):
+        keys = []
+        for s in solves:
+            v = s["vol"]
+            k = (v.nx, v.ny, v.nz, v.dx, v.dy, v.dz, v.ox, v.oy, v.oz,
+                 s["n_subsets"])
+            if k not in keys:
+                keys.append(k)
+        self.groups = []
+        self.members = []
+        for k in keys:
+            idx = [i for i, s in enumerate(solves)
+                   if (s["vol"].nx, s["vol"].ny, s["vol"].nz, s["vol"].dx,
+                       s["vol"].dy, s["vol"].dz, s["vol"].ox, s["vol"].oy,
+                       s["vol"].oz, s["n_subsets"]) == k]
+            ctx = firlib.context_for(workload, device)
+            ctx.set_uneven_subsets(True)
+            vol = solves[idx[0]]["vol"]
+            g = OSEMGroup(ctx, vol, k[-1], [solves[i]["y"] for i in idx],
+                          fwhm=fwhm, conv_interval=conv_interval, cut=cut)
+            self.groups.append(g)
+            self.members.append(idx)
+
+    def capture(self):
+        for g in self.groups:
+            g.capture()
+
+    def iteration(self):
+        for g in self.groups:
+            g.iteration()
+
+    def images(self) -> List[torch.Tensor]:
+        out = [None] * sum(len(m) for m in self.members)
+        for g, m in zip(self.groups, self.members):
+            for j, i in enumerate(m):
+                out[i] = g.x[j]
+        return out

@@ -93,6 +93,13 @@ int64_t fir_proj_n_bins(const fir_ctx* ctx);
 /* FIR_WALK_REFERENCE (default) or FIR_WALK_GEOMETRIC. */
 int fir_set_walk_mode(fir_ctx* ctx, int mode);
 
+/* Subset counts: by default n_subsets must divide the number of views, as
+ * the reference requires (ProjData.cc:166-174). With allow != 0 any count
+ * from 1 to nViews is accepted; subset s is still the views
+ * v = k * n_subsets + s, so the first nViews % n_subsets subsets hold one
+ * view more (an extension: the reference defines no such split). */
+int fir_set_uneven_subsets(fir_ctx* ctx, int allow);
+
 /* ---- device memory -------------------------------------------------- */
 
 int fir_malloc(fir_ctx* ctx, void** dptr, size_t bytes);

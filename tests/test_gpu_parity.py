@@ -313,3 +313,28 @@ def test_batched_projector_matches_single(c1ctx, c1_golden):
         ref = torch.zeros(v.n_voxels, device="cuda")
         ctx.osem_project(v, xs[i], ys[i], bias[i], ref, 1, 4)
         assert rel_max(host(bps[i]), host(ref)) < 1e-5, i
+
+
+def test_uneven_subsets_partition_the_views(c1ctx, c1_golden):
+    """With uneven subsets enabled, 5 subsets of the 64 views (interleaved,
+    13/13/13/13/12 views) cover every bin exactly once, and their
+    backprojections sum to the single-subset backprojection."""
+    w, ctx = c1ctx
+    v = firlib.VolDesc.from_spec(w.vol)
+    img = dev(c1_golden["act"].ravel())
+    with pytest.raises(firlib.FirError, match="divisor"):
+        ctx.forward(v, img, torch.zeros(ctx.n_bins, device="cuda"),
+                    subset=0, n_subsets=5)
+    ctx.set_uneven_subsets(True)
+    try:
+        sino = torch.full((ctx.n_bins,), -1.0, device="cuda")
+        for s in range(5):
+            ctx.forward(v, img, sino, subset=s, n_subsets=5)
+        assert np.array_equal(host(sino), c1_golden["y0"])
+        y = dev(c1_golden["y"])
+        parts = torch.zeros(5 * v.n_voxels, device="cuda")
+        ctx.backward(v, y, parts, 5)
+        total = host(parts).reshape(5, -1).sum(0)
+        assert rel_max(total, c1_golden["bp"]) < 1e-5
+    finally:
+        ctx.set_uneven_subsets(False)
