@@ -56,11 +56,19 @@ struct fir_ctx
   int seg_off = 0, tang_off = 0, n_views = 0, half_span = 0;
   int64_t n_bins = 0;
   std::vector<int> n_axial;
-  std::vector<int> ax_prefix;
   std::vector<long long> seg_base;
-  int2* d_ax_slices = nullptr;
-  int* d_ax_prefix = nullptr;
+  std::vector<int2> slices; // slice pair per (segment rank, axial), flattened
+  int n_pairs = 0;
+  int4* d_pairs = nullptr;  // rows in processing order (see fir_set_lor_order)
+  int4* d_pairs_bins = nullptr; // rows in sinogram order (index == bin)
+  int* d_n_axial = nullptr;
   long long* d_seg_base = nullptr;
+  int lor_order = FIR_ORDER_ZCENTER;
+
+  // private per-XCD copies of scatter targets (1: scatter directly)
+  int bp_copies = 1;
+  float* d_copies = nullptr;
+  size_t copies_floats = 0;
 
   int walk_mode = FIR_WALK_REFERENCE;
   // allow subset counts that do not divide the number of views
@@ -199,10 +207,9 @@ LorGeom lor_geom(const fir_ctx* ctx)
   LorGeom g;
   g.xy = ctx->d_xy;
   g.z = ctx->d_z;
-  g.ax_slices = ctx->d_ax_slices;
-  g.ax_prefix = ctx->d_ax_prefix;
+  g.pairs = ctx->d_pairs;
+  g.n_axial = ctx->d_n_axial;
   g.seg_base = ctx->d_seg_base;
-  g.n_seg = ctx->n_seg;
   g.n_views = ctx->n_views;
   g.n_tang = ctx->n_tang;
   g.tang_off = ctx->tang_off;
@@ -213,7 +220,65 @@ LorGeom lor_geom(const fir_ctx* ctx)
 long long n_work(const fir_ctx* ctx, int n_subsets, int subset)
 {
   const long long vpr = fir::views_in_subset(ctx->n_views, n_subsets, subset);
-  return vpr * ctx->ax_prefix.back() * (long long)ctx->n_tang;
+  return vpr * ctx->n_pairs * (long long)ctx->n_tang;
+}
+
+// Row order of the projector kernels: the (segment, axial) pairs sorted by
+// segment then axial (the sinogram's own order), or by the centre slice
+// s1 + s2 so that rows processed together cover a thin z-slab of the
+// image and their gathers and scatters stay in L2.
+std::vector<int4> pair_order(const fir_ctx* c, int order)
+{
+  std::vector<int4> pairs;
+  int flat = 0;
+  for (int k = 0; k < c->n_seg; ++k)
+    for (int a = 0; a < c->n_axial[k]; ++a, ++flat)
+      pairs.push_back(make_int4(k, a, c->slices[flat].x, c->slices[flat].y));
+  if (order == FIR_ORDER_ZCENTER)
+    std::stable_sort(pairs.begin(), pairs.end(),
+                     [&](const int4& u, const int4& v) {
+                       const int cu = u.z + u.w, cv = v.z + v.w;
+                       if (cu != cv) return cu < cv;
+                       const int su = std::abs(u.x - c->seg_off);
+                       const int sv = std::abs(v.x - c->seg_off);
+                       return su < sv;
+                     });
+  return pairs;
+}
+
+int upload_pairs(fir_ctx* c)
+{
+  const std::vector<int4> pairs = pair_order(c, c->lor_order);
+  const std::vector<int4> bins = pair_order(c, FIR_ORDER_SEGMENT);
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  (void)hipFree(c->d_pairs);
+  (void)hipFree(c->d_pairs_bins);
+  c->d_pairs = nullptr;
+  c->d_pairs_bins = nullptr;
+  FIR_HIP_CHECK(hipMalloc(&c->d_pairs, sizeof(int4) * pairs.size()));
+  FIR_HIP_CHECK(hipMalloc(&c->d_pairs_bins, sizeof(int4) * bins.size()));
+  FIR_HIP_CHECK(hipMemcpy(c->d_pairs, pairs.data(), sizeof(int4) * pairs.size(),
+                          hipMemcpyHostToDevice));
+  FIR_HIP_CHECK(hipMemcpy(c->d_pairs_bins, bins.data(),
+                          sizeof(int4) * bins.size(), hipMemcpyHostToDevice));
+  c->n_pairs = (int)pairs.size();
+  return FIR_OK;
+}
+
+// LorGeom whose work index is the bin index (sinogram order).
+LorGeom lor_geom_bins(const fir_ctx* ctx)
+{
+  LorGeom g = lor_geom(ctx);
+  g.pairs = ctx->d_pairs_bins;
+  return g;
+}
+
+// Work items of one launch must fit the kernels' 32-bit index decode.
+int check_work(long long n)
+{
+  if (n > 0x7fffffffLL)
+    return fail(FIR_E_ARG, "more than 2^31-1 LORs in one launch");
+  return FIR_OK;
 }
 
 } // namespace
@@ -254,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_forward(Grid g, LorGeom lg,
 // backprojects ones.
 __global__ __launch_bounds__(kBlock) void k_backward(Grid g, LorGeom lg,
                                                      const float* __restrict__ sino,
-                                                     float* __restrict__ img,
+                                                     fir::Scatter sc,
                                                      long long n, int n_subsets,
                                                      int subset)
 {
@@ -267,6 +332,7 @@ __global__ __launch_bounds__(kBlock) void k_backward(Grid g, LorGeom lg,
   Walk w;
   if (!fir::walk_setup(g, L.e1, L.e2, w)) return;
   const double dv = (double)val;
+  float* img = fir::scatter_target(sc);
   fir::walk_run(g, w, [&](int idx, double len) {
     unsafeAtomicAdd(img + idx, (float)(len * dv));
   });
@@ -275,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_backward(Grid g, LorGeom lg,
 // Fused OSEM projector pass (reconAlgos.cc:126-158).
 __global__ __launch_bounds__(kBlock) void k_osem_project(
   Grid g, LorGeom lg, const float* __restrict__ x, const float* __restrict__ y,
-  const float* __restrict__ bias, float* __restrict__ bp, long long n,
+  const float* __restrict__ bias, fir::Scatter sc, long long n,
   int n_subsets, int subset)
 {
   const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -296,6 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_osem_project(
   const float ratio = y[L.bin] / line;
   if (ratio == 0.0f) return;
   const double dr = (double)ratio;
+  float* bp = fir::scatter_target(sc);
   fir::walk_run(g, w, [&](int idx, double len) {
     unsafeAtomicAdd(bp + idx, (float)(len * dr));
   });
@@ -308,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_osem_project(
 // L is reduced per block in double and added to *loss.
 __global__ __launch_bounds__(kBlock) void k_poisson_grad(
   Grid g, LorGeom lg, const float* __restrict__ x, const float* __restrict__ y,
-  const float* __restrict__ r, float* __restrict__ grad,
+  const float* __restrict__ r, fir::Scatter sc,
   double* __restrict__ loss, long long n)
 {
   __shared__ double part[kBlock];
@@ -344,6 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_poisson_grad(
     if (hit && wgt != 0.0f)
     {
       const double dw = (double)wgt;
+      float* grad = fir::scatter_target(sc);
       fir::walk_run(g, w, [&](int idx, double len) {
         unsafeAtomicAdd(grad + idx, (float)(len * dw));
       });
@@ -373,6 +441,8 @@ struct BatchPtrs
   const float* y[kMaxBatch];
   const float* bias[kMaxBatch];
   float* bp[kMaxBatch];
+  long long copy_stride; // per-XCD copies: image b's copy c at bp[b] + c * stride
+  int copies;
 };
 
 template <int NB>
@@ -420,10 +490,15 @@ __global__ __launch_bounds__(kBlock) void k_osem_project_batch(
     any = any || r != 0.0f;
   }
   if (!any) return;
+  const long long off =
+    p.copies > 1 ? (long long)(__builtin_amdgcn_s_getreg((20) | (3 << 11)) %
+                               (unsigned)p.copies) * p.copy_stride
+                 : 0;
   fir::walk_run(g, w, [&](int idx, double len) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
-      if (ratio[b] != 0.0) unsafeAtomicAdd(p.bp[b] + idx, (float)(len * ratio[b]));
+      if (ratio[b] != 0.0)
+        unsafeAtomicAdd(p.bp[b] + off + idx, (float)(len * ratio[b]));
   });
 }
 
@@ -487,6 +562,24 @@ __global__ __launch_bounds__(kBlock) void k_path_count(
 }
 
 __global__ void k_zero_u64(unsigned long long* p) { *p = 0ULL; }
+
+// dst += sum of the private copies; the copies are cleared for next use.
+__global__ void k_reduce_copies(float* __restrict__ dst,
+                                float* __restrict__ copies, long long n,
+                                int n_copies, long long stride)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+  {
+    float acc = dst[i];
+    for (int c = 0; c < n_copies; ++c)
+    {
+      acc += copies[c * stride + i];
+      copies[c * stride + i] = 0.0f;
+    }
+    dst[i] = acc;
+  }
+}
 
 __global__ void k_fill(float* __restrict__ a, long long n, float v)
 {
@@ -673,6 +766,54 @@ __global__ __launch_bounds__(kBlock) void k_gauss(const float* __restrict__ in,
 
 } // namespace
 
+namespace
+{
+// Scatter set-up: with bp_copies > 1, `n_images` images of n floats get
+// bp_copies private copies each in the context's copies buffer (kept zero
+// between launches); image b's copy c sits at b * n + c * n_images * n.
+int ensure_copies(fir_ctx* c, size_t floats)
+{
+  if (c->copies_floats >= floats) return FIR_OK;
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  (void)hipFree(c->d_copies);
+  c->d_copies = nullptr;
+  c->copies_floats = 0;
+  if (hipMalloc(&c->d_copies, floats * sizeof(float)) != hipSuccess)
+    return fail(FIR_E_NOMEM, "allocation of %zu floats of scatter copies failed",
+                floats);
+  c->copies_floats = floats;
+  hipLaunchKernelGGL(k_fill, dim3(ew_blocks((long long)floats)), dim3(kBlock),
+                     0, c->stream, c->d_copies, (long long)floats, 0.0f);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+int scatter_for(fir_ctx* c, float* dst, long long n, int n_images,
+                fir::Scatter* sc)
+{
+  if (c->bp_copies <= 1)
+  {
+    *sc = fir::Scatter{dst, 0, 1};
+    return FIR_OK;
+  }
+  int rc = ensure_copies(c, (size_t)n * n_images * c->bp_copies);
+  if (rc) return rc;
+  *sc = fir::Scatter{c->d_copies, n * n_images, c->bp_copies};
+  return FIR_OK;
+}
+
+// Add the copies of image `b` (of n_images) into dst and clear them.
+int gather_copies(fir_ctx* c, float* dst, long long n, int n_images, int b)
+{
+  if (c->bp_copies <= 1) return FIR_OK;
+  hipLaunchKernelGGL(k_reduce_copies, dim3(ew_blocks(n)), dim3(kBlock), 0,
+                     c->stream, dst, c->d_copies + (long long)b * n, n,
+                     c->bp_copies, n * n_images);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+} // namespace
+
 // ---------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------
@@ -712,9 +853,11 @@ int fir_ctx_destroy(fir_ctx* c)
   (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->d_xy);
   (void)hipFree(c->d_z);
-  (void)hipFree(c->d_ax_slices);
-  (void)hipFree(c->d_ax_prefix);
+  (void)hipFree(c->d_pairs);
+  (void)hipFree(c->d_pairs_bins);
+  (void)hipFree(c->d_n_axial);
   (void)hipFree(c->d_seg_base);
+  (void)hipFree(c->d_copies);
   (void)hipFree(c->d_scratch);
   (void)hipFree(c->d_taps);
   (void)hipFree(c->d_counter);
@@ -892,23 +1035,18 @@ int fir_set_proj(fir_ctx* c, int n_rings, int n_cpr, int span, int n_seg,
 
   FIR_HIP_CHECK(hipSetDevice(c->device));
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
-  (void)hipFree(c->d_ax_slices);
-  (void)hipFree(c->d_ax_prefix);
+  (void)hipFree(c->d_n_axial);
   (void)hipFree(c->d_seg_base);
-  c->d_ax_slices = nullptr;
-  c->d_ax_prefix = nullptr;
+  c->d_n_axial = nullptr;
   c->d_seg_base = nullptr;
-  FIR_HIP_CHECK(hipMalloc(&c->d_ax_slices, sizeof(int2) * sl.size()));
-  FIR_HIP_CHECK(hipMalloc(&c->d_ax_prefix, sizeof(int) * pre.size()));
+  c->proj_set = false;
+  FIR_HIP_CHECK(hipMalloc(&c->d_n_axial, sizeof(int) * na.size()));
   FIR_HIP_CHECK(hipMalloc(&c->d_seg_base, sizeof(long long) * base.size()));
-  FIR_HIP_CHECK(hipMemcpy(c->d_ax_slices, sl.data(), sizeof(int2) * sl.size(),
-                          hipMemcpyHostToDevice));
-  FIR_HIP_CHECK(hipMemcpy(c->d_ax_prefix, pre.data(), sizeof(int) * pre.size(),
+  FIR_HIP_CHECK(hipMemcpy(c->d_n_axial, na.data(), sizeof(int) * na.size(),
                           hipMemcpyHostToDevice));
   FIR_HIP_CHECK(hipMemcpy(c->d_seg_base, base.data(),
                           sizeof(long long) * base.size(),
                           hipMemcpyHostToDevice));
-  c->proj_set = true;
   c->n_rings = n_rings;
   c->span = span;
   c->n_seg = n_seg;
@@ -919,8 +1057,11 @@ int fir_set_proj(fir_ctx* c, int n_rings, int n_cpr, int span, int n_seg,
   c->half_span = half;
   c->n_bins = nbins;
   c->n_axial = na;
-  c->ax_prefix = pre;
   c->seg_base = base;
+  c->slices = sl;
+  int rc = upload_pairs(c);
+  if (rc) return rc;
+  c->proj_set = true;
   return FIR_OK;
 }
 
@@ -1005,6 +1146,7 @@ int fir_forward_subset(fir_ctx* c, const fir_vol_desc* v, const float* img,
       return fail(FIR_E_ARG, "subset %d out of range", subset);
   }
   const long long n = n_work(c, n_subsets, subset);
+  if ((rc = check_work(n))) return rc;
   FIR_HIP_CHECK(hipSetDevice(c->device));
   hipLaunchKernelGGL(k_forward, dim3(blocks_for(n)), dim3(kBlock), 0,
                      c->stream, g, lor_geom(c), img, sino, n, n_subsets,
@@ -1031,12 +1173,16 @@ int fir_backward_subset(fir_ctx* c, const fir_vol_desc* v, const float* sino,
   if (subset < 0 || subset >= n_subsets)
     return fail(FIR_E_ARG, "subset %d out of range", subset);
   const long long n = n_work(c, n_subsets, subset);
+  if ((rc = check_work(n))) return rc;
   FIR_HIP_CHECK(hipSetDevice(c->device));
+  const long long nvox = (long long)v->nx * v->ny * v->nz;
+  fir::Scatter sc;
+  if ((rc = scatter_for(c, img, nvox, 1, &sc))) return rc;
   hipLaunchKernelGGL(k_backward, dim3(blocks_for(n)), dim3(kBlock), 0,
-                     c->stream, g, lor_geom(c), sino, img, n, n_subsets,
+                     c->stream, g, lor_geom(c), sino, sc, n, n_subsets,
                      subset);
   FIR_CHECK_LAUNCH();
-  return FIR_OK;
+  return gather_copies(c, img, nvox, 1, 0);
 }
 
 int fir_backward(fir_ctx* c, const fir_vol_desc* v, const float* sino,
@@ -1074,12 +1220,16 @@ int fir_osem_project(fir_ctx* c, const fir_vol_desc* v, const float* x,
   if (subset < 0 || subset >= n_subsets)
     return fail(FIR_E_ARG, "subset %d out of range", subset);
   const long long n = n_work(c, n_subsets, subset);
+  if ((rc = check_work(n))) return rc;
   FIR_HIP_CHECK(hipSetDevice(c->device));
+  const long long nvox = (long long)v->nx * v->ny * v->nz;
+  fir::Scatter sc;
+  if ((rc = scatter_for(c, bp, nvox, 1, &sc))) return rc;
   hipLaunchKernelGGL(k_osem_project, dim3(blocks_for(n)), dim3(kBlock), 0,
-                     c->stream, g, lor_geom(c), x, y, bias, bp, n, n_subsets,
+                     c->stream, g, lor_geom(c), x, y, bias, sc, n, n_subsets,
                      subset);
   FIR_CHECK_LAUNCH();
-  return FIR_OK;
+  return gather_copies(c, bp, nvox, 1, 0);
 }
 
 int fir_osem_update(fir_ctx* c, const fir_vol_desc* v, float* x, float* bp,
@@ -1305,8 +1455,9 @@ int fir_path_dump(fir_ctx* c, const fir_vol_desc* v, int64_t bin0,
   if (!counts || !idx || !len) return fail(FIR_E_ARG, "NULL buffer");
   if (nb == 0) return FIR_OK;
   FIR_HIP_CHECK(hipSetDevice(c->device));
+  if ((rc = check_work(c->n_bins))) return rc;
   hipLaunchKernelGGL(k_path_dump, dim3(blocks_for(nb)), dim3(kBlock), 0,
-                     c->stream, g, lor_geom(c), (long long)bin0,
+                     c->stream, g, lor_geom_bins(c), (long long)bin0,
                      (long long)nb, maxp, counts, idx, len);
   FIR_CHECK_LAUNCH();
   return FIR_OK;
@@ -1330,6 +1481,7 @@ int fir_path_count(fir_ctx* c, const fir_vol_desc* v, int subset,
   if (!c->d_counter)
     FIR_HIP_CHECK(hipMalloc(&c->d_counter, sizeof(unsigned long long)));
   const long long n = n_work(c, n_subsets, subset);
+  if ((rc = check_work(n))) return rc;
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, c->stream, c->d_counter);
   hipLaunchKernelGGL(k_path_count, dim3(blocks_for(n)), dim3(kBlock), 0,
                      c->stream, g, lor_geom(c), n, n_subsets, subset,
@@ -1358,10 +1510,13 @@ int fir_poisson_nll_grad(fir_ctx* c, const fir_vol_desc* v, const float* x,
   hipLaunchKernelGGL(k_fill, dim3(ew_blocks(nvox)), dim3(kBlock), 0,
                      c->stream, grad, nvox, 0.0f);
   hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, c->stream, loss);
+  if ((rc = check_work(n))) return rc;
+  fir::Scatter sc;
+  if ((rc = scatter_for(c, grad, nvox, 1, &sc))) return rc;
   hipLaunchKernelGGL(k_poisson_grad, dim3(blocks_for(n)), dim3(kBlock), 0,
-                     c->stream, g, lor_geom(c), x, y, r, grad, loss, n);
+                     c->stream, g, lor_geom(c), x, y, r, sc, loss, n);
   FIR_CHECK_LAUNCH();
-  return FIR_OK;
+  return gather_copies(c, grad, nvox, 1, 0);
 }
 
 int fir_osem_project_batch(fir_ctx* c, const fir_vol_desc* v, int n_images,
@@ -1381,19 +1536,25 @@ int fir_osem_project_batch(fir_ctx* c, const fir_vol_desc* v, int n_images,
   if (subset < 0 || subset >= n_subsets)
     return fail(FIR_E_ARG, "subset %d out of range", subset);
   const long long n = n_work(c, n_subsets, subset);
+  if ((rc = check_work(n))) return rc;
+  const long long nvox = (long long)v->nx * v->ny * v->nz;
   FIR_HIP_CHECK(hipSetDevice(c->device));
   for (int i0 = 0; i0 < n_images; i0 += kMaxBatch)
   {
     const int nb = std::min(kMaxBatch, n_images - i0);
     BatchPtrs p;
     std::memset(&p, 0, sizeof(p));
+    fir::Scatter sc;
+    if ((rc = scatter_for(c, nullptr, nvox, nb, &sc))) return rc;
     for (int b = 0; b < nb; ++b)
     {
       p.x[b] = x[i0 + b];
       p.y[b] = y[i0 + b];
       p.bias[b] = bias ? bias[i0 + b] : nullptr;
-      p.bp[b] = bp[i0 + b];
+      p.bp[b] = sc.copies > 1 ? sc.base + (long long)b * nvox : bp[i0 + b];
     }
+    p.copy_stride = sc.stride;
+    p.copies = sc.copies;
     const dim3 grid(blocks_for(n)), block(kBlock);
     const LorGeom lg = lor_geom(c);
     switch (nb)
@@ -1408,6 +1569,8 @@ int fir_osem_project_batch(fir_ctx* c, const fir_vol_desc* v, int n_images,
     default: hipLaunchKernelGGL(k_osem_project_batch<8>, grid, block, 0, c->stream, g, lg, p, n, n_subsets, subset); break;
     }
     FIR_CHECK_LAUNCH();
+    for (int b = 0; b < nb; ++b)
+      if ((rc = gather_copies(c, bp[i0 + b], nvox, nb, b))) return rc;
   }
   return FIR_OK;
 }
@@ -1416,6 +1579,26 @@ int fir_set_uneven_subsets(fir_ctx* c, int allow)
 {
   if (!c) return fail(FIR_E_ARG, "context is NULL");
   c->uneven_subsets = allow != 0;
+  return FIR_OK;
+}
+
+int fir_set_lor_order(fir_ctx* c, int order)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (order != FIR_ORDER_SEGMENT && order != FIR_ORDER_ZCENTER)
+    return fail(FIR_E_ARG, "unknown LOR order %d", order);
+  c->lor_order = order;
+  if (!c->proj_set) return FIR_OK;
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  return upload_pairs(c);
+}
+
+int fir_set_scatter_copies(fir_ctx* c, int copies)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (copies != 1 && copies != 2 && copies != 4 && copies != 8)
+    return fail(FIR_E_ARG, "scatter copies must be 1, 2, 4 or 8");
+  c->bp_copies = copies;
   return FIR_OK;
 }
 

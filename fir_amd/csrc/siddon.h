@@ -40,14 +40,15 @@ struct Grid
 };
 
 // Sinogram / scanner geometry needed to turn a work index into a LOR.
+// The (segment, axial) rows are visited in the order of `pairs`; within a
+// pair the views of the subset, within a view the tangential bins.
 struct LorGeom
 {
-  const double2* xy;      // [n_cpr] crystal (x, y)
-  const double* z;        // [n_slices] slice z
-  const int2* ax_slices;  // [sum n_axial] slice pair of (seg, axial)
-  const int* ax_prefix;   // [n_seg + 1] prefix sums of n_axial
+  const double2* xy;         // [n_cpr] crystal (x, y)
+  const double* z;           // [n_slices] slice z
+  const int4* pairs;         // [n_pairs] (segment rank, axial, slice1, slice2)
+  const int* n_axial;        // [n_seg] axial count per segment rank
   const long long* seg_base; // [n_seg] first bin of each segment
-  int n_seg;
   int n_views;
   int n_tang;
   int tang_off;
@@ -73,34 +74,23 @@ __host__ __device__ __forceinline__ int views_in_subset(int n_views,
                        : n_views;
 }
 
-// Work index t -> LOR. In all-bins mode (n_subsets == 0) t is the bin
-// index itself. In subset mode the rows of each segment are the views of
-// the subset (reference LORCache.cc:118-133).
+// Work index t -> LOR (t < 2^31, checked by the host). n_subsets == 0
+// selects every view. The bin index is the LOR's position in the
+// reference's segment-concatenated sinogram (ProjData.h:9-21).
 __device__ __forceinline__ void lor_from_index(const LorGeom& g, long long t,
                                                int n_subsets, int subset,
                                                Lor& L)
 {
-  const int vpr = views_in_subset(g.n_views, n_subsets, subset);
-  const long long row = t / g.n_tang;
-  const int tt = (int)(t - row * g.n_tang);
-  // largest k with vpr * ax_prefix[k] <= row
-  int lo = 0, hi = g.n_seg;
-  while (hi - lo > 1)
-  {
-    const int mid = (lo + hi) >> 1;
-    if ((long long)vpr * g.ax_prefix[mid] <= row)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  const int k = lo;
-  const int a0 = g.ax_prefix[k];
-  const int na = g.ax_prefix[k + 1] - a0;
-  const long long r = row - (long long)vpr * a0;
-  const int sv = (int)(r / na);
-  const int axial = (int)(r - (long long)sv * na);
+  const unsigned vpr = (unsigned)views_in_subset(g.n_views, n_subsets, subset);
+  const unsigned ut = (unsigned)t;
+  const unsigned row = ut / (unsigned)g.n_tang;
+  const int tt = (int)(ut - row * (unsigned)g.n_tang);
+  const unsigned p = row / vpr;
+  const int sv = (int)(row - p * vpr);
+  const int4 P = g.pairs[p];
+  const int na = g.n_axial[P.x];
   const int view = n_subsets > 0 ? sv * n_subsets + subset : sv;
-  L.bin = g.seg_base[k] + ((long long)view * na + axial) * g.n_tang + tt;
+  L.bin = g.seg_base[P.x] + ((long long)view * na + P.y) * g.n_tang + tt;
 
   // (view, tang) -> angular crystals (ProjData.cc:301-340); C++ integer
   // division truncates toward zero.
@@ -117,15 +107,31 @@ __device__ __forceinline__ void lor_from_index(const LorGeom& g, long long t,
     c2 += g.n_cpr;
   else if (c2 >= g.n_cpr)
     c2 -= g.n_cpr;
-  const int2 s = g.ax_slices[a0 + axial];
   const double2 p1 = g.xy[c1];
   const double2 p2 = g.xy[c2];
   L.e1[0] = p1.x;
   L.e1[1] = p1.y;
-  L.e1[2] = g.z[s.x];
+  L.e1[2] = g.z[P.z];
   L.e2[0] = p2.x;
   L.e2[1] = p2.y;
-  L.e2[2] = g.z[s.y];
+  L.e2[2] = g.z[P.w];
+}
+
+// Target of a scatter: the image itself, or one of `copies` private
+// copies (stride apart) chosen by the XCD the wave runs on, so that each
+// XCD's L2 holds its own copy's lines. The copies are summed afterwards.
+struct Scatter
+{
+  float* base;
+  long long stride;
+  int copies;
+};
+
+__device__ __forceinline__ float* scatter_target(const Scatter& s)
+{
+  if (s.copies <= 1) return s.base;
+  const unsigned xcc = __builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11));
+  return s.base + (long long)(xcc % (unsigned)s.copies) * s.stride;
 }
 
 // Walk state after set-up; copying it replays the identical walk.

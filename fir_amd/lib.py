@@ -18,6 +18,8 @@ LIB_PATH = os.path.join(HERE, "libfir_hip.so")
 
 FIR_WALK_REFERENCE = 0
 FIR_WALK_GEOMETRIC = 1
+FIR_ORDER_SEGMENT = 0
+FIR_ORDER_ZCENTER = 1
 
 # Every symbol declared in include/fir_hip.h.
 EXPORTS = (
@@ -30,7 +32,7 @@ EXPORTS = (
     "fir_osem_subiter", "fir_convolve", "fir_cut_circle", "fir_mul_clamped",
     "fir_div_clamped", "fir_exp_clamped", "fir_hu_to_mu", "fir_path_dump",
     "fir_path_count", "fir_poisson_nll_grad", "fir_osem_project_batch",
-    "fir_set_uneven_subsets",
+    "fir_set_uneven_subsets", "fir_set_lor_order", "fir_set_scatter_copies",
 )
 
 
@@ -82,6 +84,8 @@ def load() -> C.CDLL:
         "fir_set_proj": [vp, i, i, i, i, i],
         "fir_set_walk_mode": [vp, i],
         "fir_set_uneven_subsets": [vp, i],
+        "fir_set_lor_order": [vp, i],
+        "fir_set_scatter_copies": [vp, i],
         "fir_fill": [vp, vp, i64, f],
         "fir_forward": [vp, P, vp, vp],
         "fir_forward_subset": [vp, P, vp, vp, i, i],
@@ -180,6 +184,12 @@ class Context:
 
     def set_walk_mode(self, mode: int):
         _check(self.lib.fir_set_walk_mode(self.h, mode))
+
+    def set_lor_order(self, order: int):
+        _check(self.lib.fir_set_lor_order(self.h, order))
+
+    def set_scatter_copies(self, copies: int):
+        _check(self.lib.fir_set_scatter_copies(self.h, copies))
 
     def set_uneven_subsets(self, allow: bool):
         _check(self.lib.fir_set_uneven_subsets(self.h, int(allow)))

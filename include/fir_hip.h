@@ -46,6 +46,16 @@ enum
   FIR_WALK_GEOMETRIC = 1
 };
 
+/* Order in which the projector kernels visit the (segment, axial) rows of
+ * a sinogram (results do not depend on it). FIR_ORDER_ZCENTER groups rows
+ * by the centre slice of their LORs, so rows processed together share a
+ * thin z-slab of the image and its cache lines. */
+enum
+{
+  FIR_ORDER_SEGMENT = 0,
+  FIR_ORDER_ZCENTER = 1
+};
+
 typedef struct fir_ctx fir_ctx;
 
 /* Volume geometry: the reference VolHeader (VolHeader.h:6-34). */
@@ -99,6 +109,14 @@ int fir_set_walk_mode(fir_ctx* ctx, int mode);
  * v = k * n_subsets + s, so the first nViews % n_subsets subsets hold one
  * view more (an extension: the reference defines no such split). */
 int fir_set_uneven_subsets(fir_ctx* ctx, int allow);
+
+/* Kernel tuning (no effect on results beyond FP32 summation order):
+ * the row order (FIR_ORDER_*), and the number of private copies (1, 2, 4
+ * or 8) the scatter kernels accumulate into, one per XCD, summed into the
+ * target afterwards. Copies cost copies x N_vox floats of device memory
+ * (allocated by the library on first use). */
+int fir_set_lor_order(fir_ctx* ctx, int order);
+int fir_set_scatter_copies(fir_ctx* ctx, int copies);
 
 /* ---- device memory -------------------------------------------------- */
 

@@ -338,3 +338,32 @@ def test_uneven_subsets_partition_the_views(c1ctx, c1_golden):
         assert rel_max(total, c1_golden["bp"]) < 1e-5
     finally:
         ctx.set_uneven_subsets(False)
+
+
+@pytest.mark.parametrize("order,copies", [(0, 1), (1, 8), (0, 4)])
+def test_tuning_knobs_keep_results(c1ctx, c1_golden, order, copies):
+    """Row order and per-XCD scatter copies change only FP32 summation
+    order: forward stays bit-exact, backprojection within 1e-5."""
+    w, ctx = c1ctx
+    v = firlib.VolDesc.from_spec(w.vol)
+    ctx.set_lor_order(order)
+    ctx.set_scatter_copies(copies)
+    try:
+        img = dev(c1_golden["act"].ravel())
+        sino = torch.full((ctx.n_bins,), -1.0, device="cuda")
+        ctx.forward(v, img, sino)
+        assert np.array_equal(host(sino), c1_golden["y0"])
+        bp = torch.zeros(v.n_voxels, device="cuda")
+        ctx.backward(v, dev(c1_golden["y"]), bp, 1)
+        assert rel_max(host(bp), c1_golden["bp"]) < 1e-5
+        x = torch.ones(v.n_voxels, device="cuda")
+        a = torch.zeros(v.n_voxels, device="cuda")
+        ctx.osem_project(v, x, dev(c1_golden["y"]), None, a, 2, 4)
+        b2 = [torch.zeros(v.n_voxels, device="cuda") for _ in range(3)]
+        ctx.osem_project_batch(v, [x, x, x], [dev(c1_golden["y"])] * 3, None,
+                               b2, 2, 4)
+        for t in b2:
+            assert rel_max(host(t), host(a)) < 1e-5
+    finally:
+        ctx.set_lor_order(1)
+        ctx.set_scatter_copies(1)
