@@ -60,7 +60,8 @@ struct fir_ctx
   std::vector<int2> slices; // slice pair per (segment rank, axial), flattened
   int n_pairs = 0;
   int4* d_pairs = nullptr;  // rows in processing order (see fir_set_lor_order)
-  int4* d_pairs_bins = nullptr; // rows in sinogram order (index == bin)
+  int4* d_pairs_bins = nullptr; // rows in sinogram (segment, axial) order
+  int* d_ax_prefix = nullptr;   // offsets of each segment in d_pairs_bins
   int* d_n_axial = nullptr;
   long long* d_seg_base = nullptr;
   int lor_order = FIR_ORDER_ZCENTER;
@@ -265,7 +266,7 @@ int upload_pairs(fir_ctx* c)
   return FIR_OK;
 }
 
-// LorGeom whose work index is the bin index (sinogram order).
+// LorGeom whose pair table is in sinogram (segment, axial) order.
 LorGeom lor_geom_bins(const fir_ctx* ctx)
 {
   LorGeom g = lor_geom(ctx);
@@ -504,6 +505,8 @@ __global__ __launch_bounds__(kBlock) void k_osem_project_batch(
 
 // Path dump for parity checks.
 __global__ __launch_bounds__(kBlock) void k_path_dump(Grid g, LorGeom lg,
+                                                      const int* ax_prefix,
+                                                      int n_seg,
                                                       long long bin0,
                                                       long long n, int maxp,
                                                       int* __restrict__ counts,
@@ -513,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_path_dump(Grid g, LorGeom lg,
   const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
   if (t >= n) return;
   Lor L;
-  fir::lor_from_index(lg, bin0 + t, 0, 0, L);
+  fir::lor_from_bin(lg, ax_prefix, n_seg, bin0 + t, L);
   Walk w;
   if (!fir::walk_setup(g, L.e1, L.e2, w))
   {
@@ -855,6 +858,7 @@ int fir_ctx_destroy(fir_ctx* c)
   (void)hipFree(c->d_z);
   (void)hipFree(c->d_pairs);
   (void)hipFree(c->d_pairs_bins);
+  (void)hipFree(c->d_ax_prefix);
   (void)hipFree(c->d_n_axial);
   (void)hipFree(c->d_seg_base);
   (void)hipFree(c->d_copies);
@@ -1037,9 +1041,14 @@ int fir_set_proj(fir_ctx* c, int n_rings, int n_cpr, int span, int n_seg,
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
   (void)hipFree(c->d_n_axial);
   (void)hipFree(c->d_seg_base);
+  (void)hipFree(c->d_ax_prefix);
   c->d_n_axial = nullptr;
   c->d_seg_base = nullptr;
+  c->d_ax_prefix = nullptr;
   c->proj_set = false;
+  FIR_HIP_CHECK(hipMalloc(&c->d_ax_prefix, sizeof(int) * pre.size()));
+  FIR_HIP_CHECK(hipMemcpy(c->d_ax_prefix, pre.data(), sizeof(int) * pre.size(),
+                          hipMemcpyHostToDevice));
   FIR_HIP_CHECK(hipMalloc(&c->d_n_axial, sizeof(int) * na.size()));
   FIR_HIP_CHECK(hipMalloc(&c->d_seg_base, sizeof(long long) * base.size()));
   FIR_HIP_CHECK(hipMemcpy(c->d_n_axial, na.data(), sizeof(int) * na.size(),
@@ -1457,7 +1466,8 @@ int fir_path_dump(fir_ctx* c, const fir_vol_desc* v, int64_t bin0,
   FIR_HIP_CHECK(hipSetDevice(c->device));
   if ((rc = check_work(c->n_bins))) return rc;
   hipLaunchKernelGGL(k_path_dump, dim3(blocks_for(nb)), dim3(kBlock), 0,
-                     c->stream, g, lor_geom_bins(c), (long long)bin0,
+                     c->stream, g, lor_geom_bins(c), c->d_ax_prefix, c->n_seg,
+                     (long long)bin0,
                      (long long)nb, maxp, counts, idx, len);
   FIR_CHECK_LAUNCH();
   return FIR_OK;

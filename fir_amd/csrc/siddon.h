@@ -117,6 +117,56 @@ __device__ __forceinline__ void lor_from_index(const LorGeom& g, long long t,
   L.e2[2] = g.z[P.w];
 }
 
+// Bin index (sinogram order) -> LOR, for parity dumps: the segment by
+// binary search of seg_base, then view, axial and tangential bin. `pairs`
+// must be in sinogram (segment, axial) order and ax_prefix its per-segment
+// offsets.
+__device__ __forceinline__ void lor_from_bin(const LorGeom& g,
+                                             const int* ax_prefix, int n_seg,
+                                             long long b, Lor& L)
+{
+  int lo = 0, hi = n_seg;
+  while (hi - lo > 1)
+  {
+    const int mid = (lo + hi) >> 1;
+    if (g.seg_base[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int k = lo;
+  const int na = g.n_axial[k];
+  const long long r = b - g.seg_base[k];
+  const long long per_view = (long long)na * g.n_tang;
+  const int view = (int)(r / per_view);
+  const long long r2 = r - (long long)view * per_view;
+  const int axial = (int)(r2 / g.n_tang);
+  const int tt = (int)(r2 - (long long)axial * g.n_tang);
+  const int4 P = g.pairs[ax_prefix[k] + axial];
+  L.bin = b;
+  const int tang = tt - g.tang_off;
+  const int shift = tang >= 0 ? tang / 2 : (tang - 1) / 2;
+  const int odd = (tang < 0 ? -tang : tang) & 1;
+  int c1 = view + shift + odd;
+  int c2 = view + g.n_views - shift;
+  if (c1 < 0)
+    c1 += g.n_cpr;
+  else if (c1 >= g.n_cpr)
+    c1 -= g.n_cpr;
+  if (c2 < 0)
+    c2 += g.n_cpr;
+  else if (c2 >= g.n_cpr)
+    c2 -= g.n_cpr;
+  const double2 p1 = g.xy[c1];
+  const double2 p2 = g.xy[c2];
+  L.e1[0] = p1.x;
+  L.e1[1] = p1.y;
+  L.e1[2] = g.z[P.z];
+  L.e2[0] = p2.x;
+  L.e2[1] = p2.y;
+  L.e2[2] = g.z[P.w];
+}
+
 // Target of a scatter: the image itself, or one of `copies` private
 // copies (stride apart) chosen by the XCD the wave runs on, so that each
 // XCD's L2 holds its own copy's lines. The copies are summed afterwards.
