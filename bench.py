@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=8,
                     help="reconstructions in the batched measurement (0: skip)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run only one projector launch of each kind and a "
+                         "1 GiB device copy (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not "
@@ -147,6 +150,24 @@ def main():
     del mean, act_d, y0
 
     stream = torch.cuda.current_stream()
+
+    if args.traffic_probe:
+        # one launch of each projector kernel at steady state, then a plain
+        # copy of known size whose FETCH_SIZE/WRITE_SIZE calibrate the
+        # counters (1 GiB read + 1 GiB written)
+        ctx.osem_project(v, x, y, None, bp, 3, n_sub)
+        torch.cuda.synchronize()
+        ctx.osem_project(v, x, y, None, bp, 3, n_sub)
+        sino = torch.zeros(nb, device="cuda")
+        ctx.forward(v, x, sino)
+        img = torch.zeros(nvox, device="cuda")
+        ctx.backward_subset(v, y, img, 3, n_sub)
+        src = torch.ones(1 << 28, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("traffic probe done")
+        return
 
     def osem_iteration(evs=None):
         for s in range(n_sub):

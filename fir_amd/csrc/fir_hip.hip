@@ -64,7 +64,8 @@ struct fir_ctx
   int* d_ax_prefix = nullptr;   // offsets of each segment in d_pairs_bins
   int* d_n_axial = nullptr;
   long long* d_seg_base = nullptr;
-  int lor_order = FIR_ORDER_ZCENTER;
+  int lor_order = FIR_ORDER_SEGMENT;
+  int view_major = 1;
 
   // private per-XCD copies of scatter targets (1: scatter directly)
   int bp_copies = 1;
@@ -212,6 +213,8 @@ LorGeom lor_geom(const fir_ctx* ctx)
   g.n_axial = ctx->d_n_axial;
   g.seg_base = ctx->d_seg_base;
   g.n_views = ctx->n_views;
+  g.n_pairs = ctx->n_pairs;
+  g.view_major = ctx->view_major;
   g.n_tang = ctx->n_tang;
   g.tang_off = ctx->tang_off;
   g.n_cpr = ctx->n_cpr;
@@ -1595,9 +1598,11 @@ int fir_set_uneven_subsets(fir_ctx* c, int allow)
 int fir_set_lor_order(fir_ctx* c, int order)
 {
   if (!c) return fail(FIR_E_ARG, "context is NULL");
-  if (order != FIR_ORDER_SEGMENT && order != FIR_ORDER_ZCENTER)
+  const int pair_order_id = order & 0xff;
+  if (pair_order_id != FIR_ORDER_SEGMENT && pair_order_id != FIR_ORDER_ZCENTER)
     return fail(FIR_E_ARG, "unknown LOR order %d", order);
-  c->lor_order = order;
+  c->lor_order = pair_order_id;
+  c->view_major = (order & FIR_ORDER_PAIR_MAJOR) ? 0 : 1;
   if (!c->proj_set) return FIR_OK;
   FIR_HIP_CHECK(hipSetDevice(c->device));
   return upload_pairs(c);

@@ -20,6 +20,7 @@ FIR_WALK_REFERENCE = 0
 FIR_WALK_GEOMETRIC = 1
 FIR_ORDER_SEGMENT = 0
 FIR_ORDER_ZCENTER = 1
+FIR_ORDER_PAIR_MAJOR = 0x100
 
 # Every symbol declared in include/fir_hip.h.
 EXPORTS = (

@@ -46,14 +46,20 @@ enum
   FIR_WALK_GEOMETRIC = 1
 };
 
-/* Order in which the projector kernels visit the (segment, axial) rows of
- * a sinogram (results do not depend on it). FIR_ORDER_ZCENTER groups rows
- * by the centre slice of their LORs, so rows processed together share a
- * thin z-slab of the image and its cache lines. */
+/* Order in which the projector kernels visit the rows of a sinogram
+ * (results do not depend on it beyond FP32 summation order). The
+ * (segment, axial) pairs are ranked by segment then axial
+ * (FIR_ORDER_SEGMENT) or by the centre slice of their LORs
+ * (FIR_ORDER_ZCENTER). By default a view's rows for every pair are visited
+ * together, so concurrent LORs spread over the whole image depth (fewer
+ * atomic collisions); with FIR_ORDER_PAIR_MAJOR every view of one pair is
+ * visited together (concurrent LORs share one z-slab: better gather
+ * locality, more scatter collisions). */
 enum
 {
   FIR_ORDER_SEGMENT = 0,
-  FIR_ORDER_ZCENTER = 1
+  FIR_ORDER_ZCENTER = 1,
+  FIR_ORDER_PAIR_MAJOR = 0x100
 };
 
 typedef struct fir_ctx fir_ctx;

@@ -340,7 +340,7 @@ def test_uneven_subsets_partition_the_views(c1ctx, c1_golden):
         ctx.set_uneven_subsets(False)
 
 
-@pytest.mark.parametrize("order,copies", [(0, 1), (1, 8), (0, 4)])
+@pytest.mark.parametrize("order,copies", [(0x100, 1), (1, 8), (0x101, 4)])
 def test_tuning_knobs_keep_results(c1ctx, c1_golden, order, copies):
     """Row order and per-XCD scatter copies change only FP32 summation
     order: forward stays bit-exact, backprojection within 1e-5."""
@@ -365,5 +365,5 @@ def test_tuning_knobs_keep_results(c1ctx, c1_golden, order, copies):
         for t in b2:
             assert rel_max(host(t), host(a)) < 1e-5
     finally:
-        ctx.set_lor_order(1)
+        ctx.set_lor_order(0)
         ctx.set_scatter_copies(1)
