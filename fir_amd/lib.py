@@ -294,6 +294,25 @@ class Context:
                                       _ptr(ln)))
 
 
+_libm = None
+
+
+def _sincos(angle: float):
+    """(sin, cos) from the C library's sincos: the reference's
+    rotateXY (ScannerData.cc:234-246) calls cos and sin on the same angle,
+    which an optimising C++ compiler merges into one sincos call; separate
+    cos/sin can differ from it in the last place for some angles."""
+    global _libm
+    if _libm is None:
+        _libm = C.CDLL("libm.so.6")
+        _libm.sincos.argtypes = [C.c_double, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]
+        _libm.sincos.restype = None
+    s, c = C.c_double(), C.c_double()
+    _libm.sincos(angle, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
 def scanner_tables(scanner) -> tuple:
     """Crystal (x, y) per angular index and slice z, computed as the
     reference ScannerData does (ScannerHeader.cc:305-342,
@@ -335,7 +354,7 @@ def scanner_tables(scanner) -> tuple:
         k = i + coff
         yy = ys[k % per]
         ang = (k // per) * dang
-        ca, sa = math.cos(ang), math.sin(ang)
+        sa, ca = _sincos(ang)
         xy[2 * i] = tx * ca - yy * sa
         xy[2 * i + 1] = tx * sa + yy * ca
     for i in range(first, n_cpr):

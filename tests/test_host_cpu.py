@@ -207,3 +207,19 @@ def test_param_file_without_start_key(tmp_path):
 def test_missing_file_is_an_error(tmp_path):
     r = run("params", tmp_path / "nope.params", check=False)
     assert r.returncode != 0 and "Couldn't open file" in r.stderr
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c5"])
+def test_python_scanner_tables_match_oracle(name, tmp_path):
+    """fir_amd.lib.scanner_tables (which feeds the GPU) and the C++ host
+    tables both equal the oracle's bit for bit (same libm sincos calls)."""
+    from fir_amd import lib as firlib
+    sc = synth.WORKLOADS[name]().scanner
+    a, b = O.scanner_tables(sc)
+    c, d = firlib.scanner_tables(sc)
+    assert np.array_equal(a, c) and np.array_equal(b, d)
+    sc.write(str(tmp_path / "s.scan"))
+    out = run("scanner", tmp_path / "s.scan").stdout.split("\n")
+    n, ns = map(int, out[0].split())
+    xy = np.array([float(v) for line in out[1:1 + n] for v in line.split()])
+    assert np.array_equal(xy, a)
