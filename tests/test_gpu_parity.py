@@ -340,7 +340,8 @@ def test_uneven_subsets_partition_the_views(c1ctx, c1_golden):
         ctx.set_uneven_subsets(False)
 
 
-@pytest.mark.parametrize("order,copies", [(0x100, 1), (1, 8), (0x101, 4)])
+@pytest.mark.parametrize("order,copies", [(0x100, 1), (0x401, 8), (0x301, 4),
+                                          (0x200, 2), (0x600, 1)])
 def test_tuning_knobs_keep_results(c1ctx, c1_golden, order, copies):
     """Row order and per-XCD scatter copies change only FP32 summation
     order: forward stays bit-exact, backprojection within 1e-5."""
@@ -367,3 +368,11 @@ def test_tuning_knobs_keep_results(c1ctx, c1_golden, order, copies):
     finally:
         ctx.set_lor_order(0)
         ctx.set_scatter_copies(1)
+
+
+def test_lor_order_rejects_bad_flags(c1ctx):
+    _, ctx = c1ctx
+    for bad in (0x500, 1, 0x800, 2):
+        with pytest.raises(firlib.FirError):
+            ctx.set_lor_order(bad)
+    ctx.set_lor_order(0)
