@@ -37,10 +37,11 @@ def _run(cmd):
 
 
 def build_hip(verbose=True):
-    src = os.path.join(HERE, "csrc", "fir_hip.hip")
+    srcs = [os.path.join(HERE, "csrc", f)
+            for f in ("fir_hip.hip", "path_cache.hip")]
     out = os.path.join(HERE, "libfir_hip.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, src])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
     os.replace(tmp, out)
     return out
 

@@ -25,72 +25,20 @@
 #include <string>
 #include <vector>
 
-#include "../../include/fir_hip.h"
-#include "siddon.h"
+#include "internal.h"
 
 using fir::Grid;
 using fir::kEpsF;
 using fir::Lor;
 using fir::LorGeom;
 using fir::Walk;
+using namespace firi;
 
 // ---------------------------------------------------------------------------
-// Context and error handling
+// Context helpers (declared in internal.h)
 // ---------------------------------------------------------------------------
 
-struct fir_ctx
-{
-  int device = 0;
-  hipStream_t own_stream = nullptr;
-  hipStream_t stream = nullptr;
-
-  // scanner tables
-  double2* d_xy = nullptr;
-  double* d_z = nullptr;
-  int n_cpr = 0;
-  int n_slices = 0;
-
-  // sinogram geometry
-  bool proj_set = false;
-  int n_rings = 0, span = 1, n_seg = 0, n_tang = 0;
-  int seg_off = 0, tang_off = 0, n_views = 0, half_span = 0;
-  int64_t n_bins = 0;
-  std::vector<int> n_axial;
-  std::vector<long long> seg_base;
-  std::vector<int2> slices; // slice pair per (segment rank, axial), flattened
-  int n_pairs = 0;
-  int4* d_pairs = nullptr;  // rows in processing order (see fir_set_lor_order)
-  int4* d_pairs_bins = nullptr; // rows in sinogram (segment, axial) order
-  int* d_ax_prefix = nullptr;   // offsets of each segment in d_pairs_bins
-  int* d_n_axial = nullptr;
-  long long* d_seg_base = nullptr;
-  int lor_order = FIR_ORDER_SEGMENT;
-  int row_mode = fir::ROWS_SEGMENT_VIEW;
-  int tang_major = 0;
-
-  // private per-XCD copies of scatter targets (1: scatter directly)
-  int bp_copies = 1;
-  float* d_copies = nullptr;
-  size_t copies_floats = 0;
-
-  int walk_mode = FIR_WALK_REFERENCE;
-  // allow subset counts that do not divide the number of views
-  bool uneven_subsets = false;
-
-  // 64-bit device counter of fir_path_count
-  unsigned long long* d_counter = nullptr;
-
-  // scratch images of the Gaussian (library-allocated, grown on demand)
-  float* d_scratch = nullptr;
-  size_t scratch_floats = 0;
-  // Gaussian taps on the device: [x: 64][y: 64][z: 64]
-  float* d_taps = nullptr;
-  float taps_key[3] = {-1.f, -1.f, -1.f};
-  double taps_vox[3] = {0, 0, 0};
-  int taps_h[3] = {0, 0, 0};
-};
-
-namespace
+namespace firi
 {
 thread_local std::string g_err;
 
@@ -105,25 +53,8 @@ int fail(int code, const char* fmt, ...)
   return code;
 }
 
-#define FIR_HIP_CHECK(expr)                                                  \
-  do                                                                         \
-  {                                                                          \
-    hipError_t e_ = (expr);                                                  \
-    if (e_ != hipSuccess)                                                    \
-      return fail(FIR_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));        \
-  } while (0)
 
-#define FIR_CHECK_LAUNCH()                                                   \
-  do                                                                         \
-  {                                                                          \
-    hipError_t e_ = hipGetLastError();                                       \
-    if (e_ != hipSuccess)                                                    \
-      return fail(FIR_E_HIP, "kernel launch: %s", hipGetErrorString(e_));    \
-  } while (0)
-
-constexpr int kBlock = 256;
-
-unsigned blocks_for(long long n, int block = kBlock)
+unsigned blocks_for(long long n, int block)
 {
   return (unsigned)((n + block - 1) / block);
 }
@@ -289,7 +220,7 @@ int check_work(long long n)
   return FIR_OK;
 }
 
-} // namespace
+} // namespace firi
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -912,6 +843,7 @@ int fir_set_scanner(fir_ctx* c, const double* xy, int n_cpr, const double* z,
     hipMemcpy(c->d_z, z, sizeof(double) * n_slices, hipMemcpyHostToDevice));
   c->n_cpr = n_cpr;
   c->n_slices = n_slices;
+  ++c->generation;
   return FIR_OK;
 }
 
@@ -1078,6 +1010,7 @@ int fir_set_proj(fir_ctx* c, int n_rings, int n_cpr, int span, int n_seg,
   int rc = upload_pairs(c);
   if (rc) return rc;
   c->proj_set = true;
+  ++c->generation;
   return FIR_OK;
 }
 
@@ -1091,6 +1024,7 @@ int fir_set_walk_mode(fir_ctx* c, int mode)
   if (!c) return fail(FIR_E_ARG, "context is NULL");
   if (mode != FIR_WALK_REFERENCE && mode != FIR_WALK_GEOMETRIC)
     return fail(FIR_E_ARG, "unknown walk mode %d", mode);
+  if (c->walk_mode != mode) ++c->generation;
   c->walk_mode = mode;
   return FIR_OK;
 }

@@ -1,0 +1,109 @@
+// internal.h -- the projector context shared by the translation units of
+// libfir_hip.so, and the helpers they share. Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fir_hip.h"
+#include "siddon.h"
+
+struct RowMatrix; // LOR-major path matrix of a path cache (path_cache.hip)
+
+struct fir_ctx
+{
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+
+  // scanner tables
+  double2* d_xy = nullptr;
+  double* d_z = nullptr;
+  int n_cpr = 0;
+  int n_slices = 0;
+
+  // sinogram geometry
+  bool proj_set = false;
+  int n_rings = 0, span = 1, n_seg = 0, n_tang = 0;
+  int seg_off = 0, tang_off = 0, n_views = 0, half_span = 0;
+  int64_t n_bins = 0;
+  std::vector<int> n_axial;
+  std::vector<long long> seg_base;
+  std::vector<int2> slices; // slice pair per (segment rank, axial), flattened
+  int n_pairs = 0;
+  int4* d_pairs = nullptr;  // rows in processing order (see fir_set_lor_order)
+  int4* d_pairs_bins = nullptr; // rows in sinogram (segment, axial) order
+  int* d_ax_prefix = nullptr;   // offsets of each segment in d_pairs_bins
+  int* d_n_axial = nullptr;
+  long long* d_seg_base = nullptr;
+  int lor_order = FIR_ORDER_SEGMENT;
+  int row_mode = fir::ROWS_SEGMENT_VIEW;
+  int tang_major = 0;
+
+  // private per-XCD copies of scatter targets (1: scatter directly)
+  int bp_copies = 1;
+  float* d_copies = nullptr;
+  size_t copies_floats = 0;
+
+  int walk_mode = FIR_WALK_REFERENCE;
+  // allow subset counts that do not divide the number of views
+  bool uneven_subsets = false;
+
+  // 64-bit device counter of fir_path_count
+  unsigned long long* d_counter = nullptr;
+
+  // scratch images of the Gaussian (library-allocated, grown on demand)
+  float* d_scratch = nullptr;
+  size_t scratch_floats = 0;
+  // Gaussian taps on the device: [x: 64][y: 64][z: 64]
+  float* d_taps = nullptr;
+  float taps_key[3] = {-1.f, -1.f, -1.f};
+  double taps_vox[3] = {0, 0, 0};
+  int taps_h[3] = {0, 0, 0};
+
+  // Bumped whenever the scanner, sinogram or walk mode changes: path
+  // matrices built under another generation are not shared.
+  unsigned generation = 0;
+  // Path matrices of live caches, shared by caches of one volume, walk mode
+  // and length format.
+  std::vector<std::weak_ptr<RowMatrix>> row_matrices;
+};
+
+namespace firi
+{
+// Sets the thread's last-error message and returns code.
+int fail(int code, const char* fmt, ...);
+
+constexpr int kBlock = 256;
+
+unsigned blocks_for(long long n, int block = kBlock);
+// Grid of element-wise kernels (grid-stride loops).
+unsigned ew_blocks(long long n);
+int make_grid(const fir_ctx* ctx, const fir_vol_desc* v, fir::Grid* g);
+int need_lors(const fir_ctx* ctx);
+int check_subsets(const fir_ctx* ctx, int n_subsets);
+fir::LorGeom lor_geom(const fir_ctx* ctx);
+// LorGeom whose pair table is in sinogram (segment, axial) order.
+fir::LorGeom lor_geom_bins(const fir_ctx* ctx);
+long long n_work(const fir_ctx* ctx, int n_subsets, int subset);
+int check_work(long long n);
+} // namespace firi
+
+#define FIR_HIP_CHECK(expr)                                                  \
+  do                                                                         \
+  {                                                                          \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(FIR_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+#define FIR_CHECK_LAUNCH()                                                   \
+  do                                                                         \
+  {                                                                          \
+    hipError_t e_ = hipGetLastError();                                       \
+    if (e_ != hipSuccess)                                                    \
+      return fail(FIR_E_HIP, "kernel launch: %s", hipGetErrorString(e_));    \
+  } while (0)

@@ -1,0 +1,1390 @@
+// path_cache.hip -- the path cache: every LOR's Siddon path walked once and
+// kept in HBM, so that projector passes stream the system matrix instead of
+// walking it again.
+//
+// Two copies of the matrix are kept:
+//   - the path matrix (RowMatrix), LOR-major: the LORs that cross the
+//     volume ("rows") in (view, segment, axial, tangential) order, each
+//     with its elements (voxel index, length) in path order. Forward
+//     projections and OSEM line integrals are gathers over it, T lanes per
+//     row.
+//   - the transpose of one subset split (fir_cache), voxel-major: for
+//     every subset and voxel the (row, length) elements of the subset's
+//     LORs through that voxel, in row order. Backprojection is a gather
+//     over it, T lanes per voxel: no atomics, and the result does not
+//     depend on scheduling.
+// The reference's LORCache (LORCache.h:5-47, LORCache.cc:10-168) keeps the
+// crystal pairs of every (subset, segment) to skip their decoding; this
+// cache keeps whole paths, which the 288 GB of HBM of one GPU hold for a
+// clinical scanner (3.9e9 elements, 23 GB per copy at 6 B per element).
+//
+// Element lengths are stored as
+//   FIR_LEN_UNORM16: round(65535 * len / m), m = the LOR's longest element;
+//   FIR_LEN_FP16:    binary16 of len / m;
+//   FIR_LEN_FP32:    binary32 of len;
+// with a per-row FP32 factor (m / 65535, m, 1) that restores the length.
+#include <hip/hip_fp16.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "internal.h"
+
+using fir::Grid;
+using fir::kEpsF;
+using fir::Lor;
+using fir::LorGeom;
+using fir::Walk;
+using namespace firi;
+
+namespace
+{
+
+constexpr int kMaxCacheBatch = 8;
+
+// Elements per transpose chunk (bounds the sort's scratch: 24 B each).
+constexpr long long kChunkElems = 1LL << 28;
+
+template <int F>
+struct Len;
+
+template <>
+struct Len<FIR_LEN_UNORM16>
+{
+  using T = unsigned short;
+  static __device__ __forceinline__ float dec(T v) { return (float)v; }
+  static __device__ __forceinline__ T enc(double len, float m)
+  {
+    if (!(m > 0.0f)) return 0;
+    double q = rint(len / (double)m * 65535.0);
+    q = q < 0.0 ? 0.0 : (q > 65535.0 ? 65535.0 : q);
+    return (T)q;
+  }
+  static __device__ __forceinline__ unsigned bits(T v) { return v; }
+  static __device__ __forceinline__ T from_bits(unsigned b) { return (T)b; }
+};
+
+template <>
+struct Len<FIR_LEN_FP16>
+{
+  using T = __half;
+  static __device__ __forceinline__ float dec(T v) { return __half2float(v); }
+  static __device__ __forceinline__ T enc(double len, float m)
+  {
+    if (!(m > 0.0f)) return __float2half_rn(0.0f);
+    return __float2half_rn((float)(len / (double)m));
+  }
+  static __device__ __forceinline__ unsigned bits(T v)
+  {
+    return (unsigned)__half_as_ushort(v);
+  }
+  static __device__ __forceinline__ T from_bits(unsigned b)
+  {
+    return __ushort_as_half((unsigned short)b);
+  }
+};
+
+template <>
+struct Len<FIR_LEN_FP32>
+{
+  using T = float;
+  static __device__ __forceinline__ float dec(T v) { return v; }
+  static __device__ __forceinline__ T enc(double len, float) { return (float)len; }
+  static __device__ __forceinline__ unsigned bits(T v) { return __float_as_uint(v); }
+  static __device__ __forceinline__ T from_bits(unsigned b) { return __uint_as_float(b); }
+};
+
+size_t len_bytes(int fmt) { return fmt == FIR_LEN_FP32 ? 4 : 2; }
+
+// ---------------------------------------------------------------------------
+// Build kernels
+// ---------------------------------------------------------------------------
+
+// LOR t of the cache's row order: view by view (t / per_view); within a
+// view the decode of `lg` for a one-view subset -- (pair, tangential) or,
+// with lg.tang_major, (tangential, pair) -- with the pairs in lg's order.
+__device__ __forceinline__ void cache_lor(const LorGeom& lg, long long t,
+                                          long long per_view, Lor& L)
+{
+  const int view = (int)(t / per_view);
+  fir::lor_from_index(lg, t - (long long)view * per_view, lg.n_views, view, L);
+}
+
+// Pass 1 over all LORs in (view, segment, axial, tangential) order: element
+// count and longest element of each path.
+__global__ __launch_bounds__(kBlock) void k_pc_stats(Grid g, LorGeom lg,
+                                                     long long n,
+                                                     long long per_view,
+                                                     int* __restrict__ count,
+                                                     float* __restrict__ maxlen)
+{
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  Lor L;
+  cache_lor(lg, t, per_view, L);
+  Walk w;
+  int c = 0;
+  double m = 0.0;
+  if (fir::walk_setup(g, L.e1, L.e2, w))
+    fir::walk_run(g, w, [&](int, double l) {
+      ++c;
+      m = l > m ? l : m;
+    });
+  count[t] = c;
+  maxlen[t] = (float)m;
+}
+
+struct HitFlag
+{
+  __host__ __device__ int operator()(int c) const { return c > 0 ? 1 : 0; }
+};
+
+struct ToI64
+{
+  __host__ __device__ long long operator()(int c) const { return (long long)c; }
+  __host__ __device__ long long operator()(unsigned c) const
+  {
+    return (long long)c;
+  }
+};
+
+// Row tables: bin, length factor and first element of every crossing LOR.
+__global__ __launch_bounds__(kBlock) void k_pc_rows(
+  LorGeom lg, long long n, long long per_view, const int* __restrict__ count,
+  const float* __restrict__ maxlen, const int* __restrict__ rowid,
+  const long long* __restrict__ eoff, int fmt, int* __restrict__ bin,
+  float* __restrict__ factor, long long* __restrict__ off)
+{
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n || count[t] == 0) return;
+  Lor L;
+  cache_lor(lg, t, per_view, L);
+  const int r = rowid[t];
+  const float m = maxlen[t];
+  bin[r] = (int)L.bin;
+  off[r] = eoff[t];
+  factor[r] = fmt == FIR_LEN_UNORM16 ? m / 65535.0f
+            : fmt == FIR_LEN_FP16    ? m
+                                     : 1.0f;
+}
+
+// First row and first element of every view, and the totals.
+__global__ void k_pc_views(const int* __restrict__ count,
+                           const int* __restrict__ rowid,
+                           const long long* __restrict__ eoff, long long n,
+                           long long per_view, int n_views,
+                           int* __restrict__ view_row,
+                           long long* __restrict__ view_elem,
+                           long long* __restrict__ row_off_end)
+{
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > n_views) return;
+  if (v < n_views)
+  {
+    view_row[v] = rowid[v * per_view];
+    view_elem[v] = eoff[v * per_view];
+  }
+  else
+  {
+    const int rows = rowid[n - 1] + (count[n - 1] > 0 ? 1 : 0);
+    const long long elems = eoff[n - 1] + count[n - 1];
+    view_row[v] = rows;
+    view_elem[v] = elems;
+    row_off_end[rows] = elems; // off[R] = E
+  }
+}
+
+// Pass 2: the paths themselves.
+template <int F>
+__global__ __launch_bounds__(kBlock) void k_pc_fill(
+  Grid g, LorGeom lg, long long n, long long per_view,
+  const int* __restrict__ count,
+  const float* __restrict__ maxlen, const long long* __restrict__ eoff,
+  int* __restrict__ idx, typename Len<F>::T* __restrict__ len)
+{
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n || count[t] == 0) return;
+  Lor L;
+  cache_lor(lg, t, per_view, L);
+  Walk w;
+  if (!fir::walk_setup(g, L.e1, L.e2, w)) return;
+  const float m = maxlen[t];
+  long long e = eoff[t];
+  const long long e_end = e + count[t];
+  fir::walk_run(g, w, [&](int i, double l) {
+    if (e < e_end)
+    {
+      idx[e] = i;
+      len[e] = Len<F>::enc(l, m);
+    }
+    ++e;
+  });
+}
+
+// Rows of subset s: view k * S + s for k = blockIdx.y. With xcd_map, the
+// x-blocks of a view (a multiple of 8) are dealt so that the XCD that runs
+// block x (x mod 8, as the dispatcher deals workgroups round-robin) gets the
+// contiguous eighth (x mod 8) of the view's rows: with rows ordered by
+// centre slice, each XCD's L2 then serves one z-slab of the image.
+struct SubsetRows
+{
+  const int* view_row;
+  int n_subsets;
+  int subset;
+  int xcd_map;
+};
+
+__device__ __forceinline__ unsigned logical_block(unsigned b, unsigned n,
+                                                  int xcd_map)
+{
+  return xcd_map ? (b & 7u) * (n >> 3) + (b >> 3) : b;
+}
+
+__device__ __forceinline__ bool subset_row(const SubsetRows& sr, int lanes,
+                                           int* r, int* r_first, int* k)
+{
+  const int view = (int)blockIdx.y * sr.n_subsets + sr.subset;
+  const int r0 = sr.view_row[view];
+  const int r1 = sr.view_row[view + 1];
+  const unsigned bx = logical_block(blockIdx.x, gridDim.x, sr.xcd_map);
+  *k = (int)blockIdx.y;
+  *r_first = r0;
+  *r = r0 + (int)(((long long)bx * blockDim.x + threadIdx.x) / lanes);
+  return *r < r1;
+}
+
+// Voxel histogram of one subset's elements (thread per element).
+__global__ __launch_bounds__(kBlock) void k_tr_hist(
+  const long long* __restrict__ view_elem, int n_subsets, int subset,
+  const int* __restrict__ idx, unsigned* __restrict__ cnt)
+{
+  const int view = (int)blockIdx.y * n_subsets + subset;
+  const long long e =
+    view_elem[view] + (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= view_elem[view + 1]) return;
+  atomicAdd(cnt + idx[e], 1u);
+}
+
+// Elements of each row of the subset inside the voxel range [v0, v1);
+// local row = row_prefix[k] + (r - first row of view k).
+__global__ __launch_bounds__(kBlock) void k_tr_count(
+  SubsetRows sr, const int* __restrict__ row_prefix,
+  const long long* __restrict__ off, const int* __restrict__ idx, int v0,
+  int v1, int* __restrict__ rc)
+{
+  int r, r0, k;
+  if (!subset_row(sr, 1, &r, &r0, &k)) return;
+  int c = 0;
+  const unsigned span = (unsigned)(v1 - v0);
+  for (long long e = off[r]; e < off[r + 1]; ++e)
+    c += (unsigned)(idx[e] - v0) < span ? 1 : 0;
+  rc[row_prefix[k] + (r - r0)] = c;
+}
+
+template <int F>
+__global__ __launch_bounds__(kBlock) void k_tr_expand(
+  SubsetRows sr, const int* __restrict__ row_prefix,
+  const long long* __restrict__ off, const int* __restrict__ idx,
+  const typename Len<F>::T* __restrict__ len, int v0, int v1,
+  const int* __restrict__ pos, long long n_out, unsigned* __restrict__ keys,
+  unsigned long long* __restrict__ vals)
+{
+  int r, r0, k;
+  if (!subset_row(sr, 1, &r, &r0, &k)) return;
+  int p = pos[row_prefix[k] + (r - r0)];
+  const unsigned span = (unsigned)(v1 - v0);
+  for (long long e = off[r]; e < off[r + 1]; ++e)
+  {
+    const unsigned local = (unsigned)(idx[e] - v0);
+    if (local < span && p < n_out)
+    {
+      keys[p] = local;
+      vals[p] = ((unsigned long long)(unsigned)r << 32) |
+        (unsigned long long)Len<F>::bits(len[e]);
+      ++p;
+    }
+  }
+}
+
+template <int F>
+__global__ void k_tr_split(const unsigned long long* __restrict__ vals,
+                           long long n, int* __restrict__ trow,
+                           typename Len<F>::T* __restrict__ tlen)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+  {
+    const unsigned long long v = vals[i];
+    trow[i] = (int)(unsigned)(v >> 32);
+    tlen[i] = Len<F>::from_bits((unsigned)(v & 0xffffffffULL));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Projector kernels
+// ---------------------------------------------------------------------------
+
+template <int NB>
+struct Vec;
+template <>
+struct Vec<1>
+{
+  float v[1];
+  static __device__ __forceinline__ Vec load(const float* p, long long i)
+  {
+    Vec r;
+    r.v[0] = p[i];
+    return r;
+  }
+};
+template <>
+struct Vec<2>
+{
+  float v[2];
+  static __device__ __forceinline__ Vec load(const float* p, long long i)
+  {
+    const float2 a = reinterpret_cast<const float2*>(p)[i];
+    Vec r;
+    r.v[0] = a.x;
+    r.v[1] = a.y;
+    return r;
+  }
+};
+template <>
+struct Vec<4>
+{
+  float v[4];
+  static __device__ __forceinline__ Vec load(const float* p, long long i)
+  {
+    const float4 a = reinterpret_cast<const float4*>(p)[i];
+    Vec r;
+    r.v[0] = a.x;
+    r.v[1] = a.y;
+    r.v[2] = a.z;
+    r.v[3] = a.w;
+    return r;
+  }
+};
+template <>
+struct Vec<8>
+{
+  float v[8];
+  static __device__ __forceinline__ Vec load(const float* p, long long i)
+  {
+    const float4 a = reinterpret_cast<const float4*>(p)[2 * i];
+    const float4 b = reinterpret_cast<const float4*>(p)[2 * i + 1];
+    Vec r;
+    r.v[0] = a.x;
+    r.v[1] = a.y;
+    r.v[2] = a.z;
+    r.v[3] = a.w;
+    r.v[4] = b.x;
+    r.v[5] = b.y;
+    r.v[6] = b.z;
+    r.v[7] = b.w;
+    return r;
+  }
+};
+
+// acc[b] += sum over elements [e, e1) (lane-strided by `lanes`) of
+// dec(len[e]) * X[ind[e] * NB + b], unrolled by 4 so that four index loads
+// and then four gathers are in flight together.
+template <int F, int NB>
+__device__ __forceinline__ void gather_sum(const int* __restrict__ ind,
+                                           const typename Len<F>::T* __restrict__ len,
+                                           const float* __restrict__ X,
+                                           long long e, long long e1, int lanes,
+                                           float* acc)
+{
+  for (; e + 3 * lanes < e1; e += 4 * lanes)
+  {
+    int i[4];
+    float l[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+    {
+      i[u] = ind[e + u * lanes];
+      l[u] = Len<F>::dec(len[e + u * lanes]);
+    }
+    Vec<NB> xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xv[u] = Vec<NB>::load(X, i[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[b] = fmaf(l[u], xv[u].v[b], acc[b]);
+  }
+  for (; e < e1; e += lanes)
+  {
+    const int i = ind[e];
+    const float l = Len<F>::dec(len[e]);
+    const Vec<NB> xv = Vec<NB>::load(X, i);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = fmaf(l, xv.v[b], acc[b]);
+  }
+}
+
+template <int NB>
+__device__ __forceinline__ void lane_reduce(float* acc, int lanes)
+{
+  for (int o = lanes >> 1; o > 0; o >>= 1)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] += __shfl_xor(acc[b], o);
+}
+
+struct RowData
+{
+  const long long* off;
+  const int* idx;
+  const void* len;
+  const int* bin;
+  const float* factor;
+};
+
+// Forward projection of the rows of one subset: sino[bin] = line integral.
+template <int F>
+__global__ __launch_bounds__(kBlock) void k_pc_forward(SubsetRows sr,
+                                                       RowData rd, int lanes,
+                                                       const float* __restrict__ img,
+                                                       float* __restrict__ sino)
+{
+  int r, r0, k;
+  const bool active = subset_row(sr, lanes, &r, &r0, &k);
+  const int lane = (int)threadIdx.x & (lanes - 1);
+  float acc[1] = {0.0f};
+  if (active)
+    gather_sum<F, 1>(rd.idx, (const typename Len<F>::T*)rd.len, img,
+                     rd.off[r] + lane, rd.off[r + 1], lanes, acc);
+  lane_reduce<1>(acc, lanes);
+  if (active && lane == 0) sino[rd.bin[r]] = acc[0] * rd.factor[r];
+}
+
+struct OsemRowsIO
+{
+  const float* y[kMaxCacheBatch];
+  const float* bias[kMaxCacheBatch];
+  int n_images;
+};
+
+// OSEM line integrals of NB images (image b at X[v * NB + b]) over one
+// subset's rows, and the per-row backprojection weights
+// w[r * NB + b] = factor * y_b / (line_b + bias_b) when that line exceeds
+// 1e-5, else 0 (reconAlgos.cc:126-158).
+template <int F, int NB>
+__global__ __launch_bounds__(kBlock) void k_pc_osem_rows(SubsetRows sr,
+                                                         RowData rd, int lanes,
+                                                         const float* __restrict__ X,
+                                                         OsemRowsIO io,
+                                                         float* __restrict__ w)
+{
+  int r, r0, k;
+  const bool active = subset_row(sr, lanes, &r, &r0, &k);
+  const int lane = (int)threadIdx.x & (lanes - 1);
+  float acc[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) acc[b] = 0.0f;
+  if (active)
+    gather_sum<F, NB>(rd.idx, (const typename Len<F>::T*)rd.len, X,
+                      rd.off[r] + lane, rd.off[r + 1], lanes, acc);
+  lane_reduce<NB>(acc, lanes);
+  if (!active || lane != 0) return;
+  const float f = rd.factor[r];
+  const int bin = rd.bin[r];
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+  {
+    float out = 0.0f;
+    if (b < io.n_images)
+    {
+      float line = acc[b] * f;
+      if (io.bias[b]) line += io.bias[b][bin];
+      if (line > kEpsF) out = io.y[b][bin] / line * f;
+    }
+    w[(long long)r * NB + b] = out;
+  }
+}
+
+// Per-row backprojection weights of a sinogram (or of ones): the value
+// times the row's length factor.
+__global__ void k_pc_row_weights(const int* __restrict__ bin,
+                                 const float* __restrict__ factor, int n_rows,
+                                 const float* __restrict__ sino,
+                                 float* __restrict__ w)
+{
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+       r < n_rows; r += (long long)gridDim.x * blockDim.x)
+    w[r] = (sino ? sino[bin[r]] : 1.0f) * factor[r];
+}
+
+struct VoxOut
+{
+  float* p[kMaxCacheBatch];
+};
+
+// Backprojection of one subset: out_b[v] = sum over the subset's elements
+// through voxel v of dec(len) * w[row * NB + b] (overwrites out_b).
+template <int F, int NB>
+__global__ __launch_bounds__(kBlock) void k_pc_backproject(
+  const long long* __restrict__ voff, const int* __restrict__ trow,
+  const void* __restrict__ tlen_, long long nvox, int lanes, int xcd_map,
+  const float* __restrict__ w, VoxOut out)
+{
+  const long long v =
+    ((long long)logical_block(blockIdx.x, gridDim.x, xcd_map) * blockDim.x +
+     threadIdx.x) / lanes;
+  const int lane = (int)threadIdx.x & (lanes - 1);
+  const bool active = v < nvox;
+  float acc[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) acc[b] = 0.0f;
+  if (active)
+    gather_sum<F, NB>(trow, (const typename Len<F>::T*)tlen_, w,
+                      voff[v] + lane, voff[v + 1], lanes, acc);
+  lane_reduce<NB>(acc, lanes);
+  if (!active || lane != 0) return;
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+    if (out.p[b]) out.p[b][v] = acc[b];
+}
+
+struct ImgIn
+{
+  const float* p[kMaxCacheBatch];
+};
+
+// X[v * NB + b] = x_b[v] (0 for b >= n_images).
+template <int NB>
+__global__ void k_pc_pack(float* __restrict__ X, ImgIn in, int n_images,
+                          long long nvox)
+{
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+       v < nvox; v += (long long)gridDim.x * blockDim.x)
+  {
+    float t[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) t[b] = b < n_images ? in.p[b][v] : 0.0f;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) X[v * NB + b] = t[b];
+  }
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct RowMatrix
+{
+  int device = 0;
+  unsigned generation = 0;
+  fir_vol_desc vol{};
+  int walk_mode = 0;
+  int pair_order = 0;
+  int tang_major = 0;
+  int fmt = 0;
+  int n_views = 0;
+  int n_rows = 0;
+  long long n_elems = 0;
+  int* d_view_row = nullptr;        // [n_views + 1]
+  long long* d_view_elem = nullptr; // [n_views + 1]
+  int* d_bin = nullptr;             // [n_rows]
+  float* d_factor = nullptr;        // [n_rows]
+  long long* d_off = nullptr;       // [n_rows + 1]
+  int* d_idx = nullptr;             // [n_elems]
+  void* d_len = nullptr;            // [n_elems]
+  std::vector<int> view_row;
+  std::vector<long long> view_elem;
+
+  long long bytes() const
+  {
+    return (long long)(n_views + 1) * (4 + 8) + (long long)n_rows * (4 + 4 + 8) +
+      8 + n_elems * (4 + (long long)len_bytes(fmt));
+  }
+
+  ~RowMatrix()
+  {
+    (void)hipSetDevice(device);
+    (void)hipFree(d_view_row);
+    (void)hipFree(d_view_elem);
+    (void)hipFree(d_bin);
+    (void)hipFree(d_factor);
+    (void)hipFree(d_off);
+    (void)hipFree(d_idx);
+    (void)hipFree(d_len);
+  }
+};
+
+struct fir_cache
+{
+  fir_ctx* ctx = nullptr;
+  std::shared_ptr<RowMatrix> rm;
+  fir_vol_desc vol{};
+  long long nvox = 0;
+  int n_subsets = 0;
+  long long* d_voff = nullptr; // [n_subsets][nvox + 1], global element offsets
+  int* d_trow = nullptr;       // [n_elems]
+  void* d_tlen = nullptr;      // [n_elems]
+  float* d_w = nullptr;        // per-row weights, [n_rows * 8]
+  size_t w_floats = 0;
+  float* d_x = nullptr;        // interleaved images, [nvox * 8]
+  size_t x_floats = 0;
+  int fwd_lanes = 16;
+  int bp_lanes = 16;
+  int xcd_map = 0;
+
+  ~fir_cache()
+  {
+    (void)hipSetDevice(rm ? rm->device : 0);
+    (void)hipFree(d_voff);
+    (void)hipFree(d_trow);
+    (void)hipFree(d_tlen);
+    (void)hipFree(d_w);
+    (void)hipFree(d_x);
+  }
+};
+
+namespace
+{
+
+// Device buffers of one build step, freed on every exit path.
+struct DevBufs
+{
+  std::vector<void*> p;
+  ~DevBufs()
+  {
+    for (void* q : p) (void)hipFree(q);
+  }
+  template <class T>
+  int get(T** out, size_t count, const char* what)
+  {
+    *out = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc((void**)out, count * sizeof(T)) != hipSuccess)
+    {
+      (void)hipGetLastError();
+      return fail(FIR_E_NOMEM, "path cache: allocation of %zu bytes for %s failed",
+                  count * sizeof(T), what);
+    }
+    p.push_back(*out);
+    return FIR_OK;
+  }
+};
+
+template <class T>
+int dev_alloc(T** out, size_t count, const char* what)
+{
+  *out = nullptr;
+  if (count == 0) count = 1;
+  if (hipMalloc((void**)out, count * sizeof(T)) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return fail(FIR_E_NOMEM, "path cache: allocation of %zu bytes for %s failed",
+                count * sizeof(T), what);
+  }
+  return FIR_OK;
+}
+
+bool same_vol(const fir_vol_desc& a, const fir_vol_desc& b)
+{
+  return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.dx == b.dx &&
+    a.dy == b.dy && a.dz == b.dz && a.ox == b.ox && a.oy == b.oy &&
+    a.oz == b.oz;
+}
+
+// LOR decode of the cache's rows (see cache_lor): the context's pair
+// order and tangential-major flag (fir_set_lor_order).
+LorGeom lor_geom_views(const fir_ctx* c)
+{
+  LorGeom lg = lor_geom(c);
+  lg.row_mode = fir::ROWS_VIEW_MAJOR;
+  return lg;
+}
+
+template <int F>
+int fill_rows(fir_ctx* c, const Grid& g, const LorGeom& lg, long long n,
+              long long per_view, const int* count, const float* maxlen, const long long* eoff,
+              RowMatrix* rm)
+{
+  hipLaunchKernelGGL(k_pc_fill<F>, dim3(blocks_for(n)), dim3(kBlock), 0,
+                     c->stream, g, lg, n, per_view, count, maxlen, eoff,
+                     rm->d_idx,
+                     (typename Len<F>::T*)rm->d_len);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
+               std::shared_ptr<RowMatrix>* out)
+{
+  Grid g;
+  int rc = make_grid(c, v, &g);
+  if (rc) return rc;
+  const long long n = c->n_bins;
+  if ((rc = check_work(n))) return rc;
+  const LorGeom lg = lor_geom_views(c);
+  const long long per_view = (long long)c->n_pairs * c->n_tang;
+  if (per_view * c->n_views != n)
+    return fail(FIR_E_STATE, "path cache: inconsistent sinogram geometry");
+
+  DevBufs tmp;
+  int* count;
+  float* maxlen;
+  int* rowid;
+  long long* eoff;
+  if ((rc = tmp.get(&count, n, "element counts"))) return rc;
+  if ((rc = tmp.get(&maxlen, n, "longest elements"))) return rc;
+  if ((rc = tmp.get(&rowid, n, "row ids"))) return rc;
+  if ((rc = tmp.get(&eoff, n, "element offsets"))) return rc;
+
+  hipLaunchKernelGGL(k_pc_stats, dim3(blocks_for(n)), dim3(kBlock), 0,
+                     c->stream, g, lg, n, per_view, count, maxlen);
+  FIR_CHECK_LAUNCH();
+
+  // Exclusive scans: row id of each crossing LOR, first element of each LOR.
+  hipcub::TransformInputIterator<int, HitFlag, const int*> hits(count,
+                                                               HitFlag());
+  hipcub::TransformInputIterator<long long, ToI64, const int*> counts64(
+    count, ToI64());
+  size_t tb1 = 0, tb2 = 0;
+  FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb1, hits, rowid,
+                                                 (int)n, c->stream));
+  FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, counts64, eoff,
+                                                 (int)n, c->stream));
+  unsigned char* scan_tmp;
+  if ((rc = tmp.get(&scan_tmp, std::max(tb1, tb2), "scan scratch"))) return rc;
+  FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tb1, hits, rowid,
+                                                 (int)n, c->stream));
+  FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tb2, counts64, eoff,
+                                                 (int)n, c->stream));
+
+  auto rm = std::make_shared<RowMatrix>();
+  rm->device = c->device;
+  rm->generation = c->generation;
+  rm->vol = *v;
+  rm->walk_mode = c->walk_mode;
+  rm->pair_order = c->lor_order;
+  rm->tang_major = c->tang_major;
+  rm->fmt = fmt;
+  rm->n_views = c->n_views;
+  if ((rc = dev_alloc(&rm->d_view_row, c->n_views + 1, "view rows"))) return rc;
+  if ((rc = dev_alloc(&rm->d_view_elem, c->n_views + 1, "view elements")))
+    return rc;
+
+  // Totals first (they size the row and element arrays).
+  long long* d_end;
+  if ((rc = tmp.get(&d_end, 1, "totals"))) return rc;
+  int last_row = 0, last_count = 0;
+  long long last_eoff = 0;
+  FIR_HIP_CHECK(hipMemcpyAsync(&last_row, rowid + n - 1, sizeof(int),
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipMemcpyAsync(&last_count, count + n - 1, sizeof(int),
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipMemcpyAsync(&last_eoff, eoff + n - 1, sizeof(long long),
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  rm->n_rows = last_row + (last_count > 0 ? 1 : 0);
+  rm->n_elems = last_eoff + last_count;
+
+  size_t free_b = 0, total_b = 0;
+  FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  const long long need = rm->bytes();
+  if ((long long)free_b < need + (1LL << 30))
+    return fail(FIR_E_NOMEM,
+                "path cache: %lld elements need %.2f GB of device memory, "
+                "%.2f GB free",
+                rm->n_elems, need / 1e9, free_b / 1e9);
+  if ((rc = dev_alloc(&rm->d_bin, rm->n_rows, "row bins"))) return rc;
+  if ((rc = dev_alloc(&rm->d_factor, rm->n_rows, "row factors"))) return rc;
+  if ((rc = dev_alloc(&rm->d_off, (size_t)rm->n_rows + 1, "row offsets")))
+    return rc;
+  if ((rc = dev_alloc(&rm->d_idx, rm->n_elems, "element indices"))) return rc;
+  if ((rc = dev_alloc((unsigned char**)&rm->d_len,
+                      rm->n_elems * len_bytes(fmt), "element lengths")))
+    return rc;
+
+  hipLaunchKernelGGL(k_pc_views, dim3(blocks_for(c->n_views + 1)),
+                     dim3(kBlock), 0, c->stream, count, rowid, eoff, n,
+                     per_view, c->n_views, rm->d_view_row, rm->d_view_elem,
+                     rm->d_off);
+  FIR_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_pc_rows, dim3(blocks_for(n)), dim3(kBlock), 0,
+                     c->stream, lg, n, per_view, count, maxlen, rowid, eoff,
+                     fmt,
+                     rm->d_bin, rm->d_factor, rm->d_off);
+  FIR_CHECK_LAUNCH();
+  switch (fmt)
+  {
+  case FIR_LEN_UNORM16:
+    rc = fill_rows<FIR_LEN_UNORM16>(c, g, lg, n, per_view, count, maxlen, eoff,
+                                  rm.get());
+    break;
+  case FIR_LEN_FP16:
+    rc = fill_rows<FIR_LEN_FP16>(c, g, lg, n, per_view, count, maxlen, eoff,
+                                  rm.get());
+    break;
+  default:
+    rc = fill_rows<FIR_LEN_FP32>(c, g, lg, n, per_view, count, maxlen, eoff,
+                                  rm.get());
+    break;
+  }
+  if (rc) return rc;
+  rm->view_row.resize(c->n_views + 1);
+  rm->view_elem.resize(c->n_views + 1);
+  FIR_HIP_CHECK(hipMemcpyAsync(rm->view_row.data(), rm->d_view_row,
+                               sizeof(int) * (c->n_views + 1),
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipMemcpyAsync(rm->view_elem.data(), rm->d_view_elem,
+                               sizeof(long long) * (c->n_views + 1),
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  *out = rm;
+  return FIR_OK;
+}
+
+template <int F>
+int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
+                    const SubsetRows& sr, const int* d_prefix, int vpr,
+                    int max_rows, int n_rows_s, int v0, int v1,
+                    long long base, long long n_ch)
+{
+  DevBufs tmp; // the chunk's scratch, released on return
+  int rc;
+  int* rc_cnt;
+  int* pos;
+  if ((rc = tmp.get(&rc_cnt, (size_t)n_rows_s + 1, "row counts"))) return rc;
+  if ((rc = tmp.get(&pos, (size_t)n_rows_s + 1, "row positions"))) return rc;
+  unsigned *k0, *k1;
+  unsigned long long *val0, *val1;
+  if ((rc = tmp.get(&k0, n_ch, "sort keys"))) return rc;
+  if ((rc = tmp.get(&k1, n_ch, "sort keys"))) return rc;
+  if ((rc = tmp.get(&val0, n_ch, "sort values"))) return rc;
+  if ((rc = tmp.get(&val1, n_ch, "sort values"))) return rc;
+
+  const dim3 grid_rows(blocks_for(max_rows), vpr);
+  FIR_HIP_CHECK(hipMemsetAsync(rc_cnt, 0, sizeof(int) * ((size_t)n_rows_s + 1),
+                               c->stream));
+  hipLaunchKernelGGL(k_tr_count, grid_rows, dim3(kBlock), 0, c->stream, sr,
+                     d_prefix, rm->d_off, rm->d_idx, v0, v1, rc_cnt);
+  FIR_CHECK_LAUNCH();
+  size_t tb = 0;
+  FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, rc_cnt, pos,
+                                                 n_rows_s + 1, c->stream));
+  unsigned char* scan_tmp;
+  if ((rc = tmp.get(&scan_tmp, tb, "scan scratch"))) return rc;
+  FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tb, rc_cnt, pos,
+                                                 n_rows_s + 1, c->stream));
+  int counted = -1;
+  FIR_HIP_CHECK(hipMemcpyAsync(&counted, pos + n_rows_s, sizeof(int),
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if ((long long)counted != n_ch)
+    return fail(FIR_E_STATE, "path cache: voxels [%d, %d) hold %lld elements "
+                             "by histogram, %d by rows", v0, v1, n_ch, counted);
+  hipLaunchKernelGGL(k_tr_expand<F>, grid_rows, dim3(kBlock), 0, c->stream, sr,
+                     d_prefix, rm->d_off, rm->d_idx,
+                     (const typename Len<F>::T*)rm->d_len, v0, v1, pos, n_ch,
+                     k0, val0);
+  FIR_CHECK_LAUNCH();
+
+  // Stable radix sort by voxel: elements of one voxel keep their row order.
+  int bits = 1;
+  while ((1LL << bits) < (long long)(v1 - v0)) ++bits;
+  hipcub::DoubleBuffer<unsigned> keys(k0, k1);
+  hipcub::DoubleBuffer<unsigned long long> vals(val0, val1);
+  size_t sb = 0;
+  FIR_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
+    nullptr, sb, keys, vals, (int)n_ch, 0, bits, c->stream));
+  unsigned char* sort_tmp;
+  if ((rc = tmp.get(&sort_tmp, sb, "sort scratch"))) return rc;
+  FIR_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
+    sort_tmp, sb, keys, vals, (int)n_ch, 0, bits, c->stream));
+  hipLaunchKernelGGL(k_tr_split<F>, dim3(ew_blocks(n_ch)), dim3(kBlock), 0,
+                     c->stream, vals.Current(), n_ch, cache->d_trow + base,
+                     (typename Len<F>::T*)cache->d_tlen + base);
+  FIR_CHECK_LAUNCH();
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return FIR_OK;
+}
+
+int build_transpose(fir_ctx* c, fir_cache* cache)
+{
+  RowMatrix* rm = cache->rm.get();
+  const int S = cache->n_subsets;
+  const long long nvox = cache->nvox;
+  int rc;
+  if ((rc = dev_alloc(&cache->d_voff, (size_t)S * (nvox + 1), "voxel offsets")))
+    return rc;
+  if ((rc = dev_alloc(&cache->d_trow, rm->n_elems, "transposed rows")))
+    return rc;
+  if ((rc = dev_alloc((unsigned char**)&cache->d_tlen,
+                      rm->n_elems * len_bytes(rm->fmt), "transposed lengths")))
+    return rc;
+
+  std::vector<long long> voff(nvox + 1);
+  long long base = 0;
+  for (int s = 0; s < S; ++s)
+  {
+    DevBufs tmp;
+    const int vpr = fir::views_in_subset(rm->n_views, S, s);
+    std::vector<int> prefix(vpr + 1, 0);
+    int max_rows = 0;
+    long long max_elems = 0, elems_s = 0;
+    for (int k = 0; k < vpr; ++k)
+    {
+      const int view = k * S + s;
+      const int nr = rm->view_row[view + 1] - rm->view_row[view];
+      const long long ne = rm->view_elem[view + 1] - rm->view_elem[view];
+      prefix[k + 1] = prefix[k] + nr;
+      max_rows = std::max(max_rows, nr);
+      max_elems = std::max(max_elems, ne);
+      elems_s += ne;
+    }
+    const int n_rows_s = prefix[vpr];
+    int* d_prefix;
+    if ((rc = tmp.get(&d_prefix, vpr + 1, "subset row prefix"))) return rc;
+    FIR_HIP_CHECK(hipMemcpyAsync(d_prefix, prefix.data(), sizeof(int) * (vpr + 1),
+                                 hipMemcpyHostToDevice, c->stream));
+
+    // Voxel histogram and offsets of the subset's elements.
+    unsigned* cnt;
+    if ((rc = tmp.get(&cnt, nvox + 1, "voxel counts"))) return rc;
+    FIR_HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned) * (nvox + 1),
+                                 c->stream));
+    if (max_elems > 0)
+    {
+      hipLaunchKernelGGL(k_tr_hist, dim3(blocks_for(max_elems), vpr),
+                         dim3(kBlock), 0, c->stream, rm->d_view_elem, S, s,
+                         rm->d_idx, cnt);
+      FIR_CHECK_LAUNCH();
+    }
+    long long* voff_s = cache->d_voff + (long long)s * (nvox + 1);
+    hipcub::TransformInputIterator<long long, ToI64, const unsigned*> cnt64(
+      cnt, ToI64());
+    size_t tb = 0;
+    FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveScan(
+      nullptr, tb, cnt64, voff_s, hipcub::Sum(), base, (int)(nvox + 1),
+      c->stream));
+    unsigned char* scan_tmp;
+    if ((rc = tmp.get(&scan_tmp, tb, "scan scratch"))) return rc;
+    FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveScan(
+      scan_tmp, tb, cnt64, voff_s, hipcub::Sum(), base, (int)(nvox + 1),
+      c->stream));
+    FIR_HIP_CHECK(hipMemcpyAsync(voff.data(), voff_s,
+                                 sizeof(long long) * (nvox + 1),
+                                 hipMemcpyDeviceToHost, c->stream));
+    FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (voff[nvox] - base != elems_s)
+      return fail(FIR_E_STATE, "path cache: subset %d holds %lld elements, "
+                               "its voxel histogram %lld", s, elems_s,
+                  voff[nvox] - base);
+
+    const SubsetRows sr{rm->d_view_row, S, s, 0};
+    long long v0 = 0;
+    while (v0 < nvox && n_rows_s > 0)
+    {
+      // Largest voxel range starting at v0 with at most kChunkElems elements.
+      const long long limit = voff[v0] + kChunkElems;
+      long long v1 = std::upper_bound(voff.begin() + v0 + 1, voff.end(), limit) -
+        voff.begin() - 1;
+      if (v1 <= v0)
+        return fail(FIR_E_STATE, "path cache: voxel %lld holds more than "
+                                 "%lld elements", v0, kChunkElems);
+      if (v1 > nvox) v1 = nvox;
+      const long long n_ch = voff[v1] - voff[v0];
+      if (n_ch > 0)
+      {
+        switch (rm->fmt)
+        {
+        case FIR_LEN_UNORM16:
+          rc = transpose_chunk<FIR_LEN_UNORM16>(c, rm, cache, sr, d_prefix, vpr,
+                                                max_rows, n_rows_s, (int)v0,
+                                                (int)v1, voff[v0], n_ch);
+          break;
+        case FIR_LEN_FP16:
+          rc = transpose_chunk<FIR_LEN_FP16>(c, rm, cache, sr, d_prefix, vpr,
+                                             max_rows, n_rows_s, (int)v0,
+                                             (int)v1, voff[v0], n_ch);
+          break;
+        default:
+          rc = transpose_chunk<FIR_LEN_FP32>(c, rm, cache, sr, d_prefix, vpr,
+                                             max_rows, n_rows_s, (int)v0,
+                                             (int)v1, voff[v0], n_ch);
+          break;
+        }
+        if (rc) return rc;
+      }
+      v0 = v1;
+    }
+    base = voff[nvox];
+  }
+  if (base != rm->n_elems)
+    return fail(FIR_E_STATE, "path cache: transposed %lld of %lld elements",
+                base, rm->n_elems);
+  return FIR_OK;
+}
+
+int check_cache(const fir_cache* cache)
+{
+  if (!cache || !cache->ctx || !cache->rm)
+    return fail(FIR_E_ARG, "path cache is NULL");
+  return FIR_OK;
+}
+
+RowData row_data(const RowMatrix* rm)
+{
+  return RowData{rm->d_off, rm->d_idx, rm->d_len, rm->d_bin, rm->d_factor};
+}
+
+int max_rows_of_subset(const RowMatrix* rm, int S, int s)
+{
+  int m = 0;
+  const int vpr = fir::views_in_subset(rm->n_views, S, s);
+  for (int k = 0; k < vpr; ++k)
+  {
+    const int view = k * S + s;
+    m = std::max(m, rm->view_row[view + 1] - rm->view_row[view]);
+  }
+  return m;
+}
+
+// Blocks of a projector launch: a multiple of 8 with the XCD mapping.
+unsigned grid_blocks(const fir_cache* cache, long long threads)
+{
+  unsigned b = blocks_for(threads);
+  if (cache->xcd_map) b = (b + 7u) & ~7u;
+  return b;
+}
+
+template <int F>
+void launch_forward(fir_cache* cache, int S, int s, const float* img,
+                    float* sino)
+{
+  const RowMatrix* rm = cache->rm.get();
+  const int lanes = cache->fwd_lanes;
+  const int vpr = fir::views_in_subset(rm->n_views, S, s);
+  const long long threads = (long long)max_rows_of_subset(rm, S, s) * lanes;
+  const dim3 grid(grid_blocks(cache, std::max(threads, 1LL)), vpr);
+  hipLaunchKernelGGL((k_pc_forward<F>), grid, dim3(kBlock), 0,
+                     cache->ctx->stream,
+                     SubsetRows{rm->d_view_row, S, s, cache->xcd_map},
+                     row_data(rm), lanes, img, sino);
+}
+
+template <int F, int NB>
+void launch_bp_nb(fir_cache* cache, int s, const float* w, const VoxOut& out)
+{
+  const int lanes = cache->bp_lanes;
+  const long long threads = cache->nvox * lanes;
+  const dim3 grid(grid_blocks(cache, threads));
+  const long long* voff = cache->d_voff + (long long)s * (cache->nvox + 1);
+  hipLaunchKernelGGL((k_pc_backproject<F, NB>), grid, dim3(kBlock), 0,
+                     cache->ctx->stream, voff, cache->d_trow, cache->d_tlen,
+                     cache->nvox, lanes, cache->xcd_map, w, out);
+}
+
+template <int F>
+void launch_backproject(fir_cache* cache, int s, int nb, const float* w,
+                        const VoxOut& out)
+{
+  switch (nb)
+  {
+  case 1: launch_bp_nb<F, 1>(cache, s, w, out); break;
+  case 2: launch_bp_nb<F, 2>(cache, s, w, out); break;
+  case 4: launch_bp_nb<F, 4>(cache, s, w, out); break;
+  default: launch_bp_nb<F, 8>(cache, s, w, out); break;
+  }
+}
+
+template <int F, int NB>
+void launch_rows_nb(fir_cache* cache, int s, const float* X,
+                    const OsemRowsIO& io, float* w)
+{
+  const RowMatrix* rm = cache->rm.get();
+  const int S = cache->n_subsets;
+  const int lanes = cache->fwd_lanes;
+  const int vpr = fir::views_in_subset(rm->n_views, S, s);
+  const long long threads = (long long)max_rows_of_subset(rm, S, s) * lanes;
+  const dim3 grid(grid_blocks(cache, std::max(threads, 1LL)), vpr);
+  hipLaunchKernelGGL((k_pc_osem_rows<F, NB>), grid, dim3(kBlock), 0,
+                     cache->ctx->stream,
+                     SubsetRows{rm->d_view_row, S, s, cache->xcd_map},
+                     row_data(rm), lanes, X, io, w);
+}
+
+template <int F>
+void launch_osem_rows(fir_cache* cache, int s, int nb, const float* X,
+                      const OsemRowsIO& io, float* w)
+{
+  switch (nb)
+  {
+  case 1: launch_rows_nb<F, 1>(cache, s, X, io, w); break;
+  case 2: launch_rows_nb<F, 2>(cache, s, X, io, w); break;
+  case 4: launch_rows_nb<F, 4>(cache, s, X, io, w); break;
+  default: launch_rows_nb<F, 8>(cache, s, X, io, w); break;
+  }
+}
+
+int batch_width(int n)
+{
+  return n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : 8;
+}
+
+} // namespace
+
+extern "C" {
+
+int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
+                     int len_format, fir_cache** out)
+{
+  if (!out) return fail(FIR_E_ARG, "output pointer is NULL");
+  *out = nullptr;
+  int rc = need_lors(c);
+  if (rc) return rc;
+  if (len_format != FIR_LEN_UNORM16 && len_format != FIR_LEN_FP16 &&
+      len_format != FIR_LEN_FP32)
+    return fail(FIR_E_ARG, "unknown length format %d", len_format);
+  Grid g;
+  if ((rc = make_grid(c, v, &g))) return rc;
+  if ((rc = check_subsets(c, n_subsets))) return rc;
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+
+  std::shared_ptr<RowMatrix> rm;
+  for (auto& wp : c->row_matrices)
+  {
+    auto sp = wp.lock();
+    if (sp && sp->generation == c->generation && same_vol(sp->vol, *v) &&
+        sp->walk_mode == c->walk_mode && sp->pair_order == c->lor_order &&
+        sp->tang_major == c->tang_major &&
+        sp->fmt == len_format)
+    {
+      rm = sp;
+      break;
+    }
+  }
+  if (!rm)
+  {
+    if ((rc = build_rows(c, v, len_format, &rm))) return rc;
+    c->row_matrices.erase(
+      std::remove_if(c->row_matrices.begin(), c->row_matrices.end(),
+                     [](const std::weak_ptr<RowMatrix>& w) { return w.expired(); }),
+      c->row_matrices.end());
+    c->row_matrices.push_back(rm);
+  }
+  if (rm->n_rows >= 0x7fffffff)
+    return fail(FIR_E_ARG, "path cache: more than 2^31-1 crossing LORs");
+
+  auto* cache = new fir_cache();
+  cache->ctx = c;
+  cache->rm = rm;
+  cache->vol = *v;
+  cache->nvox = (long long)v->nx * v->ny * v->nz;
+  cache->n_subsets = n_subsets;
+  size_t free_b = 0, total_b = 0;
+  FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  const long long need = rm->n_elems * (4 + (long long)len_bytes(len_format)) +
+    (long long)n_subsets * (cache->nvox + 1) * 8 +
+    ((long long)rm->n_rows + cache->nvox) * kMaxCacheBatch * 4;
+  if ((long long)free_b < need + (1LL << 30))
+  {
+    delete cache;
+    return fail(FIR_E_NOMEM,
+                "path cache: the transpose needs %.2f GB of device memory, "
+                "%.2f GB free",
+                need / 1e9, free_b / 1e9);
+  }
+  // Per-row weights and interleaved images for up to 8 images, allocated
+  // once so that later calls (and graphs capturing them) never reallocate.
+  cache->w_floats = (size_t)rm->n_rows * kMaxCacheBatch;
+  cache->x_floats = (size_t)cache->nvox * kMaxCacheBatch;
+  if ((rc = build_transpose(c, cache)) ||
+      (rc = dev_alloc(&cache->d_w, cache->w_floats, "row weights")) ||
+      (rc = dev_alloc(&cache->d_x, cache->x_floats, "interleaved images")))
+  {
+    (void)hipStreamSynchronize(c->stream);
+    delete cache;
+    return rc;
+  }
+  *out = cache;
+  return FIR_OK;
+}
+
+int fir_cache_destroy(fir_cache* cache)
+{
+  if (!cache) return FIR_OK;
+  if (cache->ctx)
+  {
+    (void)hipSetDevice(cache->ctx->device);
+    (void)hipStreamSynchronize(cache->ctx->stream);
+  }
+  delete cache;
+  return FIR_OK;
+}
+
+int fir_cache_info(const fir_cache* cache, int64_t* n_rows, int64_t* n_elements,
+                   int64_t* device_bytes)
+{
+  int rc = check_cache(cache);
+  if (rc) return rc;
+  const RowMatrix* rm = cache->rm.get();
+  if (n_rows) *n_rows = rm->n_rows;
+  if (n_elements) *n_elements = rm->n_elems;
+  if (device_bytes)
+    *device_bytes = rm->bytes() +
+      rm->n_elems * (4 + (long long)len_bytes(rm->fmt)) +
+      (long long)cache->n_subsets * (cache->nvox + 1) * 8 +
+      (long long)(cache->w_floats + cache->x_floats) * 4;
+  return FIR_OK;
+}
+
+int fir_cache_tune(fir_cache* cache, int fwd_lanes, int bp_lanes, int flags)
+{
+  int rc = check_cache(cache);
+  if (rc) return rc;
+  auto ok = [](int t) { return t >= 1 && t <= 64 && (t & (t - 1)) == 0; };
+  if (!ok(fwd_lanes) || !ok(bp_lanes))
+    return fail(FIR_E_ARG, "lanes per LOR / voxel must be a power of two "
+                           "from 1 to 64");
+  if (flags & ~FIR_CACHE_XCD_MAP)
+    return fail(FIR_E_ARG, "unknown cache flags 0x%x", flags);
+  cache->fwd_lanes = fwd_lanes;
+  cache->bp_lanes = bp_lanes;
+  cache->xcd_map = (flags & FIR_CACHE_XCD_MAP) ? 1 : 0;
+  return FIR_OK;
+}
+
+int fir_cache_forward(fir_cache* cache, const float* img, float* sino)
+{
+  int rc = check_cache(cache);
+  if (rc) return rc;
+  if (!img || !sino) return fail(FIR_E_ARG, "NULL buffer");
+  fir_ctx* c = cache->ctx;
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  // LORs that miss the volume project to 0 (projections.cc:63-66).
+  if ((rc = fir_fill(c, sino, c->n_bins, 0.0f))) return rc;
+  const int fmt = cache->rm->fmt;
+  // One launch over all views (S = 1, subset 0 covers every view).
+  if (fmt == FIR_LEN_UNORM16)
+    launch_forward<FIR_LEN_UNORM16>(cache, 1, 0, img, sino);
+  else if (fmt == FIR_LEN_FP16)
+    launch_forward<FIR_LEN_FP16>(cache, 1, 0, img, sino);
+  else
+    launch_forward<FIR_LEN_FP32>(cache, 1, 0, img, sino);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+static int cache_backward_impl(fir_cache* cache, const float* sino,
+                               float* frames)
+{
+  int rc = check_cache(cache);
+  if (rc) return rc;
+  if (!frames) return fail(FIR_E_ARG, "NULL image");
+  fir_ctx* c = cache->ctx;
+  const RowMatrix* rm = cache->rm.get();
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_pc_row_weights, dim3(ew_blocks(rm->n_rows)),
+                     dim3(kBlock), 0, c->stream, rm->d_bin, rm->d_factor,
+                     rm->n_rows, sino, cache->d_w);
+  FIR_CHECK_LAUNCH();
+  for (int s = 0; s < cache->n_subsets; ++s)
+  {
+    VoxOut out;
+    std::memset(&out, 0, sizeof(out));
+    out.p[0] = frames + (long long)s * cache->nvox;
+    if (rm->fmt == FIR_LEN_UNORM16)
+      launch_backproject<FIR_LEN_UNORM16>(cache, s, 1, cache->d_w, out);
+    else if (rm->fmt == FIR_LEN_FP16)
+      launch_backproject<FIR_LEN_FP16>(cache, s, 1, cache->d_w, out);
+    else
+      launch_backproject<FIR_LEN_FP32>(cache, s, 1, cache->d_w, out);
+    FIR_CHECK_LAUNCH();
+  }
+  return FIR_OK;
+}
+
+int fir_cache_backward(fir_cache* cache, const float* sino, float* frames)
+{
+  if (!sino) return fail(FIR_E_ARG, "NULL sinogram");
+  return cache_backward_impl(cache, sino, frames);
+}
+
+int fir_cache_sensitivity(fir_cache* cache, float* frames)
+{
+  return cache_backward_impl(cache, nullptr, frames);
+}
+
+int fir_cache_osem_project(fir_cache* cache, int n_images,
+                           const float* const* x, const float* const* y,
+                           const float* const* bias, float* const* bp,
+                           int subset)
+{
+  int rc = check_cache(cache);
+  if (rc) return rc;
+  if (n_images <= 0) return fail(FIR_E_ARG, "n_images must be > 0");
+  if (!x || !y || !bp) return fail(FIR_E_ARG, "NULL pointer array");
+  for (int i = 0; i < n_images; ++i)
+    if (!x[i] || !y[i] || !bp[i]) return fail(FIR_E_ARG, "NULL buffer");
+  if (subset < 0 || subset >= cache->n_subsets)
+    return fail(FIR_E_ARG, "subset %d out of range", subset);
+  fir_ctx* c = cache->ctx;
+  const RowMatrix* rm = cache->rm.get();
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  for (int i0 = 0; i0 < n_images; i0 += kMaxCacheBatch)
+  {
+    const int nb = std::min(kMaxCacheBatch, n_images - i0);
+    const int wd = batch_width(nb);
+    OsemRowsIO io;
+    std::memset(&io, 0, sizeof(io));
+    ImgIn in;
+    std::memset(&in, 0, sizeof(in));
+    VoxOut out;
+    std::memset(&out, 0, sizeof(out));
+    for (int b = 0; b < nb; ++b)
+    {
+      io.y[b] = y[i0 + b];
+      io.bias[b] = bias ? bias[i0 + b] : nullptr;
+      in.p[b] = x[i0 + b];
+      out.p[b] = bp[i0 + b];
+    }
+    io.n_images = nb;
+    const float* X = x[i0];
+    if (wd > 1)
+    {
+      X = cache->d_x;
+      const unsigned blocks = ew_blocks(cache->nvox);
+      switch (wd)
+      {
+      case 2: hipLaunchKernelGGL(k_pc_pack<2>, dim3(blocks), dim3(kBlock), 0, c->stream, cache->d_x, in, nb, cache->nvox); break;
+      case 4: hipLaunchKernelGGL(k_pc_pack<4>, dim3(blocks), dim3(kBlock), 0, c->stream, cache->d_x, in, nb, cache->nvox); break;
+      default: hipLaunchKernelGGL(k_pc_pack<8>, dim3(blocks), dim3(kBlock), 0, c->stream, cache->d_x, in, nb, cache->nvox); break;
+      }
+      FIR_CHECK_LAUNCH();
+    }
+    if (rm->fmt == FIR_LEN_UNORM16)
+    {
+      launch_osem_rows<FIR_LEN_UNORM16>(cache, subset, wd, X, io, cache->d_w);
+      FIR_CHECK_LAUNCH();
+      launch_backproject<FIR_LEN_UNORM16>(cache, subset, wd, cache->d_w, out);
+    }
+    else if (rm->fmt == FIR_LEN_FP16)
+    {
+      launch_osem_rows<FIR_LEN_FP16>(cache, subset, wd, X, io, cache->d_w);
+      FIR_CHECK_LAUNCH();
+      launch_backproject<FIR_LEN_FP16>(cache, subset, wd, cache->d_w, out);
+    }
+    else
+    {
+      launch_osem_rows<FIR_LEN_FP32>(cache, subset, wd, X, io, cache->d_w);
+      FIR_CHECK_LAUNCH();
+      launch_backproject<FIR_LEN_FP32>(cache, subset, wd, cache->d_w, out);
+    }
+    FIR_CHECK_LAUNCH();
+  }
+  return FIR_OK;
+}
+
+} // extern "C"

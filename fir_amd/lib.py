@@ -23,6 +23,10 @@ FIR_ORDER_ZCENTER = 1
 FIR_ORDER_PAIR_MAJOR = 0x100
 FIR_ORDER_TANG_MAJOR = 0x200
 FIR_ORDER_VIEW_MAJOR = 0x400
+FIR_LEN_UNORM16 = 0
+FIR_LEN_FP16 = 1
+FIR_LEN_FP32 = 2
+FIR_CACHE_XCD_MAP = 1
 
 # Every symbol declared in include/fir_hip.h.
 EXPORTS = (
@@ -36,6 +40,9 @@ EXPORTS = (
     "fir_div_clamped", "fir_exp_clamped", "fir_hu_to_mu", "fir_path_dump",
     "fir_path_count", "fir_poisson_nll_grad", "fir_osem_project_batch",
     "fir_set_uneven_subsets", "fir_set_lor_order", "fir_set_scatter_copies",
+    "fir_cache_create", "fir_cache_destroy", "fir_cache_info",
+    "fir_cache_tune", "fir_cache_forward", "fir_cache_backward",
+    "fir_cache_sensitivity", "fir_cache_osem_project",
 )
 
 
@@ -108,6 +115,15 @@ def load() -> C.CDLL:
         "fir_path_count": [vp, P, i, i, C.POINTER(C.c_int64)],
         "fir_poisson_nll_grad": [vp, P, vp, vp, vp, vp, vp],
         "fir_osem_project_batch": [vp, P, i, vp, vp, vp, vp, i, i],
+        "fir_cache_create": [vp, P, i, i, C.POINTER(C.c_void_p)],
+        "fir_cache_destroy": [vp],
+        "fir_cache_info": [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                           C.POINTER(C.c_int64)],
+        "fir_cache_tune": [vp, i, i, i],
+        "fir_cache_forward": [vp, vp, vp],
+        "fir_cache_backward": [vp, vp, vp],
+        "fir_cache_sensitivity": [vp, vp],
+        "fir_cache_osem_project": [vp, i, vp, vp, vp, vp, i],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -294,6 +310,66 @@ class Context:
         _check(self.lib.fir_path_dump(self.h, C.byref(vol), bin0, n_bins,
                                       max_per_lor, _ptr(counts), _ptr(idx),
                                       _ptr(ln)))
+
+    def path_cache(self, vol: VolDesc, n_subsets: int = 1,
+                   len_format: int = FIR_LEN_UNORM16) -> "PathCache":
+        return PathCache(self, vol, n_subsets, len_format)
+
+
+def _ptr_array(ts):
+    return (C.c_void_p * len(ts))(*[_ptr(t) for t in ts])
+
+
+class PathCache:
+    """fir_cache: the Siddon paths of one volume kept in device memory (the
+    LOR-major matrix and its voxel-major transpose for one subset split),
+    an extension of the reference's LORCache (LORCache.h:5-47)."""
+
+    def __init__(self, ctx: Context, vol: VolDesc, n_subsets: int = 1,
+                 len_format: int = FIR_LEN_UNORM16):
+        self.ctx, self.lib, self.vol = ctx, ctx.lib, vol
+        self.n_subsets, self.len_format = n_subsets, len_format
+        h = C.c_void_p()
+        _check(self.lib.fir_cache_create(ctx.h, C.byref(vol), n_subsets,
+                                         len_format, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fir_cache_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        r, e, b = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(self.lib.fir_cache_info(self.h, C.byref(r), C.byref(e),
+                                       C.byref(b)))
+        return dict(rows=int(r.value), elements=int(e.value),
+                    device_bytes=int(b.value))
+
+    def tune(self, fwd_lanes: int = 16, bp_lanes: int = 16, flags: int = 0):
+        _check(self.lib.fir_cache_tune(self.h, fwd_lanes, bp_lanes, flags))
+
+    def forward(self, img, sino):
+        _check(self.lib.fir_cache_forward(self.h, _ptr(img), _ptr(sino)))
+
+    def backward(self, sino, frames):
+        _check(self.lib.fir_cache_backward(self.h, _ptr(sino), _ptr(frames)))
+
+    def sensitivity(self, frames):
+        _check(self.lib.fir_cache_sensitivity(self.h, _ptr(frames)))
+
+    def osem_project(self, xs, ys, biases, bps, subset):
+        n = len(xs)
+        pb = None if biases is None else _ptr_array(biases)
+        _check(self.lib.fir_cache_osem_project(
+            self.h, n, _ptr_array(xs), _ptr_array(ys), pb, _ptr_array(bps),
+            subset))
 
 
 _libm = None

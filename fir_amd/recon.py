@@ -8,6 +8,9 @@ up to 8 images) followed by the per-image update, Gaussian and cut. After a
 first eager iteration, one whole iteration is captured in a HIP graph and
 replayed.
 
+With a ``PathCache`` the projector pass streams the cached system matrix
+(fir_cache_osem_project) instead of walking every LOR.
+
 ``BatchedOSEM`` spreads reconstructions of uneven size (different image
 grids, subset counts and count levels) over such groups.
 
@@ -30,8 +33,13 @@ class OSEMGroup:
                  biases: Optional[Sequence[Optional[torch.Tensor]]] = None,
                  sens: Optional[torch.Tensor] = None,
                  fwhm=(0.0, 0.0, 0.0), conv_interval: int = 0,
-                 cut: float = 0.0, x0: Optional[Sequence[torch.Tensor]] = None):
+                 cut: float = 0.0, x0: Optional[Sequence[torch.Tensor]] = None,
+                 cache: Optional[firlib.PathCache] = None):
         self.ctx, self.vol, self.n_subsets = ctx, vol, n_subsets
+        if cache is not None and cache.n_subsets != n_subsets:
+            raise ValueError("the path cache was built for "
+                             f"{cache.n_subsets} subsets, not {n_subsets}")
+        self.cache = cache
         nv = vol.n_voxels
         self.ys = [y.contiguous() for y in ys]
         self.biases = list(biases) if biases is not None else None
@@ -46,7 +54,10 @@ class OSEMGroup:
         dev = self.ys[0].device
         if sens is None:
             sens = torch.zeros(n_subsets * nv, device=dev)
-            ctx.sensitivity(vol, sens, n_subsets)
+            if cache is not None:
+                cache.sensitivity(sens)
+            else:
+                ctx.sensitivity(vol, sens, n_subsets)
         self.sens = sens
         self.x = ([t.clone() for t in x0] if x0 is not None else
                   [torch.ones(nv, device=dev) for _ in self.ys])
@@ -63,8 +74,12 @@ class OSEMGroup:
     def _subiteration(self, subset: int):
         nv = self.vol.n_voxels
         ctx, vol = self.ctx, self.vol
-        ctx.osem_project_batch(vol, self.x, self.ys, self.biases, self.bp,
-                               subset, self.n_subsets)
+        if self.cache is not None:
+            self.cache.osem_project(self.x, self.ys, self.biases, self.bp,
+                                    subset)
+        else:
+            ctx.osem_project_batch(vol, self.x, self.ys, self.biases, self.bp,
+                                   subset, self.n_subsets)
         sens_s = self.sens[subset * nv:(subset + 1) * nv]
         conv = self.convolve and (subset + 1) % self.conv_interval == 0
         for x, bp in zip(self.x, self.bp):

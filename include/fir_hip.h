@@ -263,6 +263,78 @@ int fir_path_dump(fir_ctx* ctx, const fir_vol_desc* vol, int64_t bin0,
 int fir_path_count(fir_ctx* ctx, const fir_vol_desc* vol, int subset,
                    int n_subsets, int64_t* total);
 
+/* ---- path cache ------------------------------------------------------ */
+
+/* Every LOR's Siddon path, walked once and kept in device memory: the
+ * LOR-major path matrix (shared by the caches of one context, volume, walk
+ * mode and length format) plus, per cache, its voxel-major transpose for
+ * one subset split. Projector passes then stream the matrix: forward
+ * projection and OSEM line integrals gather over a LOR's elements,
+ * backprojection gathers over a voxel's elements (no atomics; results do
+ * not depend on scheduling). It extends the reference's LORCache
+ * (LORCache.h:5-47, LORCache.cc:10-168), which keeps the crystal pairs of
+ * each (subset, segment), to whole paths. A cache describes the geometry
+ * at creation; later fir_set_scanner / fir_set_proj / fir_set_walk_mode
+ * calls do not change it. Calls on one cache must not overlap (they share
+ * its per-LOR weight buffer). */
+typedef struct fir_cache fir_cache;
+
+/* Stored element length: FIR_LEN_UNORM16 (default) round(65535 * len / m)
+ * with m the LOR's longest element, FIR_LEN_FP16 binary16 of len / m,
+ * FIR_LEN_FP32 binary32 of len; each restored with a per-LOR FP32 factor. */
+enum
+{
+  FIR_LEN_UNORM16 = 0,
+  FIR_LEN_FP16 = 1,
+  FIR_LEN_FP32 = 2
+};
+
+/* Build the cache of `vol` for `n_subsets` subsets (views v with
+ * v % n_subsets == s, as LORCache.cc:49-51). Synchronizes the context's
+ * stream; FIR_E_NOMEM if the matrices do not fit in device memory. */
+int fir_cache_create(fir_ctx* ctx, const fir_vol_desc* vol, int n_subsets,
+                     int len_format, fir_cache** out);
+int fir_cache_destroy(fir_cache* cache);
+
+/* Crossing LORs, path elements, and device bytes held by the cache
+ * (including its share of the path matrix). Any pointer may be NULL. */
+int fir_cache_info(const fir_cache* cache, int64_t* n_rows,
+                   int64_t* n_elements, int64_t* device_bytes);
+
+/* Kernel tuning (results change only by FP32 summation order): lanes
+ * that share one LOR (forward) and one voxel (backprojection), powers of
+ * two from 1 to 64 (default 16 and 16); flags FIR_CACHE_XCD_MAP deals
+ * contiguous eighths
+ * of the rows of a view (of the voxels) to the eight XCDs. The path
+ * matrix's row order within a view follows the context's row order
+ * (fir_set_lor_order: pair order, FIR_ORDER_TANG_MAJOR) at creation. */
+enum
+{
+  FIR_CACHE_XCD_MAP = 1
+};
+int fir_cache_tune(fir_cache* cache, int fwd_lanes, int bp_lanes, int flags);
+
+/* projections::forward (projections.cc:15-92) over the cached paths: every
+ * bin written, LORs that miss the volume give 0. */
+int fir_cache_forward(fir_cache* cache, const float* img, float* sino);
+
+/* projections::backward (projections.cc:94-184) with the cache's subset
+ * count: frames[s * n_voxels ...] = backprojection of subset s (overwritten). */
+int fir_cache_backward(fir_cache* cache, const float* sino, float* frames);
+
+/* projections::computeSensitivityVol (projections.cc:186-197): backward of
+ * ones, one frame per subset. */
+int fir_cache_sensitivity(fir_cache* cache, float* frames);
+
+/* OSEM projector pass of subset `subset` for n_images images
+ * (reconAlgos.cc:126-158): bp[b] = A_s^T (y_b / (A_s x_b + bias_b)) over
+ * the LORs whose line exceeds 1e-5 (bp[b] is overwritten). Up to 8 images
+ * share each read of the matrix. bias may be NULL, or hold NULL entries. */
+int fir_cache_osem_project(fir_cache* cache, int n_images,
+                           const float* const* x, const float* const* y,
+                           const float* const* bias, float* const* bp,
+                           int subset);
+
 #ifdef __cplusplus
 }
 #endif
