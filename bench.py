@@ -7,14 +7,20 @@ water cylinder, attenuation-corrected Poisson data at 5e8 counts, OSEM with
 16 subsets and a 4 mm FWHM Gaussian every sub-iteration.
 
 One step = one full 16-subset OSEM iteration (every LOR forward- and
-back-projected once, fused per sub-iteration, plus the per-sub-iteration
-update and Gaussian). Inputs are resident in HBM before timing starts.
+back-projected once per iteration, plus the per-sub-iteration update and
+Gaussian) with the projector streaming the path cache (the Siddon system
+matrix kept in HBM with unorm16 lengths; OSEM images within 1e-4 relative L2
+of the reference). Inputs and the cache are resident in HBM before timing
+starts; the cache build is timed separately.
 
     value = LORs per step / step time  [M LORs/s, fwd+backproj]
 
-The JSON line also carries the OSEM iteration and sub-iteration times, the
-separately timed forward and backprojection passes, the roofline of the
-dominant kernel (the fused OSEM projector) and the reference CPU baseline.
+The JSON line also carries: the same iteration on the plain path (FP64
+Siddon walk per LOR, bit-exact forward), 8 reconstructions batched over one
+cache read per sub-iteration in a HIP graph, the separately timed forward
+and backprojection passes, the roofline of the dominant kernel pair (the
+cached OSEM projector: line integrals + transposed backprojection) and the
+reference CPU baseline.
 """
 from __future__ import annotations
 
@@ -39,14 +45,18 @@ def log(*a):
 
 
 def traffic_from_profiles():
-    """Measured HBM bytes per launch of the dominant kernel, if a profile
-    summary with them is committed (profiles/osem_project_c2_traffic.json)."""
-    p = os.path.join(ROOT, "profiles", "osem_project_c2_traffic.json")
-    if os.path.exists(p):
-        try:
-            return json.load(open(p)).get("bytes_per_launch")
-        except Exception:
-            return None
+    """Measured HBM bytes per launch of the dominant kernel pair (FETCH_SIZE
+    and WRITE_SIZE, calibrated against a copy of known size), from the
+    committed rocprofv3 summary profiles/osem_cached_c2_traffic.md."""
+    p = os.path.join(ROOT, "profiles", "osem_cached_c2_traffic.md")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("bytes_per_launch:"):
+            try:
+                return int(line.split(":", 1)[1])
+            except ValueError:
+                return None
     return None
 
 
@@ -151,17 +161,24 @@ def main():
 
     stream = torch.cuda.current_stream()
 
+    cache_t0 = time.perf_counter()
+    cache = ctx.path_cache(v, n_sub, firlib.FIR_LEN_UNORM16,
+                           row_order=firlib.CACHE_ROW_ORDER)
+    torch.cuda.synchronize()
+    cache_build_s = time.perf_counter() - cache_t0
+    cinfo = cache.info()
+    log(f"path cache: {cinfo['elements']} elements, "
+        f"{cinfo['device_bytes'] / 1e9:.2f} GB, built in {cache_build_s:.2f} s")
+    sens_c = torch.zeros(n_sub * nvox, device="cuda")
+    cache.sensitivity(sens_c)
+
     if args.traffic_probe:
-        # one launch of each projector kernel at steady state, then a plain
-        # copy of known size whose FETCH_SIZE/WRITE_SIZE calibrate the
-        # counters (1 GiB read + 1 GiB written)
-        ctx.osem_project(v, x, y, None, bp, 3, n_sub)
+        # one launch of each cached projector kernel pair at steady state,
+        # then a plain copy of known size whose FETCH_SIZE/WRITE_SIZE
+        # calibrate the counters (1 GiB read + 1 GiB written)
+        cache.osem_project([x], [y], None, [bp], 3)
         torch.cuda.synchronize()
-        ctx.osem_project(v, x, y, None, bp, 3, n_sub)
-        sino = torch.zeros(nb, device="cuda")
-        ctx.forward(v, x, sino)
-        img = torch.zeros(nvox, device="cuda")
-        ctx.backward_subset(v, y, img, 3, n_sub)
+        cache.osem_project([x], [y], None, [bp], 3)
         src = torch.ones(1 << 28, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -169,66 +186,83 @@ def main():
         log("traffic probe done")
         return
 
-    def osem_iteration(evs=None):
+    def osem_iteration(project, sens_v, evs=None):
         for s in range(n_sub):
             if evs is not None:
                 evs[s][0].record(stream)
-            ctx.osem_project(v, x, y, None, bp, s, n_sub)
+            project(s)
             if evs is not None:
                 evs[s][1].record(stream)
-            ctx.osem_update(v, x, bp, sens[s * nvox:(s + 1) * nvox], 0.0,
-                            apply_cut=False, reset=False)
+            ctx.osem_update(v, x, bp, sens_v[s * nvox:(s + 1) * nvox], 0.0,
+                            apply_cut=False, reset=True)
             ctx.convolve(v, x, fwhm, 0.0)
-            ctx.fill(bp, 0.0)
 
-    # --- warm-up and timed OSEM iterations -------------------------------
-    for _ in range(args.warmup):
-        osem_iteration()
-    torch.cuda.synchronize()
-    evs = [[(torch.cuda.Event(enable_timing=True),
-             torch.cuda.Event(enable_timing=True)) for _ in range(n_sub)]
-           for _ in range(args.steps)]
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    t_start.record(stream)
-    for k in range(args.steps):
-        osem_iteration(evs[k])
-    t_end.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_total = t_start.elapsed_time(t_end)
-    ms_step = ms_total / args.steps
-    proj_ms = [a.elapsed_time(b) for row in evs for (a, b) in row]
-    proj_avg_ms = float(np.mean(proj_ms))
-    log(f"OSEM iteration {ms_step:.3f} ms (wall {wall / args.steps * 1e3:.3f}"
-        f" ms), projector launch avg {proj_avg_ms:.3f} ms")
+    def cached_project(s):
+        cache.osem_project([x], [y], None, [bp], s)
+
+    def plain_project(s):
+        ctx.osem_project(v, x, y, None, bp, s, n_sub)
+
+    def timed_iterations(project, sens_v, steps, warmup):
+        x.fill_(1.0)
+        bp.zero_()
+        for _ in range(warmup):
+            osem_iteration(project, sens_v)
+        torch.cuda.synchronize()
+        evs = [[(torch.cuda.Event(enable_timing=True),
+                 torch.cuda.Event(enable_timing=True)) for _ in range(n_sub)]
+               for _ in range(steps)]
+        t_start = torch.cuda.Event(enable_timing=True)
+        t_end = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t_start.record(stream)
+        for k in range(steps):
+            osem_iteration(project, sens_v, evs[k])
+        t_end.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = t_start.elapsed_time(t_end) / steps
+        proj = float(np.mean([a.elapsed_time(b) for row in evs
+                              for (a, b) in row]))
+        return ms, proj, wall / steps * 1e3
+
+    # --- the measured step: cached OSEM iterations ------------------------
+    ms_step, proj_avg_ms, wall_ms = timed_iterations(
+        cached_project, sens_c, args.steps, args.warmup)
+    log(f"cached OSEM iteration {ms_step:.3f} ms (wall {wall_ms:.3f} ms), "
+        f"projector launch avg {proj_avg_ms:.3f} ms")
+
+    # --- the plain path (FP64 walk per LOR), same schedule ----------------
+    plain_ms, plain_proj_ms, _ = timed_iterations(
+        plain_project, sens, max(1, min(args.steps, 2)), 1)
+    log(f"plain OSEM iteration {plain_ms:.3f} ms, projector launch avg "
+        f"{plain_proj_ms:.3f} ms")
 
     # --- separately timed full forward and backprojection passes ---------
     sino = torch.zeros(nb, device="cuda")
     img = torch.zeros(nvox, device="cuda")
-    reps = 3
-    ctx.forward(v, x, sino)
+    frames = torch.zeros(n_sub * nvox, device="cuda")
     e0, e1 = (torch.cuda.Event(enable_timing=True) for _ in range(2))
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(reps):
-        ctx.forward(v, x, sino)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    fwd_ms = e0.elapsed_time(e1) / reps
-    ctx.backward_subset(v, y, img, 0, 1)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(reps):
-        ctx.backward_subset(v, y, img, 0, 1)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    bp_ms = e0.elapsed_time(e1) / reps
-    log(f"forward {fwd_ms:.3f} ms, backprojection {bp_ms:.3f} ms")
 
-    # --- batched: noise realisations sharing one walk per LOR ------------
+    def ev_time(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    fwd_ms = ev_time(lambda: ctx.forward(v, x, sino))
+    bp_ms = ev_time(lambda: ctx.backward_subset(v, y, img, 0, 1))
+    cfwd_ms = ev_time(lambda: cache.forward(x, sino))
+    cbp_ms = ev_time(lambda: cache.backward(y, frames))
+    log(f"plain forward {fwd_ms:.3f} ms, backprojection {bp_ms:.3f} ms; "
+        f"cached forward {cfwd_ms:.3f} ms, backprojection {cbp_ms:.3f} ms")
+
+    # --- batched: reconstructions sharing each read of the cache ---------
     batched = None
     if args.batch > 0:
         from fir_amd import recon
@@ -239,8 +273,8 @@ def main():
             yi = torch.poisson((mean_f * float(sc)), generator=g2)
             ctx.mul_clamped(yi, acf)
             ys.append(yi)
-        grp = recon.OSEMGroup(ctx, v, n_sub, ys, sens=sens, fwhm=fwhm,
-                              conv_interval=1)
+        grp = recon.OSEMGroup(ctx, v, n_sub, ys, sens=sens_c, fwhm=fwhm,
+                              conv_interval=1, cache=cache)
         grp.capture()
         grp.iteration()
         torch.cuda.synchronize()
@@ -252,26 +286,27 @@ def main():
         bms = e0.elapsed_time(e1) / args.steps
         batched = {
             "n_images": args.batch,
+            "path": "cached, interleaved images, one HIP graph per iteration",
             "osem_iter_ms": round(bms, 4),
             "osem_iter_ms_per_recon": round(bms / args.batch, 4),
             "mlors_s_per_recon_aggregate": round(
                 args.batch * w.n_lors / (bms * 1e-3) / 1e6, 3),
-            "speedup_per_recon_vs_plain": round(ms_step * args.batch / bms, 3),
-            "graph": "one HIP graph per iteration (16 sub-iterations)",
+            "speedup_per_recon_vs_plain": round(plain_ms * args.batch / bms,
+                                                3),
+            "speedup_per_recon_vs_cached": round(ms_step * args.batch / bms,
+                                                 3),
         }
         log(f"batched x{args.batch}: {bms:.3f} ms per iteration "
             f"({bms / args.batch:.3f} ms per reconstruction)")
         del grp, ys
         ctx.set_stream(stream)
 
-    # --- roofline of the dominant kernel (fused OSEM projector) ----------
+    # --- roofline of the dominant kernel pair (cached OSEM projector) ----
     # Algorithmic bytes per launch (one sub-iteration's LORs): forward +
-    # backprojection of the cached-path format, 12 B per LOR and 6 B per
-    # path element each way, plus 4 B (x read) + 8 B (bp read-modify-write)
+    # backprojection in the cached-path format, 12 B per LOR and 6 B per
+    # path element each way, plus 4 B (x read) + 8 B (bp write, sens read)
     # per voxel.
-    n_e_total = 0
-    for s in range(n_sub):
-        n_e_total += ctx.path_count(v, s, n_sub)
+    n_e_total = cinfo["elements"]
     lors_per_sub = w.n_lors / n_sub
     n_e_per_sub = n_e_total / n_sub
     bytes_launch = 2 * (12 * lors_per_sub + 6 * n_e_per_sub) + 12 * nvox
@@ -304,17 +339,30 @@ def main():
             "n_lors": w.n_lors,
             "image": "160x160x47 @ 2x2x3.27 mm",
             "osem": f"{n_sub} subsets, 4 mm FWHM Gaussian every subiter",
-            "path": "plain (on-the-fly FP64 Siddon walk, FP32 accumulate)",
+            "path": "path cache (Siddon matrix in HBM, unorm16 lengths)",
         },
         "osem_iter_ms": round(ms_step, 4),
         "osem_subiter_ms": round(ms_step / n_sub, 4),
-        "fwd_mlors_s": round(w.n_lors / (fwd_ms * 1e-3) / 1e6, 3),
-        "bp_mlors_s": round(w.n_lors / (bp_ms * 1e-3) / 1e6, 3),
-        "fwd_ms": round(fwd_ms, 4),
-        "bp_ms": round(bp_ms, 4),
+        "cache": {"build_s": round(cache_build_s, 3),
+                  "elements": int(n_e_total),
+                  "rows": cinfo["rows"],
+                  "device_gb": round(cinfo["device_bytes"] / 1e9, 3),
+                  "fwd_ms": round(cfwd_ms, 4),
+                  "fwd_mlors_s": round(w.n_lors / (cfwd_ms * 1e-3) / 1e6, 3),
+                  "bp_all_subsets_ms": round(cbp_ms, 4),
+                  "bp_mlors_s": round(w.n_lors / (cbp_ms * 1e-3) / 1e6, 3)},
+        "plain": {"osem_iter_ms": round(plain_ms, 4),
+                  "projector_launch_ms": round(plain_proj_ms, 4),
+                  "mlors_s": round(w.n_lors / (plain_ms * 1e-3) / 1e6, 3),
+                  "fwd_ms": round(fwd_ms, 4),
+                  "bp_ms": round(bp_ms, 4),
+                  "fwd_mlors_s": round(w.n_lors / (fwd_ms * 1e-3) / 1e6, 3),
+                  "bp_mlors_s": round(w.n_lors / (bp_ms * 1e-3) / 1e6, 3)},
+        "speedup_cached_vs_plain": round(plain_ms / ms_step, 3),
         "path_elements_per_iter": int(n_e_total),
         "roofline": {
-            "kernel": "k_osem_project",
+            "kernel": "k_pc_osem_rows + k_pc_backproject (one cached OSEM "
+                      "projector launch)",
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
@@ -329,7 +377,10 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu_iter"] = round(cpu["osem_iter_s"] * 1e3 / ms_step, 2)
+        out["speedup_plain_vs_cpu_iter"] = round(
+            cpu["osem_iter_s"] * 1e3 / plain_ms, 2)
     print(json.dumps(out), flush=True)
+    cache.close()
     ctx.close()
 
 

@@ -27,6 +27,9 @@ FIR_LEN_UNORM16 = 0
 FIR_LEN_FP16 = 1
 FIR_LEN_FP32 = 2
 FIR_CACHE_XCD_MAP = 1
+# Row order of path caches: per view, tangential bin then (segment, axial)
+# pair by centre slice -- the rows through one voxel are then close together.
+CACHE_ROW_ORDER = FIR_ORDER_VIEW_MAJOR | FIR_ORDER_TANG_MAJOR | FIR_ORDER_ZCENTER
 
 # Every symbol declared in include/fir_hip.h.
 EXPORTS = (
@@ -206,6 +209,7 @@ class Context:
 
     def set_lor_order(self, order: int):
         _check(self.lib.fir_set_lor_order(self.h, order))
+        self._lor_order = order
 
     def set_scatter_copies(self, copies: int):
         _check(self.lib.fir_set_scatter_copies(self.h, copies))
@@ -312,8 +316,19 @@ class Context:
                                       _ptr(ln)))
 
     def path_cache(self, vol: VolDesc, n_subsets: int = 1,
-                   len_format: int = FIR_LEN_UNORM16) -> "PathCache":
-        return PathCache(self, vol, n_subsets, len_format)
+                   len_format: int = FIR_LEN_UNORM16,
+                   row_order: Optional[int] = None) -> "PathCache":
+        """Build a path cache. row_order (FIR_ORDER_* flags) sets the order
+        of the cached rows within a view (the context's LOR order at
+        creation); by default the context's current order is used."""
+        if row_order is None:
+            return PathCache(self, vol, n_subsets, len_format)
+        old = getattr(self, "_lor_order", 0)
+        self.set_lor_order(row_order)
+        try:
+            return PathCache(self, vol, n_subsets, len_format)
+        finally:
+            self.set_lor_order(old)
 
 
 def _ptr_array(ts):
