@@ -99,6 +99,39 @@ struct Len<FIR_LEN_FP32>
 
 size_t len_bytes(int fmt) { return fmt == FIR_LEN_FP32 ? 4 : 2; }
 
+// Second image layout, y fastest: voxel (x, y, z) at (z nx + x) ny + y.
+// LORs closer to the y axis than to the x axis store their voxel indices in
+// this layout and gather from a y-fastest copy of the image, so that the
+// consecutive voxels of their paths share cache lines.
+struct XY
+{
+  int nx, ny;
+};
+
+__device__ __forceinline__ int to_yfast(const XY& g, int idx)
+{
+  const int x = idx % g.nx;
+  const int r = idx / g.nx;
+  const int y = r % g.ny;
+  const int z = r / g.ny;
+  return (z * g.nx + x) * g.ny + y;
+}
+
+__device__ __forceinline__ int from_yfast(const XY& g, int i)
+{
+  const int y = i % g.ny;
+  const int r = i / g.ny;
+  const int x = r % g.nx;
+  const int z = r / g.nx;
+  return (z * g.ny + y) * g.nx + x;
+}
+
+__device__ __forceinline__ bool y_dominant(const Lor& L)
+{
+  const double dx = L.e2[0] - L.e1[0], dy = L.e2[1] - L.e1[1];
+  return (dy < 0 ? -dy : dy) > (dx < 0 ? -dx : dx);
+}
+
 // ---------------------------------------------------------------------------
 // Build kernels
 // ---------------------------------------------------------------------------
@@ -156,7 +189,8 @@ __global__ __launch_bounds__(kBlock) void k_pc_rows(
   LorGeom lg, long long n, long long per_view, const int* __restrict__ count,
   const float* __restrict__ maxlen, const int* __restrict__ rowid,
   const long long* __restrict__ eoff, int fmt, int* __restrict__ bin,
-  float* __restrict__ factor, long long* __restrict__ off)
+  float* __restrict__ factor, long long* __restrict__ off,
+  unsigned char* __restrict__ yrow)
 {
   const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
   if (t >= n || count[t] == 0) return;
@@ -165,6 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_rows(
   const int r = rowid[t];
   const float m = maxlen[t];
   bin[r] = (int)L.bin;
+  yrow[r] = y_dominant(L) ? 1 : 0;
   off[r] = eoff[t];
   factor[r] = fmt == FIR_LEN_UNORM16 ? m / 65535.0f
             : fmt == FIR_LEN_FP16    ? m
@@ -203,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_fill(
   Grid g, LorGeom lg, long long n, long long per_view,
   const int* __restrict__ count,
   const float* __restrict__ maxlen, const long long* __restrict__ eoff,
-  int* __restrict__ idx, typename Len<F>::T* __restrict__ len)
+  XY xy, int* __restrict__ idx, typename Len<F>::T* __restrict__ len)
 {
   const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
   if (t >= n || count[t] == 0) return;
@@ -212,12 +247,13 @@ __global__ __launch_bounds__(kBlock) void k_pc_fill(
   Walk w;
   if (!fir::walk_setup(g, L.e1, L.e2, w)) return;
   const float m = maxlen[t];
+  const bool yf = y_dominant(L);
   long long e = eoff[t];
   const long long e_end = e + count[t];
   fir::walk_run(g, w, [&](int i, double l) {
     if (e < e_end)
     {
-      idx[e] = i;
+      idx[e] = yf ? to_yfast(xy, i) : i;
       len[e] = Len<F>::enc(l, m);
     }
     ++e;
@@ -256,31 +292,34 @@ __device__ __forceinline__ bool subset_row(const SubsetRows& sr, int lanes,
   return *r < r1;
 }
 
-// Voxel histogram of one subset's elements (thread per element).
+// Voxel histogram of one subset's elements (thread per row; indices of
+// y-fastest rows converted back).
 __global__ __launch_bounds__(kBlock) void k_tr_hist(
-  const long long* __restrict__ view_elem, int n_subsets, int subset,
-  const int* __restrict__ idx, unsigned* __restrict__ cnt)
+  SubsetRows sr, const long long* __restrict__ off, const int* __restrict__ idx,
+  const unsigned char* __restrict__ yrow, XY xy, unsigned* __restrict__ cnt)
 {
-  const int view = (int)blockIdx.y * n_subsets + subset;
-  const long long e =
-    view_elem[view] + (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (e >= view_elem[view + 1]) return;
-  atomicAdd(cnt + idx[e], 1u);
+  int r, r0, k;
+  if (!subset_row(sr, 1, &r, &r0, &k)) return;
+  const bool yf = yrow[r] != 0;
+  for (long long e = off[r]; e < off[r + 1]; ++e)
+    atomicAdd(cnt + (yf ? from_yfast(xy, idx[e]) : idx[e]), 1u);
 }
 
 // Elements of each row of the subset inside the voxel range [v0, v1);
 // local row = row_prefix[k] + (r - first row of view k).
 __global__ __launch_bounds__(kBlock) void k_tr_count(
   SubsetRows sr, const int* __restrict__ row_prefix,
-  const long long* __restrict__ off, const int* __restrict__ idx, int v0,
-  int v1, int* __restrict__ rc)
+  const long long* __restrict__ off, const int* __restrict__ idx,
+  const unsigned char* __restrict__ yrow, XY xy, int v0, int v1,
+  int* __restrict__ rc)
 {
   int r, r0, k;
   if (!subset_row(sr, 1, &r, &r0, &k)) return;
   int c = 0;
   const unsigned span = (unsigned)(v1 - v0);
+  const bool yf = yrow[r] != 0;
   for (long long e = off[r]; e < off[r + 1]; ++e)
-    c += (unsigned)(idx[e] - v0) < span ? 1 : 0;
+    c += (unsigned)((yf ? from_yfast(xy, idx[e]) : idx[e]) - v0) < span ? 1 : 0;
   rc[row_prefix[k] + (r - r0)] = c;
 }
 
@@ -288,6 +327,7 @@ template <int F>
 __global__ __launch_bounds__(kBlock) void k_tr_expand(
   SubsetRows sr, const int* __restrict__ row_prefix,
   const long long* __restrict__ off, const int* __restrict__ idx,
+  const unsigned char* __restrict__ yrow, XY xy,
   const typename Len<F>::T* __restrict__ len, int v0, int v1,
   const int* __restrict__ pos, long long n_out, unsigned* __restrict__ keys,
   unsigned long long* __restrict__ vals)
@@ -296,9 +336,11 @@ __global__ __launch_bounds__(kBlock) void k_tr_expand(
   if (!subset_row(sr, 1, &r, &r0, &k)) return;
   int p = pos[row_prefix[k] + (r - r0)];
   const unsigned span = (unsigned)(v1 - v0);
+  const bool yf = yrow[r] != 0;
   for (long long e = off[r]; e < off[r + 1]; ++e)
   {
-    const unsigned local = (unsigned)(idx[e] - v0);
+    const unsigned local =
+      (unsigned)((yf ? from_yfast(xy, idx[e]) : idx[e]) - v0);
     if (local < span && p < n_out)
     {
       keys[p] = local;
@@ -442,6 +484,7 @@ struct RowData
   const void* len;
   const int* bin;
   const float* factor;
+  const unsigned char* yrow;
 };
 
 // Forward projection of the rows of one subset: sino[bin] = line integral.
@@ -449,6 +492,7 @@ template <int F>
 __global__ __launch_bounds__(kBlock) void k_pc_forward(SubsetRows sr,
                                                        RowData rd, int lanes,
                                                        const float* __restrict__ img,
+                                                       const float* __restrict__ img_y,
                                                        float* __restrict__ sino)
 {
   int r, r0, k;
@@ -456,8 +500,9 @@ __global__ __launch_bounds__(kBlock) void k_pc_forward(SubsetRows sr,
   const int lane = (int)threadIdx.x & (lanes - 1);
   float acc[1] = {0.0f};
   if (active)
-    gather_sum<F, 1>(rd.idx, (const typename Len<F>::T*)rd.len, img,
-                     rd.off[r] + lane, rd.off[r + 1], lanes, acc);
+    gather_sum<F, 1>(rd.idx, (const typename Len<F>::T*)rd.len,
+                     rd.yrow[r] ? img_y : img, rd.off[r] + lane,
+                     rd.off[r + 1], lanes, acc);
   lane_reduce<1>(acc, lanes);
   if (active && lane == 0) sino[rd.bin[r]] = acc[0] * rd.factor[r];
 }
@@ -477,6 +522,7 @@ template <int F, int NB>
 __global__ __launch_bounds__(kBlock) void k_pc_osem_rows(SubsetRows sr,
                                                          RowData rd, int lanes,
                                                          const float* __restrict__ X,
+                                                         const float* __restrict__ XY_,
                                                          OsemRowsIO io,
                                                          float* __restrict__ w)
 {
@@ -487,8 +533,9 @@ __global__ __launch_bounds__(kBlock) void k_pc_osem_rows(SubsetRows sr,
 #pragma unroll
   for (int b = 0; b < NB; ++b) acc[b] = 0.0f;
   if (active)
-    gather_sum<F, NB>(rd.idx, (const typename Len<F>::T*)rd.len, X,
-                      rd.off[r] + lane, rd.off[r + 1], lanes, acc);
+    gather_sum<F, NB>(rd.idx, (const typename Len<F>::T*)rd.len,
+                      rd.yrow[r] ? XY_ : X, rd.off[r] + lane, rd.off[r + 1],
+                      lanes, acc);
   lane_reduce<NB>(acc, lanes);
   if (!active || lane != 0) return;
   const float f = rd.factor[r];
@@ -555,10 +602,11 @@ struct ImgIn
   const float* p[kMaxCacheBatch];
 };
 
-// X[v * NB + b] = x_b[v] (0 for b >= n_images).
+// X[v * NB + b] = x_b[v] (if X is not NULL) and XT[vy * NB + b] = x_b[v]
+// with vy the y-fastest index of v (0 for b >= n_images).
 template <int NB>
-__global__ void k_pc_pack(float* __restrict__ X, ImgIn in, int n_images,
-                          long long nvox)
+__global__ void k_pc_pack(float* __restrict__ X, float* __restrict__ XT,
+                          ImgIn in, int n_images, long long nvox, XY xy)
 {
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
        v < nvox; v += (long long)gridDim.x * blockDim.x)
@@ -566,8 +614,13 @@ __global__ void k_pc_pack(float* __restrict__ X, ImgIn in, int n_images,
     float t[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) t[b] = b < n_images ? in.p[b][v] : 0.0f;
+    const long long vy = to_yfast(xy, (int)v);
 #pragma unroll
-    for (int b = 0; b < NB; ++b) X[v * NB + b] = t[b];
+    for (int b = 0; b < NB; ++b)
+    {
+      if (X) X[v * NB + b] = t[b];
+      XT[vy * NB + b] = t[b];
+    }
   }
 }
 
@@ -593,6 +646,7 @@ struct RowMatrix
   long long* d_view_elem = nullptr; // [n_views + 1]
   int* d_bin = nullptr;             // [n_rows]
   float* d_factor = nullptr;        // [n_rows]
+  unsigned char* d_yrow = nullptr;  // [n_rows] 1: indices in the y-fastest layout
   long long* d_off = nullptr;       // [n_rows + 1]
   int* d_idx = nullptr;             // [n_elems]
   void* d_len = nullptr;            // [n_elems]
@@ -601,7 +655,7 @@ struct RowMatrix
 
   long long bytes() const
   {
-    return (long long)(n_views + 1) * (4 + 8) + (long long)n_rows * (4 + 4 + 8) +
+    return (long long)(n_views + 1) * (4 + 8) + (long long)n_rows * (4 + 4 + 1 + 8) +
       8 + n_elems * (4 + (long long)len_bytes(fmt));
   }
 
@@ -612,6 +666,7 @@ struct RowMatrix
     (void)hipFree(d_view_elem);
     (void)hipFree(d_bin);
     (void)hipFree(d_factor);
+    (void)hipFree(d_yrow);
     (void)hipFree(d_off);
     (void)hipFree(d_idx);
     (void)hipFree(d_len);
@@ -632,6 +687,7 @@ struct fir_cache
   size_t w_floats = 0;
   float* d_x = nullptr;        // interleaved images, [nvox * 8]
   size_t x_floats = 0;
+  float* d_xy = nullptr;       // the same in the y-fastest layout
   int fwd_lanes = 16;
   int bp_lanes = 16;
   int xcd_map = 0;
@@ -644,6 +700,7 @@ struct fir_cache
     (void)hipFree(d_tlen);
     (void)hipFree(d_w);
     (void)hipFree(d_x);
+    (void)hipFree(d_xy);
   }
 };
 
@@ -711,7 +768,7 @@ int fill_rows(fir_ctx* c, const Grid& g, const LorGeom& lg, long long n,
 {
   hipLaunchKernelGGL(k_pc_fill<F>, dim3(blocks_for(n)), dim3(kBlock), 0,
                      c->stream, g, lg, n, per_view, count, maxlen, eoff,
-                     rm->d_idx,
+                     XY{g.row, g.slice / g.row}, rm->d_idx,
                      (typename Len<F>::T*)rm->d_len);
   FIR_CHECK_LAUNCH();
   return FIR_OK;
@@ -799,6 +856,7 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
                 rm->n_elems, need / 1e9, free_b / 1e9);
   if ((rc = dev_alloc(&rm->d_bin, rm->n_rows, "row bins"))) return rc;
   if ((rc = dev_alloc(&rm->d_factor, rm->n_rows, "row factors"))) return rc;
+  if ((rc = dev_alloc(&rm->d_yrow, rm->n_rows, "row layouts"))) return rc;
   if ((rc = dev_alloc(&rm->d_off, (size_t)rm->n_rows + 1, "row offsets")))
     return rc;
   if ((rc = dev_alloc(&rm->d_idx, rm->n_elems, "element indices"))) return rc;
@@ -814,7 +872,7 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
   hipLaunchKernelGGL(k_pc_rows, dim3(blocks_for(n)), dim3(kBlock), 0,
                      c->stream, lg, n, per_view, count, maxlen, rowid, eoff,
                      fmt,
-                     rm->d_bin, rm->d_factor, rm->d_off);
+                     rm->d_bin, rm->d_factor, rm->d_off, rm->d_yrow);
   FIR_CHECK_LAUNCH();
   switch (fmt)
   {
@@ -867,8 +925,10 @@ int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
   const dim3 grid_rows(blocks_for(max_rows), vpr);
   FIR_HIP_CHECK(hipMemsetAsync(rc_cnt, 0, sizeof(int) * ((size_t)n_rows_s + 1),
                                c->stream));
+  const XY xy{cache->vol.nx, cache->vol.ny};
   hipLaunchKernelGGL(k_tr_count, grid_rows, dim3(kBlock), 0, c->stream, sr,
-                     d_prefix, rm->d_off, rm->d_idx, v0, v1, rc_cnt);
+                     d_prefix, rm->d_off, rm->d_idx, rm->d_yrow, xy, v0, v1,
+                     rc_cnt);
   FIR_CHECK_LAUNCH();
   size_t tb = 0;
   FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, rc_cnt, pos,
@@ -885,7 +945,7 @@ int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
     return fail(FIR_E_STATE, "path cache: voxels [%d, %d) hold %lld elements "
                              "by histogram, %d by rows", v0, v1, n_ch, counted);
   hipLaunchKernelGGL(k_tr_expand<F>, grid_rows, dim3(kBlock), 0, c->stream, sr,
-                     d_prefix, rm->d_off, rm->d_idx,
+                     d_prefix, rm->d_off, rm->d_idx, rm->d_yrow, xy,
                      (const typename Len<F>::T*)rm->d_len, v0, v1, pos, n_ch,
                      k0, val0);
   FIR_CHECK_LAUNCH();
@@ -954,11 +1014,13 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
     if ((rc = tmp.get(&cnt, nvox + 1, "voxel counts"))) return rc;
     FIR_HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned) * (nvox + 1),
                                  c->stream));
-    if (max_elems > 0)
+    if (max_rows > 0)
     {
-      hipLaunchKernelGGL(k_tr_hist, dim3(blocks_for(max_elems), vpr),
-                         dim3(kBlock), 0, c->stream, rm->d_view_elem, S, s,
-                         rm->d_idx, cnt);
+      hipLaunchKernelGGL(k_tr_hist, dim3(blocks_for(max_rows), vpr),
+                         dim3(kBlock), 0, c->stream,
+                         SubsetRows{rm->d_view_row, S, s, 0}, rm->d_off,
+                         rm->d_idx, rm->d_yrow,
+                         XY{cache->vol.nx, cache->vol.ny}, cnt);
       FIR_CHECK_LAUNCH();
     }
     long long* voff_s = cache->d_voff + (long long)s * (nvox + 1);
@@ -1036,7 +1098,8 @@ int check_cache(const fir_cache* cache)
 
 RowData row_data(const RowMatrix* rm)
 {
-  return RowData{rm->d_off, rm->d_idx, rm->d_len, rm->d_bin, rm->d_factor};
+  return RowData{rm->d_off, rm->d_idx, rm->d_len, rm->d_bin, rm->d_factor,
+                 rm->d_yrow};
 }
 
 int max_rows_of_subset(const RowMatrix* rm, int S, int s)
@@ -1059,6 +1122,23 @@ unsigned grid_blocks(const fir_cache* cache, long long threads)
   return b;
 }
 
+// The y-fastest copy (and, for nb > 1, the interleaved x-fastest copy) of
+// the images of one launch.
+void pack_images(fir_cache* cache, int wd, const ImgIn& in, int nb)
+{
+  const unsigned blocks = ew_blocks(cache->nvox);
+  const XY xy{cache->vol.nx, cache->vol.ny};
+  hipStream_t st = cache->ctx->stream;
+  float* X = wd > 1 ? cache->d_x : nullptr;
+  switch (wd)
+  {
+  case 1: hipLaunchKernelGGL(k_pc_pack<1>, dim3(blocks), dim3(kBlock), 0, st, X, cache->d_xy, in, nb, cache->nvox, xy); break;
+  case 2: hipLaunchKernelGGL(k_pc_pack<2>, dim3(blocks), dim3(kBlock), 0, st, X, cache->d_xy, in, nb, cache->nvox, xy); break;
+  case 4: hipLaunchKernelGGL(k_pc_pack<4>, dim3(blocks), dim3(kBlock), 0, st, X, cache->d_xy, in, nb, cache->nvox, xy); break;
+  default: hipLaunchKernelGGL(k_pc_pack<8>, dim3(blocks), dim3(kBlock), 0, st, X, cache->d_xy, in, nb, cache->nvox, xy); break;
+  }
+}
+
 template <int F>
 void launch_forward(fir_cache* cache, int S, int s, const float* img,
                     float* sino)
@@ -1071,7 +1151,7 @@ void launch_forward(fir_cache* cache, int S, int s, const float* img,
   hipLaunchKernelGGL((k_pc_forward<F>), grid, dim3(kBlock), 0,
                      cache->ctx->stream,
                      SubsetRows{rm->d_view_row, S, s, cache->xcd_map},
-                     row_data(rm), lanes, img, sino);
+                     row_data(rm), lanes, img, cache->d_xy, sino);
 }
 
 template <int F, int NB>
@@ -1112,7 +1192,7 @@ void launch_rows_nb(fir_cache* cache, int s, const float* X,
   hipLaunchKernelGGL((k_pc_osem_rows<F, NB>), grid, dim3(kBlock), 0,
                      cache->ctx->stream,
                      SubsetRows{rm->d_view_row, S, s, cache->xcd_map},
-                     row_data(rm), lanes, X, io, w);
+                     row_data(rm), lanes, X, cache->d_xy, io, w);
 }
 
 template <int F>
@@ -1187,7 +1267,7 @@ int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
   FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
   const long long need = rm->n_elems * (4 + (long long)len_bytes(len_format)) +
     (long long)n_subsets * (cache->nvox + 1) * 8 +
-    ((long long)rm->n_rows + cache->nvox) * kMaxCacheBatch * 4;
+    ((long long)rm->n_rows + 2 * cache->nvox) * kMaxCacheBatch * 4;
   if ((long long)free_b < need + (1LL << 30))
   {
     delete cache;
@@ -1202,7 +1282,8 @@ int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
   cache->x_floats = (size_t)cache->nvox * kMaxCacheBatch;
   if ((rc = build_transpose(c, cache)) ||
       (rc = dev_alloc(&cache->d_w, cache->w_floats, "row weights")) ||
-      (rc = dev_alloc(&cache->d_x, cache->x_floats, "interleaved images")))
+      (rc = dev_alloc(&cache->d_x, cache->x_floats, "interleaved images")) ||
+      (rc = dev_alloc(&cache->d_xy, cache->x_floats, "y-fastest images")))
   {
     (void)hipStreamSynchronize(c->stream);
     delete cache;
@@ -1236,7 +1317,7 @@ int fir_cache_info(const fir_cache* cache, int64_t* n_rows, int64_t* n_elements,
     *device_bytes = rm->bytes() +
       rm->n_elems * (4 + (long long)len_bytes(rm->fmt)) +
       (long long)cache->n_subsets * (cache->nvox + 1) * 8 +
-      (long long)(cache->w_floats + cache->x_floats) * 4;
+      (long long)(cache->w_floats + 2 * cache->x_floats) * 4;
   return FIR_OK;
 }
 
@@ -1265,6 +1346,11 @@ int fir_cache_forward(fir_cache* cache, const float* img, float* sino)
   FIR_HIP_CHECK(hipSetDevice(c->device));
   // LORs that miss the volume project to 0 (projections.cc:63-66).
   if ((rc = fir_fill(c, sino, c->n_bins, 0.0f))) return rc;
+  ImgIn in;
+  std::memset(&in, 0, sizeof(in));
+  in.p[0] = img;
+  pack_images(cache, 1, in, 1);
+  FIR_CHECK_LAUNCH();
   const int fmt = cache->rm->fmt;
   // One launch over all views (S = 1, subset 0 covers every view).
   if (fmt == FIR_LEN_UNORM16)
@@ -1351,19 +1437,9 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
       out.p[b] = bp[i0 + b];
     }
     io.n_images = nb;
-    const float* X = x[i0];
-    if (wd > 1)
-    {
-      X = cache->d_x;
-      const unsigned blocks = ew_blocks(cache->nvox);
-      switch (wd)
-      {
-      case 2: hipLaunchKernelGGL(k_pc_pack<2>, dim3(blocks), dim3(kBlock), 0, c->stream, cache->d_x, in, nb, cache->nvox); break;
-      case 4: hipLaunchKernelGGL(k_pc_pack<4>, dim3(blocks), dim3(kBlock), 0, c->stream, cache->d_x, in, nb, cache->nvox); break;
-      default: hipLaunchKernelGGL(k_pc_pack<8>, dim3(blocks), dim3(kBlock), 0, c->stream, cache->d_x, in, nb, cache->nvox); break;
-      }
-      FIR_CHECK_LAUNCH();
-    }
+    const float* X = wd > 1 ? cache->d_x : x[i0];
+    pack_images(cache, wd, in, nb);
+    FIR_CHECK_LAUNCH();
     if (rm->fmt == FIR_LEN_UNORM16)
     {
       launch_osem_rows<FIR_LEN_UNORM16>(cache, subset, wd, X, io, cache->d_w);
