@@ -161,14 +161,23 @@ def main():
 
     stream = torch.cuda.current_stream()
 
+    # --- path cache, row order and lanes picked by the autotuner ---------
+    from fir_amd import autotune
+    cache_t0 = time.perf_counter()
+    cache, tres = autotune.tune_cache(ctx, v, n_sub, [x], [y], reps=2)
+    torch.cuda.synchronize()
+    tune_s = time.perf_counter() - cache_t0
+    cache.close()
     cache_t0 = time.perf_counter()
     cache = ctx.path_cache(v, n_sub, firlib.FIR_LEN_UNORM16,
-                           row_order=firlib.CACHE_ROW_ORDER)
+                           row_order=tres.best["row_order"])
     torch.cuda.synchronize()
     cache_build_s = time.perf_counter() - cache_t0
+    cache.tune(tres.best["fwd_lanes"], tres.best["bp_lanes"])
     cinfo = cache.info()
     log(f"path cache: {cinfo['elements']} elements, "
-        f"{cinfo['device_bytes'] / 1e9:.2f} GB, built in {cache_build_s:.2f} s")
+        f"{cinfo['device_bytes'] / 1e9:.2f} GB, built in {cache_build_s:.2f} s;"
+        f" autotuned {tres.best} in {tune_s:.1f} s")
     sens_c = torch.zeros(n_sub * nvox, device="cuda")
     cache.sensitivity(sens_c)
 
@@ -234,6 +243,8 @@ def main():
         f"projector launch avg {proj_avg_ms:.3f} ms")
 
     # --- the plain path (FP64 walk per LOR), same schedule ----------------
+    pres = autotune.tune_plain(ctx, v, n_sub, x, y, reps=2)
+    log(f"plain projector autotuned: {pres.best}")
     plain_ms, plain_proj_ms, _ = timed_iterations(
         plain_project, sens, max(1, min(args.steps, 2)), 1)
     log(f"plain OSEM iteration {plain_ms:.3f} ms, projector launch avg "
@@ -344,6 +355,9 @@ def main():
         "osem_iter_ms": round(ms_step, 4),
         "osem_subiter_ms": round(ms_step / n_sub, 4),
         "cache": {"build_s": round(cache_build_s, 3),
+                  "autotune": {"best": tres.best,
+                               "trials_ms": tres.table(),
+                               "seconds": round(tune_s, 2)},
                   "elements": int(n_e_total),
                   "rows": cinfo["rows"],
                   "device_gb": round(cinfo["device_bytes"] / 1e9, 3),
@@ -352,6 +366,7 @@ def main():
                   "bp_all_subsets_ms": round(cbp_ms, 4),
                   "bp_mlors_s": round(w.n_lors / (cbp_ms * 1e-3) / 1e6, 3)},
         "plain": {"osem_iter_ms": round(plain_ms, 4),
+                  "autotune": {"best": pres.best, "trials_ms": pres.table()},
                   "projector_launch_ms": round(plain_proj_ms, 4),
                   "mlors_s": round(w.n_lors / (plain_ms * 1e-3) / 1e6, 3),
                   "fwd_ms": round(fwd_ms, 4),

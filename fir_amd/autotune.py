@@ -1,0 +1,120 @@
+"""Autotuner of the projector kernels: picks, by timing the real workload on
+the device, the LOR sort key and the lanes per LOR / per voxel of a path
+cache, and the row order and per-XCD scatter copies of the plain projector.
+
+Every candidate changes only the FP32 summation order of the results (the
+tests check that), so the choice is purely a matter of speed. Timings are
+HIP events around the OSEM projector pass of one subset on the caller's
+stream, best of ``reps`` repetitions after one warm-up launch.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import torch
+
+from . import lib as firlib
+
+# Row orders a cache is tried with: per view (tangential bin, pair by centre
+# slice) and the sinogram's own (pair, tangential bin).
+CACHE_ROW_ORDERS = (firlib.CACHE_ROW_ORDER, firlib.FIR_ORDER_VIEW_MAJOR)
+CACHE_LANES = ((16, 16), (8, 16), (16, 32), (32, 16))
+PLAIN_ORDERS = (firlib.FIR_ORDER_SEGMENT,
+                firlib.FIR_ORDER_SEGMENT | firlib.FIR_ORDER_TANG_MAJOR,
+                firlib.FIR_ORDER_VIEW_MAJOR | firlib.FIR_ORDER_ZCENTER)
+PLAIN_COPIES = (1, 8)
+
+
+@dataclass
+class Trial:
+    config: dict
+    ms: float
+
+
+@dataclass
+class Result:
+    best: dict
+    best_ms: float
+    trials: List[Trial] = field(default_factory=list)
+
+    def table(self) -> List[dict]:
+        return [dict(t.config, ms=round(t.ms, 4)) for t in self.trials]
+
+
+def _time(fn, reps: int) -> float:
+    stream = torch.cuda.current_stream()
+    fn()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        best = min(best, a.elapsed_time(b))
+    return best
+
+
+def tune_cache(ctx: firlib.Context, vol: firlib.VolDesc, n_subsets: int,
+               xs: Sequence[torch.Tensor], ys: Sequence[torch.Tensor],
+               len_format: int = firlib.FIR_LEN_UNORM16,
+               row_orders: Sequence[int] = CACHE_ROW_ORDERS,
+               lanes: Sequence[Tuple[int, int]] = CACHE_LANES,
+               subset: int = 0, reps: int = 3
+               ) -> Tuple[firlib.PathCache, Result]:
+    """Build a path cache per row order, time the OSEM projector pass of
+    ``len(xs)`` images for every lanes setting, and return the fastest
+    cache, tuned, with the table of trials. The other caches are freed
+    before the next one is built."""
+    bps = [torch.zeros(vol.n_voxels, device=xs[0].device) for _ in xs]
+    trials: List[Trial] = []
+    best = None
+    best_cache: Optional[firlib.PathCache] = None
+    for order in row_orders:
+        cache = ctx.path_cache(vol, n_subsets, len_format, row_order=order)
+        mine = None
+        for fl, bl in lanes:
+            cache.tune(fl, bl)
+            ms = _time(lambda: cache.osem_project(list(xs), list(ys), None,
+                                                  bps, subset), reps)
+            cfg = {"row_order": order, "fwd_lanes": fl, "bp_lanes": bl}
+            trials.append(Trial(cfg, ms))
+            if mine is None or ms < mine[1]:
+                mine = (cfg, ms)
+        if best is None or mine[1] < best[1]:
+            if best_cache is not None:
+                best_cache.close()
+            best, best_cache = mine, cache
+        else:
+            cache.close()
+        torch.cuda.synchronize()
+    cfg = best[0]
+    best_cache.tune(cfg["fwd_lanes"], cfg["bp_lanes"])
+    return best_cache, Result(cfg, best[1], trials)
+
+
+def tune_plain(ctx: firlib.Context, vol: firlib.VolDesc, n_subsets: int,
+               x: torch.Tensor, y: torch.Tensor,
+               orders: Sequence[int] = PLAIN_ORDERS,
+               copies: Sequence[int] = PLAIN_COPIES,
+               subset: int = 0, reps: int = 3) -> Result:
+    """Time the plain (on-the-fly walk) OSEM projector pass for every row
+    order and scatter-copy count, leave the fastest set on the context and
+    return the table."""
+    bp = torch.zeros(vol.n_voxels, device=x.device)
+    trials: List[Trial] = []
+    for order in orders:
+        ctx.set_lor_order(order)
+        for cp in copies:
+            ctx.set_scatter_copies(cp)
+            ms = _time(lambda: ctx.osem_project(vol, x, y, None, bp, subset,
+                                                n_subsets), reps)
+            trials.append(Trial({"lor_order": order, "scatter_copies": cp},
+                                ms))
+    best = min(trials, key=lambda t: t.ms)
+    ctx.set_lor_order(best.config["lor_order"])
+    ctx.set_scatter_copies(best.config["scatter_copies"])
+    return Result(best.config, best.ms, trials)

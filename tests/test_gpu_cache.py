@@ -277,3 +277,25 @@ def test_c2_cache_properties():
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
     cache.close()
     ctx.close()
+
+
+def test_autotuner_picks_fastest_and_keeps_results(c1, c1_golden):
+    """The autotuner returns the fastest of its trials, tuned, and the
+    chosen cache still reproduces the reference's OSEM run A."""
+    from fir_amd import autotune
+    w, ctx, v, _ = c1
+    y = dev(c1_golden["y"])
+    x = torch.ones(v.n_voxels, device="cuda")
+    cache, res = autotune.tune_cache(ctx, v, 4, [x], [y], reps=1,
+                                     lanes=((16, 16), (8, 8)))
+    assert len(res.trials) == 2 * len(autotune.CACHE_ROW_ORDERS)
+    assert res.best_ms == min(t.ms for t in res.trials)
+    g = recon.OSEMGroup(ctx, v, 4, [y], cache=cache)
+    g.iteration_eager()
+    g.iteration_eager()
+    assert rel_l2(host(g.x[0]), c1_golden["xa"]) < 1e-4
+    cache.close()
+    pres = autotune.tune_plain(ctx, v, 4, x, y, reps=1)
+    assert pres.best_ms == min(t.ms for t in pres.trials)
+    ctx.set_lor_order(0)
+    ctx.set_scatter_copies(1)

@@ -17,9 +17,10 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(ROOT, "fir_amd", "bin")
 
 
-def exe(name, *args, cwd, check=True):
+def exe(name, *args, cwd, check=True, env=None):
     r = subprocess.run([os.path.join(BIN, name), *map(str, args)], cwd=cwd,
-                       capture_output=True, text=True, timeout=120)
+                       capture_output=True, text=True, timeout=120,
+                       env=None if env is None else dict(os.environ, **env))
     if check and r.returncode != 0:
         raise AssertionError(f"{name} failed:\n{r.stdout[-2000:]}\n{r.stderr}")
     return r
@@ -131,6 +132,33 @@ def test_osem_bias(work, c1_golden):
     exe("FIR_OSEM", "c.params", cwd=work)
     xc = np.fromfile(work / "xc.i33", np.float32)
     assert rel_l2(xc, c1_golden["xc"]) < 1e-4
+
+
+@pytest.mark.parametrize("fmt", ["unorm16", "fp32"])
+def test_osem_path_cache(work, c1_golden, fmt):
+    """FIR_PATH_CACHE selects the path-cache projector: runs A and B of the
+    reference within 1e-4 relative L2."""
+    osem_params(work / f"pa_{fmt}.params", f"pa_{fmt}", 2, 4)
+    r = exe("FIR_OSEM", f"pa_{fmt}.params", cwd=work,
+            env={"FIR_PATH_CACHE": fmt})
+    assert "Building the path cache" in r.stdout
+    xa = np.fromfile(work / f"pa_{fmt}.i33", np.float32)
+    assert rel_l2(xa, c1_golden["xa"]) < 1e-4
+    osem_params(work / f"pb_{fmt}.params", f"pb_{fmt}", 2, 4,
+                attenuation_volume_in_HU="hu.h33",
+                cut_radius_in_mm="55", convolution_interval="1",
+                convolution_FHWM_XYZ_in_mm="{4,4,4}")
+    exe("FIR_OSEM", f"pb_{fmt}.params", cwd=work, env={"FIR_PATH_CACHE": fmt})
+    xb = np.fromfile(work / f"pb_{fmt}.i33", np.float32)
+    assert rel_l2(xb, c1_golden["xb"]) < 1e-4
+
+
+def test_osem_path_cache_rejects_unknown_format(work):
+    osem_params(work / "pz.params", "pz", 1, 4)
+    r = exe("FIR_OSEM", "pz.params", cwd=work, check=False,
+            env={"FIR_PATH_CACHE": "int8"})
+    assert r.returncode != 0
+    assert "FIR_PATH_CACHE" in r.stderr
 
 
 def test_osem_rejects_non_divisor_subsets(work):
