@@ -9,7 +9,7 @@ water cylinder, attenuation-corrected Poisson data at 5e8 counts, OSEM with
 One step = one full 16-subset OSEM iteration (every LOR forward- and
 back-projected once per iteration, plus the per-sub-iteration update and
 Gaussian) with the projector streaming the path cache (the Siddon system
-matrix kept in HBM with unorm16 lengths; OSEM images within 1e-4 relative L2
+matrix kept in HBM with snorm16 lengths; OSEM images within 1e-4 relative L2
 of the reference). Inputs and the cache are resident in HBM before timing
 starts; the cache build is timed separately.
 
@@ -104,12 +104,104 @@ def cpu_baseline(w, n_subsets):
                       "oracle, uniform image and sinogram"}
 
 
+def run_uneven(args):
+    """Batched uneven reconstructions: 64 OSEM solves of the 24-ring data
+    (noise realisations / dynamic frames with total counts log-uniform in
+    [1e5, 1e8], seeds 100-163) on mixed 128x128x47 and 160x160x47 grids with
+    8, 16 or 24 subsets (24 does not divide the 224 views: interleaved
+    uneven subsets), 4 mm Gaussian every sub-iteration. Groups of equal
+    (grid, subset count) share each read of their path cache (one shared
+    context; caches of one grid share the path matrix); each group's
+    iteration is one HIP graph. One step = one iteration of all 64 solves.
+    """
+    import torch
+    from fir_amd import interfile as io
+    from fir_amd import lib as firlib
+    from fir_amd import recon
+    from fir_amd import synth
+
+    w = synth.c2()
+    ctx = firlib.context_for(w)
+    v160 = firlib.VolDesc.from_spec(w.vol)
+    v128 = firlib.VolDesc.from_spec(io.VolSpec((128, 128, 47),
+                                               (2.0, 2.0, 3.27),
+                                               (-127.0, -127.0, 0.0)))
+    act, _ = synth.phantom(w)
+    y0 = torch.zeros(ctx.n_bins, device="cuda")
+    ctx.forward(v160, torch.from_numpy(act.ravel()).cuda(), y0)
+    y0 = y0 / y0.sum()
+    solves = []
+    for i in range(64):
+        seed = 100 + i
+        total = 10 ** np.random.default_rng(seed).uniform(5.0, 8.0)
+        g = torch.Generator(device="cuda").manual_seed(seed)
+        y = torch.poisson(y0 * float(total), generator=g)
+        solves.append(dict(vol=v128 if i % 2 else v160,
+                           n_subsets=(8, 16, 24)[i % 3], y=y))
+    t0 = time.perf_counter()
+    b = recon.BatchedOSEM(w, solves, fwhm=w.fwhm, conv_interval=1,
+                          cache_format=firlib.FIR_LEN_SNORM16)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    log(f"{len(b.groups)} groups, caches built in {setup_s:.1f} s")
+    b.capture()
+    for _ in range(args.warmup):
+        b.iteration()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        b.iteration()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    # the plain path, one solve of the same data alone (160 grid, 16 subsets)
+    single = recon.OSEMGroup(ctx, v160, 16, [solves[4]["y"]],
+                             fwhm=w.fwhm, conv_interval=1)
+    single.iteration_eager()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    single.iteration_eager()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    plain_ms = e0.elapsed_time(e1)
+    mem = sum(c.info()["device_bytes"] for c in b.caches)
+    out = {
+        "metric": "M LORs/s fwd+backproj & OSEM iter ms: plain vs batched "
+                  "GPU at rel err<=1e-4",
+        "value": round(64 * w.n_lors / (ms * 1e-3) / 1e6, 3),
+        "unit": "M LORs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded phantom, Poisson counts log-uniform "
+                "1e5..1e8)",
+        "config": {"workload": "uneven_64_solves_c2_data",
+                   "grids": "128x128x47 and 160x160x47 @ 2x2x3.27 mm",
+                   "subsets": [8, 16, 24], "n_solves": 64,
+                   "groups": len(b.groups),
+                   "path": "path cache (snorm16), batched, HIP graphs"},
+        "iteration_ms_all_solves": round(ms, 3),
+        "iteration_ms_per_solve": round(ms / 64, 4),
+        "plain_single_solve_iteration_ms": round(plain_ms, 3),
+        "speedup_per_solve_vs_plain": round(plain_ms * 64 / ms, 3),
+        "cache_setup_s": round(setup_s, 2),
+        "cache_device_gb": round(mem / 1e9, 2),
+    }
+    print(json.dumps(out), flush=True)
+    b.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c2",
+                    help="c2 (default) or uneven (64 batched solves)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=8,
                     help="reconstructions in the batched measurement (0: skip)")
@@ -117,6 +209,12 @@ def main():
                     help="run only one projector launch of each kind and a "
                          "1 GiB device copy (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
+    if args.workload == "uneven":
+        if args.gpus != 1:
+            print("bench.py measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        run_uneven(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not "
               "supported (multi-GPU sharding is a design note only)",
@@ -169,7 +267,7 @@ def main():
     tune_s = time.perf_counter() - cache_t0
     cache.close()
     cache_t0 = time.perf_counter()
-    cache = ctx.path_cache(v, n_sub, firlib.FIR_LEN_UNORM16,
+    cache = ctx.path_cache(v, n_sub, firlib.FIR_LEN_SNORM16,
                            row_order=tres.best["row_order"])
     torch.cuda.synchronize()
     cache_build_s = time.perf_counter() - cache_t0
@@ -350,7 +448,7 @@ def main():
             "n_lors": w.n_lors,
             "image": "160x160x47 @ 2x2x3.27 mm",
             "osem": f"{n_sub} subsets, 4 mm FWHM Gaussian every subiter",
-            "path": "path cache (Siddon matrix in HBM, unorm16 lengths)",
+            "path": "path cache (Siddon matrix in HBM, snorm16 lengths)",
         },
         "osem_iter_ms": round(ms_step, 4),
         "osem_subiter_ms": round(ms_step / n_sub, 4),

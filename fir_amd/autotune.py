@@ -60,7 +60,7 @@ def _time(fn, reps: int) -> float:
 
 def tune_cache(ctx: firlib.Context, vol: firlib.VolDesc, n_subsets: int,
                xs: Sequence[torch.Tensor], ys: Sequence[torch.Tensor],
-               len_format: int = firlib.FIR_LEN_UNORM16,
+               len_format: int = firlib.FIR_LEN_SNORM16,
                row_orders: Sequence[int] = CACHE_ROW_ORDERS,
                lanes: Sequence[Tuple[int, int]] = CACHE_LANES,
                subset: int = 0, reps: int = 3

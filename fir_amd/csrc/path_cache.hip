@@ -19,10 +19,12 @@
 // clinical scanner (3.9e9 elements, 23 GB per copy at 6 B per element).
 //
 // Element lengths are stored as
-//   FIR_LEN_UNORM16: round(65535 * len / m), m = the LOR's longest element;
+//   FIR_LEN_SNORM16: round(32767 * len / m), m = the LOR's largest |length|
+//                    (the reference walk with int-truncated steps makes some
+//                    lengths negative);
 //   FIR_LEN_FP16:    binary16 of len / m;
 //   FIR_LEN_FP32:    binary32 of len;
-// with a per-row FP32 factor (m / 65535, m, 1) that restores the length.
+// with a per-row FP32 factor (m / 32767, m, 1) that restores the length.
 #include <hip/hip_fp16.h>
 
 #include <hipcub/hipcub.hpp>
@@ -52,19 +54,25 @@ template <int F>
 struct Len;
 
 template <>
-struct Len<FIR_LEN_UNORM16>
+struct Len<FIR_LEN_SNORM16>
 {
-  using T = unsigned short;
+  using T = short;
   static __device__ __forceinline__ float dec(T v) { return (float)v; }
   static __device__ __forceinline__ T enc(double len, float m)
   {
     if (!(m > 0.0f)) return 0;
-    double q = rint(len / (double)m * 65535.0);
-    q = q < 0.0 ? 0.0 : (q > 65535.0 ? 65535.0 : q);
+    double q = rint(len / (double)m * 32767.0);
+    q = q < -32767.0 ? -32767.0 : (q > 32767.0 ? 32767.0 : q);
     return (T)q;
   }
-  static __device__ __forceinline__ unsigned bits(T v) { return v; }
-  static __device__ __forceinline__ T from_bits(unsigned b) { return (T)b; }
+  static __device__ __forceinline__ unsigned bits(T v)
+  {
+    return (unsigned)(unsigned short)v;
+  }
+  static __device__ __forceinline__ T from_bits(unsigned b)
+  {
+    return (T)(unsigned short)b;
+  }
 };
 
 template <>
@@ -146,8 +154,8 @@ __device__ __forceinline__ void cache_lor(const LorGeom& lg, long long t,
   fir::lor_from_index(lg, t - (long long)view * per_view, lg.n_views, view, L);
 }
 
-// Pass 1 over all LORs in (view, segment, axial, tangential) order: element
-// count and longest element of each path.
+// Pass 1 over all LORs in the cache's row order: element count and largest
+// |length| of each path.
 __global__ __launch_bounds__(kBlock) void k_pc_stats(Grid g, LorGeom lg,
                                                      long long n,
                                                      long long per_view,
@@ -164,7 +172,8 @@ __global__ __launch_bounds__(kBlock) void k_pc_stats(Grid g, LorGeom lg,
   if (fir::walk_setup(g, L.e1, L.e2, w))
     fir::walk_run(g, w, [&](int, double l) {
       ++c;
-      m = l > m ? l : m;
+      const double a = l < 0 ? -l : l;
+      m = a > m ? a : m;
     });
   count[t] = c;
   maxlen[t] = (float)m;
@@ -201,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_rows(
   bin[r] = (int)L.bin;
   yrow[r] = y_dominant(L) ? 1 : 0;
   off[r] = eoff[t];
-  factor[r] = fmt == FIR_LEN_UNORM16 ? m / 65535.0f
+  factor[r] = fmt == FIR_LEN_SNORM16 ? m / 32767.0f
             : fmt == FIR_LEN_FP16    ? m
                                      : 1.0f;
 }
@@ -876,8 +885,8 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
   FIR_CHECK_LAUNCH();
   switch (fmt)
   {
-  case FIR_LEN_UNORM16:
-    rc = fill_rows<FIR_LEN_UNORM16>(c, g, lg, n, per_view, count, maxlen, eoff,
+  case FIR_LEN_SNORM16:
+    rc = fill_rows<FIR_LEN_SNORM16>(c, g, lg, n, per_view, count, maxlen, eoff,
                                   rm.get());
     break;
   case FIR_LEN_FP16:
@@ -1061,8 +1070,8 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
       {
         switch (rm->fmt)
         {
-        case FIR_LEN_UNORM16:
-          rc = transpose_chunk<FIR_LEN_UNORM16>(c, rm, cache, sr, d_prefix, vpr,
+        case FIR_LEN_SNORM16:
+          rc = transpose_chunk<FIR_LEN_SNORM16>(c, rm, cache, sr, d_prefix, vpr,
                                                 max_rows, n_rows_s, (int)v0,
                                                 (int)v1, voff[v0], n_ch);
           break;
@@ -1224,7 +1233,7 @@ int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
   *out = nullptr;
   int rc = need_lors(c);
   if (rc) return rc;
-  if (len_format != FIR_LEN_UNORM16 && len_format != FIR_LEN_FP16 &&
+  if (len_format != FIR_LEN_SNORM16 && len_format != FIR_LEN_FP16 &&
       len_format != FIR_LEN_FP32)
     return fail(FIR_E_ARG, "unknown length format %d", len_format);
   Grid g;
@@ -1353,8 +1362,8 @@ int fir_cache_forward(fir_cache* cache, const float* img, float* sino)
   FIR_CHECK_LAUNCH();
   const int fmt = cache->rm->fmt;
   // One launch over all views (S = 1, subset 0 covers every view).
-  if (fmt == FIR_LEN_UNORM16)
-    launch_forward<FIR_LEN_UNORM16>(cache, 1, 0, img, sino);
+  if (fmt == FIR_LEN_SNORM16)
+    launch_forward<FIR_LEN_SNORM16>(cache, 1, 0, img, sino);
   else if (fmt == FIR_LEN_FP16)
     launch_forward<FIR_LEN_FP16>(cache, 1, 0, img, sino);
   else
@@ -1381,8 +1390,8 @@ static int cache_backward_impl(fir_cache* cache, const float* sino,
     VoxOut out;
     std::memset(&out, 0, sizeof(out));
     out.p[0] = frames + (long long)s * cache->nvox;
-    if (rm->fmt == FIR_LEN_UNORM16)
-      launch_backproject<FIR_LEN_UNORM16>(cache, s, 1, cache->d_w, out);
+    if (rm->fmt == FIR_LEN_SNORM16)
+      launch_backproject<FIR_LEN_SNORM16>(cache, s, 1, cache->d_w, out);
     else if (rm->fmt == FIR_LEN_FP16)
       launch_backproject<FIR_LEN_FP16>(cache, s, 1, cache->d_w, out);
     else
@@ -1440,11 +1449,11 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
     const float* X = wd > 1 ? cache->d_x : x[i0];
     pack_images(cache, wd, in, nb);
     FIR_CHECK_LAUNCH();
-    if (rm->fmt == FIR_LEN_UNORM16)
+    if (rm->fmt == FIR_LEN_SNORM16)
     {
-      launch_osem_rows<FIR_LEN_UNORM16>(cache, subset, wd, X, io, cache->d_w);
+      launch_osem_rows<FIR_LEN_SNORM16>(cache, subset, wd, X, io, cache->d_w);
       FIR_CHECK_LAUNCH();
-      launch_backproject<FIR_LEN_UNORM16>(cache, subset, wd, cache->d_w, out);
+      launch_backproject<FIR_LEN_SNORM16>(cache, subset, wd, cache->d_w, out);
     }
     else if (rm->fmt == FIR_LEN_FP16)
     {

@@ -10,17 +10,17 @@ namespace reconAlgos
 {
 namespace
 {
-// FIR_PATH_CACHE = unorm16 | fp16 | fp32 selects the path-cache projector
+// FIR_PATH_CACHE = snorm16 | fp16 | fp32 selects the path-cache projector
 // (Siddon paths kept in device memory) with that length format; unset, empty
 // or "off" keeps the on-the-fly walk.
 int pathCacheFormat()
 {
   const char* env = std::getenv("FIR_PATH_CACHE");
   if (!env || !*env || std::strcmp(env, "off") == 0) return -1;
-  if (std::strcmp(env, "unorm16") == 0) return FIR_LEN_UNORM16;
+  if (std::strcmp(env, "snorm16") == 0) return FIR_LEN_SNORM16;
   if (std::strcmp(env, "fp16") == 0) return FIR_LEN_FP16;
   if (std::strcmp(env, "fp32") == 0) return FIR_LEN_FP32;
-  error("FIR_PATH_CACHE must be unorm16, fp16, fp32 or off (got \"", env,
+  error("FIR_PATH_CACHE must be snorm16, fp16, fp32 or off (got \"", env,
         "\")");
   return -1;
 }

@@ -23,7 +23,7 @@ FIR_ORDER_ZCENTER = 1
 FIR_ORDER_PAIR_MAJOR = 0x100
 FIR_ORDER_TANG_MAJOR = 0x200
 FIR_ORDER_VIEW_MAJOR = 0x400
-FIR_LEN_UNORM16 = 0
+FIR_LEN_SNORM16 = 0
 FIR_LEN_FP16 = 1
 FIR_LEN_FP32 = 2
 FIR_CACHE_XCD_MAP = 1
@@ -316,7 +316,7 @@ class Context:
                                       _ptr(ln)))
 
     def path_cache(self, vol: VolDesc, n_subsets: int = 1,
-                   len_format: int = FIR_LEN_UNORM16,
+                   len_format: int = FIR_LEN_SNORM16,
                    row_order: Optional[int] = None) -> "PathCache":
         """Build a path cache. row_order (FIR_ORDER_* flags) sets the order
         of the cached rows within a view (the context's LOR order at
@@ -341,7 +341,7 @@ class PathCache:
     an extension of the reference's LORCache (LORCache.h:5-47)."""
 
     def __init__(self, ctx: Context, vol: VolDesc, n_subsets: int = 1,
-                 len_format: int = FIR_LEN_UNORM16):
+                 len_format: int = FIR_LEN_SNORM16):
         self.ctx, self.lib, self.vol = ctx, ctx.lib, vol
         self.n_subsets, self.len_format = n_subsets, len_format
         h = C.c_void_p()

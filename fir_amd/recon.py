@@ -122,11 +122,19 @@ class BatchedOSEM:
 
     ``solves`` is a list of dicts with keys ``vol`` (VolDesc), ``n_subsets``
     and ``y`` (device sinogram); every solve shares one scanner/sinogram
-    geometry, given by ``workload``. Contexts are made per group.
+    geometry, given by ``workload``.
+
+    Without a cache format every group gets its own context and the plain
+    projector. With ``cache_format`` (a FIR_LEN_* value) all groups share one
+    context and each group gets a path cache; caches of one image grid share
+    the LOR-major path matrix, so only the per-subset-count transposes are
+    extra.
     """
 
     def __init__(self, workload, solves: List[dict], fwhm=(0.0, 0.0, 0.0),
-                 conv_interval: int = 0, cut: float = 0.0, device: int = 0):
+                 conv_interval: int = 0, cut: float = 0.0, device: int = 0,
+                 cache_format: Optional[int] = None,
+                 cache_row_order: int = firlib.CACHE_ROW_ORDER):
         keys = []
         for s in solves:
             v = s["vol"]
@@ -136,16 +144,30 @@ class BatchedOSEM:
                 keys.append(k)
         self.groups = []
         self.members = []
+        self.caches = []
+        shared = None
+        if cache_format is not None:
+            shared = firlib.context_for(workload, device)
+            shared.set_uneven_subsets(True)
+        self.ctx = shared
         for k in keys:
             idx = [i for i, s in enumerate(solves)
                    if (s["vol"].nx, s["vol"].ny, s["vol"].nz, s["vol"].dx,
                        s["vol"].dy, s["vol"].dz, s["vol"].ox, s["vol"].oy,
                        s["vol"].oz, s["n_subsets"]) == k]
-            ctx = firlib.context_for(workload, device)
-            ctx.set_uneven_subsets(True)
             vol = solves[idx[0]]["vol"]
+            cache = None
+            if shared is not None:
+                ctx = shared
+                cache = ctx.path_cache(vol, k[-1], cache_format,
+                                       row_order=cache_row_order)
+                self.caches.append(cache)
+            else:
+                ctx = firlib.context_for(workload, device)
+                ctx.set_uneven_subsets(True)
             g = OSEMGroup(ctx, vol, k[-1], [solves[i]["y"] for i in idx],
-                          fwhm=fwhm, conv_interval=conv_interval, cut=cut)
+                          fwhm=fwhm, conv_interval=conv_interval, cut=cut,
+                          cache=cache)
             self.groups.append(g)
             self.members.append(idx)
 
@@ -163,3 +185,10 @@ class BatchedOSEM:
             for j, i in enumerate(m):
                 out[i] = g.x[j]
         return out
+
+    def close(self):
+        for g in self.groups:
+            g.graph = None
+        for c in self.caches:
+            c.close()
+        self.caches = []

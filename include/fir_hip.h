@@ -279,12 +279,14 @@ int fir_path_count(fir_ctx* ctx, const fir_vol_desc* vol, int subset,
  * its per-LOR weight buffer). */
 typedef struct fir_cache fir_cache;
 
-/* Stored element length: FIR_LEN_UNORM16 (default) round(65535 * len / m)
- * with m the LOR's longest element, FIR_LEN_FP16 binary16 of len / m,
- * FIR_LEN_FP32 binary32 of len; each restored with a per-LOR FP32 factor. */
+/* Stored element length: FIR_LEN_SNORM16 (default) round(32767 * len / m)
+ * with m the LOR's largest |length| (the reference walk with int-truncated
+ * steps, FIR_WALK_REFERENCE on non-integer voxel sizes, yields some negative
+ * lengths), FIR_LEN_FP16 binary16 of len / m, FIR_LEN_FP32 binary32 of len;
+ * each restored with a per-LOR FP32 factor. */
 enum
 {
-  FIR_LEN_UNORM16 = 0,
+  FIR_LEN_SNORM16 = 0,
   FIR_LEN_FP16 = 1,
   FIR_LEN_FP32 = 2
 };

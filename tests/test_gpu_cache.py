@@ -6,10 +6,10 @@ as the on-the-fly walk, every LOR that crosses the volume is a row); its
 lengths are stored in the chosen format, so its projections are compared
 with the reference's outputs at that format's precision:
   FP32 lengths:     forward and backprojection within 1e-5 relative;
-  unorm16 lengths:  within 2e-5 relative L2 (per-LOR scale, 2^-17 steps);
+  snorm16 lengths:  within 2e-5 relative L2 (per-LOR scale, 2^-16 steps);
   FP16 lengths:     within 2e-4 relative L2 (11-bit significand).
 OSEM images after the reference's schedules are within 1e-4 relative L2 of
-the reference for the FP32 and unorm16 formats. The cached forward and
+the reference for the FP32 and snorm16 formats. The cached forward and
 backward projectors are exact adjoints of each other (same stored matrix):
 <A x, y> = <x, A^T y> to FP32 summation accuracy.
 """
@@ -18,6 +18,7 @@ import pytest
 
 import pyoracle as O
 from conftest import rel_l2, rel_max
+from fir_amd import interfile as io
 from fir_amd import lib as firlib
 from fir_amd import recon
 from fir_amd import synth
@@ -25,8 +26,8 @@ from fir_amd import synth
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
-FORMATS = [firlib.FIR_LEN_FP32, firlib.FIR_LEN_UNORM16, firlib.FIR_LEN_FP16]
-FWD_TOL = {firlib.FIR_LEN_FP32: 1e-5, firlib.FIR_LEN_UNORM16: 2e-5,
+FORMATS = [firlib.FIR_LEN_FP32, firlib.FIR_LEN_SNORM16, firlib.FIR_LEN_FP16]
+FWD_TOL = {firlib.FIR_LEN_FP32: 1e-5, firlib.FIR_LEN_SNORM16: 2e-5,
            firlib.FIR_LEN_FP16: 2e-4}
 
 
@@ -100,7 +101,7 @@ def test_cached_backward_vs_reference(c1, c1_golden, fmt):
         assert rel_max(host(bp), c1_golden["bp"]) < 1e-5
 
 
-@pytest.mark.parametrize("fmt", [firlib.FIR_LEN_FP32, firlib.FIR_LEN_UNORM16])
+@pytest.mark.parametrize("fmt", [firlib.FIR_LEN_FP32, firlib.FIR_LEN_SNORM16])
 def test_cached_sensitivity_vs_reference(c1, c1_golden, fmt):
     w, ctx, v, _ = c1
     cache = cache_of(c1, 4, fmt)
@@ -139,7 +140,7 @@ def test_cached_backward_matches_atomic_path(c1, c1_golden):
     assert rel_max(host(a), host(b)) < 1e-5
 
 
-@pytest.mark.parametrize("fmt", [firlib.FIR_LEN_FP32, firlib.FIR_LEN_UNORM16])
+@pytest.mark.parametrize("fmt", [firlib.FIR_LEN_FP32, firlib.FIR_LEN_SNORM16])
 def test_cached_osem_vs_reference(c1, c1_golden, fmt):
     """Runs A (2 x 4), B (ACF, Gaussian every sub-iteration, cut 55 mm) and
     C (bias, 1 x 4) of the reference with the cached projector."""
@@ -177,7 +178,7 @@ def test_cached_batch_equals_single(c1, c1_golden, n_images):
     """Batched projector passes (interleaved images, 2/4/8 per matrix read)
     give each image exactly what a single-image pass gives."""
     w, ctx, v, _ = c1
-    cache = cache_of(c1, 4, firlib.FIR_LEN_UNORM16)
+    cache = cache_of(c1, 4, firlib.FIR_LEN_SNORM16)
     rng = np.random.default_rng(n_images)
     xs = [dev(rng.uniform(0.5, 2.0, v.n_voxels).astype(np.float32))
           for _ in range(n_images)]
@@ -194,7 +195,7 @@ def test_cached_batch_equals_single(c1, c1_golden, n_images):
 def test_cached_graph_replays_match_eager(c1, c1_golden):
     """A captured cached iteration replays identically to eager ones."""
     w, ctx, v, _ = c1
-    cache = cache_of(c1, 4, firlib.FIR_LEN_UNORM16)
+    cache = cache_of(c1, 4, firlib.FIR_LEN_SNORM16)
     rng = np.random.default_rng(5)
     ys = [dev(rng.poisson(c1_golden["y0"] * s).astype(np.float32))
           for s in (0.5, 3.0)]
@@ -237,13 +238,41 @@ def test_cache_lanes_keep_results(c1, c1_golden):
             cache.tune(*bad)
 
 
+def test_cache_keeps_negative_lengths():
+    """The reference walk with int-truncated steps (2.5 mm voxels walked in
+    2 mm steps) yields some negative lengths; the signed 16-bit cache keeps
+    them: cached projections agree with the plain path (which reproduces
+    them bit for bit) within the format's error."""
+    w = synth.c1()
+    ctx = firlib.context_for(w)
+    v = firlib.VolDesc.from_spec(io.VolSpec((48, 48, 8), (2.5, 2.5, 2.0),
+                                            (-58.75, -58.75, 0.0)))
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.rand(v.n_voxels, generator=g, device="cuda")
+    y = torch.rand(ctx.n_bins, generator=g, device="cuda")
+    a = torch.zeros(ctx.n_bins, device="cuda")
+    ctx.forward(v, x, a)
+    bp = torch.zeros(v.n_voxels, device="cuda")
+    ctx.backward(v, y, bp, 1)
+    for fmt, tol in [(firlib.FIR_LEN_SNORM16, 1e-4), (firlib.FIR_LEN_FP32, 1e-6)]:
+        cache = ctx.path_cache(v, 1, fmt)
+        ca = torch.zeros(ctx.n_bins, device="cuda")
+        cache.forward(x, ca)
+        cb = torch.zeros(v.n_voxels, device="cuda")
+        cache.backward(y, cb)
+        assert rel_l2(host(ca), host(a)) < tol
+        assert rel_l2(host(cb), host(bp)) < tol
+        cache.close()
+    ctx.close()
+
+
 def test_cache_rejects_bad_arguments(c1):
     w, ctx, v, _ = c1
     with pytest.raises(firlib.FirError):
         ctx.path_cache(v, 5)                 # 5 does not divide 64 views
     with pytest.raises(firlib.FirError):
         ctx.path_cache(v, 4, 7)              # unknown length format
-    cache = cache_of(c1, 4, firlib.FIR_LEN_UNORM16)
+    cache = cache_of(c1, 4, firlib.FIR_LEN_SNORM16)
     x = torch.ones(v.n_voxels, device="cuda")
     y = torch.ones(ctx.n_bins, device="cuda")
     with pytest.raises(firlib.FirError):
@@ -257,7 +286,7 @@ def test_c2_cache_properties():
     w = synth.c2()
     ctx = firlib.context_for(w)
     v = firlib.VolDesc.from_spec(w.vol)
-    cache = ctx.path_cache(v, 16, firlib.FIR_LEN_UNORM16)
+    cache = ctx.path_cache(v, 16, firlib.FIR_LEN_SNORM16)
     info = cache.info()
     assert info["elements"] == ctx.path_count(v)
     act, _ = synth.phantom(w)

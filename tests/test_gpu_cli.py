@@ -134,7 +134,7 @@ def test_osem_bias(work, c1_golden):
     assert rel_l2(xc, c1_golden["xc"]) < 1e-4
 
 
-@pytest.mark.parametrize("fmt", ["unorm16", "fp32"])
+@pytest.mark.parametrize("fmt", ["snorm16", "fp32"])
 def test_osem_path_cache(work, c1_golden, fmt):
     """FIR_PATH_CACHE selects the path-cache projector: runs A and B of the
     reference within 1e-4 relative L2."""

@@ -90,3 +90,37 @@ def test_batched_uneven_groups():
         assert rel_l2(out[i], single.x[0].cpu().numpy()) < 1e-5, i
         c.close()
     ctx.close()
+
+
+def test_batched_uneven_groups_cached():
+    """The same uneven solves on path caches (one shared context, caches of
+    one grid sharing their path matrix): each equals the cached solve run
+    alone, and the plain run within the snorm16 length error."""
+    w = synth.c1()
+    vols = [firlib.VolDesc.from_spec(w.vol),
+            firlib.VolDesc.from_spec(io.VolSpec((48, 48, 8), (2.5, 2.5, 2.0),
+                                                (-58.75, -58.75, 0.0)))]
+    ctx = firlib.context_for(w)
+    act = synth.phantom(w)[0].ravel()
+    y0 = torch.zeros(ctx.n_bins, device="cuda")
+    ctx.forward(vols[0], torch.from_numpy(act).cuda(), y0)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    solves = []
+    for i, (vi, s) in enumerate([(0, 8), (1, 16), (0, 24), (1, 8), (0, 8)]):
+        y = torch.poisson(y0 * (0.5 + i), generator=g)
+        solves.append(dict(vol=vols[vi], n_subsets=s, y=y))
+    b = recon.BatchedOSEM(w, solves, cache_format=firlib.FIR_LEN_SNORM16)
+    assert len(b.groups) == 4 and len(b.caches) == 4
+    b.capture()
+    b.iteration()
+    out = [x.cpu().numpy() for x in b.images()]
+    for i, s in enumerate(solves):
+        c = firlib.context_for(w)
+        c.set_uneven_subsets(True)
+        single = recon.OSEMGroup(c, s["vol"], s["n_subsets"], [s["y"]])
+        single.iteration_eager()
+        single.iteration_eager()
+        assert rel_l2(out[i], single.x[0].cpu().numpy()) < 1e-4, i
+        c.close()
+    b.close()
+    ctx.close()
