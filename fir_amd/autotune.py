@@ -19,7 +19,7 @@ from . import lib as firlib
 # Row orders a cache is tried with: per view (tangential bin, pair by centre
 # slice) and the sinogram's own (pair, tangential bin).
 CACHE_ROW_ORDERS = (firlib.CACHE_ROW_ORDER, firlib.FIR_ORDER_VIEW_MAJOR)
-CACHE_LANES = ((16, 16), (8, 16), (16, 32), (32, 16))
+CACHE_LANES = ((8, 16), (16, 16), (8, 8), (16, 32))
 PLAIN_ORDERS = (firlib.FIR_ORDER_SEGMENT,
                 firlib.FIR_ORDER_SEGMENT | firlib.FIR_ORDER_TANG_MAJOR,
                 firlib.FIR_ORDER_VIEW_MAJOR | firlib.FIR_ORDER_ZCENTER)

@@ -193,6 +193,22 @@ struct ToI64
   }
 };
 
+// Element slots of a path: its count rounded up to a multiple of 4, so that
+// every path (and every voxel's transposed list) starts 16-byte aligned and
+// a lane loads four consecutive elements at once; the padding elements
+// have index 0 and length 0.
+struct Pad4
+{
+  __host__ __device__ long long operator()(int c) const
+  {
+    return (long long)((c + 3) & ~3);
+  }
+  __host__ __device__ long long operator()(unsigned c) const
+  {
+    return (long long)((c + 3u) & ~3u);
+  }
+};
+
 // Row tables: bin, length factor and first element of every crossing LOR.
 __global__ __launch_bounds__(kBlock) void k_pc_rows(
   LorGeom lg, long long n, long long per_view, const int* __restrict__ count,
@@ -234,7 +250,7 @@ __global__ void k_pc_views(const int* __restrict__ count,
   else
   {
     const int rows = rowid[n - 1] + (count[n - 1] > 0 ? 1 : 0);
-    const long long elems = eoff[n - 1] + count[n - 1];
+    const long long elems = eoff[n - 1] + ((count[n - 1] + 3) & ~3);
     view_row[v] = rows;
     view_elem[v] = elems;
     row_off_end[rows] = elems; // off[R] = E
@@ -301,17 +317,27 @@ __device__ __forceinline__ bool subset_row(const SubsetRows& sr, int lanes,
   return *r < r1;
 }
 
+// Whether element e has a zero length (the padding elements, and elements
+// that contribute nothing): the transpose skips them. lb = bytes per length.
+__device__ __forceinline__ bool zero_len(const void* len, int lb, long long e)
+{
+  return lb == 2 ? ((const unsigned short*)len)[e] == 0
+                 : ((const unsigned*)len)[e] == 0;
+}
+
 // Voxel histogram of one subset's elements (thread per row; indices of
 // y-fastest rows converted back).
 __global__ __launch_bounds__(kBlock) void k_tr_hist(
   SubsetRows sr, const long long* __restrict__ off, const int* __restrict__ idx,
-  const unsigned char* __restrict__ yrow, XY xy, unsigned* __restrict__ cnt)
+  const void* __restrict__ len, int lb, const unsigned char* __restrict__ yrow,
+  XY xy, unsigned* __restrict__ cnt)
 {
   int r, r0, k;
   if (!subset_row(sr, 1, &r, &r0, &k)) return;
   const bool yf = yrow[r] != 0;
   for (long long e = off[r]; e < off[r + 1]; ++e)
-    atomicAdd(cnt + (yf ? from_yfast(xy, idx[e]) : idx[e]), 1u);
+    if (!zero_len(len, lb, e))
+      atomicAdd(cnt + (yf ? from_yfast(xy, idx[e]) : idx[e]), 1u);
 }
 
 // Elements of each row of the subset inside the voxel range [v0, v1);
@@ -319,8 +345,8 @@ __global__ __launch_bounds__(kBlock) void k_tr_hist(
 __global__ __launch_bounds__(kBlock) void k_tr_count(
   SubsetRows sr, const int* __restrict__ row_prefix,
   const long long* __restrict__ off, const int* __restrict__ idx,
-  const unsigned char* __restrict__ yrow, XY xy, int v0, int v1,
-  int* __restrict__ rc)
+  const void* __restrict__ len, int lb, const unsigned char* __restrict__ yrow,
+  XY xy, int v0, int v1, int* __restrict__ rc)
 {
   int r, r0, k;
   if (!subset_row(sr, 1, &r, &r0, &k)) return;
@@ -328,7 +354,10 @@ __global__ __launch_bounds__(kBlock) void k_tr_count(
   const unsigned span = (unsigned)(v1 - v0);
   const bool yf = yrow[r] != 0;
   for (long long e = off[r]; e < off[r + 1]; ++e)
-    c += (unsigned)((yf ? from_yfast(xy, idx[e]) : idx[e]) - v0) < span ? 1 : 0;
+    c += !zero_len(len, lb, e) &&
+        (unsigned)((yf ? from_yfast(xy, idx[e]) : idx[e]) - v0) < span
+      ? 1
+      : 0;
   rc[row_prefix[k] + (r - r0)] = c;
 }
 
@@ -350,7 +379,7 @@ __global__ __launch_bounds__(kBlock) void k_tr_expand(
   {
     const unsigned local =
       (unsigned)((yf ? from_yfast(xy, idx[e]) : idx[e]) - v0);
-    if (local < span && p < n_out)
+    if (local < span && p < n_out && Len<F>::bits(len[e]) != 0)
     {
       keys[p] = local;
       vals[p] = ((unsigned long long)(unsigned)r << 32) |
@@ -360,17 +389,27 @@ __global__ __launch_bounds__(kBlock) void k_tr_expand(
   }
 }
 
+// Sorted elements of voxels [v0, v0 + span) to their padded slots: the
+// element of rank i among voxel v's goes to voff_p[v] + i, where the
+// voxel's first rank in the sorted chunk is voff_u[v] - voff_u[v0].
 template <int F>
-__global__ void k_tr_split(const unsigned long long* __restrict__ vals,
-                           long long n, int* __restrict__ trow,
+__global__ void k_tr_split(const unsigned* __restrict__ keys,
+                           const unsigned long long* __restrict__ vals,
+                           long long n, int v0,
+                           const long long* __restrict__ voff_p,
+                           const long long* __restrict__ voff_u,
+                           int* __restrict__ trow,
                            typename Len<F>::T* __restrict__ tlen)
 {
+  const long long u0 = voff_u[v0];
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
   {
-    const unsigned long long v = vals[i];
-    trow[i] = (int)(unsigned)(v >> 32);
-    tlen[i] = Len<F>::from_bits((unsigned)(v & 0xffffffffULL));
+    const int v = v0 + (int)keys[i];
+    const long long dst = voff_p[v] + (i - (voff_u[v] - u0));
+    const unsigned long long val = vals[i];
+    trow[dst] = (int)(unsigned)(val >> 32);
+    tlen[dst] = Len<F>::from_bits((unsigned)(val & 0xffffffffULL));
   }
 }
 
@@ -440,41 +479,103 @@ struct Vec<8>
   }
 };
 
-// acc[b] += sum over elements [e, e1) (lane-strided by `lanes`) of
-// dec(len[e]) * X[ind[e] * NB + b], unrolled by 4 so that four index loads
-// and then four gathers are in flight together.
-template <int F, int NB>
-__device__ __forceinline__ void gather_sum(const int* __restrict__ ind,
-                                           const typename Len<F>::T* __restrict__ len,
-                                           const float* __restrict__ X,
-                                           long long e, long long e1, int lanes,
-                                           float* acc)
+// Four consecutive lengths (8 or 16 bytes, aligned) as floats.
+template <int F>
+struct Quad;
+
+template <>
+struct Quad<FIR_LEN_SNORM16>
 {
-  for (; e + 3 * lanes < e1; e += 4 * lanes)
+  static __device__ __forceinline__ void load(const void* len, long long e,
+                                              float* l)
   {
-    int i[4];
+    const uint2 q = *reinterpret_cast<const uint2*>((const short*)len + e);
+    l[0] = (float)(short)(q.x & 0xffffu);
+    l[1] = (float)(short)(q.x >> 16);
+    l[2] = (float)(short)(q.y & 0xffffu);
+    l[3] = (float)(short)(q.y >> 16);
+  }
+};
+
+template <>
+struct Quad<FIR_LEN_FP16>
+{
+  static __device__ __forceinline__ void load(const void* len, long long e,
+                                              float* l)
+  {
+    const uint2 q =
+      *reinterpret_cast<const uint2*>((const unsigned short*)len + e);
+    l[0] = __half2float(__ushort_as_half((unsigned short)(q.x & 0xffffu)));
+    l[1] = __half2float(__ushort_as_half((unsigned short)(q.x >> 16)));
+    l[2] = __half2float(__ushort_as_half((unsigned short)(q.y & 0xffffu)));
+    l[3] = __half2float(__ushort_as_half((unsigned short)(q.y >> 16)));
+  }
+};
+
+template <>
+struct Quad<FIR_LEN_FP32>
+{
+  static __device__ __forceinline__ void load(const void* len, long long e,
+                                              float* l)
+  {
+    const float4 q = *reinterpret_cast<const float4*>((const float*)len + e);
+    l[0] = q.x;
+    l[1] = q.y;
+    l[2] = q.z;
+    l[3] = q.w;
+  }
+};
+
+// acc[b] += sum over the elements of [e, e1) that this lane owns of
+// dec(len[i]) * X[ind[i] * NB + b]. e and e1 are multiples of 4 (every path
+// and voxel list is padded to whole quads); the lane owns the quads at
+// e, e + 4 lanes, ... and loads each with one 16-byte index load and one
+// 8-byte length load. With one or two images two quads are in flight per
+// trip; with more, one (the gathers of 4+ images per element would cost
+// registers and occupancy).
+template <int F, int NB>
+__device__ __forceinline__ void gather_sum4(const int* __restrict__ ind,
+                                            const void* __restrict__ len,
+                                            const float* __restrict__ X,
+                                            long long e, long long e1,
+                                            int lanes, float* acc)
+{
+  const long long step = 4LL * lanes;
+  for (; NB <= 2 && e + step < e1; e += 2 * step)
+  {
+    const int4 i0 = *reinterpret_cast<const int4*>(ind + e);
+    const int4 i1 = *reinterpret_cast<const int4*>(ind + e + step);
+    float l[8];
+    Quad<F>::load(len, e, l);
+    Quad<F>::load(len, e + step, l + 4);
+    Vec<NB> xv[8];
+    xv[0] = Vec<NB>::load(X, i0.x);
+    xv[1] = Vec<NB>::load(X, i0.y);
+    xv[2] = Vec<NB>::load(X, i0.z);
+    xv[3] = Vec<NB>::load(X, i0.w);
+    xv[4] = Vec<NB>::load(X, i1.x);
+    xv[5] = Vec<NB>::load(X, i1.y);
+    xv[6] = Vec<NB>::load(X, i1.z);
+    xv[7] = Vec<NB>::load(X, i1.w);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[b] = fmaf(l[u], xv[u].v[b], acc[b]);
+  }
+  for (; e < e1; e += step)
+  {
+    const int4 i0 = *reinterpret_cast<const int4*>(ind + e);
     float l[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-    {
-      i[u] = ind[e + u * lanes];
-      l[u] = Len<F>::dec(len[e + u * lanes]);
-    }
+    Quad<F>::load(len, e, l);
     Vec<NB> xv[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) xv[u] = Vec<NB>::load(X, i[u]);
+    xv[0] = Vec<NB>::load(X, i0.x);
+    xv[1] = Vec<NB>::load(X, i0.y);
+    xv[2] = Vec<NB>::load(X, i0.z);
+    xv[3] = Vec<NB>::load(X, i0.w);
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
       for (int b = 0; b < NB; ++b) acc[b] = fmaf(l[u], xv[u].v[b], acc[b]);
-  }
-  for (; e < e1; e += lanes)
-  {
-    const int i = ind[e];
-    const float l = Len<F>::dec(len[e]);
-    const Vec<NB> xv = Vec<NB>::load(X, i);
-#pragma unroll
-    for (int b = 0; b < NB; ++b) acc[b] = fmaf(l, xv.v[b], acc[b]);
   }
 }
 
@@ -509,9 +610,8 @@ __global__ __launch_bounds__(kBlock) void k_pc_forward(SubsetRows sr,
   const int lane = (int)threadIdx.x & (lanes - 1);
   float acc[1] = {0.0f};
   if (active)
-    gather_sum<F, 1>(rd.idx, (const typename Len<F>::T*)rd.len,
-                     rd.yrow[r] ? img_y : img, rd.off[r] + lane,
-                     rd.off[r + 1], lanes, acc);
+    gather_sum4<F, 1>(rd.idx, rd.len, rd.yrow[r] ? img_y : img,
+                      rd.off[r] + 4LL * lane, rd.off[r + 1], lanes, acc);
   lane_reduce<1>(acc, lanes);
   if (active && lane == 0) sino[rd.bin[r]] = acc[0] * rd.factor[r];
 }
@@ -542,9 +642,8 @@ __global__ __launch_bounds__(kBlock) void k_pc_osem_rows(SubsetRows sr,
 #pragma unroll
   for (int b = 0; b < NB; ++b) acc[b] = 0.0f;
   if (active)
-    gather_sum<F, NB>(rd.idx, (const typename Len<F>::T*)rd.len,
-                      rd.yrow[r] ? XY_ : X, rd.off[r] + lane, rd.off[r + 1],
-                      lanes, acc);
+    gather_sum4<F, NB>(rd.idx, rd.len, rd.yrow[r] ? XY_ : X,
+                       rd.off[r] + 4LL * lane, rd.off[r + 1], lanes, acc);
   lane_reduce<NB>(acc, lanes);
   if (!active || lane != 0) return;
   const float f = rd.factor[r];
@@ -597,8 +696,8 @@ __global__ __launch_bounds__(kBlock) void k_pc_backproject(
 #pragma unroll
   for (int b = 0; b < NB; ++b) acc[b] = 0.0f;
   if (active)
-    gather_sum<F, NB>(trow, (const typename Len<F>::T*)tlen_, w,
-                      voff[v] + lane, voff[v + 1], lanes, acc);
+    gather_sum4<F, NB>(trow, tlen_, w, voff[v] + 4LL * lane, voff[v + 1],
+                       lanes, acc);
   lane_reduce<NB>(acc, lanes);
   if (!active || lane != 0) return;
 #pragma unroll
@@ -650,22 +749,23 @@ struct RowMatrix
   int fmt = 0;
   int n_views = 0;
   int n_rows = 0;
-  long long n_elems = 0;
+  long long n_elems = 0;            // path elements
+  long long n_slots = 0;            // with each path padded to a multiple of 4
   int* d_view_row = nullptr;        // [n_views + 1]
   long long* d_view_elem = nullptr; // [n_views + 1]
   int* d_bin = nullptr;             // [n_rows]
   float* d_factor = nullptr;        // [n_rows]
   unsigned char* d_yrow = nullptr;  // [n_rows] 1: indices in the y-fastest layout
   long long* d_off = nullptr;       // [n_rows + 1]
-  int* d_idx = nullptr;             // [n_elems]
-  void* d_len = nullptr;            // [n_elems]
+  int* d_idx = nullptr;             // [n_slots]
+  void* d_len = nullptr;            // [n_slots]
   std::vector<int> view_row;
   std::vector<long long> view_elem;
 
   long long bytes() const
   {
     return (long long)(n_views + 1) * (4 + 8) + (long long)n_rows * (4 + 4 + 1 + 8) +
-      8 + n_elems * (4 + (long long)len_bytes(fmt));
+      8 + n_slots * (4 + (long long)len_bytes(fmt));
   }
 
   ~RowMatrix()
@@ -690,14 +790,15 @@ struct fir_cache
   long long nvox = 0;
   int n_subsets = 0;
   long long* d_voff = nullptr; // [n_subsets][nvox + 1], global element offsets
-  int* d_trow = nullptr;       // [n_elems]
-  void* d_tlen = nullptr;      // [n_elems]
+  int* d_trow = nullptr;       // [t_slots]
+  void* d_tlen = nullptr;      // [t_slots]
+  long long t_slots = 0;
   float* d_w = nullptr;        // per-row weights, [n_rows * 8]
   size_t w_floats = 0;
   float* d_x = nullptr;        // interleaved images, [nvox * 8]
   size_t x_floats = 0;
   float* d_xy = nullptr;       // the same in the y-fastest layout
-  int fwd_lanes = 16;
+  int fwd_lanes = 8;
   int bp_lanes = 16;
   int xcd_map = 0;
 
@@ -813,8 +914,10 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
   // Exclusive scans: row id of each crossing LOR, first element of each LOR.
   hipcub::TransformInputIterator<int, HitFlag, const int*> hits(count,
                                                                HitFlag());
-  hipcub::TransformInputIterator<long long, ToI64, const int*> counts64(
-    count, ToI64());
+  hipcub::TransformInputIterator<long long, Pad4, const int*> counts64(count,
+                                                                       Pad4());
+  hipcub::TransformInputIterator<long long, ToI64, const int*> real64(count,
+                                                                     ToI64());
   size_t tb1 = 0, tb2 = 0;
   FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb1, hits, rowid,
                                                  (int)n, c->stream));
@@ -843,6 +946,16 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
   // Totals first (they size the row and element arrays).
   long long* d_end;
   if ((rc = tmp.get(&d_end, 1, "totals"))) return rc;
+  size_t tb3 = 0;
+  FIR_HIP_CHECK(hipcub::DeviceReduce::Sum(nullptr, tb3, real64, d_end, (int)n,
+                                          c->stream));
+  unsigned char* red_tmp;
+  if ((rc = tmp.get(&red_tmp, tb3, "reduction scratch"))) return rc;
+  FIR_HIP_CHECK(hipcub::DeviceReduce::Sum(red_tmp, tb3, real64, d_end, (int)n,
+                                          c->stream));
+  long long real_total = 0;
+  FIR_HIP_CHECK(hipMemcpyAsync(&real_total, d_end, sizeof(long long),
+                               hipMemcpyDeviceToHost, c->stream));
   int last_row = 0, last_count = 0;
   long long last_eoff = 0;
   FIR_HIP_CHECK(hipMemcpyAsync(&last_row, rowid + n - 1, sizeof(int),
@@ -853,7 +966,8 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
                                hipMemcpyDeviceToHost, c->stream));
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
   rm->n_rows = last_row + (last_count > 0 ? 1 : 0);
-  rm->n_elems = last_eoff + last_count;
+  rm->n_elems = real_total;
+  rm->n_slots = last_eoff + ((last_count + 3) & ~3);
 
   size_t free_b = 0, total_b = 0;
   FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
@@ -868,10 +982,15 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
   if ((rc = dev_alloc(&rm->d_yrow, rm->n_rows, "row layouts"))) return rc;
   if ((rc = dev_alloc(&rm->d_off, (size_t)rm->n_rows + 1, "row offsets")))
     return rc;
-  if ((rc = dev_alloc(&rm->d_idx, rm->n_elems, "element indices"))) return rc;
+  if ((rc = dev_alloc(&rm->d_idx, rm->n_slots, "element indices"))) return rc;
   if ((rc = dev_alloc((unsigned char**)&rm->d_len,
-                      rm->n_elems * len_bytes(fmt), "element lengths")))
+                      rm->n_slots * len_bytes(fmt), "element lengths")))
     return rc;
+  // padding elements: index 0, length 0
+  FIR_HIP_CHECK(hipMemsetAsync(rm->d_idx, 0, sizeof(int) * rm->n_slots,
+                               c->stream));
+  FIR_HIP_CHECK(hipMemsetAsync(rm->d_len, 0, rm->n_slots * len_bytes(fmt),
+                               c->stream));
 
   hipLaunchKernelGGL(k_pc_views, dim3(blocks_for(c->n_views + 1)),
                      dim3(kBlock), 0, c->stream, count, rowid, eoff, n,
@@ -916,7 +1035,8 @@ template <int F>
 int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
                     const SubsetRows& sr, const int* d_prefix, int vpr,
                     int max_rows, int n_rows_s, int v0, int v1,
-                    long long base, long long n_ch)
+                    const long long* voff_p, const long long* voff_u,
+                    long long n_ch)
 {
   DevBufs tmp; // the chunk's scratch, released on return
   int rc;
@@ -935,9 +1055,10 @@ int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
   FIR_HIP_CHECK(hipMemsetAsync(rc_cnt, 0, sizeof(int) * ((size_t)n_rows_s + 1),
                                c->stream));
   const XY xy{cache->vol.nx, cache->vol.ny};
+  const int lb = (int)len_bytes(rm->fmt);
   hipLaunchKernelGGL(k_tr_count, grid_rows, dim3(kBlock), 0, c->stream, sr,
-                     d_prefix, rm->d_off, rm->d_idx, rm->d_yrow, xy, v0, v1,
-                     rc_cnt);
+                     d_prefix, rm->d_off, rm->d_idx, rm->d_len, lb, rm->d_yrow,
+                     xy, v0, v1, rc_cnt);
   FIR_CHECK_LAUNCH();
   size_t tb = 0;
   FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, rc_cnt, pos,
@@ -972,8 +1093,9 @@ int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
   FIR_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
     sort_tmp, sb, keys, vals, (int)n_ch, 0, bits, c->stream));
   hipLaunchKernelGGL(k_tr_split<F>, dim3(ew_blocks(n_ch)), dim3(kBlock), 0,
-                     c->stream, vals.Current(), n_ch, cache->d_trow + base,
-                     (typename Len<F>::T*)cache->d_tlen + base);
+                     c->stream, keys.Current(), vals.Current(), n_ch, v0,
+                     voff_p, voff_u, cache->d_trow,
+                     (typename Len<F>::T*)cache->d_tlen);
   FIR_CHECK_LAUNCH();
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
   return FIR_OK;
@@ -985,32 +1107,34 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
   const int S = cache->n_subsets;
   const long long nvox = cache->nvox;
   int rc;
+  // Padding adds at most 3 slots per (subset, voxel).
+  const long long slots = rm->n_elems + 3LL * S * nvox;
   if ((rc = dev_alloc(&cache->d_voff, (size_t)S * (nvox + 1), "voxel offsets")))
     return rc;
-  if ((rc = dev_alloc(&cache->d_trow, rm->n_elems, "transposed rows")))
-    return rc;
+  if ((rc = dev_alloc(&cache->d_trow, slots, "transposed rows"))) return rc;
   if ((rc = dev_alloc((unsigned char**)&cache->d_tlen,
-                      rm->n_elems * len_bytes(rm->fmt), "transposed lengths")))
+                      slots * len_bytes(rm->fmt), "transposed lengths")))
     return rc;
+  FIR_HIP_CHECK(hipMemsetAsync(cache->d_trow, 0, sizeof(int) * slots,
+                               c->stream));
+  FIR_HIP_CHECK(hipMemsetAsync(cache->d_tlen, 0, slots * len_bytes(rm->fmt),
+                               c->stream));
+  cache->t_slots = slots;
 
   std::vector<long long> voff(nvox + 1);
-  long long base = 0;
+  long long base = 0, real = 0;
   for (int s = 0; s < S; ++s)
   {
     DevBufs tmp;
     const int vpr = fir::views_in_subset(rm->n_views, S, s);
     std::vector<int> prefix(vpr + 1, 0);
     int max_rows = 0;
-    long long max_elems = 0, elems_s = 0;
     for (int k = 0; k < vpr; ++k)
     {
       const int view = k * S + s;
       const int nr = rm->view_row[view + 1] - rm->view_row[view];
-      const long long ne = rm->view_elem[view + 1] - rm->view_elem[view];
       prefix[k + 1] = prefix[k] + nr;
       max_rows = std::max(max_rows, nr);
-      max_elems = std::max(max_elems, ne);
-      elems_s += ne;
     }
     const int n_rows_s = prefix[vpr];
     int* d_prefix;
@@ -1018,9 +1142,13 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
     FIR_HIP_CHECK(hipMemcpyAsync(d_prefix, prefix.data(), sizeof(int) * (vpr + 1),
                                  hipMemcpyHostToDevice, c->stream));
 
-    // Voxel histogram and offsets of the subset's elements.
+    // Voxel histogram of the subset's (non-zero-length) elements; padded
+    // offsets (stored) and unpadded offsets (chunk planning, ranks).
     unsigned* cnt;
-    if ((rc = tmp.get(&cnt, nvox + 1, "voxel counts"))) return rc;
+    long long* voff_u;
+    if ((rc = tmp.get(&cnt, nvox + 1, "voxel counts")) ||
+        (rc = tmp.get(&voff_u, nvox + 1, "voxel offsets")))
+      return rc;
     FIR_HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned) * (nvox + 1),
                                  c->stream));
     if (max_rows > 0)
@@ -1028,30 +1156,37 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
       hipLaunchKernelGGL(k_tr_hist, dim3(blocks_for(max_rows), vpr),
                          dim3(kBlock), 0, c->stream,
                          SubsetRows{rm->d_view_row, S, s, 0}, rm->d_off,
-                         rm->d_idx, rm->d_yrow,
-                         XY{cache->vol.nx, cache->vol.ny}, cnt);
+                         rm->d_idx, rm->d_len, (int)len_bytes(rm->fmt),
+                         rm->d_yrow, XY{cache->vol.nx, cache->vol.ny}, cnt);
       FIR_CHECK_LAUNCH();
     }
     long long* voff_s = cache->d_voff + (long long)s * (nvox + 1);
-    hipcub::TransformInputIterator<long long, ToI64, const unsigned*> cnt64(
+    hipcub::TransformInputIterator<long long, Pad4, const unsigned*> cnt_p(
+      cnt, Pad4());
+    hipcub::TransformInputIterator<long long, ToI64, const unsigned*> cnt_u(
       cnt, ToI64());
-    size_t tb = 0;
+    size_t tb = 0, tb2 = 0;
     FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveScan(
-      nullptr, tb, cnt64, voff_s, hipcub::Sum(), base, (int)(nvox + 1),
+      nullptr, tb, cnt_p, voff_s, hipcub::Sum(), base, (int)(nvox + 1),
       c->stream));
+    FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, cnt_u, voff_u,
+                                                   (int)(nvox + 1), c->stream));
     unsigned char* scan_tmp;
-    if ((rc = tmp.get(&scan_tmp, tb, "scan scratch"))) return rc;
+    if ((rc = tmp.get(&scan_tmp, std::max(tb, tb2), "scan scratch"))) return rc;
     FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveScan(
-      scan_tmp, tb, cnt64, voff_s, hipcub::Sum(), base, (int)(nvox + 1),
+      scan_tmp, tb, cnt_p, voff_s, hipcub::Sum(), base, (int)(nvox + 1),
       c->stream));
-    FIR_HIP_CHECK(hipMemcpyAsync(voff.data(), voff_s,
+    FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tb2, cnt_u, voff_u,
+                                                   (int)(nvox + 1), c->stream));
+    long long end_p = 0;
+    FIR_HIP_CHECK(hipMemcpyAsync(&end_p, voff_s + nvox, sizeof(long long),
+                                 hipMemcpyDeviceToHost, c->stream));
+    FIR_HIP_CHECK(hipMemcpyAsync(voff.data(), voff_u,
                                  sizeof(long long) * (nvox + 1),
                                  hipMemcpyDeviceToHost, c->stream));
     FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
-    if (voff[nvox] - base != elems_s)
-      return fail(FIR_E_STATE, "path cache: subset %d holds %lld elements, "
-                               "its voxel histogram %lld", s, elems_s,
-                  voff[nvox] - base);
+    if (end_p > slots)
+      return fail(FIR_E_STATE, "path cache: transposed slots overflow");
 
     const SubsetRows sr{rm->d_view_row, S, s, 0};
     long long v0 = 0;
@@ -1073,28 +1208,29 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
         case FIR_LEN_SNORM16:
           rc = transpose_chunk<FIR_LEN_SNORM16>(c, rm, cache, sr, d_prefix, vpr,
                                                 max_rows, n_rows_s, (int)v0,
-                                                (int)v1, voff[v0], n_ch);
+                                                (int)v1, voff_s, voff_u, n_ch);
           break;
         case FIR_LEN_FP16:
           rc = transpose_chunk<FIR_LEN_FP16>(c, rm, cache, sr, d_prefix, vpr,
                                              max_rows, n_rows_s, (int)v0,
-                                             (int)v1, voff[v0], n_ch);
+                                             (int)v1, voff_s, voff_u, n_ch);
           break;
         default:
           rc = transpose_chunk<FIR_LEN_FP32>(c, rm, cache, sr, d_prefix, vpr,
                                              max_rows, n_rows_s, (int)v0,
-                                             (int)v1, voff[v0], n_ch);
+                                             (int)v1, voff_s, voff_u, n_ch);
           break;
         }
         if (rc) return rc;
       }
       v0 = v1;
     }
-    base = voff[nvox];
+    base = end_p;
+    real += voff[nvox];
   }
-  if (base != rm->n_elems)
+  if (real > rm->n_elems)
     return fail(FIR_E_STATE, "path cache: transposed %lld of %lld elements",
-                base, rm->n_elems);
+                real, rm->n_elems);
   return FIR_OK;
 }
 
@@ -1274,7 +1410,9 @@ int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
   cache->n_subsets = n_subsets;
   size_t free_b = 0, total_b = 0;
   FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-  const long long need = rm->n_elems * (4 + (long long)len_bytes(len_format)) +
+  const long long need =
+    (rm->n_elems + 3LL * n_subsets * cache->nvox) *
+      (4 + (long long)len_bytes(len_format)) +
     (long long)n_subsets * (cache->nvox + 1) * 8 +
     ((long long)rm->n_rows + 2 * cache->nvox) * kMaxCacheBatch * 4;
   if ((long long)free_b < need + (1LL << 30))
@@ -1324,7 +1462,7 @@ int fir_cache_info(const fir_cache* cache, int64_t* n_rows, int64_t* n_elements,
   if (n_elements) *n_elements = rm->n_elems;
   if (device_bytes)
     *device_bytes = rm->bytes() +
-      rm->n_elems * (4 + (long long)len_bytes(rm->fmt)) +
+      cache->t_slots * (4 + (long long)len_bytes(rm->fmt)) +
       (long long)cache->n_subsets * (cache->nvox + 1) * 8 +
       (long long)(cache->w_floats + 2 * cache->x_floats) * 4;
   return FIR_OK;

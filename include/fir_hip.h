@@ -305,7 +305,7 @@ int fir_cache_info(const fir_cache* cache, int64_t* n_rows,
 
 /* Kernel tuning (results change only by FP32 summation order): lanes
  * that share one LOR (forward) and one voxel (backprojection), powers of
- * two from 1 to 64 (default 16 and 16); flags FIR_CACHE_XCD_MAP deals
+ * two from 1 to 64 (default 8 and 16); flags FIR_CACHE_XCD_MAP deals
  * contiguous eighths
  * of the rows of a view (of the voxels) to the eight XCDs. The path
  * matrix's row order within a view follows the context's row order
