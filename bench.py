@@ -267,11 +267,11 @@ def main():
     tune_s = time.perf_counter() - cache_t0
     cache.close()
     cache_t0 = time.perf_counter()
+    ctx.set_cache_lanes(tres.best["fwd_lanes"], tres.best["bp_lanes"])
     cache = ctx.path_cache(v, n_sub, firlib.FIR_LEN_SNORM16,
                            row_order=tres.best["row_order"])
     torch.cuda.synchronize()
     cache_build_s = time.perf_counter() - cache_t0
-    cache.tune(tres.best["fwd_lanes"], tres.best["bp_lanes"])
     cinfo = cache.info()
     log(f"path cache: {cinfo['elements']} elements, "
         f"{cinfo['device_bytes'] / 1e9:.2f} GB, built in {cache_build_s:.2f} s;"

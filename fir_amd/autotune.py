@@ -19,7 +19,7 @@ from . import lib as firlib
 # Row orders a cache is tried with: per view (tangential bin, pair by centre
 # slice) and the sinogram's own (pair, tangential bin).
 CACHE_ROW_ORDERS = (firlib.CACHE_ROW_ORDER, firlib.FIR_ORDER_VIEW_MAJOR)
-CACHE_LANES = ((8, 16), (16, 16), (8, 8), (16, 32))
+CACHE_LANES = ((8, 16), (16, 16), (8, 8))
 PLAIN_ORDERS = (firlib.FIR_ORDER_SEGMENT,
                 firlib.FIR_ORDER_SEGMENT | firlib.FIR_ORDER_TANG_MAJOR,
                 firlib.FIR_ORDER_VIEW_MAJOR | firlib.FIR_ORDER_ZCENTER)
@@ -65,35 +65,35 @@ def tune_cache(ctx: firlib.Context, vol: firlib.VolDesc, n_subsets: int,
                lanes: Sequence[Tuple[int, int]] = CACHE_LANES,
                subset: int = 0, reps: int = 3
                ) -> Tuple[firlib.PathCache, Result]:
-    """Build a path cache per row order, time the OSEM projector pass of
-    ``len(xs)`` images for every lanes setting, and return the fastest
-    cache, tuned, with the table of trials. The other caches are freed
-    before the next one is built."""
+    """Build a path cache per (row order, lanes) -- the lanes set the cache's
+    layout (Context.set_cache_lanes) -- time the OSEM projector pass of
+    ``len(xs)`` images on each, and return the fastest cache with the table
+    of trials. Only the best cache so far is kept while the next is built."""
     bps = [torch.zeros(vol.n_voxels, device=xs[0].device) for _ in xs]
     trials: List[Trial] = []
     best = None
     best_cache: Optional[firlib.PathCache] = None
-    for order in row_orders:
-        cache = ctx.path_cache(vol, n_subsets, len_format, row_order=order)
-        mine = None
-        for fl, bl in lanes:
-            cache.tune(fl, bl)
-            ms = _time(lambda: cache.osem_project(list(xs), list(ys), None,
-                                                  bps, subset), reps)
-            cfg = {"row_order": order, "fwd_lanes": fl, "bp_lanes": bl}
-            trials.append(Trial(cfg, ms))
-            if mine is None or ms < mine[1]:
-                mine = (cfg, ms)
-        if best is None or mine[1] < best[1]:
-            if best_cache is not None:
-                best_cache.close()
-            best, best_cache = mine, cache
-        else:
-            cache.close()
-        torch.cuda.synchronize()
-    cfg = best[0]
-    best_cache.tune(cfg["fwd_lanes"], cfg["bp_lanes"])
-    return best_cache, Result(cfg, best[1], trials)
+    old = getattr(ctx, "_cache_lanes", (8, 16))
+    try:
+        for order in row_orders:
+            for fl, bl in lanes:
+                ctx.set_cache_lanes(fl, bl)
+                cache = ctx.path_cache(vol, n_subsets, len_format,
+                                       row_order=order)
+                ms = _time(lambda: cache.osem_project(list(xs), list(ys),
+                                                      None, bps, subset), reps)
+                cfg = {"row_order": order, "fwd_lanes": fl, "bp_lanes": bl}
+                trials.append(Trial(cfg, ms))
+                if best is None or ms < best[1]:
+                    if best_cache is not None:
+                        best_cache.close()
+                    best, best_cache = (cfg, ms), cache
+                else:
+                    cache.close()
+                torch.cuda.synchronize()
+    finally:
+        ctx.set_cache_lanes(*old)
+    return best_cache, Result(best[0], best[1], trials)
 
 
 def tune_plain(ctx: firlib.Context, vol: firlib.VolDesc, n_subsets: int,

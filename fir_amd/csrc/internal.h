@@ -67,6 +67,9 @@ struct fir_ctx
   // Bumped whenever the scanner, sinogram or walk mode changes: path
   // matrices built under another generation are not shared.
   unsigned generation = 0;
+  // Lanes per LOR and per voxel the path caches built on this context are
+  // laid out for (fir_set_cache_lanes).
+  int cache_lanes[2] = {8, 16};
   // Path matrices of live caches, shared by caches of one volume, walk mode
   // and length format.
   std::vector<std::weak_ptr<RowMatrix>> row_matrices;

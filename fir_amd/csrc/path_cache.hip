@@ -209,6 +209,25 @@ struct Pad4
   }
 };
 
+// Slot of element p of a list of n slots (n a multiple of 4) that T lanes
+// read quad by quad (lane j takes quads j, j + T, ...): within each round of
+// T quads, element k of lane j's quad holds list element k * r + j of the
+// round (r = quads in the round). Each load instruction of the T lanes
+// then reads consecutive list elements -- consecutive voxels of a path,
+// consecutive rows of a voxel -- so their gathers share cache lines.
+__host__ __device__ __forceinline__ long long quad_slot(long long p,
+                                                        long long n, int T)
+{
+  const long long q = n >> 2;
+  const long long m = p / (4LL * T);
+  const long long pl = p - m * 4LL * T;
+  const long long left = q - m * T;
+  const long long r = left < T ? left : T;
+  const long long k = pl / r;
+  const long long j = pl - k * r;
+  return 4 * (m * T + j) + k;
+}
+
 // Row tables: bin, length factor and first element of every crossing LOR.
 __global__ __launch_bounds__(kBlock) void k_pc_rows(
   LorGeom lg, long long n, long long per_view, const int* __restrict__ count,
@@ -263,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_fill(
   Grid g, LorGeom lg, long long n, long long per_view,
   const int* __restrict__ count,
   const float* __restrict__ maxlen, const long long* __restrict__ eoff,
-  XY xy, int* __restrict__ idx, typename Len<F>::T* __restrict__ len)
+  XY xy, int lanes, int* __restrict__ idx, typename Len<F>::T* __restrict__ len)
 {
   const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
   if (t >= n || count[t] == 0) return;
@@ -273,15 +292,18 @@ __global__ __launch_bounds__(kBlock) void k_pc_fill(
   if (!fir::walk_setup(g, L.e1, L.e2, w)) return;
   const float m = maxlen[t];
   const bool yf = y_dominant(L);
-  long long e = eoff[t];
-  const long long e_end = e + count[t];
+  const long long base = eoff[t];
+  const int cnt = count[t];
+  const long long slots = (cnt + 3) & ~3;
+  int p = 0;
   fir::walk_run(g, w, [&](int i, double l) {
-    if (e < e_end)
+    if (p < cnt)
     {
+      const long long e = base + quad_slot(p, slots, lanes);
       idx[e] = yf ? to_yfast(xy, i) : i;
       len[e] = Len<F>::enc(l, m);
     }
-    ++e;
+    ++p;
   });
 }
 
@@ -390,14 +412,14 @@ __global__ __launch_bounds__(kBlock) void k_tr_expand(
 }
 
 // Sorted elements of voxels [v0, v0 + span) to their padded slots: the
-// element of rank i among voxel v's goes to voff_p[v] + i, where the
-// voxel's first rank in the sorted chunk is voff_u[v] - voff_u[v0].
+// element of rank i among voxel v's goes to voff_p[v] + quad_slot(i), where
+// the voxel's first rank in the sorted chunk is voff_u[v] - voff_u[v0].
 template <int F>
 __global__ void k_tr_split(const unsigned* __restrict__ keys,
                            const unsigned long long* __restrict__ vals,
                            long long n, int v0,
                            const long long* __restrict__ voff_p,
-                           const long long* __restrict__ voff_u,
+                           const long long* __restrict__ voff_u, int lanes,
                            int* __restrict__ trow,
                            typename Len<F>::T* __restrict__ tlen)
 {
@@ -406,7 +428,9 @@ __global__ void k_tr_split(const unsigned* __restrict__ keys,
        i += (long long)gridDim.x * blockDim.x)
   {
     const int v = v0 + (int)keys[i];
-    const long long dst = voff_p[v] + (i - (voff_u[v] - u0));
+    const long long dst =
+      voff_p[v] + quad_slot(i - (voff_u[v] - u0), voff_p[v + 1] - voff_p[v],
+                            lanes);
     const unsigned long long val = vals[i];
     trow[dst] = (int)(unsigned)(val >> 32);
     tlen[dst] = Len<F>::from_bits((unsigned)(val & 0xffffffffULL));
@@ -746,6 +770,7 @@ struct RowMatrix
   int walk_mode = 0;
   int pair_order = 0;
   int tang_major = 0;
+  int lanes = 0;   // quad layout lanes (fir_set_cache_lanes)
   int fmt = 0;
   int n_views = 0;
   int n_rows = 0;
@@ -878,7 +903,7 @@ int fill_rows(fir_ctx* c, const Grid& g, const LorGeom& lg, long long n,
 {
   hipLaunchKernelGGL(k_pc_fill<F>, dim3(blocks_for(n)), dim3(kBlock), 0,
                      c->stream, g, lg, n, per_view, count, maxlen, eoff,
-                     XY{g.row, g.slice / g.row}, rm->d_idx,
+                     XY{g.row, g.slice / g.row}, c->cache_lanes[0], rm->d_idx,
                      (typename Len<F>::T*)rm->d_len);
   FIR_CHECK_LAUNCH();
   return FIR_OK;
@@ -937,6 +962,7 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
   rm->walk_mode = c->walk_mode;
   rm->pair_order = c->lor_order;
   rm->tang_major = c->tang_major;
+  rm->lanes = c->cache_lanes[0];
   rm->fmt = fmt;
   rm->n_views = c->n_views;
   if ((rc = dev_alloc(&rm->d_view_row, c->n_views + 1, "view rows"))) return rc;
@@ -1094,7 +1120,7 @@ int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
     sort_tmp, sb, keys, vals, (int)n_ch, 0, bits, c->stream));
   hipLaunchKernelGGL(k_tr_split<F>, dim3(ew_blocks(n_ch)), dim3(kBlock), 0,
                      c->stream, keys.Current(), vals.Current(), n_ch, v0,
-                     voff_p, voff_u, cache->d_trow,
+                     voff_p, voff_u, cache->bp_lanes, cache->d_trow,
                      (typename Len<F>::T*)cache->d_tlen);
   FIR_CHECK_LAUNCH();
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1362,6 +1388,18 @@ int batch_width(int n)
 
 extern "C" {
 
+int fir_set_cache_lanes(fir_ctx* c, int fwd_lanes, int bp_lanes)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  auto ok = [](int t) { return t >= 1 && t <= 64 && (t & (t - 1)) == 0; };
+  if (!ok(fwd_lanes) || !ok(bp_lanes))
+    return fail(FIR_E_ARG, "lanes per LOR / voxel must be a power of two "
+                           "from 1 to 64");
+  c->cache_lanes[0] = fwd_lanes;
+  c->cache_lanes[1] = bp_lanes;
+  return FIR_OK;
+}
+
 int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
                      int len_format, fir_cache** out)
 {
@@ -1383,7 +1421,7 @@ int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
     auto sp = wp.lock();
     if (sp && sp->generation == c->generation && same_vol(sp->vol, *v) &&
         sp->walk_mode == c->walk_mode && sp->pair_order == c->lor_order &&
-        sp->tang_major == c->tang_major &&
+        sp->tang_major == c->tang_major && sp->lanes == c->cache_lanes[0] &&
         sp->fmt == len_format)
     {
       rm = sp;
@@ -1408,6 +1446,8 @@ int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
   cache->vol = *v;
   cache->nvox = (long long)v->nx * v->ny * v->nz;
   cache->n_subsets = n_subsets;
+  cache->fwd_lanes = c->cache_lanes[0];
+  cache->bp_lanes = c->cache_lanes[1];
   size_t free_b = 0, total_b = 0;
   FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
   const long long need =

@@ -291,6 +291,13 @@ enum
   FIR_LEN_FP32 = 2
 };
 
+/* Lanes per LOR (forward) and per voxel (backprojection) that the caches
+ * created afterwards on this context are laid out for, powers of two from 1
+ * to 64 (default 8 and 16); also their initial fir_cache_tune setting.
+ * Within every path and voxel list, each round of `lanes` quads is stored
+ * so that the lanes' loads read consecutive elements. */
+int fir_set_cache_lanes(fir_ctx* ctx, int fwd_lanes, int bp_lanes);
+
 /* Build the cache of `vol` for `n_subsets` subsets (views v with
  * v % n_subsets == s, as LORCache.cc:49-51). Synchronizes the context's
  * stream; FIR_E_NOMEM if the matrices do not fit in device memory. */
@@ -305,7 +312,8 @@ int fir_cache_info(const fir_cache* cache, int64_t* n_rows,
 
 /* Kernel tuning (results change only by FP32 summation order): lanes
  * that share one LOR (forward) and one voxel (backprojection), powers of
- * two from 1 to 64 (default 8 and 16); flags FIR_CACHE_XCD_MAP deals
+ * two from 1 to 64 (default: the layout's, fir_set_cache_lanes; other
+ * values give the same results, with less locality); flags FIR_CACHE_XCD_MAP deals
  * contiguous eighths
  * of the rows of a view (of the voxels) to the eight XCDs. The path
  * matrix's row order within a view follows the context's row order

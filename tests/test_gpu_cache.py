@@ -217,6 +217,36 @@ def test_cached_graph_replays_match_eager(c1, c1_golden):
             assert np.array_equal(host(a), host(b)), rep
 
 
+def test_cache_layout_lanes_keep_results(c1, c1_golden):
+    """Caches laid out for other lanes (fir_set_cache_lanes), run with their
+    own or other runtime lanes, give the same projections."""
+    w, ctx, v, _ = c1
+    ref = cache_of(c1, 4, firlib.FIR_LEN_FP32)
+    img = dev(c1_golden["act"].ravel())
+    y = dev(c1_golden["y"])
+    a = torch.zeros(ctx.n_bins, device="cuda")
+    ref.forward(img, a)
+    fa = torch.zeros(4 * v.n_voxels, device="cuda")
+    ref.backward(y, fa)
+    for fl, bl in [(4, 64), (32, 2), (1, 1)]:
+        ctx.set_cache_lanes(fl, bl)
+        try:
+            c = ctx.path_cache(v, 4, firlib.FIR_LEN_FP32)
+        finally:
+            ctx.set_cache_lanes(8, 16)
+        for rl in [(fl, bl), (8, 16)]:
+            c.tune(*rl)
+            b = torch.zeros(ctx.n_bins, device="cuda")
+            c.forward(img, b)
+            fb = torch.zeros(4 * v.n_voxels, device="cuda")
+            c.backward(y, fb)
+            assert rel_max(host(b), host(a)) < 1e-6
+            assert rel_max(host(fb), host(fa)) < 1e-6
+        c.close()
+    with pytest.raises(firlib.FirError):
+        ctx.set_cache_lanes(3, 16)
+
+
 def test_cache_lanes_keep_results(c1, c1_golden):
     w, ctx, v, _ = c1
     cache = cache_of(c1, 4, firlib.FIR_LEN_FP32)
@@ -318,6 +348,7 @@ def test_autotuner_picks_fastest_and_keeps_results(c1, c1_golden):
     cache, res = autotune.tune_cache(ctx, v, 4, [x], [y], reps=1,
                                      lanes=((16, 16), (8, 8)))
     assert len(res.trials) == 2 * len(autotune.CACHE_ROW_ORDERS)
+    assert ctx._cache_lanes == (8, 16)
     assert res.best_ms == min(t.ms for t in res.trials)
     g = recon.OSEMGroup(ctx, v, 4, [y], cache=cache)
     g.iteration_eager()
