@@ -503,17 +503,21 @@ struct Vec<8>
   }
 };
 
-// Four consecutive lengths (8 or 16 bytes, aligned) as floats.
+// Four consecutive lengths (8 or 16 bytes, aligned): raw load, then
+// decode to floats.
 template <int F>
 struct Quad;
 
 template <>
 struct Quad<FIR_LEN_SNORM16>
 {
-  static __device__ __forceinline__ void load(const void* len, long long e,
-                                              float* l)
+  using Raw = uint2;
+  static __device__ __forceinline__ Raw load(const void* len, long long e)
   {
-    const uint2 q = *reinterpret_cast<const uint2*>((const short*)len + e);
+    return *reinterpret_cast<const uint2*>((const short*)len + e);
+  }
+  static __device__ __forceinline__ void dec(Raw q, float* l)
+  {
     l[0] = (float)(short)(q.x & 0xffffu);
     l[1] = (float)(short)(q.x >> 16);
     l[2] = (float)(short)(q.y & 0xffffu);
@@ -524,11 +528,13 @@ struct Quad<FIR_LEN_SNORM16>
 template <>
 struct Quad<FIR_LEN_FP16>
 {
-  static __device__ __forceinline__ void load(const void* len, long long e,
-                                              float* l)
+  using Raw = uint2;
+  static __device__ __forceinline__ Raw load(const void* len, long long e)
   {
-    const uint2 q =
-      *reinterpret_cast<const uint2*>((const unsigned short*)len + e);
+    return *reinterpret_cast<const uint2*>((const unsigned short*)len + e);
+  }
+  static __device__ __forceinline__ void dec(Raw q, float* l)
+  {
     l[0] = __half2float(__ushort_as_half((unsigned short)(q.x & 0xffffu)));
     l[1] = __half2float(__ushort_as_half((unsigned short)(q.x >> 16)));
     l[2] = __half2float(__ushort_as_half((unsigned short)(q.y & 0xffffu)));
@@ -539,10 +545,13 @@ struct Quad<FIR_LEN_FP16>
 template <>
 struct Quad<FIR_LEN_FP32>
 {
-  static __device__ __forceinline__ void load(const void* len, long long e,
-                                              float* l)
+  using Raw = float4;
+  static __device__ __forceinline__ Raw load(const void* len, long long e)
   {
-    const float4 q = *reinterpret_cast<const float4*>((const float*)len + e);
+    return *reinterpret_cast<const float4*>((const float*)len + e);
+  }
+  static __device__ __forceinline__ void dec(Raw q, float* l)
+  {
     l[0] = q.x;
     l[1] = q.y;
     l[2] = q.z;
@@ -555,8 +564,10 @@ struct Quad<FIR_LEN_FP32>
 // and voxel list is padded to whole quads); the lane owns the quads at
 // e, e + 4 lanes, ... and loads each with one 16-byte index load and one
 // 8-byte length load. With one or two images two quads are in flight per
-// trip; with more, one (the gathers of 4+ images per element would cost
-// registers and occupancy).
+// trip, with more images one quad (the gathers of 4+ images per element
+// would cost registers and occupancy); either way the next trip's indices
+// and lengths are loaded before this trip's gathers are summed, so that the
+// latency of the streamed matrix overlaps that of the gathers.
 template <int F, int NB>
 __device__ __forceinline__ void gather_sum4(const int* __restrict__ ind,
                                             const void* __restrict__ len,
@@ -564,33 +575,89 @@ __device__ __forceinline__ void gather_sum4(const int* __restrict__ ind,
                                             long long e, long long e1,
                                             int lanes, float* acc)
 {
+  using Raw = typename Quad<F>::Raw;
   const long long step = 4LL * lanes;
-  for (; NB <= 2 && e + step < e1; e += 2 * step)
+  if (NB <= 2 && e + step < e1)
   {
-    const int4 i0 = *reinterpret_cast<const int4*>(ind + e);
-    const int4 i1 = *reinterpret_cast<const int4*>(ind + e + step);
-    float l[8];
-    Quad<F>::load(len, e, l);
-    Quad<F>::load(len, e + step, l + 4);
-    Vec<NB> xv[8];
-    xv[0] = Vec<NB>::load(X, i0.x);
-    xv[1] = Vec<NB>::load(X, i0.y);
-    xv[2] = Vec<NB>::load(X, i0.z);
-    xv[3] = Vec<NB>::load(X, i0.w);
-    xv[4] = Vec<NB>::load(X, i1.x);
-    xv[5] = Vec<NB>::load(X, i1.y);
-    xv[6] = Vec<NB>::load(X, i1.z);
-    xv[7] = Vec<NB>::load(X, i1.w);
+    int4 i0 = *reinterpret_cast<const int4*>(ind + e);
+    int4 i1 = *reinterpret_cast<const int4*>(ind + e + step);
+    Raw q0 = Quad<F>::load(len, e);
+    Raw q1 = Quad<F>::load(len, e + step);
+    for (;;)
+    {
+      Vec<NB> xv[8];
+      xv[0] = Vec<NB>::load(X, i0.x);
+      xv[1] = Vec<NB>::load(X, i0.y);
+      xv[2] = Vec<NB>::load(X, i0.z);
+      xv[3] = Vec<NB>::load(X, i0.w);
+      xv[4] = Vec<NB>::load(X, i1.x);
+      xv[5] = Vec<NB>::load(X, i1.y);
+      xv[6] = Vec<NB>::load(X, i1.z);
+      xv[7] = Vec<NB>::load(X, i1.w);
+      const long long en = e + 2 * step;
+      const bool more = en + step < e1;
+      int4 n0 = i0, n1 = i1;
+      Raw p0 = q0, p1 = q1;
+      if (more)
+      {
+        n0 = *reinterpret_cast<const int4*>(ind + en);
+        n1 = *reinterpret_cast<const int4*>(ind + en + step);
+        p0 = Quad<F>::load(len, en);
+        p1 = Quad<F>::load(len, en + step);
+      }
+      float l[8];
+      Quad<F>::dec(q0, l);
+      Quad<F>::dec(q1, l + 4);
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+      for (int u = 0; u < 8; ++u)
 #pragma unroll
-      for (int b = 0; b < NB; ++b) acc[b] = fmaf(l[u], xv[u].v[b], acc[b]);
+        for (int b = 0; b < NB; ++b) acc[b] = fmaf(l[u], xv[u].v[b], acc[b]);
+      e = en;
+      if (!more) break;
+      i0 = n0;
+      i1 = n1;
+      q0 = p0;
+      q1 = p1;
+    }
+  }
+  if (NB > 2 && e < e1)
+  {
+    int4 i0 = *reinterpret_cast<const int4*>(ind + e);
+    Raw q0 = Quad<F>::load(len, e);
+    for (;;)
+    {
+      Vec<NB> xv[4];
+      xv[0] = Vec<NB>::load(X, i0.x);
+      xv[1] = Vec<NB>::load(X, i0.y);
+      xv[2] = Vec<NB>::load(X, i0.z);
+      xv[3] = Vec<NB>::load(X, i0.w);
+      const long long en = e + step;
+      const bool more = en < e1;
+      int4 n0 = i0;
+      Raw p0 = q0;
+      if (more)
+      {
+        n0 = *reinterpret_cast<const int4*>(ind + en);
+        p0 = Quad<F>::load(len, en);
+      }
+      float l[4];
+      Quad<F>::dec(q0, l);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[b] = fmaf(l[u], xv[u].v[b], acc[b]);
+      e = en;
+      if (!more) break;
+      i0 = n0;
+      q0 = p0;
+    }
+    return;
   }
   for (; e < e1; e += step)
   {
     const int4 i0 = *reinterpret_cast<const int4*>(ind + e);
     float l[4];
-    Quad<F>::load(len, e, l);
+    Quad<F>::dec(Quad<F>::load(len, e), l);
     Vec<NB> xv[4];
     xv[0] = Vec<NB>::load(X, i0.x);
     xv[1] = Vec<NB>::load(X, i0.y);
