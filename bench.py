@@ -410,6 +410,35 @@ def main():
         del grp, ys
         ctx.set_stream(stream)
 
+    # --- list-mode events into the sinogram (crystal pairs -> bins) ------
+    n_ev = 100_000_000
+    gen = torch.Generator(device="cuda").manual_seed(w.seed + 2)
+    ncr, nr = w.proj.n_crystals_per_ring, w.proj.n_rings
+    c1 = torch.randint(0, ncr, (n_ev,), generator=gen, device="cuda",
+                       dtype=torch.int32)
+    sep = torch.randint(-(w.proj.n_tang_coords // 2),
+                        w.proj.n_tang_coords // 2 + 1, (n_ev,),
+                        generator=gen, device="cuda", dtype=torch.int32)
+    events = torch.stack([
+        torch.randint(0, nr, (n_ev,), generator=gen, device="cuda",
+                      dtype=torch.int32), c1,
+        torch.randint(0, nr, (n_ev,), generator=gen, device="cuda",
+                      dtype=torch.int32),
+        (c1 + ncr // 2 + sep) % ncr], 1).contiguous()
+    del c1, sep
+    hist = torch.zeros(ctx.n_bins, device="cuda")
+    kept = ctx.histogram_events(events, hist, count=True)
+    lm_ms = ev_time(lambda: ctx.histogram_events(events, hist))
+    listmode = {
+        "events": n_ev, "kept": kept, "ms": round(lm_ms, 4),
+        "m_events_s": round(n_ev / (lm_ms * 1e-3) / 1e6, 1),
+        "events_gb_s": round(16 * n_ev / (lm_ms * 1e-3) / 1e9, 1),
+        "source": "uniform ring pairs, crystal pairs within the stored "
+                  "tangential range"}
+    log(f"list-mode histogram: {n_ev} events in {lm_ms:.3f} ms "
+        f"({listmode['m_events_s']} M events/s)")
+    del events, hist
+
     # --- roofline of the dominant kernel pair (cached OSEM projector) ----
     # Algorithmic bytes per launch (one sub-iteration's LORs): forward +
     # backprojection in the cached-path format, 12 B per LOR and 6 B per
@@ -486,6 +515,7 @@ def main():
             "algorithmic_bytes_per_launch": int(bytes_launch),
         },
         "batched": batched,
+        "listmode": listmode,
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -38,7 +38,7 @@ def _run(cmd):
 
 def build_hip(verbose=True):
     srcs = [os.path.join(HERE, "csrc", f)
-            for f in ("fir_hip.hip", "path_cache.hip")]
+            for f in ("fir_hip.hip", "path_cache.hip", "listmode.hip")]
     out = os.path.join(HERE, "libfir_hip.so")
     tmp = out + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])

@@ -5,6 +5,12 @@
 //   scanner IN.scan     print the crystal/slice tables (%.17g)
 //   geom R C SPAN SEG T print ProjGeometry::fill results, or the error
 //   lors IN.hs NSUB     print every LOR of the cache, in order
+//   bins IN.hs OUT      ProjData::getBinCoordinates of every (ring1,
+//                       crystal1, ring2, crystal2), rings and crystals
+//                       ascending in that nesting: int32 records (ok, seg,
+//                       view, axial, tang)
+//   rebin IN.hs OUT.hs  read a sinogram, ProjData::rebinWeight, write it
+//                       (needs a GPU)
 //   key TEXT            print the standardised keyword
 //   params FILE         parse an OSEM parameter file and print the values
 #include <KeyParser.h>
@@ -19,6 +25,7 @@
 #include <cstdlib>
 #include <iostream>
 #include <string>
+#include <vector>
 
 int main(int argc, char** argv)
 {
@@ -64,6 +71,31 @@ int main(int argc, char** argv)
                   g.maxRingDiff, h.nTangCoords);
       for (int n : g.nAxialCoords) std::printf(" %d", n);
       std::printf("\n");
+    }
+    else if (mode == "bins" && argc >= 4)
+    {
+      ProjData p(argv[2], ProjData::ConstructionMode::ALLOCATE);
+      const auto& h = p.getHeader();
+      std::vector<int> out;
+      for (int r1 = 0; r1 < h.nRings; ++r1)
+        for (int c1 = 0; c1 < h.nCrystalsPerRing; ++c1)
+          for (int r2 = 0; r2 < h.nRings; ++r2)
+            for (int c2 = 0; c2 < h.nCrystalsPerRing; ++c2)
+            {
+              int s = 0, v = 0, a = 0, t = 0;
+              const bool ok = p.getBinCoordinates(r1, c1, r2, c2, &s, &v, &a, &t);
+              out.insert(out.end(), {ok ? 1 : 0, s, v, a, t});
+            }
+      FILE* f = std::fopen(argv[3], "wb");
+      if (!f) error("cannot write ", argv[3]);
+      std::fwrite(out.data(), 4, out.size(), f);
+      std::fclose(f);
+    }
+    else if (mode == "rebin" && argc >= 4)
+    {
+      ProjData p(argv[2], ProjData::ConstructionMode::READ_DATA);
+      p.rebinWeight();
+      p.write(argv[3]);
     }
     else if (mode == "lors" && argc >= 4)
     {

@@ -53,6 +53,19 @@ public:
   // (reference ProjData.cc:301-340).
   std::tuple<int, int> getCrystalAngCoord(int view, int tangCoord) const;
 
+  // Sinogram coordinates of the LOR between crystal (crystalAxialCoord1 =
+  // ring, crystalAngCoord1) and crystal (ring2, crystal2); false if the
+  // sinogram does not hold it (reference ProjData.cc:342-479).
+  bool getBinCoordinates(int crystalAxialCoord1, int crystalAngCoord1,
+                         int crystalAxialCoord2, int crystalAngCoord2,
+                         int* seg, int* view, int* axialCoord,
+                         int* tangCoord) const;
+
+  // Span > 1: weight every bin by 1 / (ring pairs fused into its
+  // (segment, axial) position) where that is above 1 (reference
+  // ProjData.cc:541-615), on the GPU.
+  void rebinWeight();
+
   // Clamped product and ACF exponential, run on the GPU.
   ProjData& operator*=(const ProjData& inputProj);
   void exponential();

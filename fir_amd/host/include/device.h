@@ -47,6 +47,8 @@ fir_vol_desc describe(const VolHeader& header);
 // Make the context's scanner tables and sinogram geometry those of
 // (scanner, proj); no-op if they already are.
 void useGeometry(const ScannerData& scanner, const ProjHeader& proj);
+// The same for the sinogram geometry alone.
+void useProj(const ProjHeader& proj);
 
 enum class ClampedOp
 {
@@ -58,4 +60,6 @@ enum class ClampedOp
 void clampedOp(ClampedOp op, float* a, const float* b, long long n);
 void expClamped(float* a, long long n);
 void huToMu(float* a, long long n);
+// ProjData::rebinWeight of a whole sinogram (n bins of geometry proj).
+void rebinWeight(const ProjHeader& proj, float* a, long long n);
 }

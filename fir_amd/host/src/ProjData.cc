@@ -179,6 +179,62 @@ std::tuple<int, int> ProjData::getCrystalAngCoord(int view, int t) const
   return {a, b};
 }
 
+bool ProjData::getBinCoordinates(int r1, int c1, int r2, int c2, int* seg,
+                                 int* view, int* axialCoord,
+                                 int* tangCoord) const
+{
+  const int n = mHeader.nCrystalsPerRing;
+  const int d = r1 > r2 ? r1 - r2 : r2 - r1;
+  if (d > mGeometry.maxRingDiff) return false;
+  // tangential: signed distance of the crystal separation from n/2, the
+  // sign given by the half-turn the crystal sum lies in
+  const int sum = c1 + c2;
+  const int s1 = (sum >= n / 2 && sum < 3 * n / 2) ? -1 : 1;
+  const int t = s1 * ((c2 > c1 ? c2 - c1 : c1 - c2) - n / 2);
+  if (t < -mGeometry.tangCoordOffset ||
+      t >= mHeader.nTangCoords - mGeometry.tangCoordOffset)
+    return false;
+  // axial: lower ring (span 1), or ring sum minus the first ring sum of
+  // the fused segment
+  int m, ax;
+  const int half = mGeometry.halfSegmentSpan;
+  if (mHeader.segmentSpan == 1)
+  {
+    m = d;
+    ax = (r1 + r2 - d) / 2;
+  }
+  else
+  {
+    m = d <= half ? 0 : 1 + (d - half - 1) / mHeader.segmentSpan;
+    ax = r1 + r2 - (m == 0 ? 0 : 1 + half + (m - 1) * mHeader.segmentSpan);
+  }
+  int s = m;
+  if (m != 0)
+  {
+    // sign: both crystals moved to the parallel LOR through the centre
+    // (tangential 0 or 1), compared, and the rings compared
+    const int s2 = c1 < c2 ? 1 : -1;
+    const int u = (t % 2 == 0) ? -t / 2 : -(t - 1) / 2;
+    int k1 = c1 - s1 * s2 * u;
+    int k2 = c2 + s1 * s2 * u;
+    k1 += k1 >= n ? -n : (k1 < 0 ? n : 0);
+    k2 += k2 >= n ? -n : (k2 < 0 ? n : 0);
+    s = m * (r1 < r2 ? -1 : 1) * (k1 < k2 ? 1 : -1);
+  }
+  *seg = s;
+  *view = ((sum + n / 2) % n) / 2;
+  *axialCoord = ax;
+  *tangCoord = t;
+  return true;
+}
+
+void ProjData::rebinWeight()
+{
+  if (mStorage.empty()) error("Projection not allocated");
+  device::rebinWeight(mHeader, mStorage.data(),
+                      static_cast<long long>(mStorage.size()));
+}
+
 ProjData& ProjData::operator*=(const ProjData& in)
 {
   if (mHeader != in.mHeader) error("Projections do not have the same size");

@@ -126,6 +126,13 @@ void useGeometry(const ScannerData& scanner, const ProjHeader& proj)
     s.xy = xy;
     s.z = z;
   }
+  useProj(proj);
+}
+
+void useProj(const ProjHeader& proj)
+{
+  State& s = state();
+  fir_ctx* c = context();
   if (!s.projSet || s.proj != proj)
   {
     check(fir_set_proj(c, proj.nRings, proj.nCrystalsPerRing,
@@ -154,6 +161,17 @@ void expClamped(float* a, long long n)
   Buffer da(bytes);
   da.upload(a, bytes);
   check(fir_exp_clamped(context(), da.f(), n));
+  da.download(a, bytes);
+}
+
+void rebinWeight(const ProjHeader& proj, float* a, long long n)
+{
+  if (n <= 0 || proj.segmentSpan == 1) return;
+  useProj(proj);
+  const size_t bytes = sizeof(float) * static_cast<size_t>(n);
+  Buffer da(bytes);
+  da.upload(a, bytes);
+  check(fir_rebin_weight(context(), da.f()));
   da.download(a, bytes);
 }
 

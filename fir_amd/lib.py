@@ -47,6 +47,7 @@ EXPORTS = (
     "fir_cache_info",
     "fir_cache_tune", "fir_cache_forward", "fir_cache_backward",
     "fir_cache_sensitivity", "fir_cache_osem_project",
+    "fir_histogram_events", "fir_rebin_weight",
 )
 
 
@@ -117,6 +118,8 @@ def load() -> C.CDLL:
         "fir_hu_to_mu": [vp, vp, i64],
         "fir_path_dump": [vp, P, i64, i64, i, vp, vp, vp],
         "fir_path_count": [vp, P, i, i, C.POINTER(C.c_int64)],
+        "fir_histogram_events": [vp, vp, i64, vp, vp, C.POINTER(C.c_int64)],
+        "fir_rebin_weight": [vp, vp],
         "fir_poisson_nll_grad": [vp, P, vp, vp, vp, vp, vp],
         "fir_osem_project_batch": [vp, P, i, vp, vp, vp, vp, i, i],
         "fir_cache_create": [vp, P, i, i, C.POINTER(C.c_void_p)],
@@ -316,6 +319,24 @@ class Context:
         _check(self.lib.fir_path_count(self.h, C.byref(vol), subset,
                                        n_subsets, C.byref(out)))
         return int(out.value)
+
+    def histogram_events(self, events, sino, weights=None,
+                         count: bool = False) -> Optional[int]:
+        """sino += histogram of list-mode events (int32 [n, 4]: ring1,
+        crystal1, ring2, crystal2) (ProjData::getBinCoordinates +
+        incrementBin); with count=True returns the number of events kept."""
+        if events.dtype.itemsize != 4 or events.dim() != 2 or \
+                events.shape[1] != 4:
+            raise FirError("events must be int32 [n, 4]")
+        out = C.c_int64() if count else None
+        _check(self.lib.fir_histogram_events(
+            self.h, _ptr(events), events.shape[0], _ptr(weights), _ptr(sino),
+            C.byref(out) if count else None))
+        return int(out.value) if count else None
+
+    def rebin_weight(self, sino):
+        """Span > 1 fused-bin weights, in place (ProjData::rebinWeight)."""
+        _check(self.lib.fir_rebin_weight(self.h, _ptr(sino)))
 
     def path_dump(self, vol, bin0, n_bins, max_per_lor, counts, idx, ln):
         _check(self.lib.fir_path_dump(self.h, C.byref(vol), bin0, n_bins,

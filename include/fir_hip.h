@@ -263,6 +263,27 @@ int fir_path_dump(fir_ctx* ctx, const fir_vol_desc* vol, int64_t bin0,
 int fir_path_count(fir_ctx* ctx, const fir_vol_desc* vol, int subset,
                    int n_subsets, int64_t* total);
 
+/* ---- list-mode events and span weighting ----------------------------- */
+
+/* List-mode events into sinogram counts: ProjData::getBinCoordinates
+ * (ProjData.cc:342-479) followed by ProjData::incrementBin
+ * (ProjData.inl:47-56), for every event at once. events: n_events x 4
+ * int32 on the device (ring1, crystal1, ring2, crystal2), 16-byte aligned.
+ * sino[bin] += weights ? weights[i] : 1 for every event whose crystal pair
+ * the sinogram holds; events with rings or crystals out of range, a ring
+ * difference beyond the last segment or a tangential coordinate outside
+ * the stored range are skipped. sino is not cleared. If n_counted is not
+ * NULL it receives the number of events counted (the call then waits for
+ * the context's stream). */
+int fir_histogram_events(fir_ctx* ctx, const int* events, int64_t n_events,
+                         const float* weights, float* sino,
+                         int64_t* n_counted);
+
+/* ProjData::rebinWeight (ProjData.cc:541-615) on a device sinogram: with a
+ * segment span above 1, every bin of a (segment, axial) position that fuses
+ * k > 1 ring pairs is multiplied by (float)(1.0 / k). No-op for span 1. */
+int fir_rebin_weight(fir_ctx* ctx, float* sino);
+
 /* ---- path cache ------------------------------------------------------ */
 
 /* Every LOR's Siddon path, walked once and kept in device memory: the

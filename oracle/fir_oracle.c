@@ -32,6 +32,11 @@
  *                        (ProjHeaderUnitTest.cc:181-232).
  *   orc_crystal_*        closed forms (separation = nViews - |t|) and golden
  *                        LOR tables.
+ *   orc_bin_coords       round trip with orc_crystal_* over every bin of
+ *                        span-1 and span-3 geometries; the closed forms of
+ *                        the tangential coordinate and the view.
+ *   orc_rebin_weight     brute-force count of the ring pairs that
+ *                        orc_bin_coords fuses into each (segment, axial).
  *   orc_path             the 28 reference unit-test paths
  *                        (SiddonUnitTest.cc:80-423), chord-length closed
  *                        form, brute-force crossing enumeration on tiny
@@ -213,6 +218,153 @@ void orc_crystal_ang(const orc_proj* p, int view, int tang, int* c1, int* c2)
   else if (b >= n) b -= n;
   *c1 = a;
   *c2 = b;
+}
+
+/* Crystal pair -> sinogram bin coordinates (reference
+ * ProjData::getBinCoordinates, ProjData.cc:342-479). r = ring (axial
+ * crystal index), c = angular crystal index. Returns 0 when the pair lies
+ * outside the sinogram (ring difference above the largest one stored, or
+ * tangential coordinate outside the stored range), else 1 with
+ * (seg, view, axial, tang). C integer division truncates toward zero, as
+ * C++'s does. */
+int orc_bin_coords(const orc_proj* p, int r1, int c1, int r2, int c2,
+                   int* seg, int* view, int* axial, int* tang)
+{
+  const int n = p->n_cpr;
+  int aseg = r1 > r2 ? r1 - r2 : r2 - r1;
+  if (aseg > p->max_ring_diff) return 0;
+  /* tangential: the crystal sum decides which half-turn the LOR is in */
+  const int sum = c1 + c2;
+  const int s1 = (sum >= n / 2 && sum < 3 * n / 2) ? -1 : 1;
+  const int dc = c2 > c1 ? c2 - c1 : c1 - c2;
+  const int t = s1 * (dc - n / 2);
+  if (t < -p->tang_off || t >= -p->tang_off + p->n_tang) return 0;
+  /* axial: span 1 = lower ring; span > 1: ring sum minus the first ring
+   * sum of the (fused) segment */
+  int ax;
+  if (p->span == 1)
+    ax = (r1 + r2 - aseg) / 2;
+  else
+  {
+    const int central = aseg <= p->half_span;
+    aseg = central ? 0 : 1 + (aseg - p->half_span - 1) / p->span;
+    const int first = central ? 0 : 1 + p->half_span + (aseg - 1) * p->span;
+    ax = r1 + r2 - first;
+  }
+  const int v = ((sum + n / 2) % n) / 2;
+  int sg = aseg;
+  if (sg != 0)
+  {
+    /* the sign follows from the pair's crystals moved to the parallel LOR
+     * through the centre (tangential 0 or 1) */
+    const int s2 = c1 < c2 ? 1 : -1;
+    const int at = t < 0 ? -t : t;
+    const int u = at % 2 == 0 ? -t / 2 : -(t - 1) / 2;
+    int k1 = c1 - s1 * s2 * u;
+    int k2 = c2 + s1 * s2 * u;
+    if (k1 >= n) k1 -= n;
+    else if (k1 < 0) k1 += n;
+    if (k2 >= n) k2 -= n;
+    else if (k2 < 0) k2 += n;
+    const int ring_sign = r1 < r2 ? -1 : 1;
+    const int flip = k1 < k2 ? 1 : -1;
+    sg = sg * ring_sign * flip;
+  }
+  *seg = sg;
+  *view = v;
+  *axial = ax;
+  *tang = t;
+  return 1;
+}
+
+/* Flat bin index of (seg, view, axial, tang) in file order, or -1 when the
+ * coordinates are outside the sinogram. */
+long long orc_bin_index(const orc_proj* p, int seg, int view, int axial,
+                        int tang)
+{
+  const int k = seg + p->seg_off;
+  if (k < 0 || k >= p->n_seg || view < 0 || view >= p->n_views) return -1;
+  if (axial < 0 || axial >= p->n_axial[k]) return -1;
+  const int t = tang + p->tang_off;
+  if (t < 0 || t >= p->n_tang) return -1;
+  return p->seg_base[k] +
+    ((long long)view * p->n_axial[k] + axial) * p->n_tang + t;
+}
+
+/* Histogram of list-mode events (4 ints each: r1, c1, r2, c2) into the
+ * sinogram, one count (or weight w[i]) per event whose pair maps to a bin
+ * (getBinCoordinates + incrementBin, ProjData.cc:342-479, ProjData.inl:47-56).
+ * Events with crystal indices out of range are skipped. Returns the number
+ * of events counted. */
+long long orc_histogram_events(const orc_proj* p, const int* ev, long long n,
+                               const float* w, float* sino)
+{
+  long long counted = 0;
+  for (long long i = 0; i < n; ++i)
+  {
+    const int* e = ev + 4 * i;
+    if (e[0] < 0 || e[0] >= p->n_rings || e[2] < 0 || e[2] >= p->n_rings ||
+        e[1] < 0 || e[1] >= p->n_cpr || e[3] < 0 || e[3] >= p->n_cpr)
+      continue;
+    int sg, v, ax, t;
+    if (!orc_bin_coords(p, e[0], e[1], e[2], e[3], &sg, &v, &ax, &t))
+      continue;
+    const long long b = orc_bin_index(p, sg, v, ax, t);
+    if (b < 0) continue;
+    sino[b] += w ? w[i] : 1.0f;
+    ++counted;
+  }
+  return counted;
+}
+
+/* Span > 1: every bin of a (segment, axial) position that fuses k > 1 ring
+ * pairs is multiplied by (float)(1.0 / k) (reference ProjData::rebinWeight,
+ * ProjData.cc:541-615, walking ring sums and, within each, ring differences
+ * of its parity segment by segment). Bins are addressed with the correct
+ * per-segment axial count (the reference's ProjData::getInd,
+ * ProjData.inl:67-82, offsets the segment twice). No-op for span 1. */
+void orc_rebin_weight(const orc_proj* p, float* sino)
+{
+  if (p->span == 1) return;
+  const int nr = p->n_rings;
+  for (int rs = 0; rs <= 2 * (nr - 1); ++rs)
+  {
+    int lmax = rs <= nr - 1 ? rs : 2 * (nr - 1) - rs;
+    if (lmax > p->max_ring_diff) lmax = p->max_ring_diff;
+    if ((rs + lmax) % 2) --lmax;
+    int sg = -p->seg_off - 1;
+    int next = -p->max_ring_diff - 1;
+    while (next < -lmax)
+    {
+      ++sg;
+      next += p->span;
+    }
+    int k = 1;
+    for (int d = -lmax; d <= lmax; d += 2)
+    {
+      if (d == lmax || d + 1 >= next)
+      {
+        if (k > 1)
+        {
+          const int asg = sg < 0 ? -sg : sg;
+          const int ax = sg == 0 ? rs
+                                 : rs - (p->span + 1) / 2 - (asg - 1) * p->span;
+          const float wt = (float)(1.0 / k);
+          for (int v = 0; v < p->n_views; ++v)
+            for (int t = -p->tang_off; t < -p->tang_off + p->n_tang; ++t)
+            {
+              const long long b = orc_bin_index(p, sg, v, ax, t);
+              if (b >= 0) sino[b] *= wt;
+            }
+        }
+        k = 1;
+        ++sg;
+        next += p->span;
+      }
+      else
+        ++k;
+    }
+  }
 }
 
 /* ------------------------------------------------------------------------ */

@@ -96,6 +96,44 @@ def crystal_ang(p: OrcProj, view, tang):
     return a.value, b.value
 
 
+def bin_coords(p: OrcProj, r1, c1, r2, c2):
+    """(seg, view, axial, tang) of a crystal pair, or None if outside."""
+    sg, v, a, t = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    ok = lib().orc_bin_coords(C.byref(p), int(r1), int(c1), int(r2), int(c2),
+                              C.byref(sg),
+                              C.byref(v), C.byref(a), C.byref(t))
+    return (sg.value, v.value, a.value, t.value) if ok else None
+
+
+def bin_index(p: OrcProj, seg, view, axial, tang):
+    lib().orc_bin_index.restype = C.c_longlong
+    return lib().orc_bin_index(C.byref(p), int(seg), int(view), int(axial),
+                               int(tang))
+
+
+def histogram_events(proj, events, weights=None, sino=None):
+    """Sinogram counts of list-mode events (int32 [n, 4]: r1, c1, r2, c2);
+    returns (sino float32, number counted)."""
+    p = proj if isinstance(proj, OrcProj) else proj_desc(proj)
+    ev = np.ascontiguousarray(events, np.int32)
+    if sino is None:
+        sino = np.zeros(p.n_bins, np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    lib().orc_histogram_events.restype = C.c_longlong
+    n = lib().orc_histogram_events(C.byref(p), _p(ev),
+                                   C.c_longlong(ev.shape[0]),
+                                   None if w is None else _p(w), _p(sino))
+    return sino, int(n)
+
+
+def rebin_weight(proj, sino):
+    """In place: span > 1 fused-bin weights (ProjData::rebinWeight)."""
+    p = proj if isinstance(proj, OrcProj) else proj_desc(proj)
+    assert sino.dtype == np.float32 and sino.flags.c_contiguous
+    lib().orc_rebin_weight(C.byref(p), _p(sino))
+    return sino
+
+
 def path(vol, p1, p2, geometric=False):
     """Siddon path of one segment: (idx int32[n], len float64[n]) or None
     for a miss."""

@@ -16,6 +16,14 @@
 //       int32 idx[total], float64 len[total].
 //   tables scanner.scan out.bin
 //       int32 nCrystalsPerRing, int32 nSlices, float64 xy[2n], float64 z[s].
+//   bins proj.hs out.bin
+//       ProjData::getBinCoordinates of every (ring1, crystal1, ring2,
+//       crystal2), rings and crystals ascending in that nesting: int32
+//       records (ok, seg, view, axial, tang).
+//   rebin proj.hs out.bin
+//       ProjData::rebinWeight of a sinogram of ones (single-segment
+//       headers only: the reference's bin addressing offsets the segment
+//       twice, ProjData.inl:67-82); float32 bins in file order.
 //   time op vol.h33 scanner.scan proj.hs nSubsets reps
 //       op in {fwd, bp, sens, subiter}; uniform image and sinogram of ones;
 //       prints one JSON line with the median seconds over reps runs after a
@@ -126,6 +134,45 @@ int paths(int argc, char** a)
   return 0;
 }
 
+int bins(char** a)
+{
+  ProjData proj(a[0], ProjData::ConstructionMode::ALLOCATE);
+  const auto& ph = proj.getHeader();
+  std::vector<int> out;
+  for (int r1 = 0; r1 < ph.nRings; ++r1)
+    for (int c1 = 0; c1 < ph.nCrystalsPerRing; ++c1)
+      for (int r2 = 0; r2 < ph.nRings; ++r2)
+        for (int c2 = 0; c2 < ph.nCrystalsPerRing; ++c2)
+        {
+          int seg = 0, view = 0, ax = 0, t = 0;
+          const bool ok =
+            proj.getBinCoordinates(r1, c1, r2, c2, &seg, &view, &ax, &t);
+          out.insert(out.end(), {ok ? 1 : 0, seg, view, ax, t});
+        }
+  FILE* f = std::fopen(a[1], "wb");
+  if (!f) return 2;
+  std::fwrite(out.data(), 4, out.size(), f);
+  std::fclose(f);
+  return 0;
+}
+
+int rebin(char** a)
+{
+  ProjData proj(a[0], ProjData::ConstructionMode::INITIALIZE, 1.0f);
+  const auto& g = proj.getGeometry();
+  if (g.segOffset != 0)
+  {
+    std::fprintf(stderr, "rebin: single-segment headers only\n");
+    return 3;
+  }
+  proj.rebinWeight();
+  FILE* f = std::fopen(a[1], "wb");
+  if (!f) return 2;
+  std::fwrite(proj.getDataArray()[0], 4, g.nBins, f);
+  std::fclose(f);
+  return 0;
+}
+
 int tables(char** a)
 {
   ScannerData scanner(a[0]);
@@ -218,6 +265,8 @@ int main(int argc, char** argv)
     if (mode == "segpath" && argc >= 17) return segpath(argv + 2);
     if (mode == "paths" && argc >= 6) return paths(argc - 2, argv + 2);
     if (mode == "tables" && argc >= 6) return tables(argv + 2);
+    if (mode == "bins" && argc >= 4) return bins(argv + 2);
+    if (mode == "rebin" && argc >= 4) return rebin(argv + 2);
     if (mode == "time" && argc >= 8) return timeit(argv + 2);
     std::fprintf(stderr, "bad arguments\n");
     return 1;

@@ -22,6 +22,11 @@ Outputs (all data: inputs and the reference's outputs):
   scanners (c1, c2 and a multi-module scanner with gaps).
 * ``ref_headers.json`` -- the text of .h33/.hs headers written by the
   reference executables (byte-compatibility of the writers).
+* ``bins_golden.npz`` -- ProjData::getBinCoordinates of a random sample of
+  crystal pairs of a span-1 (4 x 128, 7 segments) and a span-3 (8 x 64,
+  5 segments) sinogram, and ProjData::rebinWeight of sinograms of ones
+  with one segment at spans 3 and 5 (``python make_golden.py bins`` writes
+  this file alone).
 """
 from __future__ import annotations
 
@@ -284,5 +289,63 @@ def main():
     print("golden fixtures written to", HERE)
 
 
+BIN_GEOMS = {
+    "span1": dict(n_rings=4, n_crystals_per_ring=128, n_segments=7,
+                  n_tang_coords=65),
+    "span3": dict(n_rings=8, n_crystals_per_ring=64, n_segments=5,
+                  n_tang_coords=49, segment_span=3),
+}
+REBIN_GEOMS = {
+    "rebin3": dict(n_rings=8, n_crystals_per_ring=64, n_segments=1,
+                   n_tang_coords=31, segment_span=3),
+    "rebin5": dict(n_rings=6, n_crystals_per_ring=32, n_segments=1,
+                   n_tang_coords=15, segment_span=5),
+}
+
+
+def bins_fixture():
+    """bins_golden.npz from ref_driver bins / rebin."""
+    rng = np.random.default_rng(11)
+    arrays = {}
+    with tempfile.TemporaryDirectory() as d:
+        for name, kw in BIN_GEOMS.items():
+            spec = io.ProjSpec(**kw)
+            spec.write(os.path.join(d, f"{name}.hs"))
+            run([os.path.join(REF, "ref_driver"), "bins", f"{name}.hs",
+                 f"{name}.bin"], d)
+            rec = np.fromfile(os.path.join(d, f"{name}.bin"),
+                              np.int32).reshape(-1, 5)
+            nr, nc = spec.n_rings, spec.n_crystals_per_ring
+            pick = np.sort(rng.choice(rec.shape[0], 6000, replace=False))
+            r1, rest = np.divmod(pick, nc * nr * nc)
+            c1, rest = np.divmod(rest, nr * nc)
+            r2, c2 = np.divmod(rest, nc)
+            arrays[f"{name}_pairs"] = np.stack([r1, c1, r2, c2], 1).astype(
+                np.int16)
+            arrays[f"{name}_bins"] = rec[pick].astype(np.int16)
+            arrays[f"{name}_accepted"] = np.int64(rec[:, 0].sum())
+            arrays[f"{name}_hdr"] = np.array(
+                [kw["n_rings"], kw["n_crystals_per_ring"],
+                 kw.get("segment_span", 1), kw["n_segments"],
+                 kw["n_tang_coords"]], np.int32)
+        for name, kw in REBIN_GEOMS.items():
+            spec = io.ProjSpec(**kw)
+            spec.write(os.path.join(d, f"{name}.hs"))
+            run([os.path.join(REF, "ref_driver"), "rebin", f"{name}.hs",
+                 f"{name}.bin"], d)
+            w = np.fromfile(os.path.join(d, f"{name}.bin"), np.float32)
+            arrays[f"{name}_bins"] = w
+            arrays[f"{name}_hdr"] = np.array(
+                [kw["n_rings"], kw["n_crystals_per_ring"],
+                 kw["segment_span"], kw["n_segments"], kw["n_tang_coords"]],
+                np.int32)
+    np.savez_compressed(os.path.join(HERE, "bins_golden.npz"), **arrays)
+    print("bins_golden.npz written")
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "bins":
+        bins_fixture()
+    else:
+        main()
+        bins_fixture()

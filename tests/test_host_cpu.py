@@ -153,6 +153,35 @@ def test_lor_cache_order(tmp_path):
     assert k == len(rows)
 
 
+@pytest.mark.parametrize("geom", [
+    dict(n_rings=4, n_crystals_per_ring=128, n_segments=7, n_tang_coords=65),
+    dict(n_rings=8, n_crystals_per_ring=64, n_segments=5, n_tang_coords=49,
+         segment_span=3),
+    dict(n_rings=8, n_crystals_per_ring=32, n_segments=3, n_tang_coords=15,
+         segment_span=5)])
+def test_bin_coordinates_match_oracle(tmp_path, geom):
+    """ProjData::getBinCoordinates of the host layer equals the oracle's
+    (itself pinned to the compiled reference) for every crystal pair."""
+    spec = io.ProjSpec(**geom)
+    spec.write(str(tmp_path / "p.hs"))
+    run("bins", tmp_path / "p.hs", tmp_path / "b.bin")
+    got = np.fromfile(tmp_path / "b.bin", np.int32).reshape(-1, 5)
+    p = O.proj_desc(spec)
+    nr, nc = spec.n_rings, spec.n_crystals_per_ring
+    assert got.shape[0] == (nr * nc) ** 2
+    k = 0
+    for r1 in range(nr):
+        for c1 in range(nc):
+            for r2 in range(nr):
+                for c2 in range(nc):
+                    b = O.bin_coords(p, r1, c1, r2, c2)
+                    g = got[k]
+                    k += 1
+                    assert (b is None) == (g[0] == 0)
+                    if b is not None:
+                        assert tuple(g[1:]) == b
+
+
 @pytest.mark.parametrize("raw,std", [
     ("!Number_of   Rings", "number of rings"),
     ("  matrix size[1] ", "matrix size [1]"),

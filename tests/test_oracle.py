@@ -129,6 +129,131 @@ def test_crystal_mapping_closed_form():
             assert lo == 2 * ax and hi == 2 * (ax + abs(seg))
 
 
+@pytest.fixture(scope="module")
+def bins_golden():
+    return np.load(os.path.join(os.path.dirname(__file__), "golden",
+                                "bins_golden.npz"))
+
+
+def _proj_from_hdr(h):
+    return O.proj_fill(int(h[0]), int(h[1]), int(h[2]), int(h[3]),
+                       int(h[4]))[1]
+
+
+@pytest.mark.parametrize("name", ["span1", "span3"])
+def test_bin_coords_golden(bins_golden, name):
+    """getBinCoordinates of the compiled reference (ProjData.cc:342-479) on
+    6000 random crystal pairs, rejected pairs included."""
+    p = _proj_from_hdr(bins_golden[f"{name}_hdr"])
+    pairs = bins_golden[f"{name}_pairs"].astype(int)
+    ref = bins_golden[f"{name}_bins"].astype(int)
+    assert ref[:, 0].sum() > 1000 and (ref[:, 0] == 0).sum() > 100
+    for (r1, c1, r2, c2), exp in zip(pairs, ref):
+        got = O.bin_coords(p, r1, c1, r2, c2)
+        if exp[0]:
+            assert got == tuple(exp[1:]), (r1, c1, r2, c2)
+        else:
+            assert got is None, (r1, c1, r2, c2)
+
+
+def test_bin_coords_round_trip_span1():
+    """Every bin's crystals (getCrystalAxialCoord/AngCoord) map back to the
+    bin, in either crystal order."""
+    p = O.proj_desc(synth.c1().proj)
+    for seg in range(-p.seg_off, p.seg_off + 1):
+        for ax in range(p.n_axial[seg + p.seg_off]):
+            s1, s2 = O.crystal_axial(p, seg, ax)
+            for view in range(0, p.n_views, 3):
+                for t in range(-p.tang_off, -p.tang_off + p.n_tang):
+                    c1, c2 = O.crystal_ang(p, view, t)
+                    exp = (seg, view, ax, t)
+                    assert O.bin_coords(p, s1 // 2, c1, s2 // 2, c2) == exp
+                    assert O.bin_coords(p, s2 // 2, c2, s1 // 2, c1) == exp
+
+
+def test_bin_coords_cover_span3():
+    """Span 3: every crystal pair within the stored ring differences and
+    tangential range lands on a valid bin, every bin receives pairs, and
+    the tangential coordinate and view follow their closed forms."""
+    p = O.proj_fill(8, 64, 3, 5, 49)[1]
+    nc = p.n_cpr
+    hit = np.zeros(p.n_bins, np.int64)
+    for r1 in range(p.n_rings):
+        for r2 in range(p.n_rings):
+            for c1 in range(nc):
+                for c2 in range(nc):
+                    b = O.bin_coords(p, r1, c1, r2, c2)
+                    if b is None:
+                        continue
+                    seg, view, ax, t = b
+                    i = O.bin_index(p, seg, view, ax, t)
+                    assert i >= 0, (r1, c1, r2, c2, b)
+                    hit[i] += 1
+                    assert (c1 + c2 + nc // 2) % nc // 2 == view
+                    assert abs(t) == abs(abs(c2 - c1) - nc // 2)
+    assert hit.min() >= 1
+
+
+@pytest.mark.parametrize("name", ["rebin3", "rebin5"])
+def test_rebin_weight_golden(bins_golden, name):
+    """rebinWeight of the compiled reference on one-segment sinograms of
+    ones (ProjData.cc:541-615), bit for bit."""
+    p = _proj_from_hdr(bins_golden[f"{name}_hdr"])
+    sino = O.rebin_weight(p, np.ones(p.n_bins, np.float32))
+    assert np.array_equal(sino, bins_golden[f"{name}_bins"])
+
+
+@pytest.mark.parametrize("geom", [(8, 64, 3, 5, 49), (8, 64, 3, 3, 49),
+                                  (8, 32, 5, 3, 15)])
+def test_rebin_weight_brute_force(geom):
+    """Span 3 and 5, with every ring difference stored or not: each
+    (segment, axial) bin is weighted by 1 / (number of ordered ring pairs
+    getBinCoordinates fuses into it)."""
+    p = O.proj_fill(*geom)[1]
+    count = {}
+    for r1 in range(p.n_rings):
+        for r2 in range(p.n_rings):
+            b = O.bin_coords(p, r1, 0, r2, p.n_cpr // 2)
+            if b is not None:
+                count[(b[0], b[2])] = count.get((b[0], b[2]), 0) + 1
+    sino = O.rebin_weight(p, np.ones(p.n_bins, np.float32))
+    for seg in range(-p.seg_off, p.seg_off + 1):
+        for ax in range(p.n_axial[seg + p.seg_off]):
+            k = count.get((seg, ax), 0)
+            assert k > 0
+            exp = np.float32(1.0 / k) if k > 1 else np.float32(1.0)
+            for view in (0, 7, p.n_views - 1):
+                for t in (-p.tang_off, 0, -p.tang_off + p.n_tang - 1):
+                    assert sino[O.bin_index(p, seg, view, ax, t)] == exp
+
+
+def test_histogram_events_counts():
+    """Histogram of events = sum over events of the one-hot of their bin;
+    rejected events (outside rings, crystals, ring difference or
+    tangential range) count nothing; weights scale counts."""
+    p = O.proj_fill(8, 64, 3, 5, 49)[1]
+    rng = np.random.default_rng(4)
+    ev = np.stack([rng.integers(-1, 9, 5000), rng.integers(-1, 65, 5000),
+                   rng.integers(-1, 9, 5000), rng.integers(-1, 65, 5000)],
+                  1).astype(np.int32)
+    sino, n = O.histogram_events(p, ev)
+    exp = np.zeros(p.n_bins, np.float32)
+    used = np.zeros(len(ev), bool)
+    for i, (r1, c1, r2, c2) in enumerate(ev):
+        if not (0 <= r1 < 8 and 0 <= r2 < 8 and 0 <= c1 < 64 and 0 <= c2 < 64):
+            continue
+        b = O.bin_coords(p, r1, c1, r2, c2)
+        if b is not None:
+            exp[O.bin_index(p, *b)] += 1
+            used[i] = True
+    assert n == used.sum() and 0 < n < 5000
+    assert np.array_equal(sino, exp)
+    w = rng.uniform(0.5, 2.0, 5000).astype(np.float32)
+    sw, nw = O.histogram_events(p, ev, w)
+    assert nw == n
+    assert abs(float(sw.sum()) - float(w[used].sum())) < 1e-2
+
+
 def test_forward_golden_bitexact(c1_golden):
     w = synth.c1()
     xy, z = O.scanner_tables(w.scanner)
