@@ -19,6 +19,7 @@
 #include <reconAlgos.h>
 
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <optional>
@@ -172,8 +173,17 @@ int main(int argc, char** argv)
       inputProj *= acf;
     }
 
-    reconAlgos::OSEM(inputProj, scanner, outputVol, params.outputVolFileName,
-                     params.algo, sensVol, biasProj);
+    // The reference selects OSEM_ResoReco with a constant that is always
+    // false (OSEM.cc:340-341); here FIR_RESO_RECO=1 selects it.
+    const char* reso = std::getenv("FIR_RESO_RECO");
+    if (reso && *reso && std::strcmp(reso, "0") != 0)
+      reconAlgos::OSEM_ResoReco(inputProj, scanner, outputVol,
+                                params.outputVolFileName, params.algo, sensVol,
+                                biasProj);
+    else
+      reconAlgos::OSEM(inputProj, scanner, outputVol,
+                       params.outputVolFileName, params.algo, sensVol,
+                       biasProj);
 
     printQuotedValue("Saving reconstructed volume to file",
                      params.outputVolFileName);

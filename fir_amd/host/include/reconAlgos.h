@@ -32,4 +32,15 @@ void OSEM(const ProjData& inputProj, const ScannerData& scanner,
           VolData& outputVol, const std::string& outputVolFileName,
           const OSEMCoreParams& params, const VolData& sensitivityMap,
           const std::optional<ProjData>& biasProj);
+
+// Resolution-modelled OSEM as the reference computes it (reference
+// reconAlgos.cc:203-367): the sensitivity frames are convolved once (in
+// place), and each sub-iteration convolves the backprojection, then
+// x /= sensitivity frame, x *= bp, Gaussian every convolutionInterval
+// sub-iterations, circular cut. The forward projection uses x itself, as
+// the reference's does.
+void OSEM_ResoReco(const ProjData& inputProj, const ScannerData& scanner,
+                   VolData& outputVol, const std::string& outputVolFileName,
+                   const OSEMCoreParams& params, VolData& sensitivityMap,
+                   const std::optional<ProjData>& biasProj);
 }

@@ -33,14 +33,17 @@ struct CacheHolder
     if (cache) fir_cache_destroy(cache);
   }
 };
-}
 
-void OSEM(const ProjData& inputProj, const ScannerData& scanner,
-          VolData& outputVol, const std::string& outputVolFileName,
-          const OSEMCoreParams& p, const VolData& sens,
-          const std::optional<ProjData>& biasProj)
+// Shared body of OSEM and OSEM_ResoReco. With reso (reconAlgos.cc:203-367):
+// every sensitivity frame is convolved once (and written back to sensOut),
+// and each sub-iteration convolves the backprojection and updates
+// x /= sens_s, x *= bp (clamped) instead of bp /= sens_s, x *= bp.
+void osemImpl(const ProjData& inputProj, const ScannerData& scanner,
+              VolData& outputVol, const std::string& outputVolFileName,
+              const OSEMCoreParams& p, const VolData& sens,
+              const std::optional<ProjData>& biasProj, bool reso,
+              VolData* sensOut)
 {
-  echo("OSEM:");
   scanner.checkProjData(inputProj);
   inputProj.checkNSubsets(p.nSubsets);
   if (sens.getNFrames() != p.nSubsets)
@@ -75,6 +78,12 @@ void OSEM(const ProjData& inputProj, const ScannerData& scanner,
   }
   device::check(fir_fill(ctx, bp.f(), static_cast<int64_t>(nv), 0.0f));
   device::check(fir_cut_circle(ctx, &v, x.f(), 1, p.cutRadius));
+  if (reso)
+  {
+    device::check(fir_convolve(ctx, &v, s.f(), p.nSubsets, p.fwhmXYZ.data(),
+                               p.cutRadius));
+    if (sensOut) s.download(sensOut->getAllFrames(), vb * p.nSubsets);
+  }
 
   CacheHolder cached;
   const int cacheFormat = pathCacheFormat();
@@ -89,6 +98,21 @@ void OSEM(const ProjData& inputProj, const ScannerData& scanner,
     device::check(fir_set_lor_order(ctx, FIR_ORDER_SEGMENT));
     device::check(rc);
   }
+  auto projector = [&](int subset) {
+    if (cached.cache)
+    {
+      const float* xs[1] = {x.f()};
+      const float* ys[1] = {y.f()};
+      const float* bs[1] = {biasProj ? bias.f() : nullptr};
+      float* bps[1] = {bp.f()};
+      device::check(
+        fir_cache_osem_project(cached.cache, 1, xs, ys, bs, bps, subset));
+    }
+    else
+      device::check(fir_osem_project(ctx, &v, x.f(), y.f(),
+                                     biasProj ? bias.f() : nullptr, bp.f(),
+                                     subset, p.nSubsets));
+  };
 
   const int total = p.nIterations * p.nSubsets;
   for (int iter = 0; iter < p.nIterations; ++iter)
@@ -101,17 +125,28 @@ void OSEM(const ProjData& inputProj, const ScannerData& scanner,
         print("  Sub-iteration ", subset + 1, " of ", p.nSubsets);
       const int conv =
         convolveFlag && subiter % p.convolutionInterval == 0 ? 1 : 0;
-      if (cached.cache)
+      const int reset = subiter != total ? 1 : 0;
+      float* sens_s = s.f() + nv * subset;
+      if (reso)
       {
-        const float* xs[1] = {x.f()};
-        const float* ys[1] = {y.f()};
-        const float* bs[1] = {biasProj ? bias.f() : nullptr};
-        float* bps[1] = {bp.f()};
+        // The reference also convolves a copy of x here that no later step
+        // reads (reconAlgos.cc:246-247, 271-274, 349-353); it is skipped.
+        projector(subset);
         device::check(
-          fir_cache_osem_project(cached.cache, 1, xs, ys, bs, bps, subset));
-        device::check(fir_osem_update(ctx, &v, x.f(), bp.f(),
-                                      s.f() + nv * subset, p.cutRadius,
-                                      conv ? 0 : 1, subiter != total ? 1 : 0));
+          fir_convolve(ctx, &v, bp.f(), 1, p.fwhmXYZ.data(), p.cutRadius));
+        device::check(fir_div_clamped(ctx, x.f(), sens_s, (int64_t)nv));
+        device::check(fir_mul_clamped(ctx, x.f(), bp.f(), (int64_t)nv));
+        if (reset) device::check(fir_fill(ctx, bp.f(), (int64_t)nv, 0.0f));
+        if (conv)
+          device::check(
+            fir_convolve(ctx, &v, x.f(), 1, p.fwhmXYZ.data(), p.cutRadius));
+        device::check(fir_cut_circle(ctx, &v, x.f(), 1, p.cutRadius));
+      }
+      else if (cached.cache)
+      {
+        projector(subset);
+        device::check(fir_osem_update(ctx, &v, x.f(), bp.f(), sens_s,
+                                      p.cutRadius, conv ? 0 : 1, reset));
         if (conv)
         {
           device::check(
@@ -121,9 +156,9 @@ void OSEM(const ProjData& inputProj, const ScannerData& scanner,
       }
       else
         device::check(fir_osem_subiter(
-          ctx, &v, x.f(), y.f(), biasProj ? bias.f() : nullptr,
-          s.f() + nv * subset, bp.f(), subset, p.nSubsets, conv,
-          p.fwhmXYZ.data(), p.cutRadius, subiter != total ? 1 : 0));
+          ctx, &v, x.f(), y.f(), biasProj ? bias.f() : nullptr, sens_s,
+          bp.f(), subset, p.nSubsets, conv, p.fwhmXYZ.data(), p.cutRadius,
+          reset));
       if (p.saveInterval > 0 && subiter % p.saveInterval == 0 &&
           subiter != total)
       {
@@ -134,5 +169,26 @@ void OSEM(const ProjData& inputProj, const ScannerData& scanner,
     }
   }
   x.download(outputVol.getDataArray(), vb);
+}
+}
+
+void OSEM(const ProjData& inputProj, const ScannerData& scanner,
+          VolData& outputVol, const std::string& outputVolFileName,
+          const OSEMCoreParams& p, const VolData& sens,
+          const std::optional<ProjData>& biasProj)
+{
+  echo("OSEM:");
+  osemImpl(inputProj, scanner, outputVol, outputVolFileName, p, sens,
+           biasProj, false, nullptr);
+}
+
+void OSEM_ResoReco(const ProjData& inputProj, const ScannerData& scanner,
+                   VolData& outputVol, const std::string& outputVolFileName,
+                   const OSEMCoreParams& p, VolData& sensitivityMap,
+                   const std::optional<ProjData>& biasProj)
+{
+  echo("OSEM_ResoReco:");
+  osemImpl(inputProj, scanner, outputVol, outputVolFileName, p,
+           sensitivityMap, biasProj, true, &sensitivityMap);
 }
 }

@@ -44,6 +44,9 @@
  *   orc_forward/backward golden sinograms/images of the compiled reference;
  *                        adjoint identity <Ax,y> = <x,A^T y> in FP64.
  *   orc_osem             golden images of the compiled reference.
+ *   orc_osem_reso        golden images of the compiled reference's
+ *                        reconAlgos::OSEM_ResoReco (called directly: no
+ *                        executable reaches it).
  *   orc_convolve         delta-image closed form; golden output.
  *   orc_cut_circle, orc_hu_to_mu, orc_proj_exp, orc_*_mul/div
  *                        closed forms on hand-built inputs.

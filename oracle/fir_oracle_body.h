@@ -346,3 +346,48 @@ int ORC_SFX(orc_osem)(const orc_vol* v, int geometric, const orc_proj* p,
   free(bp);
   return 0;
 }
+
+/* Resolution-modelled OSEM as the reference computes it
+ * (reconAlgos.cc:203-367), on x in place: cut; every sensitivity frame
+ * convolved once (operations::convolve loops over frames,
+ * operations.cc:77-80); per sub-iteration the projector pass of x, bp
+ * convolved, x /= sens_s then x *= bp (clamped), bp reset, the Gaussian of
+ * x every conv_interval sub-iterations, cut. The reference also convolves
+ * a copy of x every sub-iteration that no later step reads
+ * (reconAlgos.cc:246-247, 271-274, 349-353); it is left out. sens is
+ * modified. Returns as orc_osem. */
+int ORC_SFX(orc_osem_reso)(const orc_vol* v, int geometric, const orc_proj* p,
+                           const double* xy, const double* z,
+                           const ORC_REAL* y, const ORC_REAL* bias,
+                           ORC_REAL* sens, ORC_REAL* x, int n_iter,
+                           int n_subsets, float cut, int conv_interval,
+                           const float* fwhm, int nthreads)
+{
+  if (n_subsets <= 0 || p->n_views % n_subsets != 0) return 1;
+  orc_grid g;
+  grid_init(&g, v, geometric);
+  set_threads(nthreads);
+  const long long nvox = (long long)v->nx * v->ny * v->nz;
+  const int conv = conv_interval > 0 && fwhm[0] > 0.0 && fwhm[1] > 0.0 &&
+    fwhm[2] > 0.0;
+  ORC_REAL* bp = (ORC_REAL*)calloc(nvox, sizeof(ORC_REAL));
+  ORC_SFX(orc_cut_circle)(v, x, cut);
+  for (int s = 0; s < n_subsets; ++s)
+    ORC_SFX(orc_convolve)(v, sens + nvox * s, fwhm, cut);
+  const int total = n_iter * n_subsets;
+  for (int it = 0; it < n_iter; ++it)
+    for (int s = 0; s < n_subsets; ++s)
+    {
+      const int sub = it * n_subsets + s + 1;
+      ORC_SFX(osem_pass)(&g, p, xy, z, x, y, bias, bp, n_subsets, s);
+      ORC_SFX(orc_convolve)(v, bp, fwhm, cut);
+      ORC_SFX(orc_div_clamped)(x, sens + nvox * s, nvox);
+      ORC_SFX(orc_mul_clamped)(x, bp, nvox);
+      if (sub != total) memset(bp, 0, sizeof(ORC_REAL) * nvox);
+      if (conv && sub % conv_interval == 0)
+        ORC_SFX(orc_convolve)(v, x, fwhm, cut);
+      ORC_SFX(orc_cut_circle)(v, x, cut);
+    }
+  free(bp);
+  return 0;
+}

@@ -227,6 +227,27 @@ def osem(vol, proj, xy, z, y, sens, x0, n_iter, n_subsets, cut=0.0,
     return x
 
 
+def osem_reso(vol, proj, xy, z, y, sens, x0, n_iter, n_subsets, cut=0.0,
+              conv_interval=0, fwhm=(0.0, 0.0, 0.0), bias=None,
+              geometric=False, nthreads=0):
+    """reconAlgos::OSEM_ResoReco (reconAlgos.cc:203-367); sens is not
+    modified (a copy is convolved)."""
+    v, p = vol_desc(vol), proj_desc(proj)
+    dt = np.asarray(y).dtype
+    y = np.ascontiguousarray(y, dt).ravel()
+    sens = np.ascontiguousarray(sens, dt).ravel().copy()
+    x = np.ascontiguousarray(x0, dt).ravel().copy()
+    fw = np.asarray(fwhm, np.float32)
+    b = None if bias is None else np.ascontiguousarray(bias, dt).ravel()
+    fn = getattr(lib(), "orc_osem_reso" + _sfx(dt))
+    rc = fn(C.byref(v), int(geometric), C.byref(p), _p(xy), _p(z), _p(y),
+            None if b is None else _p(b), _p(sens), _p(x), n_iter, n_subsets,
+            C.c_float(cut), conv_interval, _p(fw), nthreads)
+    if rc:
+        raise ValueError("number of subsets must divide the number of views")
+    return x
+
+
 def convolve(vol, img, fwhm, cut=0.0):
     v = vol_desc(vol)
     x = np.ascontiguousarray(img).ravel().copy()

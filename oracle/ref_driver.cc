@@ -24,6 +24,10 @@
 //       ProjData::rebinWeight of a sinogram of ones (single-segment
 //       headers only: the reference's bin addressing offsets the segment
 //       twice, ProjData.inl:67-82); float32 bins in file order.
+//   reso vol.h33 scanner.scan proj.hs nIter nSub fwhm conv cut out.bin
+//       reconAlgos::OSEM_ResoReco (no executable reaches it: OSEM.cc:341)
+//       from the image of vol.h33, with the sensitivity computed as
+//       FIR_OSEM computes it; writes the float32 image.
 //   time op vol.h33 scanner.scan proj.hs nSubsets reps
 //       op in {fwd, bp, sens, subiter}; uniform image and sinogram of ones;
 //       prints one JSON line with the median seconds over reps runs after a
@@ -173,6 +177,30 @@ int rebin(char** a)
   return 0;
 }
 
+int reso(char** a)
+{
+  VolData vol(a[0], VolData::ConstructionMode::READ_DATA);
+  ScannerData scanner(a[1]);
+  ProjData proj(a[2], ProjData::ConstructionMode::READ_DATA);
+  OSEMCoreParams p;
+  p.nIterations = std::atoi(a[3]);
+  p.nSubsets = std::atoi(a[4]);
+  const float fw = (float)std::atof(a[5]);
+  p.fwhmXYZ = {fw, fw, fw};
+  p.convolutionInterval = std::atoi(a[6]);
+  p.cutRadius = (float)std::atof(a[7]);
+  VolData sens;
+  sens.allocateAsMultiVol(vol, p.nSubsets);
+  projections::computeSensitivityVol(proj, scanner, sens, p.nSubsets);
+  reconAlgos::OSEM_ResoReco(proj, scanner, vol, "unused", p, sens,
+                            std::nullopt);
+  FILE* f = std::fopen(a[8], "wb");
+  if (!f) return 2;
+  std::fwrite(vol.getDataArray(), 4, (size_t)vol.getNVoxelsPerFrame(), f);
+  std::fclose(f);
+  return 0;
+}
+
 int tables(char** a)
 {
   ScannerData scanner(a[0]);
@@ -267,6 +295,7 @@ int main(int argc, char** argv)
     if (mode == "tables" && argc >= 6) return tables(argv + 2);
     if (mode == "bins" && argc >= 4) return bins(argv + 2);
     if (mode == "rebin" && argc >= 4) return rebin(argv + 2);
+    if (mode == "reso" && argc >= 11) return reso(argv + 2);
     if (mode == "time" && argc >= 8) return timeit(argv + 2);
     std::fprintf(stderr, "bad arguments\n");
     return 1;

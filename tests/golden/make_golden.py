@@ -27,6 +27,11 @@ Outputs (all data: inputs and the reference's outputs):
   5 segments) sinogram, and ProjData::rebinWeight of sinograms of ones
   with one segment at spans 3 and 5 (``python make_golden.py bins`` writes
   this file alone).
+* ``reso_golden.npz`` -- reconAlgos::OSEM_ResoReco of the compiled
+  reference (called directly by ref_driver: no executable reaches it) on
+  the C1 counts from an image of ones, 2 x 4 sub-iterations, 4 mm
+  Gaussian: run D with the image convolved every sub-iteration, run E
+  without it and a 55 mm cut (``python make_golden.py reso``).
 """
 from __future__ import annotations
 
@@ -343,9 +348,35 @@ def bins_fixture():
     print("bins_golden.npz written")
 
 
+RESO_RUNS = {"xd": (2, 4, 4.0, 1, 0.0), "xe": (2, 4, 4.0, 0, 55.0)}
+
+
+def reso_fixture():
+    """reso_golden.npz from ref_driver reso on the committed C1 counts."""
+    w = synth.c1()
+    g = np.load(os.path.join(HERE, "c1_golden.npz"))
+    arrays = {}
+    with tempfile.TemporaryDirectory() as d:
+        w.scanner.write(os.path.join(d, "s.scan"))
+        io.write_proj(os.path.join(d, "y.hs"), w.proj, g["y"])
+        w.vol.write(os.path.join(d, "x0.h33"),
+                    np.ones(w.vol.n_voxels, np.float32))
+        for name, (nit, ns, fw, conv, cut) in RESO_RUNS.items():
+            run([os.path.join(REF, "ref_driver"), "reso", "x0.h33", "s.scan",
+                 "y.hs", str(nit), str(ns), str(fw), str(conv), str(cut),
+                 "out.bin"], d)
+            arrays[name] = np.fromfile(os.path.join(d, "out.bin"), np.float32)
+            arrays[name + "_params"] = np.array([nit, ns, fw, conv, cut])
+    np.savez_compressed(os.path.join(HERE, "reso_golden.npz"), **arrays)
+    print("reso_golden.npz written")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "bins":
         bins_fixture()
+    elif len(sys.argv) > 1 and sys.argv[1] == "reso":
+        reso_fixture()
     else:
         main()
         bins_fixture()
+        reso_fixture()

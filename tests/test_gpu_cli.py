@@ -153,6 +153,27 @@ def test_osem_path_cache(work, c1_golden, fmt):
     assert rel_l2(xb, c1_golden["xb"]) < 1e-4
 
 
+@pytest.mark.parametrize("cache", ["off", "snorm16"])
+@pytest.mark.parametrize("run", ["xd", "xe"])
+def test_osem_reso_reco(work, run, cache):
+    """FIR_RESO_RECO=1 selects reconAlgos::OSEM_ResoReco (a constant false
+    in the reference's OSEM.cc:340-341): runs D and E of the compiled
+    reference's OSEM_ResoReco within 1e-4 relative L2, on the plain and the
+    cached projector."""
+    g = np.load(os.path.join(GOLDEN, "reso_golden.npz"))
+    nit, ns, fw, conv, cut = g[run + "_params"]
+    name = f"r_{run}_{cache}"
+    osem_params(work / f"{name}.params", name, int(nit), int(ns),
+                convolution_interval=str(int(conv)),
+                convolution_FHWM_XYZ_in_mm=f"{{{fw},{fw},{fw}}}",
+                cut_radius_in_mm=str(float(cut)))
+    r = exe("FIR_OSEM", f"{name}.params", cwd=work,
+            env={"FIR_RESO_RECO": "1", "FIR_PATH_CACHE": cache})
+    assert "OSEM_ResoReco" in r.stdout
+    x = np.fromfile(work / f"{name}.i33", np.float32)
+    assert rel_l2(x, g[run]) < 1e-4
+
+
 def test_osem_path_cache_rejects_unknown_format(work):
     osem_params(work / "pz.params", "pz", 1, 4)
     r = exe("FIR_OSEM", "pz.params", cwd=work, check=False,
