@@ -449,6 +449,14 @@ def main():
     n_e_per_sub = n_e_total / n_sub
     bytes_launch = 2 * (12 * lors_per_sub + 6 * n_e_per_sub) + 12 * nvox
     achieved = bytes_launch / (proj_avg_ms * 1e-3) / 1e9
+    if batched is not None:
+        # The batch reads the matrix once for all its images: per
+        # reconstruction the same algorithmic bytes as one launch, so the
+        # equivalent rate (sum over reconstructions) can exceed HBM's peak.
+        per_sub_ms = batched["osem_iter_ms"] / n_sub
+        eq = args.batch * bytes_launch / (per_sub_ms * 1e-3) / 1e9
+        batched["equivalent_gb_s"] = round(eq, 1)
+        batched["equivalent_frac_of_hbm_peak"] = round(eq / HBM_PEAK_GBS, 3)
     traffic = traffic_from_profiles()
 
     cpu = None

@@ -776,11 +776,15 @@ template <int F, int NB>
 __global__ __launch_bounds__(kBlock) void k_pc_backproject(
   const long long* __restrict__ voff, const int* __restrict__ trow,
   const void* __restrict__ tlen_, long long nvox, int lanes, int xcd_map,
-  const float* __restrict__ w, VoxOut out)
+  int zfast, const float* __restrict__ w, VoxOut out)
 {
-  const long long v =
+  long long v =
     ((long long)logical_block(blockIdx.x, gridDim.x, xcd_map) * blockDim.x +
      threadIdx.x) / lanes;
+  // zfast = nz: work items walk the voxels z fastest, so that a wave's
+  // voxels are z-neighbours (crossed by ring pairs of adjacent centre
+  // slices, whose rows lie next to each other).
+  if (zfast > 0 && v < nvox) v = (v % zfast) * (nvox / zfast) + v / zfast;
   const int lane = (int)threadIdx.x & (lanes - 1);
   const bool active = v < nvox;
   float acc[NB];
@@ -893,6 +897,9 @@ struct fir_cache
   int fwd_lanes = 8;
   int bp_lanes = 16;
   int xcd_map = 0;
+  // Backprojections of one or two images walk the voxels z fastest (with
+  // more images the scattered per-image stores cost more than it saves).
+  int zfast = 1;
 
   ~fir_cache()
   {
@@ -1401,7 +1408,8 @@ void launch_bp_nb(fir_cache* cache, int s, const float* w, const VoxOut& out)
   const long long* voff = cache->d_voff + (long long)s * (cache->nvox + 1);
   hipLaunchKernelGGL((k_pc_backproject<F, NB>), grid, dim3(kBlock), 0,
                      cache->ctx->stream, voff, cache->d_trow, cache->d_tlen,
-                     cache->nvox, lanes, cache->xcd_map, w, out);
+                     cache->nvox, lanes, cache->xcd_map,
+                     cache->zfast && NB <= 2 ? cache->vol.nz : 0, w, out);
 }
 
 template <int F>
@@ -1583,8 +1591,9 @@ int fir_cache_tune(fir_cache* cache, int fwd_lanes, int bp_lanes, int flags)
   if (!ok(fwd_lanes) || !ok(bp_lanes))
     return fail(FIR_E_ARG, "lanes per LOR / voxel must be a power of two "
                            "from 1 to 64");
-  if (flags & ~FIR_CACHE_XCD_MAP)
+  if (flags & ~(FIR_CACHE_XCD_MAP | FIR_CACHE_STORED_VOXELS))
     return fail(FIR_E_ARG, "unknown cache flags 0x%x", flags);
+  cache->zfast = (flags & FIR_CACHE_STORED_VOXELS) ? 0 : 1;
   cache->fwd_lanes = fwd_lanes;
   cache->bp_lanes = bp_lanes;
   cache->xcd_map = (flags & FIR_CACHE_XCD_MAP) ? 1 : 0;

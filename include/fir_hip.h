@@ -334,14 +334,17 @@ int fir_cache_info(const fir_cache* cache, int64_t* n_rows,
 /* Kernel tuning (results change only by FP32 summation order): lanes
  * that share one LOR (forward) and one voxel (backprojection), powers of
  * two from 1 to 64 (default: the layout's, fir_set_cache_lanes; other
- * values give the same results, with less locality); flags FIR_CACHE_XCD_MAP deals
- * contiguous eighths
- * of the rows of a view (of the voxels) to the eight XCDs. The path
- * matrix's row order within a view follows the context's row order
- * (fir_set_lor_order: pair order, FIR_ORDER_TANG_MAJOR) at creation. */
+ * values give the same results, with less locality). The path matrix's
+ * row order within a view follows the context's row order
+ * (fir_set_lor_order: pair order, FIR_ORDER_TANG_MAJOR) at creation.
+ * Backprojections of one or two images walk the voxels z fastest (a wave's
+ * voxels are then crossed by neighbouring rows). Flags: FIR_CACHE_XCD_MAP
+ * deals contiguous eighths of the rows of a view (of the voxels) to the
+ * eight XCDs; FIR_CACHE_STORED_VOXELS walks the voxels in storage order. */
 enum
 {
-  FIR_CACHE_XCD_MAP = 1
+  FIR_CACHE_XCD_MAP = 1,
+  FIR_CACHE_STORED_VOXELS = 2
 };
 int fir_cache_tune(fir_cache* cache, int fwd_lanes, int bp_lanes, int flags);
 

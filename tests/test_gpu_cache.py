@@ -252,7 +252,9 @@ def test_cache_lanes_keep_results(c1, c1_golden):
     cache = cache_of(c1, 4, firlib.FIR_LEN_FP32)
     img = dev(c1_golden["act"].ravel())
     outs = []
-    for fl, bl, fg in [(1, 1, 0), (4, 64, 1), (16, 16, 1), (64, 8, 0)]:
+    # flags: XCD map (1), voxels walked in storage order (2)
+    for fl, bl, fg in [(1, 1, 0), (4, 64, 1), (16, 16, 2), (64, 8, 3),
+                       (8, 16, 0)]:
         cache.tune(fl, bl, fg)
         s = torch.zeros(ctx.n_bins, device="cuda")
         cache.forward(img, s)
@@ -263,7 +265,7 @@ def test_cache_lanes_keep_results(c1, c1_golden):
     for s, b in outs[1:]:
         assert rel_max(s, outs[0][0]) < 1e-6
         assert rel_max(b, outs[0][1]) < 1e-6
-    for bad in [(0, 16, 0), (16, 3, 0), (128, 16, 0), (16, 16, 2)]:
+    for bad in [(0, 16, 0), (16, 3, 0), (128, 16, 0), (16, 16, 4)]:
         with pytest.raises(firlib.FirError):
             cache.tune(*bad)
 
