@@ -195,6 +195,32 @@ def run_uneven(args):
     ctx.close()
 
 
+def host_io_rates(w, sino):
+    """Upload/download rates of the C2 sinogram through the host layer
+    (fir_host_selftest upload), page-locked (default) and ordinary memory."""
+    import subprocess
+    import tempfile
+    from fir_amd import interfile as io
+    sel = os.path.join(ROOT, "fir_amd", "bin", "fir_host_selftest")
+    if not os.path.exists(sel):
+        return None
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        io.write_proj(os.path.join(d, "y.hs"), w.proj, sino.cpu().numpy())
+        for flag, key in (("1", "pinned"), ("0", "pageable")):
+            r = subprocess.run([sel, "upload", "y.hs", "5"], cwd=d,
+                               capture_output=True, text=True, timeout=300,
+                               env=dict(os.environ, FIR_PINNED=flag))
+            if r.returncode != 0:
+                return {"error": r.stderr[-300:]}
+            out[key] = json.loads(r.stdout.strip().splitlines()[-1])
+    log(f"host I/O (sinogram {out['pinned']['bytes'] / 1e6:.0f} MB): "
+        f"page-locked {out['pinned']['h2d_gb_s']} / "
+        f"{out['pinned']['d2h_gb_s']} GB/s up/down, ordinary "
+        f"{out['pageable']['h2d_gb_s']} / {out['pageable']['d2h_gb_s']} GB/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -439,6 +465,9 @@ def main():
         f"({listmode['m_events_s']} M events/s)")
     del events, hist
 
+    # --- host I/O: sinogram transfers from page-locked vs ordinary memory
+    host_io = host_io_rates(w, sino)
+
     # --- roofline of the dominant kernel pair (cached OSEM projector) ----
     # Algorithmic bytes per launch (one sub-iteration's LORs): forward +
     # backprojection in the cached-path format, 12 B per LOR and 6 B per
@@ -524,6 +553,7 @@ def main():
         },
         "batched": batched,
         "listmode": listmode,
+        "host_io": host_io,
         "cpu_baseline": cpu,
     }
     if cpu:

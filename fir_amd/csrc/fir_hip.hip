@@ -1041,6 +1041,26 @@ int fir_malloc(fir_ctx* c, void** p, size_t bytes)
   return FIR_OK;
 }
 
+int fir_host_alloc(void** p, size_t bytes)
+{
+  if (!p) return fail(FIR_E_ARG, "NULL argument");
+  *p = nullptr;
+  if (hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(FIR_E_NOMEM, "hipHostMalloc of %zu bytes failed", bytes);
+  }
+  return FIR_OK;
+}
+
+int fir_host_free(void* p)
+{
+  if (!p) return FIR_OK;
+  FIR_HIP_CHECK(hipHostFree(p));
+  return FIR_OK;
+}
+
 int fir_free(fir_ctx* c, void* p)
 {
   if (!c) return fail(FIR_E_ARG, "context is NULL");

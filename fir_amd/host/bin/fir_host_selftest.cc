@@ -11,6 +11,9 @@
 //                       view, axial, tang)
 //   rebin IN.hs OUT.hs  read a sinogram, ProjData::rebinWeight, write it
 //                       (needs a GPU)
+//   upload IN.hs REPS   read a sinogram and time REPS uploads to and
+//                       downloads from the GPU of its data (one JSON line;
+//                       needs a GPU)
 //   key TEXT            print the standardised keyword
 //   params FILE         parse an OSEM parameter file and print the values
 #include <KeyParser.h>
@@ -19,8 +22,11 @@
 #include <ScannerData.h>
 #include <VolData.h>
 #include <console.h>
+#include <device.h>
+#include <pinned.h>
 #include <reconAlgos.h>
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
@@ -96,6 +102,29 @@ int main(int argc, char** argv)
       ProjData p(argv[2], ProjData::ConstructionMode::READ_DATA);
       p.rebinWeight();
       p.write(argv[3]);
+    }
+    else if (mode == "upload" && argc >= 4)
+    {
+      ProjData p(argv[2], ProjData::ConstructionMode::READ_DATA);
+      const int reps = std::atoi(argv[3]);
+      const size_t bytes =
+        sizeof(float) * static_cast<size_t>(p.getGeometry().nBins);
+      float* data = p.getContiguousData();
+      device::Buffer b(bytes);
+      b.upload(data, bytes);
+      b.download(data, bytes);
+      using clk = std::chrono::steady_clock;
+      const auto t0 = clk::now();
+      for (int r = 0; r < reps; ++r) b.upload(data, bytes);
+      const auto t1 = clk::now();
+      for (int r = 0; r < reps; ++r) b.download(data, bytes);
+      const auto t2 = clk::now();
+      const double up = std::chrono::duration<double>(t1 - t0).count() / reps;
+      const double dn = std::chrono::duration<double>(t2 - t1).count() / reps;
+      std::printf("{\"pinned\": %d, \"bytes\": %zu, \"h2d_gb_s\": %.2f, "
+                  "\"d2h_gb_s\": %.2f}\n",
+                  pinned::isPinned(data) ? 1 : 0, bytes, bytes / up / 1e9,
+                  bytes / dn / 1e9);
     }
     else if (mode == "lors" && argc >= 4)
     {

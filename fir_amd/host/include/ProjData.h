@@ -6,6 +6,7 @@
 #pragma once
 
 #include <ProjHeader.h>
+#include <pinned.h>
 #include <types.h>
 
 #include <string>
@@ -98,6 +99,7 @@ private:
 
   ProjHeader mHeader{};
   ProjGeometry mGeometry{};
-  std::vector<types::BinValue> mStorage;
+  // page-locked when a GPU is present (direct DMA to and from the device)
+  std::vector<types::BinValue, pinned::Allocator<types::BinValue>> mStorage;
   std::vector<types::BinValue*> mSegments;
 };

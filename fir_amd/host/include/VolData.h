@@ -4,6 +4,7 @@
 #pragma once
 
 #include <VolHeader.h>
+#include <pinned.h>
 #include <types.h>
 
 #include <string>
@@ -85,5 +86,7 @@ private:
   mutable int mActiveFrame = 0;
   mutable types::VoxelValue* mDataArray = nullptr;
   std::vector<types::VoxelValue*> mFrames;
-  std::vector<types::VoxelValue> mStorage;
+  // page-locked when a GPU is present (direct DMA to and from the device)
+  std::vector<types::VoxelValue, pinned::Allocator<types::VoxelValue>>
+    mStorage;
 };

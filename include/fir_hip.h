@@ -136,6 +136,14 @@ int fir_malloc(fir_ctx* ctx, void** dptr, size_t bytes);
 int fir_free(fir_ctx* ctx, void* dptr);
 int fir_memcpy_h2d(fir_ctx* ctx, void* dst, const void* src, size_t bytes);
 int fir_memcpy_d2h(fir_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Page-locked host memory (hipHostMalloc): copies to and from it run as
+ * direct DMA instead of through a pageable staging copy. No counterpart in
+ * the reference (its containers malloc their data, ProjData.cc:623-651,
+ * VolData.cc:393-418); the host containers keep sinograms and volumes in
+ * it. FIR_E_NOMEM if it cannot be allocated (for instance with no device
+ * present). */
+int fir_host_alloc(void** ptr, size_t bytes);
+int fir_host_free(void* ptr);
 /* Fill n floats with value (a kernel, safe under graph capture). */
 int fir_fill(fir_ctx* ctx, float* dst, int64_t n, float value);
 

@@ -174,6 +174,27 @@ def test_osem_reso_reco(work, run, cache):
     assert rel_l2(x, g[run]) < 1e-4
 
 
+def test_host_containers_use_pinned_memory(work, c1_golden):
+    """Sinograms and volumes of the host layer live in page-locked memory
+    when a GPU is present (FIR_PINNED=0 keeps ordinary memory); transfers
+    and results are the same either way."""
+    sel = os.path.join(BIN, "fir_host_selftest")
+    outs = {}
+    for flag in ("1", "0"):
+        r = subprocess.run([sel, "upload", "y.hs", "3"], cwd=work,
+                           capture_output=True, text=True, timeout=120,
+                           env=dict(os.environ, FIR_PINNED=flag))
+        assert r.returncode == 0, r.stderr
+        outs[flag] = json.loads(r.stdout.strip().splitlines()[-1])
+    assert outs["1"]["pinned"] == 1 and outs["0"]["pinned"] == 0
+    assert outs["1"]["bytes"] == 4 * len(c1_golden["y"])
+    osem_params(work / "pp.params", "pp", 1, 4)
+    exe("FIR_OSEM", "pp.params", cwd=work, env={"FIR_PINNED": "0"})
+    exe("FIR_OSEM", "pp.params", cwd=work)
+    a = np.fromfile(work / "pp.i33", np.float32)
+    assert np.isfinite(a).all() and a.max() > 0
+
+
 def test_osem_path_cache_rejects_unknown_format(work):
     osem_params(work / "pz.params", "pz", 1, 4)
     r = exe("FIR_OSEM", "pz.params", cwd=work, check=False,
