@@ -716,7 +716,7 @@ int ensure_copies(fir_ctx* c, size_t floats)
 {
   if (c->copies_floats >= floats) return FIR_OK;
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
-  (void)hipFree(c->d_copies);
+  if (c->d_copies) c->retired.push_back(c->d_copies);
   c->d_copies = nullptr;
   c->copies_floats = 0;
   if (hipMalloc(&c->d_copies, floats * sizeof(float)) != hipSuccess)
@@ -801,7 +801,8 @@ int fir_ctx_destroy(fir_ctx* c)
   (void)hipFree(c->d_seg_base);
   (void)hipFree(c->d_copies);
   (void)hipFree(c->d_scratch);
-  (void)hipFree(c->d_taps);
+  for (auto& t : c->tap_sets) (void)hipFree(t.d);
+  for (void* p : c->retired) (void)hipFree(p);
   (void)hipFree(c->d_counter);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
@@ -1221,15 +1222,20 @@ int fir_osem_update(fir_ctx* c, const fir_vol_desc* v, float* x, float* bp,
 
 namespace
 {
-// Gaussian taps of operations::convolve (operations.cc:31-75), uploaded to
-// the context when the FWHM or voxel size changes. X and Y use a float
-// sigma, Z a double sigma; taps are computed in double, stored as float.
-int prepare_taps(fir_ctx* c, const fir_vol_desc* v, const float* fwhm)
+// Gaussian taps of operations::convolve (operations.cc:31-75) for (fwhm,
+// voxel size): the context's set for that key, uploaded on first use. X and
+// Y use a float sigma, Z a double sigma; taps are computed in double, stored
+// as float.
+int prepare_taps(fir_ctx* c, const fir_vol_desc* v, const float* fwhm,
+                 const fir_ctx::TapSet** out)
 {
-  if (c->d_taps && c->taps_key[0] == fwhm[0] && c->taps_key[1] == fwhm[1] &&
-      c->taps_key[2] == fwhm[2] && c->taps_vox[0] == v->dx &&
-      c->taps_vox[1] == v->dy && c->taps_vox[2] == v->dz)
-    return FIR_OK;
+  for (const auto& t : c->tap_sets)
+    if (t.key[0] == fwhm[0] && t.key[1] == fwhm[1] && t.key[2] == fwhm[2] &&
+        t.vox[0] == v->dx && t.vox[1] == v->dy && t.vox[2] == v->dz)
+    {
+      *out = &t;
+      return FIR_OK;
+    }
   const float sx = fwhm[0] / 2.3548f;
   const float sy = fwhm[1] / 2.3548f;
   const double sz = fwhm[2] / 2.3548;
@@ -1250,18 +1256,25 @@ int prepare_taps(fir_ctx* c, const fir_vol_desc* v, const float* fwhm)
   for (int k = -h[2]; k <= h[2]; ++k)
     taps[128 + h[2] + k] =
       (float)std::exp(-(k * k * v->dz * v->dz) / (2 * sz * sz));
-  if (!c->d_taps) FIR_HIP_CHECK(hipMalloc(&c->d_taps, sizeof(taps)));
-  FIR_HIP_CHECK(hipMemcpyAsync(c->d_taps, taps, sizeof(taps),
-                               hipMemcpyHostToDevice, c->stream));
-  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  fir_ctx::TapSet t;
+  FIR_HIP_CHECK(hipMalloc(&t.d, sizeof(taps)));
+  if (hipMemcpyAsync(t.d, taps, sizeof(taps), hipMemcpyHostToDevice,
+                     c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+  {
+    (void)hipFree(t.d);
+    return fail(FIR_E_HIP, "upload of the Gaussian taps failed");
+  }
   for (int d = 0; d < 3; ++d)
   {
-    c->taps_key[d] = fwhm[d];
-    c->taps_h[d] = h[d];
+    t.key[d] = fwhm[d];
+    t.h[d] = h[d];
   }
-  c->taps_vox[0] = v->dx;
-  c->taps_vox[1] = v->dy;
-  c->taps_vox[2] = v->dz;
+  t.vox[0] = v->dx;
+  t.vox[1] = v->dy;
+  t.vox[2] = v->dz;
+  c->tap_sets.push_back(t);
+  *out = &c->tap_sets.back();
   return FIR_OK;
 }
 
@@ -1269,7 +1282,7 @@ int ensure_scratch(fir_ctx* c, size_t floats)
 {
   if (c->scratch_floats >= floats) return FIR_OK;
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
-  (void)hipFree(c->d_scratch);
+  if (c->d_scratch) c->retired.push_back(c->d_scratch);
   c->d_scratch = nullptr;
   c->scratch_floats = 0;
   if (hipMalloc(&c->d_scratch, floats * sizeof(float)) != hipSuccess)
@@ -1281,7 +1294,8 @@ int ensure_scratch(fir_ctx* c, size_t floats)
 
 // Gaussian of one frame plus edge restore and cut (fused in the Z pass).
 int convolve_frame(fir_ctx* c, const fir_vol_desc* v, float* img,
-                   const float* fwhm, float restore_cut, float final_cut)
+                   const fir_ctx::TapSet* ts, const float* fwhm,
+                   float restore_cut, float final_cut)
 {
   const long long n = (long long)v->nx * v->ny * v->nz;
   float* t1 = c->d_scratch;
@@ -1292,15 +1306,15 @@ int convolve_frame(fir_ctx* c, const fir_vol_desc* v, float* img,
   const double rr = (double)restore_cut - 5.0 * fw;
   const unsigned nb = ew_blocks(n);
   hipLaunchKernelGGL((k_gauss<0, false>), dim3(nb), dim3(kBlock), 0,
-                     c->stream, img, t1, c->d_taps, c->taps_h[0], v->nx,
+                     c->stream, img, t1, ts->d, ts->h[0], v->nx,
                      v->ny, v->nz, cg, 0, 0.0, 0.0f);
   hipLaunchKernelGGL((k_gauss<1, false>), dim3(nb), dim3(kBlock), 0,
-                     c->stream, t1, t2, c->d_taps + 64, c->taps_h[1], v->nx,
+                     c->stream, t1, t2, ts->d + 64, ts->h[1], v->nx,
                      v->ny, v->nz, cg, 0, 0.0, 0.0f);
   // Z pass writes img in place: each voxel reads its own img value (the
   // pre-convolution copy) only for the edge restore.
   hipLaunchKernelGGL((k_gauss<2, true>), dim3(nb), dim3(kBlock), 0, c->stream,
-                     t2, img, c->d_taps + 128, c->taps_h[2], v->nx, v->ny,
+                     t2, img, ts->d + 128, ts->h[2], v->nx, v->ny,
                      v->nz, cg, restore, rr, final_cut);
   FIR_CHECK_LAUNCH();
   return FIR_OK;
@@ -1318,10 +1332,12 @@ int fir_convolve(fir_ctx* c, const fir_vol_desc* v, float* img, int n_frames,
   if (!(fwhm[0] > 0.0 && fwhm[1] > 0.0 && fwhm[2] > 0.0)) return FIR_OK;
   FIR_HIP_CHECK(hipSetDevice(c->device));
   const long long n = (long long)v->nx * v->ny * v->nz;
-  if ((rc = prepare_taps(c, v, fwhm))) return rc;
+  const fir_ctx::TapSet* ts = nullptr;
+  if ((rc = prepare_taps(c, v, fwhm, &ts))) return rc;
   if ((rc = ensure_scratch(c, 2 * (size_t)n))) return rc;
   for (int f = 0; f < n_frames; ++f)
-    if ((rc = convolve_frame(c, v, img + n * f, fwhm, cut, 0.0f))) return rc;
+    if ((rc = convolve_frame(c, v, img + n * f, ts, fwhm, cut, 0.0f)))
+      return rc;
   return FIR_OK;
 }
 
@@ -1357,9 +1373,10 @@ int fir_osem_subiter(fir_ctx* c, const fir_vol_desc* v, float* x,
   if (conv)
   {
     const long long n = (long long)v->nx * v->ny * v->nz;
-    if ((rc = prepare_taps(c, v, fwhm))) return rc;
+    const fir_ctx::TapSet* ts = nullptr;
+    if ((rc = prepare_taps(c, v, fwhm, &ts))) return rc;
     if ((rc = ensure_scratch(c, 2 * (size_t)n))) return rc;
-    if ((rc = convolve_frame(c, v, x, fwhm, cut, cut))) return rc;
+    if ((rc = convolve_frame(c, v, x, ts, fwhm, cut, cut))) return rc;
   }
   return FIR_OK;
 }

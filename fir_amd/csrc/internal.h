@@ -59,10 +59,20 @@ struct fir_ctx
   float* d_scratch = nullptr;
   size_t scratch_floats = 0;
   // Gaussian taps on the device: [x: 64][y: 64][z: 64]
-  float* d_taps = nullptr;
-  float taps_key[3] = {-1.f, -1.f, -1.f};
-  double taps_vox[3] = {0, 0, 0};
-  int taps_h[3] = {0, 0, 0};
+  // Gaussian taps, one set per (FWHM, voxel size), each uploaded once and
+  // kept until the context is destroyed (captured graphs hold their
+  // addresses): X taps at [0, 64), Y at [64, 128), Z at [128, 192).
+  struct TapSet
+  {
+    float key[3];
+    double vox[3];
+    int h[3];
+    float* d = nullptr;
+  };
+  std::vector<TapSet> tap_sets;
+  // Scratch and scatter-copy buffers replaced by larger ones: freed with
+  // the context, not before, since captured graphs may still use them.
+  std::vector<void*> retired;
 
   // Bumped whenever the scanner, sinogram or walk mode changes: path
   // matrices built under another generation are not shared.

@@ -12,7 +12,8 @@ With a ``PathCache`` the projector pass streams the cached system matrix
 (fir_cache_osem_project) instead of walking every LOR.
 
 ``BatchedOSEM`` spreads reconstructions of uneven size (different image
-grids, subset counts and count levels) over such groups.
+grids, subset counts and count levels) over such groups and captures one
+iteration of all of them in a single HIP graph.
 
 Per sub-iteration each image follows reconAlgos::OSEM
 (reconAlgos.cc:119-186): projector pass, bp /= sens, x *= bp, Gaussian every
@@ -145,6 +146,7 @@ class BatchedOSEM:
         self.groups = []
         self.members = []
         self.caches = []
+        self.graph = None
         shared = None
         if cache_format is not None:
             shared = firlib.context_for(workload, device)
@@ -172,12 +174,33 @@ class BatchedOSEM:
             self.members.append(idx)
 
     def capture(self):
+        """Capture one iteration of every group in a single HIP graph: one
+        launch replays the sub-iterations of all solves. Every group first
+        runs one eager iteration, so that the contexts' Gaussian scratch and
+        taps are sized for all of them before capture."""
         for g in self.groups:
-            g.capture()
+            if g.iterations_done == 0:
+                g.iteration_eager()
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            st = torch.cuda.current_stream()
+            for g in self.groups:
+                g.ctx.set_stream(st)
+                for s in range(g.n_subsets):
+                    g._subiteration(s)
+        for g in self.groups:
+            g.ctx.set_stream(torch.cuda.current_stream())
+        self.graph = graph
 
     def iteration(self):
-        for g in self.groups:
-            g.iteration()
+        if self.graph is None:
+            for g in self.groups:
+                g.iteration()
+        else:
+            self.graph.replay()
+            for g in self.groups:
+                g.iterations_done += 1
 
     def images(self) -> List[torch.Tensor]:
         out = [None] * sum(len(m) for m in self.members)
@@ -187,6 +210,7 @@ class BatchedOSEM:
         return out
 
     def close(self):
+        self.graph = None
         for g in self.groups:
             g.graph = None
         for c in self.caches:

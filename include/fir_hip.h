@@ -14,6 +14,13 @@
  *   - Work is enqueued asynchronously on the context's stream
  *     (fir_set_stream); fir_synchronize() waits for it. Calls on one
  *     context must not overlap; different contexts may run concurrently.
+ *   - HIP graph capture: projector, update, Gaussian, cut and path-cache
+ *     calls may be captured once the same call has run eagerly on the
+ *     context (which uploads the Gaussian taps of that FWHM and voxel size
+ *     and sizes the scratch). Taps are kept per (FWHM, voxel size), and
+ *     scratch buffers replaced by larger ones stay allocated until the
+ *     context is destroyed, so later calls never invalidate a captured
+ *     graph.
  *   - Images are FP32, x fastest, then y, then z (reference VolData.inl:53-56),
  *     frames contiguous. Sinograms are FP32 in the concatenated segment order
  *     of the reference .s file: segment -S..S, each [view][axial][tang]

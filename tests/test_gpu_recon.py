@@ -92,6 +92,43 @@ def test_batched_uneven_groups():
     ctx.close()
 
 
+def test_batched_graph_with_gaussian_on_mixed_grids():
+    """One graph for all groups, with the Gaussian every sub-iteration on
+    grids of different voxel sizes (different taps) and sizes (the smaller
+    grid first, so the larger one regrows the context's scratch after the
+    smaller one's eager iteration): three replays later every solve still
+    equals the same solve run alone."""
+    w = synth.c1()
+    vols = [firlib.VolDesc.from_spec(io.VolSpec((48, 48, 8), (2.5, 2.5, 2.0),
+                                                (-58.75, -58.75, 0.0))),
+            firlib.VolDesc.from_spec(w.vol)]
+    ctx = firlib.context_for(w)
+    act = synth.phantom(w)[0].ravel()
+    y0 = torch.zeros(ctx.n_bins, device="cuda")
+    ctx.forward(vols[1], torch.from_numpy(act).cuda(), y0)
+    g = torch.Generator(device="cuda").manual_seed(13)
+    solves = []
+    for i, (vi, s) in enumerate([(0, 8), (1, 16), (0, 16), (1, 8)]):
+        y = torch.poisson(y0 * (0.5 + i), generator=g)
+        solves.append(dict(vol=vols[vi], n_subsets=s, y=y))
+    kw = dict(fwhm=(4.0, 4.0, 4.0), conv_interval=1, cut=50.0)
+    b = recon.BatchedOSEM(w, solves, **kw)
+    b.capture()
+    for _ in range(3):
+        b.iteration()
+        torch.cuda.synchronize()
+    out = [x.cpu().numpy() for x in b.images()]
+    for i, s in enumerate(solves):
+        c = firlib.context_for(w)
+        single = recon.OSEMGroup(c, s["vol"], s["n_subsets"], [s["y"]], **kw)
+        for _ in range(4):
+            single.iteration_eager()
+        assert rel_l2(out[i], single.x[0].cpu().numpy()) < 1e-5, i
+        c.close()
+    b.close()
+    ctx.close()
+
+
 def test_batched_uneven_groups_cached():
     """The same uneven solves on path caches (one shared context, caches of
     one grid sharing their path matrix). Each batched solve equals the same
