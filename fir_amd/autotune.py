@@ -1,6 +1,7 @@
 """Autotuner of the projector kernels: picks, by timing the real workload on
-the device, the LOR sort key and the lanes per LOR / per voxel of a path
-cache, and the row order and per-XCD scatter copies of the plain projector.
+the device, the LOR sort key, the lanes per LOR / per voxel and the z-slice
+interleave of a path cache's layout, and the row order and per-XCD scatter
+copies of the plain projector.
 
 Every candidate changes only the FP32 summation order of the results (the
 tests check that), so the choice is purely a matter of speed. Timings are
@@ -20,6 +21,14 @@ from . import lib as firlib
 # slice) and the sinogram's own (pair, tangential bin).
 CACHE_ROW_ORDERS = (firlib.CACHE_ROW_ORDER, firlib.FIR_ORDER_VIEW_MAJOR)
 CACHE_LANES = ((8, 16), (16, 16), (8, 8))
+# Layouts tried: (row order, lanes per LOR, lanes per voxel, z-slices).
+CACHE_LAYOUTS = ((firlib.CACHE_ROW_ORDER, 8, 16, 1),
+                 (firlib.CACHE_ROW_ORDER, 8, 16, 2),
+                 (firlib.CACHE_ROW_ORDER, 8, 16, 4),
+                 (firlib.CACHE_ROW_ORDER, 8, 16, 8),
+                 (firlib.CACHE_ROW_ORDER, 16, 16, 4),
+                 (firlib.CACHE_ROW_ORDER, 8, 8, 4),
+                 (firlib.FIR_ORDER_VIEW_MAJOR, 8, 16, 4))
 PLAIN_ORDERS = (firlib.FIR_ORDER_SEGMENT,
                 firlib.FIR_ORDER_SEGMENT | firlib.FIR_ORDER_TANG_MAJOR,
                 firlib.FIR_ORDER_VIEW_MAJOR | firlib.FIR_ORDER_ZCENTER)
@@ -61,38 +70,40 @@ def _time(fn, reps: int) -> float:
 def tune_cache(ctx: firlib.Context, vol: firlib.VolDesc, n_subsets: int,
                xs: Sequence[torch.Tensor], ys: Sequence[torch.Tensor],
                len_format: int = firlib.FIR_LEN_SNORM16,
-               row_orders: Sequence[int] = CACHE_ROW_ORDERS,
-               lanes: Sequence[Tuple[int, int]] = CACHE_LANES,
+               layouts: Sequence[Tuple[int, int, int, int]] = CACHE_LAYOUTS,
                subset: int = 0, reps: int = 3
                ) -> Tuple[firlib.PathCache, Result]:
-    """Build a path cache per (row order, lanes) -- the lanes set the cache's
-    layout (Context.set_cache_lanes) -- time the OSEM projector pass of
-    ``len(xs)`` images on each, and return the fastest cache with the table
-    of trials. Only the best cache so far is kept while the next is built."""
+    """Build a path cache per layout (row order, lanes per LOR and per voxel,
+    z-slice interleave: Context.set_cache_lanes / set_cache_zslices), time
+    the OSEM projector pass of ``len(xs)`` images on each, and return the
+    fastest cache with the table of trials. Only the best cache so far is
+    kept while the next is built."""
     bps = [torch.zeros(vol.n_voxels, device=xs[0].device) for _ in xs]
     trials: List[Trial] = []
     best = None
     best_cache: Optional[firlib.PathCache] = None
-    old = getattr(ctx, "_cache_lanes", (8, 16))
+    old_lanes = getattr(ctx, "_cache_lanes", (8, 16))
+    old_zq = getattr(ctx, "_cache_zq", 4)
     try:
-        for order in row_orders:
-            for fl, bl in lanes:
-                ctx.set_cache_lanes(fl, bl)
-                cache = ctx.path_cache(vol, n_subsets, len_format,
-                                       row_order=order)
-                ms = _time(lambda: cache.osem_project(list(xs), list(ys),
-                                                      None, bps, subset), reps)
-                cfg = {"row_order": order, "fwd_lanes": fl, "bp_lanes": bl}
-                trials.append(Trial(cfg, ms))
-                if best is None or ms < best[1]:
-                    if best_cache is not None:
-                        best_cache.close()
-                    best, best_cache = (cfg, ms), cache
-                else:
-                    cache.close()
-                torch.cuda.synchronize()
+        for order, fl, bl, zq in layouts:
+            ctx.set_cache_lanes(fl, bl)
+            ctx.set_cache_zslices(zq)
+            cache = ctx.path_cache(vol, n_subsets, len_format, row_order=order)
+            ms = _time(lambda: cache.osem_project(list(xs), list(ys), None,
+                                                  bps, subset), reps)
+            cfg = {"row_order": order, "fwd_lanes": fl, "bp_lanes": bl,
+                   "zslices": zq}
+            trials.append(Trial(cfg, ms))
+            if best is None or ms < best[1]:
+                if best_cache is not None:
+                    best_cache.close()
+                best, best_cache = (cfg, ms), cache
+            else:
+                cache.close()
+            torch.cuda.synchronize()
     finally:
-        ctx.set_cache_lanes(*old)
+        ctx.set_cache_lanes(*old_lanes)
+        ctx.set_cache_zslices(old_zq)
     return best_cache, Result(best[0], best[1], trials)
 
 

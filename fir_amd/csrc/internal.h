@@ -80,6 +80,9 @@ struct fir_ctx
   // Lanes per LOR and per voxel the path caches built on this context are
   // laid out for (fir_set_cache_lanes).
   int cache_lanes[2] = {8, 16};
+  // z-slices interleaved in the gather layouts of the path caches built on
+  // this context (fir_set_cache_zslices).
+  int cache_zq = 4;
   // Path matrices of live caches, shared by caches of one volume, walk mode
   // and length format.
   std::vector<std::weak_ptr<RowMatrix>> row_matrices;

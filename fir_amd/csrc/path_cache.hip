@@ -107,30 +107,49 @@ struct Len<FIR_LEN_FP32>
 
 size_t len_bytes(int fmt) { return fmt == FIR_LEN_FP32 ? 4 : 2; }
 
-// Second image layout, y fastest: voxel (x, y, z) at (z nx + x) ny + y.
-// LORs closer to the y axis than to the x axis store their voxel indices in
-// this layout and gather from a y-fastest copy of the image, so that the
-// consecutive voxels of their paths share cache lines.
+// Image layouts of the gathers. The projector kernels gather from two
+// copies of each image: x fastest for LORs closer to the x axis, y fastest
+// for the others, so that the consecutive voxels of a path share cache
+// lines. In both, zq consecutive z-slices are interleaved innermost:
+// voxel (x, y, z) of the x-fastest copy sits at
+// (((z / zq) ny + y) nx + x) zq + z % zq (x and y swapped in the other).
+// The rows a wave gathers for together are LORs through one line of the
+// xy-plane at neighbouring centre slices, a few slices apart, whose voxels
+// then share cache lines too (zq is fixed per path matrix,
+// fir_set_cache_zslices).
 struct XY
 {
   int nx, ny;
+  int zq = 1;
 };
 
-__device__ __forceinline__ int to_yfast(const XY& g, int idx)
+// Slices of the copies: nz rounded up to a multiple of zq.
+long long layout_voxels(const XY& g, int nz)
+{
+  return (long long)g.nx * g.ny * ((nz + g.zq - 1) / g.zq * g.zq);
+}
+
+__device__ __forceinline__ int to_layout(const XY& g, int idx, bool yf)
 {
   const int x = idx % g.nx;
   const int r = idx / g.nx;
   const int y = r % g.ny;
   const int z = r / g.ny;
-  return (z * g.nx + x) * g.ny + y;
+  const int a = yf ? y : x, b = yf ? x : y;
+  const int na = yf ? g.ny : g.nx, nb = yf ? g.nx : g.ny;
+  return (((z / g.zq) * nb + b) * na + a) * g.zq + z % g.zq;
 }
 
-__device__ __forceinline__ int from_yfast(const XY& g, int i)
+__device__ __forceinline__ int from_layout(const XY& g, int li, bool yf)
 {
-  const int y = i % g.ny;
-  const int r = i / g.ny;
-  const int x = r % g.nx;
-  const int z = r / g.nx;
+  const int na = yf ? g.ny : g.nx, nb = yf ? g.nx : g.ny;
+  const int zl = li % g.zq;
+  const int r = li / g.zq;
+  const int a = r % na;
+  const int r2 = r / na;
+  const int b = r2 % nb;
+  const int z = (r2 / nb) * g.zq + zl;
+  const int x = yf ? b : a, y = yf ? a : b;
   return (z * g.ny + y) * g.nx + x;
 }
 
@@ -300,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_fill(
     if (p < cnt)
     {
       const long long e = base + quad_slot(p, slots, lanes);
-      idx[e] = yf ? to_yfast(xy, i) : i;
+      idx[e] = to_layout(xy, i, yf);
       len[e] = Len<F>::enc(l, m);
     }
     ++p;
@@ -359,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_tr_hist(
   const bool yf = yrow[r] != 0;
   for (long long e = off[r]; e < off[r + 1]; ++e)
     if (!zero_len(len, lb, e))
-      atomicAdd(cnt + (yf ? from_yfast(xy, idx[e]) : idx[e]), 1u);
+      atomicAdd(cnt + from_layout(xy, idx[e], yf), 1u);
 }
 
 // Elements of each row of the subset inside the voxel range [v0, v1);
@@ -377,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_tr_count(
   const bool yf = yrow[r] != 0;
   for (long long e = off[r]; e < off[r + 1]; ++e)
     c += !zero_len(len, lb, e) &&
-        (unsigned)((yf ? from_yfast(xy, idx[e]) : idx[e]) - v0) < span
+        (unsigned)(from_layout(xy, idx[e], yf) - v0) < span
       ? 1
       : 0;
   rc[row_prefix[k] + (r - r0)] = c;
@@ -400,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_tr_expand(
   for (long long e = off[r]; e < off[r + 1]; ++e)
   {
     const unsigned local =
-      (unsigned)((yf ? from_yfast(xy, idx[e]) : idx[e]) - v0);
+      (unsigned)(from_layout(xy, idx[e], yf) - v0);
     if (local < span && p < n_out && Len<F>::bits(len[e]) != 0)
     {
       keys[p] = local;
@@ -805,8 +824,9 @@ struct ImgIn
   const float* p[kMaxCacheBatch];
 };
 
-// X[v * NB + b] = x_b[v] (if X is not NULL) and XT[vy * NB + b] = x_b[v]
-// with vy the y-fastest index of v (0 for b >= n_images).
+// X[vx * NB + b] = x_b[v] (if X is not NULL) and XT[vy * NB + b] = x_b[v]
+// with vx, vy the indices of v in the x- and y-fastest layouts (0 for
+// b >= n_images).
 template <int NB>
 __global__ void k_pc_pack(float* __restrict__ X, float* __restrict__ XT,
                           ImgIn in, int n_images, long long nvox, XY xy)
@@ -817,11 +837,12 @@ __global__ void k_pc_pack(float* __restrict__ X, float* __restrict__ XT,
     float t[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) t[b] = b < n_images ? in.p[b][v] : 0.0f;
-    const long long vy = to_yfast(xy, (int)v);
+    const long long vx = to_layout(xy, (int)v, false);
+    const long long vy = to_layout(xy, (int)v, true);
 #pragma unroll
     for (int b = 0; b < NB; ++b)
     {
-      if (X) X[v * NB + b] = t[b];
+      if (X) X[vx * NB + b] = t[b];
       XT[vy * NB + b] = t[b];
     }
   }
@@ -842,6 +863,7 @@ struct RowMatrix
   int pair_order = 0;
   int tang_major = 0;
   int lanes = 0;   // quad layout lanes (fir_set_cache_lanes)
+  int zq = 1;      // z-slices interleaved in the gather layouts
   int fmt = 0;
   int n_views = 0;
   int n_rows = 0;
@@ -916,6 +938,12 @@ struct fir_cache
 namespace
 {
 
+// The gather layouts of a cache's images.
+XY cache_xy(const fir_cache* cache)
+{
+  return XY{cache->vol.nx, cache->vol.ny, cache->rm->zq};
+}
+
 // Device buffers of one build step, freed on every exit path.
 struct DevBufs
 {
@@ -977,7 +1005,8 @@ int fill_rows(fir_ctx* c, const Grid& g, const LorGeom& lg, long long n,
 {
   hipLaunchKernelGGL(k_pc_fill<F>, dim3(blocks_for(n)), dim3(kBlock), 0,
                      c->stream, g, lg, n, per_view, count, maxlen, eoff,
-                     XY{g.row, g.slice / g.row}, c->cache_lanes[0], rm->d_idx,
+                     XY{g.row, g.slice / g.row, rm->zq}, c->cache_lanes[0],
+                     rm->d_idx,
                      (typename Len<F>::T*)rm->d_len);
   FIR_CHECK_LAUNCH();
   return FIR_OK;
@@ -1037,6 +1066,7 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
   rm->pair_order = c->lor_order;
   rm->tang_major = c->tang_major;
   rm->lanes = c->cache_lanes[0];
+  rm->zq = c->cache_zq;
   rm->fmt = fmt;
   rm->n_views = c->n_views;
   if ((rc = dev_alloc(&rm->d_view_row, c->n_views + 1, "view rows"))) return rc;
@@ -1154,7 +1184,7 @@ int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
   const dim3 grid_rows(blocks_for(max_rows), vpr);
   FIR_HIP_CHECK(hipMemsetAsync(rc_cnt, 0, sizeof(int) * ((size_t)n_rows_s + 1),
                                c->stream));
-  const XY xy{cache->vol.nx, cache->vol.ny};
+  const XY xy = cache_xy(cache);
   const int lb = (int)len_bytes(rm->fmt);
   hipLaunchKernelGGL(k_tr_count, grid_rows, dim3(kBlock), 0, c->stream, sr,
                      d_prefix, rm->d_off, rm->d_idx, rm->d_len, lb, rm->d_yrow,
@@ -1257,7 +1287,7 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
                          dim3(kBlock), 0, c->stream,
                          SubsetRows{rm->d_view_row, S, s, 0}, rm->d_off,
                          rm->d_idx, rm->d_len, (int)len_bytes(rm->fmt),
-                         rm->d_yrow, XY{cache->vol.nx, cache->vol.ny}, cnt);
+                         rm->d_yrow, cache_xy(cache), cnt);
       FIR_CHECK_LAUNCH();
     }
     long long* voff_s = cache->d_voff + (long long)s * (nvox + 1);
@@ -1367,14 +1397,14 @@ unsigned grid_blocks(const fir_cache* cache, long long threads)
   return b;
 }
 
-// The y-fastest copy (and, for nb > 1, the interleaved x-fastest copy) of
-// the images of one launch.
+// The y-fastest copy (and, for nb > 1 or interleaved slices, the
+// x-fastest copy) of the images of one launch.
 void pack_images(fir_cache* cache, int wd, const ImgIn& in, int nb)
 {
   const unsigned blocks = ew_blocks(cache->nvox);
-  const XY xy{cache->vol.nx, cache->vol.ny};
+  const XY xy = cache_xy(cache);
   hipStream_t st = cache->ctx->stream;
-  float* X = wd > 1 ? cache->d_x : nullptr;
+  float* X = wd > 1 || xy.zq > 1 ? cache->d_x : nullptr;
   switch (wd)
   {
   case 1: hipLaunchKernelGGL(k_pc_pack<1>, dim3(blocks), dim3(kBlock), 0, st, X, cache->d_xy, in, nb, cache->nvox, xy); break;
@@ -1475,6 +1505,16 @@ int fir_set_cache_lanes(fir_ctx* c, int fwd_lanes, int bp_lanes)
   return FIR_OK;
 }
 
+int fir_set_cache_zslices(fir_ctx* c, int zq)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (zq < 1 || zq > 16 || (zq & (zq - 1)) != 0)
+    return fail(FIR_E_ARG, "z-slices per layout group must be a power of two "
+                           "from 1 to 16");
+  c->cache_zq = zq;
+  return FIR_OK;
+}
+
 int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
                      int len_format, fir_cache** out)
 {
@@ -1497,6 +1537,7 @@ int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
     if (sp && sp->generation == c->generation && same_vol(sp->vol, *v) &&
         sp->walk_mode == c->walk_mode && sp->pair_order == c->lor_order &&
         sp->tang_major == c->tang_major && sp->lanes == c->cache_lanes[0] &&
+        sp->zq == c->cache_zq &&
         sp->fmt == len_format)
     {
       rm = sp;
@@ -1541,7 +1582,8 @@ int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
   // Per-row weights and interleaved images for up to 8 images, allocated
   // once so that later calls (and graphs capturing them) never reallocate.
   cache->w_floats = (size_t)rm->n_rows * kMaxCacheBatch;
-  cache->x_floats = (size_t)cache->nvox * kMaxCacheBatch;
+  cache->x_floats = (size_t)layout_voxels(cache_xy(cache), v->nz) *
+    kMaxCacheBatch;
   if ((rc = build_transpose(c, cache)) ||
       (rc = dev_alloc(&cache->d_w, cache->w_floats, "row weights")) ||
       (rc = dev_alloc(&cache->d_x, cache->x_floats, "interleaved images")) ||
@@ -1550,6 +1592,16 @@ int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
     (void)hipStreamSynchronize(c->stream);
     delete cache;
     return rc;
+  }
+  // the image copies' padding slices are never gathered; keep them zero
+  if (hipMemsetAsync(cache->d_x, 0, cache->x_floats * sizeof(float),
+                     c->stream) != hipSuccess ||
+      hipMemsetAsync(cache->d_xy, 0, cache->x_floats * sizeof(float),
+                     c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+  {
+    delete cache;
+    return fail(FIR_E_HIP, "path cache: clearing the image copies failed");
   }
   *out = cache;
   return FIR_OK;
@@ -1616,12 +1668,13 @@ int fir_cache_forward(fir_cache* cache, const float* img, float* sino)
   FIR_CHECK_LAUNCH();
   const int fmt = cache->rm->fmt;
   // One launch over all views (S = 1, subset 0 covers every view).
+  const float* X = cache->rm->zq > 1 ? cache->d_x : img;
   if (fmt == FIR_LEN_SNORM16)
-    launch_forward<FIR_LEN_SNORM16>(cache, 1, 0, img, sino);
+    launch_forward<FIR_LEN_SNORM16>(cache, 1, 0, X, sino);
   else if (fmt == FIR_LEN_FP16)
-    launch_forward<FIR_LEN_FP16>(cache, 1, 0, img, sino);
+    launch_forward<FIR_LEN_FP16>(cache, 1, 0, X, sino);
   else
-    launch_forward<FIR_LEN_FP32>(cache, 1, 0, img, sino);
+    launch_forward<FIR_LEN_FP32>(cache, 1, 0, X, sino);
   FIR_CHECK_LAUNCH();
   return FIR_OK;
 }
@@ -1700,7 +1753,7 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
       out.p[b] = bp[i0 + b];
     }
     io.n_images = nb;
-    const float* X = wd > 1 ? cache->d_x : x[i0];
+    const float* X = wd > 1 || rm->zq > 1 ? cache->d_x : x[i0];
     pack_images(cache, wd, in, nb);
     FIR_CHECK_LAUNCH();
     if (rm->fmt == FIR_LEN_SNORM16)

@@ -45,7 +45,8 @@ EXPORTS = (
     "fir_div_clamped", "fir_exp_clamped", "fir_hu_to_mu", "fir_path_dump",
     "fir_path_count", "fir_poisson_nll_grad", "fir_osem_project_batch",
     "fir_set_uneven_subsets", "fir_set_lor_order", "fir_set_scatter_copies",
-    "fir_set_cache_lanes", "fir_cache_create", "fir_cache_destroy",
+    "fir_set_cache_lanes", "fir_set_cache_zslices", "fir_cache_create",
+    "fir_cache_destroy",
     "fir_cache_info",
     "fir_cache_tune", "fir_cache_forward", "fir_cache_backward",
     "fir_cache_sensitivity", "fir_cache_osem_project",
@@ -130,6 +131,7 @@ def load() -> C.CDLL:
                            C.POINTER(C.c_int64)],
         "fir_cache_tune": [vp, i, i, i],
         "fir_set_cache_lanes": [vp, i, i],
+        "fir_set_cache_zslices": [vp, i],
         "fir_cache_forward": [vp, vp, vp],
         "fir_cache_backward": [vp, vp, vp],
         "fir_cache_sensitivity": [vp, vp],
@@ -225,6 +227,11 @@ class Context:
         """Lanes per LOR / per voxel the next path caches are laid out for."""
         _check(self.lib.fir_set_cache_lanes(self.h, fwd_lanes, bp_lanes))
         self._cache_lanes = (fwd_lanes, bp_lanes)
+
+    def set_cache_zslices(self, zq: int):
+        """z-slices interleaved in the gather layouts of the next caches."""
+        _check(self.lib.fir_set_cache_zslices(self.h, zq))
+        self._cache_zq = zq
 
     def set_uneven_subsets(self, allow: bool):
         _check(self.lib.fir_set_uneven_subsets(self.h, int(allow)))

@@ -129,13 +129,15 @@ class BatchedOSEM:
     projector. With ``cache_format`` (a FIR_LEN_* value) all groups share one
     context and each group gets a path cache; caches of one image grid share
     the LOR-major path matrix, so only the per-subset-count transposes are
-    extra.
+    extra. ``cache_zslices`` is their layout's z-slice interleave
+    (Context.set_cache_zslices).
     """
 
     def __init__(self, workload, solves: List[dict], fwhm=(0.0, 0.0, 0.0),
                  conv_interval: int = 0, cut: float = 0.0, device: int = 0,
                  cache_format: Optional[int] = None,
-                 cache_row_order: int = firlib.CACHE_ROW_ORDER):
+                 cache_row_order: int = firlib.CACHE_ROW_ORDER,
+                 cache_zslices: int = 2):
         keys = []
         for s in solves:
             v = s["vol"]
@@ -151,6 +153,9 @@ class BatchedOSEM:
         if cache_format is not None:
             shared = firlib.context_for(workload, device)
             shared.set_uneven_subsets(True)
+            # groups project several images per launch, for which 2
+            # interleaved z-slices suit the gathers best
+            shared.set_cache_zslices(cache_zslices)
         self.ctx = shared
         for k in keys:
             idx = [i for i, s in enumerate(solves)

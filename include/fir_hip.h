@@ -334,6 +334,14 @@ enum
  * so that the lanes' loads read consecutive elements. */
 int fir_set_cache_lanes(fir_ctx* ctx, int fwd_lanes, int bp_lanes);
 
+/* z-slices interleaved innermost in the image layouts the caches created
+ * afterwards on this context gather from: 1, 2, 4, 8 or 16 (default 4).
+ * The LORs a wave projects together cross one line of the xy-plane at
+ * neighbouring centre slices; interleaving z lets their voxels share cache
+ * lines (one image: -8% per projector pass at 4; with 8 interleaved images
+ * 1 or 2 is faster). Results do not depend on it. */
+int fir_set_cache_zslices(fir_ctx* ctx, int zslices);
+
 /* Build the cache of `vol` for `n_subsets` subsets (views v with
  * v % n_subsets == s, as LORCache.cc:49-51). Synchronizes the context's
  * stream; FIR_E_NOMEM if the matrices do not fit in device memory. */

@@ -218,8 +218,10 @@ def test_cached_graph_replays_match_eager(c1, c1_golden):
 
 
 def test_cache_layout_lanes_keep_results(c1, c1_golden):
-    """Caches laid out for other lanes (fir_set_cache_lanes), run with their
-    own or other runtime lanes, give the same projections."""
+    """Caches laid out for other lanes (fir_set_cache_lanes) and z-slice
+    interleaves (fir_set_cache_zslices; 16 exceeds the 8 slices of the
+    image, so the layout pads them), run with their own or other runtime
+    lanes, give the same projections."""
     w, ctx, v, _ = c1
     ref = cache_of(c1, 4, firlib.FIR_LEN_FP32)
     img = dev(c1_golden["act"].ravel())
@@ -228,12 +230,14 @@ def test_cache_layout_lanes_keep_results(c1, c1_golden):
     ref.forward(img, a)
     fa = torch.zeros(4 * v.n_voxels, device="cuda")
     ref.backward(y, fa)
-    for fl, bl in [(4, 64), (32, 2), (1, 1)]:
+    for fl, bl, zq in [(4, 64, 1), (32, 2, 2), (1, 1, 8), (8, 16, 16)]:
         ctx.set_cache_lanes(fl, bl)
+        ctx.set_cache_zslices(zq)
         try:
             c = ctx.path_cache(v, 4, firlib.FIR_LEN_FP32)
         finally:
             ctx.set_cache_lanes(8, 16)
+            ctx.set_cache_zslices(4)
         for rl in [(fl, bl), (8, 16)]:
             c.tune(*rl)
             b = torch.zeros(ctx.n_bins, device="cuda")
@@ -245,6 +249,8 @@ def test_cache_layout_lanes_keep_results(c1, c1_golden):
         c.close()
     with pytest.raises(firlib.FirError):
         ctx.set_cache_lanes(3, 16)
+    with pytest.raises(firlib.FirError):
+        ctx.set_cache_zslices(3)
 
 
 def test_cache_lanes_keep_results(c1, c1_golden):
@@ -347,10 +353,13 @@ def test_autotuner_picks_fastest_and_keeps_results(c1, c1_golden):
     w, ctx, v, _ = c1
     y = dev(c1_golden["y"])
     x = torch.ones(v.n_voxels, device="cuda")
+    layouts = ((firlib.CACHE_ROW_ORDER, 16, 16, 1),
+               (firlib.CACHE_ROW_ORDER, 8, 8, 4),
+               (firlib.FIR_ORDER_VIEW_MAJOR, 8, 16, 2))
     cache, res = autotune.tune_cache(ctx, v, 4, [x], [y], reps=1,
-                                     lanes=((16, 16), (8, 8)))
-    assert len(res.trials) == 2 * len(autotune.CACHE_ROW_ORDERS)
-    assert ctx._cache_lanes == (8, 16)
+                                     layouts=layouts)
+    assert len(res.trials) == len(layouts)
+    assert ctx._cache_lanes == (8, 16) and ctx._cache_zq == 4
     assert res.best_ms == min(t.ms for t in res.trials)
     g = recon.OSEMGroup(ctx, v, 4, [y], cache=cache)
     g.iteration_eager()
