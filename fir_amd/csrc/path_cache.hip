@@ -295,6 +295,65 @@ __global__ void k_pc_views(const int* __restrict__ count,
   }
 }
 
+// Element slots, path elements and crossing LORs of each view (one block
+// per view): the device memory each subset's share of a cache takes.
+__global__ __launch_bounds__(kBlock) void k_pc_view_totals(
+  const int* __restrict__ count, long long per_view,
+  unsigned long long* __restrict__ totals) // [n_views][3]
+{
+  const int* c = count + (long long)blockIdx.x * per_view;
+  unsigned long long t[3] = {0, 0, 0};
+  for (long long i = threadIdx.x; i < per_view; i += kBlock)
+  {
+    const int k = c[i];
+    t[0] += (unsigned)((k + 3) & ~3);
+    t[1] += (unsigned)k;
+    t[2] += k > 0 ? 1u : 0u;
+  }
+  __shared__ unsigned long long sh[3][kBlock];
+  for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] = t[j];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1)
+  {
+    if ((int)threadIdx.x < s)
+      for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] += sh[j][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3)
+    totals[3LL * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// A partial cache holds subsets 0 .. k-1 of S: the LORs of the other
+// subsets' views (view % S >= k, LORCache.cc:49-51) get no row.
+__global__ void k_pc_keep_subsets(int* __restrict__ count, long long n,
+                                  long long per_view, int S, int k)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n && (int)(t / per_view) % S >= k) count[t] = 0;
+}
+
+// sino[b] = 0 for the bins of the views of subsets 0 .. k-1 of S; the
+// sinogram is [segment][view][axial][tangential] (ProjData.h:16-21).
+__global__ void k_pc_zero_subset_bins(float* __restrict__ sino, long long n_bins,
+                                      const long long* __restrict__ seg_base,
+                                      const int* __restrict__ n_axial,
+                                      int n_seg, int n_tang, int S, int k)
+{
+  for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+       b < n_bins; b += (long long)gridDim.x * blockDim.x)
+  {
+    int lo = 0, hi = n_seg - 1; // last segment starting at or before b
+    while (lo < hi)
+    {
+      const int mid = (lo + hi + 1) / 2;
+      if (seg_base[mid] <= b) lo = mid;
+      else hi = mid - 1;
+    }
+    const long long view = (b - seg_base[lo]) / n_tang / n_axial[lo];
+    if ((int)(view % S) < k) sino[b] = 0.0f;
+  }
+}
+
 // Pass 2: the paths themselves.
 template <int F>
 __global__ __launch_bounds__(kBlock) void k_pc_fill(
@@ -865,6 +924,7 @@ struct RowMatrix
   int lanes = 0;   // quad layout lanes (fir_set_cache_lanes)
   int zq = 1;      // z-slices interleaved in the gather layouts
   int fmt = 0;
+  int partial = 0; // holds the rows of some subsets only (not shared)
   int n_views = 0;
   int n_rows = 0;
   long long n_elems = 0;            // path elements
@@ -907,7 +967,8 @@ struct fir_cache
   fir_vol_desc vol{};
   long long nvox = 0;
   int n_subsets = 0;
-  long long* d_voff = nullptr; // [n_subsets][nvox + 1], global element offsets
+  int n_cached = 0;            // subsets 0 .. n_cached-1 are held
+  long long* d_voff = nullptr; // [n_cached][nvox + 1], global element offsets
   int* d_trow = nullptr;       // [t_slots]
   void* d_tlen = nullptr;      // [t_slots]
   long long t_slots = 0;
@@ -1012,8 +1073,80 @@ int fill_rows(fir_ctx* c, const Grid& g, const LorGeom& lg, long long n,
   return FIR_OK;
 }
 
+// A partial cache: subsets 0 .. n_cached-1 of S, n_cached at most
+// max_cached (<= 0: no cap), the largest count whose device memory (build
+// temporaries included) fits in max_bytes (<= 0: no limit) and in the free
+// device memory.
+struct PartialPlan
+{
+  int S = 0;
+  int max_cached = 0;
+  long long max_bytes = 0;
+  long long nvox = 0;
+  long long image_bytes = 0; // the cache's image copies
+  int n_cached = 0;          // chosen
+};
+
+// Device bytes of subsets 0 .. k-1 of a cache, from per-subset totals.
+int plan_subsets(fir_ctx* c, const int* count, long long n, long long per_view,
+                 int fmt, PartialPlan* plan)
+{
+  const int V = c->n_views, S = plan->S;
+  DevBufs tmp;
+  unsigned long long* d_tot;
+  int rc;
+  if ((rc = tmp.get(&d_tot, 3 * (size_t)V, "view totals"))) return rc;
+  hipLaunchKernelGGL(k_pc_view_totals, dim3(V), dim3(kBlock), 0, c->stream,
+                     count, per_view, d_tot);
+  FIR_CHECK_LAUNCH();
+  std::vector<unsigned long long> tot(3 * (size_t)V);
+  FIR_HIP_CHECK(hipMemcpyAsync(tot.data(), d_tot, tot.size() * sizeof(tot[0]),
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  std::vector<long long> slots(S, 0), elems(S, 0), rows(S, 0);
+  for (int v = 0; v < V; ++v)
+  {
+    slots[v % S] += (long long)tot[3 * (size_t)v];
+    elems[v % S] += (long long)tot[3 * (size_t)v + 1];
+    rows[v % S] += (long long)tot[3 * (size_t)v + 2];
+  }
+  size_t free_b = 0, total_b = 0;
+  FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  const long long lb = (long long)len_bytes(fmt);
+  const long long held = 20LL * n; // count, maxlen, row ids, offsets
+  const long long avail = (long long)free_b - (1LL << 30);
+  const int cap = plan->max_cached > 0 ? std::min(plan->max_cached, S) : S;
+  long long R = 0, SL = 0, E = 0, maxE = 0, cost1 = 0;
+  int k = 0;
+  for (int s = 0; s < cap; ++s)
+  {
+    R += rows[s];
+    SL += slots[s];
+    E += elems[s];
+    maxE = std::max(maxE, elems[s]);
+    const long long cost =
+      R * (4 + 4 + 1 + 8) + SL * (4 + lb) + (V + 1LL) * 12 +         // rows
+      (E + 3LL * (s + 1) * plan->nvox) * (4 + lb) +                     // lists
+      (s + 1LL) * (plan->nvox + 1) * 8 +                                // offsets
+      R * kMaxCacheBatch * 4 + plan->image_bytes +                      // weights
+      std::max(held, 24 * std::min(kChunkElems, maxE) + 16 * plan->nvox); // temps
+    if (s == 0) cost1 = cost;
+    if ((plan->max_bytes > 0 && cost > plan->max_bytes) || cost - held > avail)
+      break;
+    k = s + 1;
+  }
+  if (k == 0)
+    return fail(FIR_E_NOMEM,
+                "path cache: subset 0 alone needs %.2f GB of device memory "
+                "(limit %.2f GB, %.2f GB free)",
+                cost1 / 1e9, plan->max_bytes > 0 ? plan->max_bytes / 1e9 : 0.0,
+                free_b / 1e9);
+  plan->n_cached = k;
+  return FIR_OK;
+}
+
 int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
-               std::shared_ptr<RowMatrix>* out)
+               std::shared_ptr<RowMatrix>* out, PartialPlan* plan = nullptr)
 {
   Grid g;
   int rc = make_grid(c, v, &g);
@@ -1038,6 +1171,17 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
   hipLaunchKernelGGL(k_pc_stats, dim3(blocks_for(n)), dim3(kBlock), 0,
                      c->stream, g, lg, n, per_view, count, maxlen);
   FIR_CHECK_LAUNCH();
+  if (plan)
+  {
+    if ((rc = plan_subsets(c, count, n, per_view, fmt, plan))) return rc;
+    if (plan->n_cached < plan->S)
+    {
+      hipLaunchKernelGGL(k_pc_keep_subsets, dim3(blocks_for(n)), dim3(kBlock),
+                         0, c->stream, count, n, per_view, plan->S,
+                         plan->n_cached);
+      FIR_CHECK_LAUNCH();
+    }
+  }
 
   // Exclusive scans: row id of each crossing LOR, first element of each LOR.
   hipcub::TransformInputIterator<int, HitFlag, const int*> hits(count,
@@ -1068,6 +1212,7 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
   rm->lanes = c->cache_lanes[0];
   rm->zq = c->cache_zq;
   rm->fmt = fmt;
+  rm->partial = plan && plan->n_cached < plan->S ? 1 : 0;
   rm->n_views = c->n_views;
   if ((rc = dev_alloc(&rm->d_view_row, c->n_views + 1, "view rows"))) return rc;
   if ((rc = dev_alloc(&rm->d_view_elem, c->n_views + 1, "view elements")))
@@ -1235,11 +1380,12 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
 {
   RowMatrix* rm = cache->rm.get();
   const int S = cache->n_subsets;
+  const int K = cache->n_cached;
   const long long nvox = cache->nvox;
   int rc;
   // Padding adds at most 3 slots per (subset, voxel).
-  const long long slots = rm->n_elems + 3LL * S * nvox;
-  if ((rc = dev_alloc(&cache->d_voff, (size_t)S * (nvox + 1), "voxel offsets")))
+  const long long slots = rm->n_elems + 3LL * K * nvox;
+  if ((rc = dev_alloc(&cache->d_voff, (size_t)K * (nvox + 1), "voxel offsets")))
     return rc;
   if ((rc = dev_alloc(&cache->d_trow, slots, "transposed rows"))) return rc;
   if ((rc = dev_alloc((unsigned char**)&cache->d_tlen,
@@ -1253,7 +1399,7 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
 
   std::vector<long long> voff(nvox + 1);
   long long base = 0, real = 0;
-  for (int s = 0; s < S; ++s)
+  for (int s = 0; s < K; ++s)
   {
     DevBufs tmp;
     const int vpr = fir::views_in_subset(rm->n_views, S, s);
@@ -1515,8 +1661,12 @@ int fir_set_cache_zslices(fir_ctx* c, int zq)
   return FIR_OK;
 }
 
-int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
-                     int len_format, fir_cache** out)
+} // extern "C"
+
+// Shared body of fir_cache_create (plan NULL: every subset, the path matrix
+// shared with the context's other caches) and fir_cache_create_partial.
+static int cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
+                        int len_format, PartialPlan* plan, fir_cache** out)
 {
   if (!out) return fail(FIR_E_ARG, "output pointer is NULL");
   *out = nullptr;
@@ -1533,6 +1683,7 @@ int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
   std::shared_ptr<RowMatrix> rm;
   for (auto& wp : c->row_matrices)
   {
+    if (plan) break;
     auto sp = wp.lock();
     if (sp && sp->generation == c->generation && same_vol(sp->vol, *v) &&
         sp->walk_mode == c->walk_mode && sp->pair_order == c->lor_order &&
@@ -1544,7 +1695,15 @@ int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
       break;
     }
   }
-  if (!rm)
+  if (plan)
+  {
+    plan->S = n_subsets;
+    plan->nvox = (long long)v->nx * v->ny * v->nz;
+    plan->image_bytes = 2LL * 4 * kMaxCacheBatch *
+      layout_voxels(XY{v->nx, v->ny, c->cache_zq}, v->nz);
+    if ((rc = build_rows(c, v, len_format, &rm, plan))) return rc;
+  }
+  else if (!rm)
   {
     if ((rc = build_rows(c, v, len_format, &rm))) return rc;
     c->row_matrices.erase(
@@ -1562,14 +1721,15 @@ int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
   cache->vol = *v;
   cache->nvox = (long long)v->nx * v->ny * v->nz;
   cache->n_subsets = n_subsets;
+  cache->n_cached = plan ? plan->n_cached : n_subsets;
   cache->fwd_lanes = c->cache_lanes[0];
   cache->bp_lanes = c->cache_lanes[1];
   size_t free_b = 0, total_b = 0;
   FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
   const long long need =
-    (rm->n_elems + 3LL * n_subsets * cache->nvox) *
+    (rm->n_elems + 3LL * cache->n_cached * cache->nvox) *
       (4 + (long long)len_bytes(len_format)) +
-    (long long)n_subsets * (cache->nvox + 1) * 8 +
+    (long long)cache->n_cached * (cache->nvox + 1) * 8 +
     ((long long)rm->n_rows + 2 * cache->nvox) * kMaxCacheBatch * 4;
   if ((long long)free_b < need + (1LL << 30))
   {
@@ -1607,6 +1767,35 @@ int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
   return FIR_OK;
 }
 
+extern "C" {
+
+int fir_cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
+                     int len_format, fir_cache** out)
+{
+  return cache_create(c, v, n_subsets, len_format, nullptr, out);
+}
+
+int fir_cache_create_partial(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
+                             int len_format, int max_subsets,
+                             int64_t max_bytes, fir_cache** out)
+{
+  if (out) *out = nullptr;
+  if (max_subsets < 0) return fail(FIR_E_ARG, "max_subsets must be >= 0");
+  PartialPlan plan;
+  plan.max_cached = max_subsets;
+  plan.max_bytes = max_bytes;
+  return cache_create(c, v, n_subsets, len_format, &plan, out);
+}
+
+int fir_cache_subsets(const fir_cache* cache, int* n_subsets, int* n_cached)
+{
+  int rc = check_cache(cache);
+  if (rc) return rc;
+  if (n_subsets) *n_subsets = cache->n_subsets;
+  if (n_cached) *n_cached = cache->n_cached;
+  return FIR_OK;
+}
+
 int fir_cache_destroy(fir_cache* cache)
 {
   if (!cache) return FIR_OK;
@@ -1630,7 +1819,7 @@ int fir_cache_info(const fir_cache* cache, int64_t* n_rows, int64_t* n_elements,
   if (device_bytes)
     *device_bytes = rm->bytes() +
       cache->t_slots * (4 + (long long)len_bytes(rm->fmt)) +
-      (long long)cache->n_subsets * (cache->nvox + 1) * 8 +
+      (long long)cache->n_cached * (cache->nvox + 1) * 8 +
       (long long)(cache->w_floats + 2 * cache->x_floats) * 4;
   return FIR_OK;
 }
@@ -1659,8 +1848,23 @@ int fir_cache_forward(fir_cache* cache, const float* img, float* sino)
   if (!img || !sino) return fail(FIR_E_ARG, "NULL buffer");
   fir_ctx* c = cache->ctx;
   FIR_HIP_CHECK(hipSetDevice(c->device));
-  // LORs that miss the volume project to 0 (projections.cc:63-66).
-  if ((rc = fir_fill(c, sino, c->n_bins, 0.0f))) return rc;
+  // LORs that miss the volume project to 0 (projections.cc:63-66); a
+  // partial cache writes the bins of its subsets' views only.
+  if (cache->n_cached == cache->n_subsets)
+  {
+    if ((rc = fir_fill(c, sino, c->n_bins, 0.0f))) return rc;
+  }
+  else
+  {
+    if (cache->rm->generation != c->generation)
+      return fail(FIR_E_STATE, "path cache: the sinogram geometry changed "
+                               "since the partial cache was built");
+    hipLaunchKernelGGL(k_pc_zero_subset_bins, dim3(ew_blocks(c->n_bins)),
+                       dim3(kBlock), 0, c->stream, sino, (long long)c->n_bins,
+                       c->d_seg_base, c->d_n_axial, c->n_seg, c->n_tang,
+                       cache->n_subsets, cache->n_cached);
+    FIR_CHECK_LAUNCH();
+  }
   ImgIn in;
   std::memset(&in, 0, sizeof(in));
   in.p[0] = img;
@@ -1692,7 +1896,7 @@ static int cache_backward_impl(fir_cache* cache, const float* sino,
                      dim3(kBlock), 0, c->stream, rm->d_bin, rm->d_factor,
                      rm->n_rows, sino, cache->d_w);
   FIR_CHECK_LAUNCH();
-  for (int s = 0; s < cache->n_subsets; ++s)
+  for (int s = 0; s < cache->n_cached; ++s)
   {
     VoxOut out;
     std::memset(&out, 0, sizeof(out));
@@ -1732,6 +1936,11 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
     if (!x[i] || !y[i] || !bp[i]) return fail(FIR_E_ARG, "NULL buffer");
   if (subset < 0 || subset >= cache->n_subsets)
     return fail(FIR_E_ARG, "subset %d out of range", subset);
+  if (subset >= cache->n_cached)
+    return fail(FIR_E_ARG,
+                "subset %d is not in the path cache (it holds subsets 0 to "
+                "%d); project it with fir_osem_project",
+                subset, cache->n_cached - 1);
   fir_ctx* c = cache->ctx;
   const RowMatrix* rm = cache->rm.get();
   FIR_HIP_CHECK(hipSetDevice(c->device));

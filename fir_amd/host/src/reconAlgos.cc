@@ -25,6 +25,21 @@ int pathCacheFormat()
   return -1;
 }
 
+// FIR_PATH_CACHE_SUBSETS = n caches subsets 0 .. n-1 only and
+// FIR_PATH_CACHE_GB = g caps the cache's device memory at g GB; either
+// makes the cache partial (fir_cache_create_partial), and so does a whole
+// cache that does not fit. Subsets outside the cache use the on-the-fly walk.
+long long envNumber(const char* name, double scale)
+{
+  const char* env = std::getenv(name);
+  if (!env || !*env) return 0;
+  char* end = nullptr;
+  const double d = std::strtod(env, &end);
+  if (end == env || *end || d <= 0)
+    error(name, " must be a positive number (got \"", env, "\")");
+  return static_cast<long long>(d * scale);
+}
+
 struct CacheHolder
 {
   fir_cache* cache = nullptr;
@@ -86,6 +101,7 @@ void osemImpl(const ProjData& inputProj, const ScannerData& scanner,
   }
 
   CacheHolder cached;
+  int nCached = 0;
   const int cacheFormat = pathCacheFormat();
   if (cacheFormat >= 0)
   {
@@ -93,13 +109,24 @@ void osemImpl(const ProjData& inputProj, const ScannerData& scanner,
     // Cache rows per view: tangential bin, then pair by centre slice.
     device::check(fir_set_lor_order(
       ctx, FIR_ORDER_VIEW_MAJOR | FIR_ORDER_TANG_MAJOR | FIR_ORDER_ZCENTER));
-    const int rc =
-      fir_cache_create(ctx, &v, p.nSubsets, cacheFormat, &cached.cache);
+    const long long maxSubsets = envNumber("FIR_PATH_CACHE_SUBSETS", 1.0);
+    const long long maxBytes = envNumber("FIR_PATH_CACHE_GB", 1e9);
+    int rc = FIR_E_NOMEM;
+    if (maxSubsets == 0 && maxBytes == 0)
+      rc = fir_cache_create(ctx, &v, p.nSubsets, cacheFormat, &cached.cache);
+    if (rc == FIR_E_NOMEM)
+      rc = fir_cache_create_partial(ctx, &v, p.nSubsets, cacheFormat,
+                                    static_cast<int>(maxSubsets), maxBytes,
+                                    &cached.cache);
     device::check(fir_set_lor_order(ctx, FIR_ORDER_SEGMENT));
     device::check(rc);
+    device::check(fir_cache_subsets(cached.cache, nullptr, &nCached));
+    if (nCached < p.nSubsets)
+      print("The path cache holds subsets 1 to ", nCached, " of ", p.nSubsets,
+            "; the others are walked on the fly");
   }
   auto projector = [&](int subset) {
-    if (cached.cache)
+    if (cached.cache && subset < nCached)
     {
       const float* xs[1] = {x.f()};
       const float* ys[1] = {y.f()};
@@ -142,7 +169,7 @@ void osemImpl(const ProjData& inputProj, const ScannerData& scanner,
             fir_convolve(ctx, &v, x.f(), 1, p.fwhmXYZ.data(), p.cutRadius));
         device::check(fir_cut_circle(ctx, &v, x.f(), 1, p.cutRadius));
       }
-      else if (cached.cache)
+      else if (cached.cache && subset < nCached)
       {
         projector(subset);
         device::check(fir_osem_update(ctx, &v, x.f(), bp.f(), sens_s,

@@ -46,7 +46,7 @@ EXPORTS = (
     "fir_path_count", "fir_poisson_nll_grad", "fir_osem_project_batch",
     "fir_set_uneven_subsets", "fir_set_lor_order", "fir_set_scatter_copies",
     "fir_set_cache_lanes", "fir_set_cache_zslices", "fir_cache_create",
-    "fir_cache_destroy",
+    "fir_cache_create_partial", "fir_cache_subsets", "fir_cache_destroy",
     "fir_cache_info",
     "fir_cache_tune", "fir_cache_forward", "fir_cache_backward",
     "fir_cache_sensitivity", "fir_cache_osem_project",
@@ -126,6 +126,9 @@ def load() -> C.CDLL:
         "fir_poisson_nll_grad": [vp, P, vp, vp, vp, vp, vp],
         "fir_osem_project_batch": [vp, P, i, vp, vp, vp, vp, i, i],
         "fir_cache_create": [vp, P, i, i, C.POINTER(C.c_void_p)],
+        "fir_cache_create_partial": [vp, P, i, i, i, i64,
+                                     C.POINTER(C.c_void_p)],
+        "fir_cache_subsets": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "fir_cache_destroy": [vp],
         "fir_cache_info": [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                            C.POINTER(C.c_int64)],
@@ -354,16 +357,24 @@ class Context:
 
     def path_cache(self, vol: VolDesc, n_subsets: int = 1,
                    len_format: int = FIR_LEN_SNORM16,
-                   row_order: Optional[int] = None) -> "PathCache":
+                   row_order: Optional[int] = None,
+                   max_subsets: Optional[int] = None,
+                   max_bytes: Optional[int] = None) -> "PathCache":
         """Build a path cache. row_order (FIR_ORDER_* flags) sets the order
         of the cached rows within a view (the context's LOR order at
-        creation); by default the context's current order is used."""
+        creation); by default the context's current order is used. With
+        max_subsets or max_bytes the cache is partial
+        (fir_cache_create_partial): it holds subsets 0 .. n_cached-1 only,
+        as many as fit (0 or None: no cap, no limit beyond free memory)."""
+        partial = None
+        if max_subsets is not None or max_bytes is not None:
+            partial = (int(max_subsets or 0), int(max_bytes or 0))
         if row_order is None:
-            return PathCache(self, vol, n_subsets, len_format)
+            return PathCache(self, vol, n_subsets, len_format, partial)
         old = getattr(self, "_lor_order", 0)
         self.set_lor_order(row_order)
         try:
-            return PathCache(self, vol, n_subsets, len_format)
+            return PathCache(self, vol, n_subsets, len_format, partial)
         finally:
             self.set_lor_order(old)
 
@@ -378,13 +389,21 @@ class PathCache:
     an extension of the reference's LORCache (LORCache.h:5-47)."""
 
     def __init__(self, ctx: Context, vol: VolDesc, n_subsets: int = 1,
-                 len_format: int = FIR_LEN_SNORM16):
+                 len_format: int = FIR_LEN_SNORM16, partial=None):
         self.ctx, self.lib, self.vol = ctx, ctx.lib, vol
         self.n_subsets, self.len_format = n_subsets, len_format
         h = C.c_void_p()
-        _check(self.lib.fir_cache_create(ctx.h, C.byref(vol), n_subsets,
-                                         len_format, C.byref(h)))
+        if partial is None:
+            _check(self.lib.fir_cache_create(ctx.h, C.byref(vol), n_subsets,
+                                             len_format, C.byref(h)))
+        else:
+            _check(self.lib.fir_cache_create_partial(
+                ctx.h, C.byref(vol), n_subsets, len_format, partial[0],
+                partial[1], C.byref(h)))
         self.h = h
+        s, k = C.c_int(), C.c_int()
+        _check(self.lib.fir_cache_subsets(h, C.byref(s), C.byref(k)))
+        self.n_cached = int(k.value)
 
     def close(self):
         if getattr(self, "h", None):

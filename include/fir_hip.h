@@ -347,6 +347,26 @@ int fir_set_cache_zslices(fir_ctx* ctx, int zslices);
  * stream; FIR_E_NOMEM if the matrices do not fit in device memory. */
 int fir_cache_create(fir_ctx* ctx, const fir_vol_desc* vol, int n_subsets,
                      int len_format, fir_cache** out);
+
+/* A cache of subsets 0 .. k-1 of n_subsets only, for scanners whose whole
+ * matrix does not fit in device memory (the reference's LORCache holds
+ * every subset, LORCache.cc:10-96): k is the largest count, at most
+ * max_subsets (<= 0: no cap), whose matrices and build temporaries fit in
+ * max_bytes (<= 0: no limit) and in the free device memory; FIR_E_NOMEM
+ * if subset 0 alone does not. The other subsets are projected with the
+ * on-the-fly walk (fir_osem_project, fir_forward, fir_backward): on a
+ * partial cache fir_cache_osem_project of such a subset fails with
+ * FIR_E_ARG, fir_cache_forward writes the bins of the cached subsets'
+ * views only, and fir_cache_backward / fir_cache_sensitivity write the
+ * frames of the cached subsets only (other bins and frames are left as
+ * they are). Its path matrix is its own, not shared. */
+int fir_cache_create_partial(fir_ctx* ctx, const fir_vol_desc* vol,
+                             int n_subsets, int len_format, int max_subsets,
+                             int64_t max_bytes, fir_cache** out);
+
+/* Subset count of the cache and how many (0 .. n_cached-1) it holds. */
+int fir_cache_subsets(const fir_cache* cache, int* n_subsets, int* n_cached);
+
 int fir_cache_destroy(fir_cache* cache);
 
 /* Crossing LORs, path elements, and device bytes held by the cache

@@ -8,9 +8,13 @@ Not collected by pytest (a long measurement, not a test). Two steps:
         CPU, FP64 oracle: forward projection of the phantom (every 997th bin
         kept), backprojection of the count sinogram, 24-subset sensitivity
         and OSEM 2 x 24 from a uniform image. Writes DIR/*.npy.
-    python tests/precision_c5.py gpu --ref DIR
+    python tests/precision_c5.py gpu --ref DIR [--partial-cache]
         GPU: the same four results through the C-ABI; prints one JSON line
-        with their errors against DIR and the GPU timings.
+        with their errors against DIR and the GPU timings. With
+        --partial-cache, also a partial path cache (snorm16 lengths with a
+        per-LOR FP32 scale; the subsets that fit in device memory, the
+        others walked on the fly): its forward projection over the cached
+        subsets' sampled bins, the backprojection and the OSEM 2 x 24 run.
 
 The count sinogram is y[b] = (b * 2654435761 mod 2^32) >> 27 (integers 0-31):
 deterministic and identical on both sides without shipping 1.2 GB of data.
@@ -91,7 +95,101 @@ def rel_max(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
-def gpu(ref):
+def sample_views(spec, n_bins):
+    """View of each sampled bin b = 0, SAMPLE, 2 SAMPLE, ...: segments in
+    order, each [view][axial][tang] (ProjData.h:16-21)."""
+    import pyoracle as O
+    p = O.proj_desc(spec)
+    na = np.asarray(p.n_axial, np.int64)
+    base = np.concatenate([[0], np.cumsum(na * p.n_views * p.n_tang)])
+    b = np.arange(0, n_bins, SAMPLE, dtype=np.int64)
+    k = np.searchsorted(base, b, side="right") - 1
+    return (b - base[k]) // p.n_tang // na[k]
+
+
+def hybrid(w, ctx, v, x, y, sens, ref, out):
+    """Partial path cache + on-the-fly walk for the other subsets."""
+    import torch
+    from fir_amd import lib as firlib
+    nb, nv = ctx.n_bins, v.n_voxels
+    st = torch.cuda.current_stream()
+    sino = torch.zeros(nb, device="cuda")
+    frames = torch.zeros(N_SUB * nv, device="cuda")
+    xo = torch.ones(nv, device="cuda")
+    bpo = torch.zeros(nv, device="cuda")
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    t = time.perf_counter()
+    try:
+        cache = ctx.path_cache(v, N_SUB, firlib.FIR_LEN_SNORM16,
+                               row_order=firlib.CACHE_ROW_ORDER, max_bytes=0)
+    except firlib.FirError as e:
+        out["partial_cache"] = {"error": str(e)}
+        return
+    torch.cuda.synchronize()
+    k = cache.n_cached
+    info = cache.info()
+    h = out["partial_cache"] = {
+        "len_format": "snorm16 (per-LOR FP32 scale)", "n_cached": k,
+        "n_subsets": N_SUB, "elements": info["elements"],
+        "device_gb": round(info["device_bytes"] / 1e9, 2),
+        "build_s": round(time.perf_counter() - t, 2)}
+    log(f"partial cache: {k} of {N_SUB} subsets, {h['device_gb']} GB")
+
+    def timed(fn):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    h["fwd_cached_subsets_ms"] = timed(lambda: cache.forward(x, sino))
+    f = sino[::SAMPLE].cpu().numpy()
+    del sino
+
+    def bp_all():
+        frames.zero_()
+        for s in range(k, N_SUB):
+            ctx.backward_subset(v, y, frames[s * nv:(s + 1) * nv], s, N_SUB)
+        cache.backward(y, frames)
+    h["bp_ms"] = timed(bp_all)
+    bp = frames.view(N_SUB, -1).sum(0).cpu().numpy()
+
+    def osem():
+        total = N_ITER * N_SUB
+        for it in range(N_ITER):
+            for s in range(N_SUB):
+                sub = it * N_SUB + s + 1
+                sens_s = sens[s * nv:(s + 1) * nv]
+                if s < k:
+                    cache.osem_project([xo], [y], None, [bpo], s)
+                    ctx.osem_update(v, xo, bpo, sens_s, reset=sub != total)
+                else:
+                    ctx.osem_subiter(v, xo, y, None, sens_s, bpo, s, N_SUB,
+                                     reset=sub != total)
+    h["osem_2x24_ms"] = timed(osem)
+    h["cached_subiter_ms"] = timed(
+        lambda: cache.osem_project([xo], [y], None, [bpo], 0))
+    if ref:
+        rf = np.load(os.path.join(ref, "fwd_samples.npy"))
+        cached = sample_views(w.proj, nb) % N_SUB < k
+        fc, rc = f[cached], rf[cached]
+        big = rc > 1e-3 * rc.max()
+        h["fwd_vs_fp64"] = {
+            "rel_l2": rel_l2(fc, rc),
+            "rel_max_over_significant_bins":
+                float((np.abs(fc - rc)[big] / rc[big]).max()),
+            "bins_compared": int(cached.sum())}
+        rb = np.load(os.path.join(ref, "bp.npy"))
+        h["bp_vs_fp64"] = {"rel_l2": rel_l2(bp, rb), "rel_max": rel_max(bp, rb)}
+        h["osem_2x24_vs_fp64"] = {"rel_l2": rel_l2(
+            xo.cpu().numpy(), np.load(os.path.join(ref, "osem.npy")))}
+    cache.close()
+
+
+def gpu(ref, partial_cache=False):
     import torch
     from fir_amd import lib as firlib
     w = synth.c5()
@@ -153,6 +251,9 @@ def gpu(ref):
             np.load(os.path.join(ref, "sens_sum.npy")))}
         out["osem_2x24_vs_fp64"] = {"rel_l2": rel_l2(
             xo.cpu().numpy(), np.load(os.path.join(ref, "osem.npy")))}
+    if partial_cache:
+        del bp, bpo, xo
+        hybrid(w, ctx, v, x, y, sens, ref, out)
     print(json.dumps(out), flush=True)
     ctx.close()
 
@@ -163,11 +264,12 @@ def main():
     ap.add_argument("--out", default="precision_c5_ref")
     ap.add_argument("--ref", default=None)
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--partial-cache", action="store_true")
     a = ap.parse_args()
     if a.mode == "reference":
         reference(a.out, a.threads)
     else:
-        gpu(a.ref)
+        gpu(a.ref, a.partial_cache)
 
 
 if __name__ == "__main__":

@@ -153,6 +153,24 @@ def test_osem_path_cache(work, c1_golden, fmt):
     assert rel_l2(xb, c1_golden["xb"]) < 1e-4
 
 
+def test_osem_partial_path_cache(work, c1_golden):
+    """FIR_PATH_CACHE_SUBSETS=2: subsets 1-2 of 4 from a partial cache, 3-4
+    walked on the fly; run B of the reference within 1e-4 relative L2. A
+    budget too small for one subset fails with the cache's message."""
+    osem_params(work / "pp.params", "pp", 2, 4,
+                attenuation_volume_in_HU="hu.h33",
+                cut_radius_in_mm="55", convolution_interval="1",
+                convolution_FHWM_XYZ_in_mm="{4,4,4}")
+    r = exe("FIR_OSEM", "pp.params", cwd=work,
+            env={"FIR_PATH_CACHE": "snorm16", "FIR_PATH_CACHE_SUBSETS": "2"})
+    assert "holds subsets 1 to 2 of 4" in r.stdout
+    xb = np.fromfile(work / "pp.i33", np.float32)
+    assert rel_l2(xb, c1_golden["xb"]) < 1e-4
+    r = exe("FIR_OSEM", "pp.params", cwd=work, check=False,
+            env={"FIR_PATH_CACHE": "snorm16", "FIR_PATH_CACHE_GB": "1e-9"})
+    assert r.returncode != 0 and "subset 0 alone needs" in r.stderr
+
+
 @pytest.mark.parametrize("cache", ["off", "snorm16"])
 @pytest.mark.parametrize("run", ["xd", "xe"])
 def test_osem_reso_reco(work, run, cache):
