@@ -195,6 +195,48 @@ def run_uneven(args):
     ctx.close()
 
 
+def cli_end_to_end(w, counts, hu, n_iter=3):
+    """The drop-in FIR_OSEM executable run on the workload's files, end to
+    end: process start, Interfile reads, ACF from the HU map, sensitivity,
+    n_iter iterations of the workload's subsets with its Gaussian, image
+    write. Wall time with the on-the-fly walk and with the path cache."""
+    from fir_amd import interfile as io
+    exe = os.path.join(ROOT, "fir_amd", "bin", "FIR_OSEM")
+    fw = ",".join(f"{f:g}" for f in w.fwhm)
+    res = {"iterations": n_iter, "subsets": w.n_subsets,
+           "run": "FIR_OSEM osem.params: counts + HU map read from "
+                  "Interfile, ACF, sensitivity, Gaussian every "
+                  "sub-iteration, image written"}
+    with tempfile.TemporaryDirectory() as d:
+        w.scanner.write(os.path.join(d, "scanner.scan"))
+        w.vol.write(os.path.join(d, "tmpl.h33"))
+        w.vol.write(os.path.join(d, "hu.h33"), hu)
+        io.write_proj(os.path.join(d, "y.hs"), w.proj, counts)
+        io.write_params(os.path.join(d, "osem.params"), "OSEM PARAMETERS", [
+            ("input projection file", "y.hs"),
+            ("scanner file", "scanner.scan"),
+            ("output volume header", "tmpl.h33"),
+            ("output volume file name", "x"),
+            ("number of iterations", str(n_iter)),
+            ("number of subsets", str(w.n_subsets)),
+            ("attenuation volume in HU", "hu.h33"),
+            ("convolution interval", "1"),
+            ("convolution FHWM XYZ in mm", "{" + fw + "}")])
+        for name, env in (("plain_s", {"FIR_PATH_CACHE": "off"}),
+                          ("path_cache_s", {"FIR_PATH_CACHE": "snorm16"})):
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, "osem.params"], cwd=d,
+                               env=dict(os.environ, **env),
+                               capture_output=True, text=True, timeout=300)
+            dt = time.perf_counter() - t0
+            if r.returncode != 0:
+                log("FIR_OSEM failed:", r.stderr[-500:])
+                return None
+            res[name] = round(dt, 3)
+            log(f"FIR_OSEM end to end ({name[:-2]}): {dt:.2f} s")
+    return res
+
+
 def host_io_rates(w, sino):
     """Upload/download rates of the C2 sinogram through the host layer
     (fir_host_selftest upload), page-locked (default) and ordinary memory."""
@@ -229,6 +271,8 @@ def main():
     ap.add_argument("--workload", default="c2",
                     help="c2 (default) or uneven (64 batched solves)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cli", action="store_true",
+                    help="skip the end-to-end FIR_OSEM runs")
     ap.add_argument("--batch", type=int, default=8,
                     help="reconstructions in the batched measurement (0: skip)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -274,6 +318,7 @@ def main():
     mean = mean * (w.counts / mean.sum())
     g = torch.Generator(device="cuda").manual_seed(w.seed)
     y = torch.poisson(mean.float(), generator=g)
+    counts = None if args.no_cli or args.traffic_probe else y.cpu().numpy()
     ctx.mul_clamped(y, acf)  # FIR_OSEM: inputProj *= ACF (OSEM.cc:336-337)
     mean_f = mean.float()
     sens = torch.zeros(n_sub * nvox, device="cuda")
@@ -479,6 +524,13 @@ def main():
     # --- host I/O: sinogram transfers from page-locked vs ordinary memory
     host_io = host_io_rates(w, sino)
 
+    # --- the drop-in executable, end to end ------------------------------
+    cli = None
+    if counts is not None:
+        cache.close()
+        torch.cuda.empty_cache()
+        cli = cli_end_to_end(w, counts, hu)
+
     # --- roofline of the dominant kernel pair (cached OSEM projector) ----
     # Algorithmic bytes per launch (one sub-iteration's LORs): forward +
     # backprojection in the cached-path format, 12 B per LOR and 6 B per
@@ -565,6 +617,7 @@ def main():
         "batched": batched,
         "listmode": listmode,
         "host_io": host_io,
+        "fir_osem_end_to_end": cli,
         "cpu_baseline": cpu,
     }
     if cpu:
