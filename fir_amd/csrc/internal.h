@@ -83,6 +83,9 @@ struct fir_ctx
   // z-slices interleaved in the gather layouts of the path caches built on
   // this context (fir_set_cache_zslices).
   int cache_zq = 4;
+  // Largest number of path elements one transpose step sorts (its
+  // temporaries take 24 bytes per element).
+  long long cache_chunk = 1LL << 28;
   // Path matrices of live caches, shared by caches of one volume, walk mode
   // and length format.
   std::vector<std::weak_ptr<RowMatrix>> row_matrices;

@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <memory>
 
 #include "internal.h"
@@ -48,7 +49,6 @@ namespace
 constexpr int kMaxCacheBatch = 8;
 
 // Elements per transpose chunk (bounds the sort's scratch: 24 B each).
-constexpr long long kChunkElems = 1LL << 28;
 
 template <int F>
 struct Len;
@@ -425,19 +425,38 @@ __device__ __forceinline__ bool zero_len(const void* len, int lb, long long e)
                  : ((const unsigned*)len)[e] == 0;
 }
 
-// Voxel histogram of one subset's elements (thread per row; indices of
-// y-fastest rows converted back).
+// The transpose passes read each row with kTrLanes lanes: lane j takes
+// slots j, j + kTrLanes, ... of the row, so that the group's loads are
+// contiguous. Rows are whole per group (groups never straddle a row).
+constexpr int kTrLanes = 16;
+
+// Slot e's voxel when it holds a contributing element (length != 0), else -1.
+__device__ __forceinline__ int tr_voxel(const int* __restrict__ idx,
+                                        const void* __restrict__ len, int lb,
+                                        const XY& xy, bool yf, long long e,
+                                        long long end)
+{
+  if (e >= end || zero_len(len, lb, e)) return -1;
+  return from_layout(xy, idx[e], yf);
+}
+
+// Voxel histogram of one subset's elements (indices of y-fastest rows
+// converted back).
 __global__ __launch_bounds__(kBlock) void k_tr_hist(
   SubsetRows sr, const long long* __restrict__ off, const int* __restrict__ idx,
   const void* __restrict__ len, int lb, const unsigned char* __restrict__ yrow,
   XY xy, unsigned* __restrict__ cnt)
 {
   int r, r0, k;
-  if (!subset_row(sr, 1, &r, &r0, &k)) return;
+  if (!subset_row(sr, kTrLanes, &r, &r0, &k)) return;
+  const int j = threadIdx.x % kTrLanes;
   const bool yf = yrow[r] != 0;
-  for (long long e = off[r]; e < off[r + 1]; ++e)
-    if (!zero_len(len, lb, e))
-      atomicAdd(cnt + from_layout(xy, idx[e], yf), 1u);
+  const long long end = off[r + 1];
+  for (long long e = off[r] + j; e - j < end; e += kTrLanes)
+  {
+    const int v = tr_voxel(idx, len, lb, xy, yf, e, end);
+    if (v >= 0) atomicAdd(cnt + v, 1u);
+  }
 }
 
 // Elements of each row of the subset inside the voxel range [v0, v1);
@@ -449,18 +468,27 @@ __global__ __launch_bounds__(kBlock) void k_tr_count(
   XY xy, int v0, int v1, int* __restrict__ rc)
 {
   int r, r0, k;
-  if (!subset_row(sr, 1, &r, &r0, &k)) return;
+  if (!subset_row(sr, kTrLanes, &r, &r0, &k)) return;
+  const int j = threadIdx.x % kTrLanes;
   int c = 0;
   const unsigned span = (unsigned)(v1 - v0);
   const bool yf = yrow[r] != 0;
-  for (long long e = off[r]; e < off[r + 1]; ++e)
-    c += !zero_len(len, lb, e) &&
-        (unsigned)(from_layout(xy, idx[e], yf) - v0) < span
-      ? 1
-      : 0;
-  rc[row_prefix[k] + (r - r0)] = c;
+  const long long end = off[r + 1];
+  for (long long e = off[r] + j; e - j < end; e += kTrLanes)
+  {
+    const int v = tr_voxel(idx, len, lb, xy, yf, e, end);
+    c += v >= 0 && (unsigned)(v - v0) < span ? 1 : 0;
+  }
+#pragma unroll
+  for (int d = kTrLanes / 2; d > 0; d >>= 1) c += __shfl_xor(c, d, kTrLanes);
+  if (j == 0) rc[row_prefix[k] + (r - r0)] = c;
 }
 
+// (voxel - v0, row | length bits) of each element of the subset inside
+// [v0, v1), at the row's positions pos[...]: the group's kept elements in
+// slot order. Within a row every voxel appears once, so the stable sort
+// by voxel that follows orders each voxel's elements by row whatever the
+// order within a row.
 template <int F>
 __global__ __launch_bounds__(kBlock) void k_tr_expand(
   SubsetRows sr, const int* __restrict__ row_prefix,
@@ -471,22 +499,55 @@ __global__ __launch_bounds__(kBlock) void k_tr_expand(
   unsigned long long* __restrict__ vals)
 {
   int r, r0, k;
-  if (!subset_row(sr, 1, &r, &r0, &k)) return;
-  int p = pos[row_prefix[k] + (r - r0)];
+  if (!subset_row(sr, kTrLanes, &r, &r0, &k)) return;
+  const int lane = threadIdx.x % warpSize;
+  const int j = lane % kTrLanes;
+  const int g0 = lane - j; // first lane of the group
+  long long p = pos[row_prefix[k] + (r - r0)];
   const unsigned span = (unsigned)(v1 - v0);
   const bool yf = yrow[r] != 0;
-  for (long long e = off[r]; e < off[r + 1]; ++e)
+  const long long end = off[r + 1];
+  for (long long e = off[r] + j; e - j < end; e += kTrLanes)
   {
-    const unsigned local =
-      (unsigned)(from_layout(xy, idx[e], yf) - v0);
-    if (local < span && p < n_out && Len<F>::bits(len[e]) != 0)
+    const int v = tr_voxel(idx, len, (int)sizeof(typename Len<F>::T), xy, yf,
+                           e, end);
+    const unsigned local = (unsigned)(v - v0);
+    const bool keep = v >= 0 && local < span;
+    const unsigned long long grp =
+      (__ballot(keep) >> g0) & ((1ULL << kTrLanes) - 1);
+    const long long q = p + __popcll(grp & ((1ULL << j) - 1));
+    if (keep && q < n_out)
     {
-      keys[p] = local;
-      vals[p] = ((unsigned long long)(unsigned)r << 32) |
+      keys[q] = local;
+      vals[q] = ((unsigned long long)(unsigned)r << 32) |
         (unsigned long long)Len<F>::bits(len[e]);
-      ++p;
     }
+    p += __popcll(grp);
   }
+}
+
+// cnt[v] = elements of voxel v in the sorted keys (cnt[nvox] = 0): two
+// binary searches per voxel.
+__global__ void k_tr_counts_sorted(const unsigned* __restrict__ keys,
+                                   long long n, int nvox,
+                                   unsigned* __restrict__ cnt)
+{
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > nvox) return;
+  long long b[2];
+  for (int j = 0; j < 2; ++j)
+  {
+    const unsigned key = (unsigned)(v + j);
+    long long lo = 0, hi = n; // first position with keys[i] >= key
+    while (lo < hi)
+    {
+      const long long mid = (lo + hi) / 2;
+      if (keys[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+    b[j] = lo;
+  }
+  cnt[v] = v == nvox ? 0u : (unsigned)(b[1] - b[0]);
 }
 
 // Sorted elements of voxels [v0, v0 + span) to their padded slots: the
@@ -1129,7 +1190,7 @@ int plan_subsets(fir_ctx* c, const int* count, long long n, long long per_view,
       (E + 3LL * (s + 1) * plan->nvox) * (4 + lb) +                     // lists
       (s + 1LL) * (plan->nvox + 1) * 8 +                                // offsets
       R * kMaxCacheBatch * 4 + plan->image_bytes +                      // weights
-      std::max(held, 24 * std::min(kChunkElems, maxE) + 16 * plan->nvox); // temps
+      std::max(held, 24 * std::min(c->cache_chunk, maxE) + 16 * plan->nvox);
     if (s == 0) cost1 = cost;
     if ((plan->max_bytes > 0 && cost > plan->max_bytes) || cost - held > avail)
       break;
@@ -1311,22 +1372,21 @@ int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
                     const SubsetRows& sr, const int* d_prefix, int vpr,
                     int max_rows, int n_rows_s, int v0, int v1,
                     const long long* voff_p, const long long* voff_u,
-                    long long n_ch)
+                    long long n_ch,
+                    const std::function<int(const unsigned*, long long)>&
+                      offsets_from_sorted)
 {
+  // n_ch < 0: the chunk is the whole subset and its element count comes
+  // from the rows; the voxel offsets are then computed from the sorted keys
+  // (offsets_from_sorted) before the split.
   DevBufs tmp; // the chunk's scratch, released on return
   int rc;
   int* rc_cnt;
   int* pos;
   if ((rc = tmp.get(&rc_cnt, (size_t)n_rows_s + 1, "row counts"))) return rc;
   if ((rc = tmp.get(&pos, (size_t)n_rows_s + 1, "row positions"))) return rc;
-  unsigned *k0, *k1;
-  unsigned long long *val0, *val1;
-  if ((rc = tmp.get(&k0, n_ch, "sort keys"))) return rc;
-  if ((rc = tmp.get(&k1, n_ch, "sort keys"))) return rc;
-  if ((rc = tmp.get(&val0, n_ch, "sort values"))) return rc;
-  if ((rc = tmp.get(&val1, n_ch, "sort values"))) return rc;
 
-  const dim3 grid_rows(blocks_for(max_rows), vpr);
+  const dim3 grid_rows(blocks_for((long long)max_rows * kTrLanes), vpr);
   FIR_HIP_CHECK(hipMemsetAsync(rc_cnt, 0, sizeof(int) * ((size_t)n_rows_s + 1),
                                c->stream));
   const XY xy = cache_xy(cache);
@@ -1346,9 +1406,18 @@ int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
   FIR_HIP_CHECK(hipMemcpyAsync(&counted, pos + n_rows_s, sizeof(int),
                                hipMemcpyDeviceToHost, c->stream));
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
-  if ((long long)counted != n_ch)
+  if (n_ch < 0) n_ch = counted;
+  if (counted < 0 || (long long)counted != n_ch)
     return fail(FIR_E_STATE, "path cache: voxels [%d, %d) hold %lld elements "
                              "by histogram, %d by rows", v0, v1, n_ch, counted);
+  unsigned *k0, *k1;
+  unsigned long long *val0, *val1;
+  if ((rc = tmp.get(&k0, n_ch, "sort keys"))) return rc;
+  if ((rc = tmp.get(&k1, n_ch, "sort keys"))) return rc;
+  if ((rc = tmp.get(&val0, n_ch, "sort values"))) return rc;
+  if ((rc = tmp.get(&val1, n_ch, "sort values"))) return rc;
+  if (n_ch == 0)
+    return offsets_from_sorted ? offsets_from_sorted(k0, 0) : FIR_OK;
   hipLaunchKernelGGL(k_tr_expand<F>, grid_rows, dim3(kBlock), 0, c->stream, sr,
                      d_prefix, rm->d_off, rm->d_idx, rm->d_yrow, xy,
                      (const typename Len<F>::T*)rm->d_len, v0, v1, pos, n_ch,
@@ -1367,6 +1436,9 @@ int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
   if ((rc = tmp.get(&sort_tmp, sb, "sort scratch"))) return rc;
   FIR_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
     sort_tmp, sb, keys, vals, (int)n_ch, 0, bits, c->stream));
+  if (offsets_from_sorted &&
+      (rc = offsets_from_sorted(keys.Current(), n_ch)))
+    return rc;
   hipLaunchKernelGGL(k_tr_split<F>, dim3(ew_blocks(n_ch)), dim3(kBlock), 0,
                      c->stream, keys.Current(), vals.Current(), n_ch, v0,
                      voff_p, voff_u, cache->bp_lanes, cache->d_trow,
@@ -1418,88 +1490,116 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
     FIR_HIP_CHECK(hipMemcpyAsync(d_prefix, prefix.data(), sizeof(int) * (vpr + 1),
                                  hipMemcpyHostToDevice, c->stream));
 
-    // Voxel histogram of the subset's (non-zero-length) elements; padded
-    // offsets (stored) and unpadded offsets (chunk planning, ranks).
+    // Voxel counts of the subset's (non-zero-length) elements give the
+    // padded offsets (stored) and unpadded offsets (chunk planning, ranks).
+    // A subset of at most cache_chunk slots is one chunk: its counts come
+    // from the sorted keys; larger subsets take a histogram first.
     unsigned* cnt;
     long long* voff_u;
     if ((rc = tmp.get(&cnt, nvox + 1, "voxel counts")) ||
         (rc = tmp.get(&voff_u, nvox + 1, "voxel offsets")))
       return rc;
-    FIR_HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned) * (nvox + 1),
-                                 c->stream));
-    if (max_rows > 0)
-    {
-      hipLaunchKernelGGL(k_tr_hist, dim3(blocks_for(max_rows), vpr),
-                         dim3(kBlock), 0, c->stream,
-                         SubsetRows{rm->d_view_row, S, s, 0}, rm->d_off,
-                         rm->d_idx, rm->d_len, (int)len_bytes(rm->fmt),
-                         rm->d_yrow, cache_xy(cache), cnt);
-      FIR_CHECK_LAUNCH();
-    }
     long long* voff_s = cache->d_voff + (long long)s * (nvox + 1);
-    hipcub::TransformInputIterator<long long, Pad4, const unsigned*> cnt_p(
-      cnt, Pad4());
-    hipcub::TransformInputIterator<long long, ToI64, const unsigned*> cnt_u(
-      cnt, ToI64());
-    size_t tb = 0, tb2 = 0;
-    FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveScan(
-      nullptr, tb, cnt_p, voff_s, hipcub::Sum(), base, (int)(nvox + 1),
-      c->stream));
-    FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, cnt_u, voff_u,
-                                                   (int)(nvox + 1), c->stream));
-    unsigned char* scan_tmp;
-    if ((rc = tmp.get(&scan_tmp, std::max(tb, tb2), "scan scratch"))) return rc;
-    FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveScan(
-      scan_tmp, tb, cnt_p, voff_s, hipcub::Sum(), base, (int)(nvox + 1),
-      c->stream));
-    FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tb2, cnt_u, voff_u,
-                                                   (int)(nvox + 1), c->stream));
     long long end_p = 0;
-    FIR_HIP_CHECK(hipMemcpyAsync(&end_p, voff_s + nvox, sizeof(long long),
-                                 hipMemcpyDeviceToHost, c->stream));
-    FIR_HIP_CHECK(hipMemcpyAsync(voff.data(), voff_u,
-                                 sizeof(long long) * (nvox + 1),
-                                 hipMemcpyDeviceToHost, c->stream));
-    FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
-    if (end_p > slots)
-      return fail(FIR_E_STATE, "path cache: transposed slots overflow");
-
-    const SubsetRows sr{rm->d_view_row, S, s, 0};
-    long long v0 = 0;
-    while (v0 < nvox && n_rows_s > 0)
+    auto offsets = [&]() -> int {
+      hipcub::TransformInputIterator<long long, Pad4, const unsigned*> cnt_p(
+        cnt, Pad4());
+      hipcub::TransformInputIterator<long long, ToI64, const unsigned*> cnt_u(
+        cnt, ToI64());
+      size_t tb = 0, tb2 = 0;
+      FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveScan(
+        nullptr, tb, cnt_p, voff_s, hipcub::Sum(), base, (int)(nvox + 1),
+        c->stream));
+      FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(
+        nullptr, tb2, cnt_u, voff_u, (int)(nvox + 1), c->stream));
+      unsigned char* scan_tmp;
+      int r2;
+      if ((r2 = tmp.get(&scan_tmp, std::max(tb, tb2), "scan scratch")))
+        return r2;
+      FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveScan(
+        scan_tmp, tb, cnt_p, voff_s, hipcub::Sum(), base, (int)(nvox + 1),
+        c->stream));
+      FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(
+        scan_tmp, tb2, cnt_u, voff_u, (int)(nvox + 1), c->stream));
+      FIR_HIP_CHECK(hipMemcpyAsync(&end_p, voff_s + nvox, sizeof(long long),
+                                   hipMemcpyDeviceToHost, c->stream));
+      FIR_HIP_CHECK(hipMemcpyAsync(voff.data(), voff_u,
+                                   sizeof(long long) * (nvox + 1),
+                                   hipMemcpyDeviceToHost, c->stream));
+      FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+      if (end_p > slots)
+        return fail(FIR_E_STATE, "path cache: transposed slots overflow");
+      return FIR_OK;
+    };
+    long long sub_slots = 0;
+    for (int k = 0; k < vpr; ++k)
     {
-      // Largest voxel range starting at v0 with at most kChunkElems elements.
-      const long long limit = voff[v0] + kChunkElems;
-      long long v1 = std::upper_bound(voff.begin() + v0 + 1, voff.end(), limit) -
-        voff.begin() - 1;
-      if (v1 <= v0)
-        return fail(FIR_E_STATE, "path cache: voxel %lld holds more than "
-                                 "%lld elements", v0, kChunkElems);
-      if (v1 > nvox) v1 = nvox;
-      const long long n_ch = voff[v1] - voff[v0];
-      if (n_ch > 0)
+      const int view = k * S + s;
+      sub_slots += rm->view_elem[view + 1] - rm->view_elem[view];
+    }
+    const SubsetRows sr{rm->d_view_row, S, s, 0};
+    auto chunk = [&](long long v0, long long v1, long long n_ch,
+                     const std::function<int(const unsigned*, long long)>&
+                       from_sorted) -> int {
+      switch (rm->fmt)
       {
-        switch (rm->fmt)
-        {
-        case FIR_LEN_SNORM16:
-          rc = transpose_chunk<FIR_LEN_SNORM16>(c, rm, cache, sr, d_prefix, vpr,
-                                                max_rows, n_rows_s, (int)v0,
-                                                (int)v1, voff_s, voff_u, n_ch);
-          break;
-        case FIR_LEN_FP16:
-          rc = transpose_chunk<FIR_LEN_FP16>(c, rm, cache, sr, d_prefix, vpr,
-                                             max_rows, n_rows_s, (int)v0,
-                                             (int)v1, voff_s, voff_u, n_ch);
-          break;
-        default:
-          rc = transpose_chunk<FIR_LEN_FP32>(c, rm, cache, sr, d_prefix, vpr,
-                                             max_rows, n_rows_s, (int)v0,
-                                             (int)v1, voff_s, voff_u, n_ch);
-          break;
-        }
-        if (rc) return rc;
+      case FIR_LEN_SNORM16:
+        return transpose_chunk<FIR_LEN_SNORM16>(
+          c, rm, cache, sr, d_prefix, vpr, max_rows, n_rows_s, (int)v0,
+          (int)v1, voff_s, voff_u, n_ch, from_sorted);
+      case FIR_LEN_FP16:
+        return transpose_chunk<FIR_LEN_FP16>(
+          c, rm, cache, sr, d_prefix, vpr, max_rows, n_rows_s, (int)v0,
+          (int)v1, voff_s, voff_u, n_ch, from_sorted);
+      default:
+        return transpose_chunk<FIR_LEN_FP32>(
+          c, rm, cache, sr, d_prefix, vpr, max_rows, n_rows_s, (int)v0,
+          (int)v1, voff_s, voff_u, n_ch, from_sorted);
       }
-      v0 = v1;
+    };
+    const long long chunk_max = c->cache_chunk;
+    if (n_rows_s > 0 && sub_slots <= chunk_max)
+    {
+      rc = chunk(0, nvox, -1, [&](const unsigned* keys, long long n) -> int {
+        hipLaunchKernelGGL(k_tr_counts_sorted, dim3(blocks_for(nvox + 1)),
+                           dim3(kBlock), 0, c->stream, keys, n, (int)nvox,
+                           cnt);
+        FIR_CHECK_LAUNCH();
+        return offsets();
+      });
+      if (rc) return rc;
+    }
+    else
+    {
+      FIR_HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned) * (nvox + 1),
+                                   c->stream));
+      if (max_rows > 0)
+      {
+        hipLaunchKernelGGL(k_tr_hist,
+                           dim3(blocks_for((long long)max_rows * kTrLanes), vpr),
+                           dim3(kBlock), 0, c->stream, sr, rm->d_off,
+                           rm->d_idx, rm->d_len, (int)len_bytes(rm->fmt),
+                           rm->d_yrow, cache_xy(cache), cnt);
+        FIR_CHECK_LAUNCH();
+      }
+      if ((rc = offsets())) return rc;
+      long long v0 = 0;
+      while (v0 < nvox && n_rows_s > 0)
+      {
+        // Largest voxel range starting at v0 with at most chunk_max
+        // elements.
+        const long long limit = voff[v0] + chunk_max;
+        long long v1 =
+          std::upper_bound(voff.begin() + v0 + 1, voff.end(), limit) -
+          voff.begin() - 1;
+        if (v1 <= v0)
+          return fail(FIR_E_STATE, "path cache: voxel %lld holds more than "
+                                   "%lld elements", v0, chunk_max);
+        if (v1 > nvox) v1 = nvox;
+        const long long n_ch = voff[v1] - voff[v0];
+        if (n_ch > 0 && (rc = chunk(v0, v1, n_ch, {}))) return rc;
+        v0 = v1;
+      }
     }
     base = end_p;
     real += voff[nvox];
@@ -1648,6 +1748,15 @@ int fir_set_cache_lanes(fir_ctx* c, int fwd_lanes, int bp_lanes)
                            "from 1 to 64");
   c->cache_lanes[0] = fwd_lanes;
   c->cache_lanes[1] = bp_lanes;
+  return FIR_OK;
+}
+
+int fir_set_cache_chunk(fir_ctx* c, int64_t max_elements)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (max_elements < 1024 || max_elements > (1LL << 30))
+    return fail(FIR_E_ARG, "transpose chunk must hold 1024 to 2^30 elements");
+  c->cache_chunk = max_elements;
   return FIR_OK;
 }
 

@@ -45,7 +45,8 @@ EXPORTS = (
     "fir_div_clamped", "fir_exp_clamped", "fir_hu_to_mu", "fir_path_dump",
     "fir_path_count", "fir_poisson_nll_grad", "fir_osem_project_batch",
     "fir_set_uneven_subsets", "fir_set_lor_order", "fir_set_scatter_copies",
-    "fir_set_cache_lanes", "fir_set_cache_zslices", "fir_cache_create",
+    "fir_set_cache_lanes", "fir_set_cache_zslices", "fir_set_cache_chunk",
+    "fir_cache_create",
     "fir_cache_create_partial", "fir_cache_subsets", "fir_cache_destroy",
     "fir_cache_info",
     "fir_cache_tune", "fir_cache_forward", "fir_cache_backward",
@@ -135,6 +136,7 @@ def load() -> C.CDLL:
         "fir_cache_tune": [vp, i, i, i],
         "fir_set_cache_lanes": [vp, i, i],
         "fir_set_cache_zslices": [vp, i],
+        "fir_set_cache_chunk": [vp, i64],
         "fir_cache_forward": [vp, vp, vp],
         "fir_cache_backward": [vp, vp, vp],
         "fir_cache_sensitivity": [vp, vp],
@@ -235,6 +237,11 @@ class Context:
         """z-slices interleaved in the gather layouts of the next caches."""
         _check(self.lib.fir_set_cache_zslices(self.h, zq))
         self._cache_zq = zq
+
+    def set_cache_chunk(self, max_elements: int):
+        """Largest number of path elements one transpose step of the next
+        cache builds sorts (24 bytes of temporaries per element)."""
+        _check(self.lib.fir_set_cache_chunk(self.h, int(max_elements)))
 
     def set_uneven_subsets(self, allow: bool):
         _check(self.lib.fir_set_uneven_subsets(self.h, int(allow)))

@@ -342,6 +342,12 @@ int fir_set_cache_lanes(fir_ctx* ctx, int fwd_lanes, int bp_lanes);
  * 1 or 2 is faster). Results do not depend on it. */
 int fir_set_cache_zslices(fir_ctx* ctx, int zslices);
 
+/* Largest number of path elements one step of a cache build's transpose
+ * sorts, 1024 to 2^30 (default 2^28); the step's temporaries take 24 bytes
+ * per element, so smaller steps fit beside larger caches. A subset's
+ * voxel lists come out the same whatever the step. */
+int fir_set_cache_chunk(fir_ctx* ctx, int64_t max_elements);
+
 /* Build the cache of `vol` for `n_subsets` subsets (views v with
  * v % n_subsets == s, as LORCache.cc:49-51). Synchronizes the context's
  * stream; FIR_E_NOMEM if the matrices do not fit in device memory. */

@@ -117,3 +117,32 @@ def test_partial_cache_budget(c1):
     assert big.n_cached == S
     assert big.info()["elements"] == whole.info()["elements"]
     big.close()
+
+
+def test_chunked_transpose_gives_the_same_cache(c1):
+    """A transpose in many small steps (voxel histogram first, then sorted
+    voxel ranges) stores the same voxel lists as the one-step transpose
+    (counts taken from the sorted keys): bit-identical projections."""
+    w, ctx, v, whole, part, g = c1
+    x = torch.from_numpy(g["act"].ravel() + 0.1).cuda()
+    y = torch.from_numpy(g["y"]).cuda()
+    ctx.set_cache_chunk(20_000)
+    try:
+        small = ctx.path_cache(v, S)
+    finally:
+        ctx.set_cache_chunk(1 << 28)
+    assert small.info()["elements"] > 10 * 20_000
+    fa = torch.zeros(S * v.n_voxels, device="cuda")
+    fb = torch.zeros(S * v.n_voxels, device="cuda")
+    whole.backward(y, fa)
+    small.backward(y, fb)
+    assert torch.equal(fa, fb)
+    for s in range(S):
+        a = torch.zeros(v.n_voxels, device="cuda")
+        b = torch.zeros(v.n_voxels, device="cuda")
+        whole.osem_project([x], [y], None, [a], s)
+        small.osem_project([x], [y], None, [b], s)
+        assert torch.equal(a, b)
+    with pytest.raises(firlib.FirError):
+        ctx.set_cache_chunk(10)
+    small.close()
