@@ -659,27 +659,44 @@ __global__ void k_osem_update(float* __restrict__ x, float* __restrict__ bp,
 // axis 0: x, 1: y, 2: z. For axis 2 with `restore`, voxels at radius >=
 // restore_r take keep[] (edge restore, operations.cc:158-185); with
 // cut > 0 the circular cut follows (operations.cc:194-220).
-template <int AXIS, bool EPILOGUE>
-__global__ __launch_bounds__(kBlock) void k_gauss(const float* __restrict__ in,
-                                                  float* __restrict__ out,
-                                                  const float* __restrict__ taps,
-                                                  int h, int nx, int ny, int nz,
-                                                  CutGeom c, int restore,
-                                                  double restore_r, float cut)
+constexpr int kGaussX = 32, kGaussY = kBlock / kGaussX;
+constexpr int kGaussMaxH = 7; // taps per side loaded up front (15 taps)
+
+// Sum over the in-bounds taps of one axis, in tap order: s = sum(k * v),
+// w = sum(k) (operations.cc:83-156). The taps' loads come first (clamped to
+// the image, so always in bounds) so that they are in flight together.
+__device__ __forceinline__ void gauss_sum(const float* __restrict__ in,
+                                          long long i, int coord, int lim,
+                                          long long stride,
+                                          const float* __restrict__ taps,
+                                          int h, float* s_out, float* w_out)
 {
-  const long long n = (long long)nx * ny * nz;
-  const long long plane = (long long)nx * ny;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
+  float s = 0.0f, w = 0.0f;
+  if (h <= kGaussMaxH)
   {
-    const int k = (int)(i / plane);
-    const long long r = i - (long long)k * plane;
-    const int j = (int)(r / nx);
-    const int ii = (int)(r - (long long)j * nx);
-    const int coord = AXIS == 0 ? ii : (AXIS == 1 ? j : k);
-    const int lim = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
-    const long long stride = AXIS == 0 ? 1 : (AXIS == 1 ? nx : plane);
-    float s = 0.0f, w = 0.0f;
+    float t[2 * kGaussMaxH + 1];
+#pragma unroll
+    for (int q = 0; q <= 2 * kGaussMaxH; ++q)
+      if (q <= 2 * h)
+      {
+        int cq = coord + q - h;
+        cq = cq < 0 ? 0 : (cq >= lim ? lim - 1 : cq);
+        t[q] = in[i + (long long)(cq - coord) * stride];
+      }
+#pragma unroll
+    for (int q = 0; q <= 2 * kGaussMaxH; ++q)
+      if (q <= 2 * h)
+      {
+        const int cq = coord + q - h;
+        if (cq >= 0 && cq < lim)
+        {
+          const float kv = taps[q];
+          s += kv * t[q];
+          w += kv;
+        }
+      }
+  }
+  else
     for (int q = -h; q <= h; ++q)
     {
       const int cq = coord + q;
@@ -690,6 +707,32 @@ __global__ __launch_bounds__(kBlock) void k_gauss(const float* __restrict__ in,
         w += kv;
       }
     }
+  *s_out = s;
+  *w_out = w;
+}
+
+template <int AXIS, bool EPILOGUE>
+__global__ __launch_bounds__(kBlock) void k_gauss(const float* __restrict__ in,
+                                                  float* __restrict__ out,
+                                                  const float* __restrict__ taps,
+                                                  int h, int nx, int ny, int nz,
+                                                  CutGeom c, int restore,
+                                                  double restore_r, float cut)
+{
+  // One thread per voxel, its (x, y, z) taken from the 3D grid (no index
+  // divisions): blocks of kGaussX x kGaussY voxels of one xy-plane.
+  const int ii = blockIdx.x * kGaussX + (int)threadIdx.x % kGaussX;
+  const int j = blockIdx.y * kGaussY + (int)threadIdx.x / kGaussX;
+  const int k = blockIdx.z;
+  const long long plane = (long long)nx * ny;
+  if (ii < nx && j < ny)
+  {
+    const long long i = (long long)k * plane + (long long)j * nx + ii;
+    const int coord = AXIS == 0 ? ii : (AXIS == 1 ? j : k);
+    const int lim = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
+    const long long stride = AXIS == 0 ? 1 : (AXIS == 1 ? nx : plane);
+    float s, w;
+    gauss_sum(in, i, coord, lim, stride, taps, h, &s, &w);
     float o;
     if (w > 0.0f)
       o = s / w;
@@ -1304,16 +1347,17 @@ int convolve_frame(fir_ctx* c, const fir_vol_desc* v, float* img,
   const float fw = fwhm[0] > fwhm[1] ? fwhm[0] : fwhm[1];
   const int restore = restore_cut > 0.0f ? 1 : 0;
   const double rr = (double)restore_cut - 5.0 * fw;
-  const unsigned nb = ew_blocks(n);
-  hipLaunchKernelGGL((k_gauss<0, false>), dim3(nb), dim3(kBlock), 0,
+  const dim3 nb((v->nx + kGaussX - 1) / kGaussX,
+                (v->ny + kGaussY - 1) / kGaussY, v->nz);
+  hipLaunchKernelGGL((k_gauss<0, false>), nb, dim3(kBlock), 0,
                      c->stream, img, t1, ts->d, ts->h[0], v->nx,
                      v->ny, v->nz, cg, 0, 0.0, 0.0f);
-  hipLaunchKernelGGL((k_gauss<1, false>), dim3(nb), dim3(kBlock), 0,
+  hipLaunchKernelGGL((k_gauss<1, false>), nb, dim3(kBlock), 0,
                      c->stream, t1, t2, ts->d + 64, ts->h[1], v->nx,
                      v->ny, v->nz, cg, 0, 0.0, 0.0f);
   // Z pass writes img in place: each voxel reads its own img value (the
   // pre-convolution copy) only for the edge restore.
-  hipLaunchKernelGGL((k_gauss<2, true>), dim3(nb), dim3(kBlock), 0, c->stream,
+  hipLaunchKernelGGL((k_gauss<2, true>), nb, dim3(kBlock), 0, c->stream,
                      t2, img, ts->d + 128, ts->h[2], v->nx, v->ny,
                      v->nz, cg, restore, rr, final_cut);
   FIR_CHECK_LAUNCH();

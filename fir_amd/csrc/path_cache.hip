@@ -947,25 +947,35 @@ struct ImgIn
 // X[vx * NB + b] = x_b[v] (if X is not NULL) and XT[vy * NB + b] = x_b[v]
 // with vx, vy the indices of v in the x- and y-fastest layouts (0 for
 // b >= n_images).
+// The gather copies of the images of one launch, written in their own
+// order (so that each wave's stores are contiguous): blockIdx.y < nb_x
+// fills row b = blockIdx.y of the x-fastest copy (a = x), the others the
+// y-fastest copy (b = x, a = y); blockIdx.z is a group of zq slices; a
+// thread writes one (a, z-slice, image) element, gathering it from the
+// x-fastest images. Slices past nz (layout padding) get 0.
 template <int NB>
 __global__ void k_pc_pack(float* __restrict__ X, float* __restrict__ XT,
-                          ImgIn in, int n_images, long long nvox, XY xy)
+                          ImgIn in, int n_images, XY xy, int zq_shift, int nz)
 {
-  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-       v < nvox; v += (long long)gridDim.x * blockDim.x)
-  {
-    float t[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) t[b] = b < n_images ? in.p[b][v] : 0.0f;
-    const long long vx = to_layout(xy, (int)v, false);
-    const long long vy = to_layout(xy, (int)v, true);
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-    {
-      if (X) X[vx * NB + b] = t[b];
-      XT[vy * NB + b] = t[b];
-    }
-  }
+  const bool xcopy = (int)blockIdx.y < xy.ny;
+  if (xcopy && !X) return;
+  const int b = xcopy ? (int)blockIdx.y : (int)blockIdx.y - xy.ny;
+  const int na = xcopy ? xy.nx : xy.ny, nbb = xcopy ? xy.ny : xy.nx;
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= na * xy.zq * NB) return;
+  const int img = t % NB;
+  const int r = t / NB;
+  const int zl = r & (xy.zq - 1);
+  const int a = r >> zq_shift;
+  const int zb = blockIdx.z;
+  const int z = (zb << zq_shift) + zl;
+  const int x = xcopy ? a : b, y = xcopy ? b : a;
+  float v = 0.0f;
+  if (z < nz && img < n_images)
+    v = in.p[img][((long long)z * xy.ny + y) * xy.nx + x];
+  const long long dst =
+    ((((long long)zb * nbb + b) * na + a) << zq_shift) + zl;
+  (xcopy ? X : XT)[dst * NB + img] = v;
 }
 
 } // namespace
@@ -1647,16 +1657,22 @@ unsigned grid_blocks(const fir_cache* cache, long long threads)
 // x-fastest copy) of the images of one launch.
 void pack_images(fir_cache* cache, int wd, const ImgIn& in, int nb)
 {
-  const unsigned blocks = ew_blocks(cache->nvox);
   const XY xy = cache_xy(cache);
+  int zq_shift = 0;
+  while ((1 << zq_shift) < xy.zq) ++zq_shift;
+  const int na = std::max(xy.nx, xy.ny);
+  const dim3 grid(blocks_for((long long)na * xy.zq * wd),
+                  (unsigned)(xy.nx + xy.ny),
+                  (unsigned)((cache->vol.nz + xy.zq - 1) / xy.zq));
   hipStream_t st = cache->ctx->stream;
   float* X = wd > 1 || xy.zq > 1 ? cache->d_x : nullptr;
+  const int nz = cache->vol.nz;
   switch (wd)
   {
-  case 1: hipLaunchKernelGGL(k_pc_pack<1>, dim3(blocks), dim3(kBlock), 0, st, X, cache->d_xy, in, nb, cache->nvox, xy); break;
-  case 2: hipLaunchKernelGGL(k_pc_pack<2>, dim3(blocks), dim3(kBlock), 0, st, X, cache->d_xy, in, nb, cache->nvox, xy); break;
-  case 4: hipLaunchKernelGGL(k_pc_pack<4>, dim3(blocks), dim3(kBlock), 0, st, X, cache->d_xy, in, nb, cache->nvox, xy); break;
-  default: hipLaunchKernelGGL(k_pc_pack<8>, dim3(blocks), dim3(kBlock), 0, st, X, cache->d_xy, in, nb, cache->nvox, xy); break;
+  case 1: hipLaunchKernelGGL(k_pc_pack<1>, grid, dim3(kBlock), 0, st, X, cache->d_xy, in, nb, xy, zq_shift, nz); break;
+  case 2: hipLaunchKernelGGL(k_pc_pack<2>, grid, dim3(kBlock), 0, st, X, cache->d_xy, in, nb, xy, zq_shift, nz); break;
+  case 4: hipLaunchKernelGGL(k_pc_pack<4>, grid, dim3(kBlock), 0, st, X, cache->d_xy, in, nb, xy, zq_shift, nz); break;
+  default: hipLaunchKernelGGL(k_pc_pack<8>, grid, dim3(kBlock), 0, st, X, cache->d_xy, in, nb, xy, zq_shift, nz); break;
   }
 }
 
