@@ -330,6 +330,13 @@ def main():
 
     stream = torch.cuda.current_stream()
 
+    # --- the drop-in executable, end to end (before this process takes the
+    # device memory the caches need) -------------------------------------
+    cli = None
+    if counts is not None:
+        cli = cli_end_to_end(w, counts, hu)
+        del counts
+
     # --- path cache, row order and lanes picked by the autotuner ---------
     from fir_amd import autotune
     cache_t0 = time.perf_counter()
@@ -523,13 +530,6 @@ def main():
 
     # --- host I/O: sinogram transfers from page-locked vs ordinary memory
     host_io = host_io_rates(w, sino)
-
-    # --- the drop-in executable, end to end ------------------------------
-    cli = None
-    if counts is not None:
-        cache.close()
-        torch.cuda.empty_cache()
-        cli = cli_end_to_end(w, counts, hu)
 
     # --- roofline of the dominant kernel pair (cached OSEM projector) ----
     # Algorithmic bytes per launch (one sub-iteration's LORs): forward +

@@ -53,7 +53,8 @@ def osem_params(path, out, n_iter, n_sub, **kw):
              ("output volume file name", out),
              ("number of iterations", str(n_iter)),
              ("number of subsets", str(n_sub))]
-    items += [(k.replace("_", " "), v) for k, v in kw.items()]
+    extra = {k.replace("_", " "): v for k, v in kw.items()}
+    items = [(k, extra.pop(k, v)) for k, v in items] + list(extra.items())
     io.write_params(str(path), "OSEM PARAMETERS", items)
 
 
@@ -226,6 +227,48 @@ def test_osem_rejects_non_divisor_subsets(work):
     r = exe("FIR_OSEM", "bad.params", cwd=work, check=False)
     assert r.returncode != 0
     assert "Number of subsets must be a divisor of 64" in r.stderr
+
+
+def test_forward_proj_rejects_bad_frame(work):
+    """A frame index outside the volume is an error (ForwardProj.cc:170-178)."""
+    io.write_params(str(work / "fbad.params"), "FORWARD PROJECTION PARAMETERS",
+                    [("input volume file", "act.h33"),
+                     ("scanner file", "scanner.scan"),
+                     ("output projection header", "tmpl.hs"),
+                     ("output projection file name", "yb"),
+                     ("frames to project", "{3}")])
+    r = exe("FIR_ForwardProj", "fbad.params", cwd=work, check=False)
+    assert r.returncode != 0
+    assert "Frame indices must be between 0 and 0" in r.stderr
+
+
+def test_osem_rejects_scanner_projection_mismatch(work):
+    """A sinogram of another crystal count than the scanner's is an error
+    (ScannerData.cc:31-44)."""
+    import dataclasses
+    spec = dataclasses.replace(synth.c1().proj, n_crystals_per_ring=64,
+                               n_tang_coords=33)
+    n = sum(4 - abs(s) for s in range(-3, 4)) * 32 * 33
+    io.write_proj(str(work / "y64.hs"), spec, np.zeros(n, np.float32))
+    osem_params(work / "mm.params", "xmm", 1, 4,
+                input_projection_file="y64.hs")
+    r = exe("FIR_OSEM", "mm.params", cwd=work, check=False)
+    assert r.returncode != 0
+    assert "same number of crystals per ring (128)" in r.stderr
+
+
+def test_osem_rejects_mismatched_sensitivity_file(work):
+    """A sensitivity map file (read, recompute flag 0) whose header differs
+    from the output volume's is an error (OSEM.cc:238-242)."""
+    import dataclasses
+    vol = dataclasses.replace(synth.c1().vol, size=(32, 32, 8), n_frames=4)
+    vol.write(str(work / "sens32.h33"),
+              np.ones(4 * 32 * 32 * 8, np.float32))
+    osem_params(work / "sm.params", "xsm", 1, 4,
+                sensitivity_map_volume="sens32.h33")
+    r = exe("FIR_OSEM", "sm.params", "0", cwd=work, check=False)
+    assert r.returncode != 0
+    assert "doesn't fit with output volume" in r.stderr
 
 
 def test_missing_parameter_file(work):
