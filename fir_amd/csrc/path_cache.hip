@@ -1100,6 +1100,60 @@ struct DevBufs
   }
 };
 
+// Device buffers reused by the steps of one transpose: a step asks for at
+// least `count` elements of buffer `slot` and gets that slot's buffer,
+// reallocated only when it is too small. This avoids a multi-GB
+// hipMalloc/hipFree pair per subset: freed device memory is cleared before
+// it is handed out again, which costs seconds once most of it is in use.
+enum PoolSlot
+{
+  kPoolRowCounts,
+  kPoolRowPos,
+  kPoolRowScan,
+  kPoolKeys0,
+  kPoolKeys1,
+  kPoolVals0,
+  kPoolVals1,
+  kPoolSort,
+  kPoolPrefix,
+  kPoolVoxCounts,
+  kPoolVoxOffsets,
+  kPoolVoxScan,
+  kPoolSlots
+};
+
+struct Pool
+{
+  void* p[kPoolSlots] = {};
+  size_t bytes[kPoolSlots] = {};
+  ~Pool()
+  {
+    for (void* q : p) (void)hipFree(q);
+  }
+  template <class T>
+  int get(T** out, size_t count, PoolSlot slot, const char* what)
+  {
+    const size_t need = (count ? count : 1) * sizeof(T);
+    if (bytes[slot] < need)
+    {
+      (void)hipFree(p[slot]);
+      p[slot] = nullptr;
+      bytes[slot] = 0;
+      if (hipMalloc(&p[slot], need) != hipSuccess)
+      {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return fail(FIR_E_NOMEM,
+                    "path cache: allocation of %zu bytes for %s failed", need,
+                    what);
+      }
+      bytes[slot] = need;
+    }
+    *out = static_cast<T*>(p[slot]);
+    return FIR_OK;
+  }
+};
+
 template <class T>
 int dev_alloc(T** out, size_t count, const char* what)
 {
@@ -1187,20 +1241,20 @@ int plan_subsets(fir_ctx* c, const int* count, long long n, long long per_view,
   const long long held = 20LL * n; // count, maxlen, row ids, offsets
   const long long avail = (long long)free_b - (1LL << 30);
   const int cap = plan->max_cached > 0 ? std::min(plan->max_cached, S) : S;
-  long long R = 0, SL = 0, E = 0, maxE = 0, cost1 = 0;
+  long long R = 0, SL = 0, E = 0, maxSL = 0, cost1 = 0;
   int k = 0;
   for (int s = 0; s < cap; ++s)
   {
     R += rows[s];
     SL += slots[s];
     E += elems[s];
-    maxE = std::max(maxE, elems[s]);
+    maxSL = std::max(maxSL, slots[s]);
     const long long cost =
       R * (4 + 4 + 1 + 8) + SL * (4 + lb) + (V + 1LL) * 12 +         // rows
       (E + 3LL * (s + 1) * plan->nvox) * (4 + lb) +                     // lists
       (s + 1LL) * (plan->nvox + 1) * 8 +                                // offsets
       R * kMaxCacheBatch * 4 + plan->image_bytes +                      // weights
-      std::max(held, 24 * std::min(c->cache_chunk, maxE) + 16 * plan->nvox);
+      std::max(held, 24 * std::min(c->cache_chunk, maxSL) + 16 * plan->nvox);
     if (s == 0) cost1 = cost;
     if ((plan->max_bytes > 0 && cost > plan->max_bytes) || cost - held > avail)
       break;
@@ -1384,17 +1438,19 @@ int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
                     const long long* voff_p, const long long* voff_u,
                     long long n_ch,
                     const std::function<int(const unsigned*, long long)>&
-                      offsets_from_sorted)
+                      offsets_from_sorted,
+                    Pool& tmp)
 {
   // n_ch < 0: the chunk is the whole subset and its element count comes
   // from the rows; the voxel offsets are then computed from the sorted keys
   // (offsets_from_sorted) before the split.
-  DevBufs tmp; // the chunk's scratch, released on return
   int rc;
   int* rc_cnt;
   int* pos;
-  if ((rc = tmp.get(&rc_cnt, (size_t)n_rows_s + 1, "row counts"))) return rc;
-  if ((rc = tmp.get(&pos, (size_t)n_rows_s + 1, "row positions"))) return rc;
+  if ((rc = tmp.get(&rc_cnt, (size_t)n_rows_s + 1, kPoolRowCounts,
+                    "row counts")) ||
+      (rc = tmp.get(&pos, (size_t)n_rows_s + 1, kPoolRowPos, "row positions")))
+    return rc;
 
   const dim3 grid_rows(blocks_for((long long)max_rows * kTrLanes), vpr);
   FIR_HIP_CHECK(hipMemsetAsync(rc_cnt, 0, sizeof(int) * ((size_t)n_rows_s + 1),
@@ -1409,7 +1465,7 @@ int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
   FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, rc_cnt, pos,
                                                  n_rows_s + 1, c->stream));
   unsigned char* scan_tmp;
-  if ((rc = tmp.get(&scan_tmp, tb, "scan scratch"))) return rc;
+  if ((rc = tmp.get(&scan_tmp, tb, kPoolRowScan, "scan scratch"))) return rc;
   FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tb, rc_cnt, pos,
                                                  n_rows_s + 1, c->stream));
   int counted = -1;
@@ -1422,10 +1478,11 @@ int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
                              "by histogram, %d by rows", v0, v1, n_ch, counted);
   unsigned *k0, *k1;
   unsigned long long *val0, *val1;
-  if ((rc = tmp.get(&k0, n_ch, "sort keys"))) return rc;
-  if ((rc = tmp.get(&k1, n_ch, "sort keys"))) return rc;
-  if ((rc = tmp.get(&val0, n_ch, "sort values"))) return rc;
-  if ((rc = tmp.get(&val1, n_ch, "sort values"))) return rc;
+  if ((rc = tmp.get(&k0, n_ch, kPoolKeys0, "sort keys")) ||
+      (rc = tmp.get(&k1, n_ch, kPoolKeys1, "sort keys")) ||
+      (rc = tmp.get(&val0, n_ch, kPoolVals0, "sort values")) ||
+      (rc = tmp.get(&val1, n_ch, kPoolVals1, "sort values")))
+    return rc;
   if (n_ch == 0)
     return offsets_from_sorted ? offsets_from_sorted(k0, 0) : FIR_OK;
   hipLaunchKernelGGL(k_tr_expand<F>, grid_rows, dim3(kBlock), 0, c->stream, sr,
@@ -1443,7 +1500,7 @@ int transpose_chunk(fir_ctx* c, const RowMatrix* rm, fir_cache* cache,
   FIR_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
     nullptr, sb, keys, vals, (int)n_ch, 0, bits, c->stream));
   unsigned char* sort_tmp;
-  if ((rc = tmp.get(&sort_tmp, sb, "sort scratch"))) return rc;
+  if ((rc = tmp.get(&sort_tmp, sb, kPoolSort, "sort scratch"))) return rc;
   FIR_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
     sort_tmp, sb, keys, vals, (int)n_ch, 0, bits, c->stream));
   if (offsets_from_sorted &&
@@ -1481,9 +1538,41 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
 
   std::vector<long long> voff(nvox + 1);
   long long base = 0, real = 0;
+  Pool tmp;
+  // The largest sort any subset needs, reserved up front so that the
+  // buffers are allocated once.
+  {
+    long long max_slots = 0;
+    int max_rows_s = 0;
+    for (int s = 0; s < K; ++s)
+    {
+      long long sl = 0;
+      int nr = 0;
+      for (int k = 0; k < fir::views_in_subset(rm->n_views, S, s); ++k)
+      {
+        const int view = k * S + s;
+        sl += rm->view_elem[view + 1] - rm->view_elem[view];
+        nr += rm->view_row[view + 1] - rm->view_row[view];
+      }
+      max_slots = std::max(max_slots, sl);
+      max_rows_s = std::max(max_rows_s, nr);
+    }
+    const size_t n_sort = (size_t)std::min(max_slots, c->cache_chunk);
+    unsigned* ku;
+    unsigned long long* vu;
+    int* ri;
+    if ((rc = tmp.get(&ri, (size_t)max_rows_s + 1, kPoolRowCounts,
+                      "row counts")) ||
+        (rc = tmp.get(&ri, (size_t)max_rows_s + 1, kPoolRowPos,
+                      "row positions")) ||
+        (rc = tmp.get(&ku, n_sort, kPoolKeys0, "sort keys")) ||
+        (rc = tmp.get(&ku, n_sort, kPoolKeys1, "sort keys")) ||
+        (rc = tmp.get(&vu, n_sort, kPoolVals0, "sort values")) ||
+        (rc = tmp.get(&vu, n_sort, kPoolVals1, "sort values")))
+      return rc;
+  }
   for (int s = 0; s < K; ++s)
   {
-    DevBufs tmp;
     const int vpr = fir::views_in_subset(rm->n_views, S, s);
     std::vector<int> prefix(vpr + 1, 0);
     int max_rows = 0;
@@ -1496,7 +1585,8 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
     }
     const int n_rows_s = prefix[vpr];
     int* d_prefix;
-    if ((rc = tmp.get(&d_prefix, vpr + 1, "subset row prefix"))) return rc;
+    if ((rc = tmp.get(&d_prefix, vpr + 1, kPoolPrefix, "subset row prefix")))
+      return rc;
     FIR_HIP_CHECK(hipMemcpyAsync(d_prefix, prefix.data(), sizeof(int) * (vpr + 1),
                                  hipMemcpyHostToDevice, c->stream));
 
@@ -1506,8 +1596,8 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
     // from the sorted keys; larger subsets take a histogram first.
     unsigned* cnt;
     long long* voff_u;
-    if ((rc = tmp.get(&cnt, nvox + 1, "voxel counts")) ||
-        (rc = tmp.get(&voff_u, nvox + 1, "voxel offsets")))
+    if ((rc = tmp.get(&cnt, nvox + 1, kPoolVoxCounts, "voxel counts")) ||
+        (rc = tmp.get(&voff_u, nvox + 1, kPoolVoxOffsets, "voxel offsets")))
       return rc;
     long long* voff_s = cache->d_voff + (long long)s * (nvox + 1);
     long long end_p = 0;
@@ -1524,7 +1614,8 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
         nullptr, tb2, cnt_u, voff_u, (int)(nvox + 1), c->stream));
       unsigned char* scan_tmp;
       int r2;
-      if ((r2 = tmp.get(&scan_tmp, std::max(tb, tb2), "scan scratch")))
+      if ((r2 = tmp.get(&scan_tmp, std::max(tb, tb2), kPoolVoxScan,
+                        "scan scratch")))
         return r2;
       FIR_HIP_CHECK(hipcub::DeviceScan::ExclusiveScan(
         scan_tmp, tb, cnt_p, voff_s, hipcub::Sum(), base, (int)(nvox + 1),
@@ -1556,15 +1647,15 @@ int build_transpose(fir_ctx* c, fir_cache* cache)
       case FIR_LEN_SNORM16:
         return transpose_chunk<FIR_LEN_SNORM16>(
           c, rm, cache, sr, d_prefix, vpr, max_rows, n_rows_s, (int)v0,
-          (int)v1, voff_s, voff_u, n_ch, from_sorted);
+          (int)v1, voff_s, voff_u, n_ch, from_sorted, tmp);
       case FIR_LEN_FP16:
         return transpose_chunk<FIR_LEN_FP16>(
           c, rm, cache, sr, d_prefix, vpr, max_rows, n_rows_s, (int)v0,
-          (int)v1, voff_s, voff_u, n_ch, from_sorted);
+          (int)v1, voff_s, voff_u, n_ch, from_sorted, tmp);
       default:
         return transpose_chunk<FIR_LEN_FP32>(
           c, rm, cache, sr, d_prefix, vpr, max_rows, n_rows_s, (int)v0,
-          (int)v1, voff_s, voff_u, n_ch, from_sorted);
+          (int)v1, voff_s, voff_u, n_ch, from_sorted, tmp);
       }
     };
     const long long chunk_max = c->cache_chunk;
