@@ -86,6 +86,8 @@ struct fir_ctx
   // Largest number of path elements one transpose step sorts (its
   // temporaries take 24 bytes per element).
   long long cache_chunk = 1LL << 28;
+  // Whether the path matrices built afterwards get the sheet form.
+  int cache_sheets = 1;
   // Path matrices of live caches, shared by caches of one volume, walk mode
   // and length format.
   std::vector<std::weak_ptr<RowMatrix>> row_matrices;

@@ -978,6 +978,292 @@ __global__ void k_pc_pack(float* __restrict__ X, float* __restrict__ XT,
   (xcopy ? X : XT)[dst * NB + img] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Sheet form of the path matrix
+// ---------------------------------------------------------------------------
+// With rows in tangential-major order, the rows of one (view, tangential
+// bin) -- a family, every ring pair of one line of the xy-plane -- are
+// contiguous, and all of them cross the same pixels of the xy-plane: the
+// family's sheet, those pixels times all slices (at most nx + ny - 1 pixels,
+// so 15 K voxels at 160 x 160 x 47). An element is then indexed within its
+// family's sheet (pixel rank * nz + z, 16 bits), and a projector stages the
+// sheet in LDS once per family instead of gathering each element from the
+// image through the vector L1.
+
+// Largest sheet, in floats, that a block stages (64 KB of LDS).
+constexpr int kSheetMaxFloats = 16384;
+constexpr int kSheetBlock = 1024;
+constexpr int kSheetQuads = 2; // quads in flight per lane (2 measured best)
+
+// First row of every family: the rows before the family's first LOR.
+__global__ void k_sh_fam_rows(const int* __restrict__ rowid, int n_pairs,
+                              int n_fam, int n_rows, int* __restrict__ fam_row)
+{
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < n_fam) fam_row[f] = rowid[(long long)f * n_pairs];
+  else if (f == n_fam) fam_row[f] = n_rows;
+}
+
+__device__ __forceinline__ int family_of_row(const int* __restrict__ fam_row,
+                                             int n_fam, int r)
+{
+  int lo = 0, hi = n_fam - 1; // last family whose first row is <= r
+  while (lo < hi)
+  {
+    const int mid = (lo + hi + 1) >> 1;
+    if (fam_row[mid] <= r) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Pixel bitmap of every family: the pixels its rows' (contributing)
+// elements cross. kTrLanes lanes per row.
+__global__ __launch_bounds__(kBlock) void k_sh_mark(
+  int n_rows, const int* __restrict__ fam_row, int n_fam,
+  const long long* __restrict__ off, const int* __restrict__ idx,
+  const void* __restrict__ len, int lb, const unsigned char* __restrict__ yrow,
+  XY xy, int words, unsigned long long* __restrict__ bits)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(t / kTrLanes);
+  if (r >= n_rows) return;
+  const int j = (int)(t % kTrLanes);
+  const int f = family_of_row(fam_row, n_fam, r);
+  const bool yf = yrow[r] != 0;
+  const int nxy = xy.nx * xy.ny;
+  const long long end = off[r + 1];
+  for (long long e = off[r] + j; e < end; e += kTrLanes)
+  {
+    const int v = tr_voxel(idx, len, lb, xy, yf, e, end);
+    if (v < 0) continue;
+    const int px = v % nxy;
+    // The rows of a family cross the same pixels: after the first few the
+    // bit is set already, and the atomic (contended) is skipped.
+    const long long wi = (long long)f * words + px / 64;
+    const unsigned long long m = 1ULL << (px % 64);
+    if (!(bits[wi] & m)) atomicOr(bits + wi, m);
+  }
+}
+
+// Pixels per family and, per 64-pixel word, the pixels before it.
+__global__ void k_sh_count(const unsigned long long* __restrict__ bits,
+                           int n_fam, int words,
+                           unsigned short* __restrict__ wpre,
+                           int* __restrict__ n_px)
+{
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_fam) return;
+  int c = 0;
+  for (int k = 0; k < words; ++k)
+  {
+    wpre[(long long)f * words + k] = (unsigned short)c;
+    c += __popcll(bits[(long long)f * words + k]);
+  }
+  n_px[f] = c;
+}
+
+// The pixels of every family's sheet, in pixel order.
+__global__ void k_sh_pixels(const unsigned long long* __restrict__ bits,
+                            int n_fam, int words,
+                            const int* __restrict__ fam_off,
+                            int* __restrict__ fam_xy)
+{
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_fam) return;
+  int o = fam_off[f];
+  for (int k = 0; k < words; ++k)
+  {
+    unsigned long long b = bits[(long long)f * words + k];
+    while (b)
+    {
+      const int i = __ffsll((long long)b) - 1;
+      fam_xy[o++] = k * 64 + i;
+      b &= b - 1;
+    }
+  }
+}
+
+// Sheet index of every slot: pixel rank * nz + z (0 for the padding and
+// zero-length elements, which contribute nothing).
+__global__ __launch_bounds__(kBlock) void k_sh_encode(
+  int n_rows, const int* __restrict__ fam_row, int n_fam,
+  const long long* __restrict__ off, const int* __restrict__ idx,
+  const void* __restrict__ len, int lb, const unsigned char* __restrict__ yrow,
+  XY xy, int nz, int words, const unsigned long long* __restrict__ bits,
+  const unsigned short* __restrict__ wpre, unsigned short* __restrict__ sidx)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(t / kTrLanes);
+  if (r >= n_rows) return;
+  const int j = (int)(t % kTrLanes);
+  const int f = family_of_row(fam_row, n_fam, r);
+  const bool yf = yrow[r] != 0;
+  const int nxy = xy.nx * xy.ny;
+  const long long end = off[r + 1];
+  for (long long e = off[r] + j; e < end; e += kTrLanes)
+  {
+    const int v = tr_voxel(idx, len, lb, xy, yf, e, end);
+    unsigned short out = 0;
+    if (v >= 0)
+    {
+      const int px = v % nxy, z = v / nxy;
+      const long long wi = (long long)f * words + px / 64;
+      const int rank =
+        wpre[wi] + __popcll(bits[wi] & ((1ULL << (px % 64)) - 1ULL));
+      out = (unsigned short)(rank * nz + z);
+    }
+    sidx[e] = out;
+  }
+}
+
+// acc += sum over the elements of [e, e1) this lane owns of
+// dec(len[i]) * sh[sidx[i]]: the quads and the order of gather_sum4's
+// single-image path (so the sums are the same bit for bit), gathering
+// from the staged sheet.
+template <int F>
+__device__ __forceinline__ void sheet_sum(const unsigned short* __restrict__ sidx,
+                                          const void* __restrict__ len,
+                                          const float* sh, long long e,
+                                          long long e1, int lanes, float* acc)
+{
+  using Raw = typename Quad<F>::Raw;
+  constexpr int Q = kSheetQuads;
+  const long long step = 4LL * lanes;
+  if (e + (Q - 1) * step < e1)
+  {
+    // Q quads per trip, the next Q loaded before this trip's sums
+    uint2 i[Q];
+    Raw q[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k)
+    {
+      i[k] = *reinterpret_cast<const uint2*>(sidx + e + k * step);
+      q[k] = Quad<F>::load(len, e + k * step);
+    }
+    for (;;)
+    {
+      float x[4 * Q];
+#pragma unroll
+      for (int k = 0; k < Q; ++k)
+      {
+        x[4 * k] = sh[i[k].x & 0xffffu];
+        x[4 * k + 1] = sh[i[k].x >> 16];
+        x[4 * k + 2] = sh[i[k].y & 0xffffu];
+        x[4 * k + 3] = sh[i[k].y >> 16];
+      }
+      const long long en = e + Q * step;
+      const bool more = en + (Q - 1) * step < e1;
+      uint2 ni[Q];
+      Raw nq[Q];
+#pragma unroll
+      for (int k = 0; k < Q; ++k)
+      {
+        ni[k] = i[k];
+        nq[k] = q[k];
+      }
+      if (more)
+#pragma unroll
+        for (int k = 0; k < Q; ++k)
+        {
+          ni[k] = *reinterpret_cast<const uint2*>(sidx + en + k * step);
+          nq[k] = Quad<F>::load(len, en + k * step);
+        }
+      float l[4 * Q];
+#pragma unroll
+      for (int k = 0; k < Q; ++k) Quad<F>::dec(q[k], l + 4 * k);
+#pragma unroll
+      for (int u = 0; u < 4 * Q; ++u) acc[0] = fmaf(l[u], x[u], acc[0]);
+      e = en;
+      if (!more) break;
+#pragma unroll
+      for (int k = 0; k < Q; ++k)
+      {
+        i[k] = ni[k];
+        q[k] = nq[k];
+      }
+    }
+  }
+  for (; e < e1; e += step)
+  {
+    const uint2 i0 = *reinterpret_cast<const uint2*>(sidx + e);
+    float l[4];
+    Quad<F>::dec(Quad<F>::load(len, e), l);
+    acc[0] = fmaf(l[0], sh[i0.x & 0xffffu], acc[0]);
+    acc[0] = fmaf(l[1], sh[i0.x >> 16], acc[0]);
+    acc[0] = fmaf(l[2], sh[i0.y & 0xffffu], acc[0]);
+    acc[0] = fmaf(l[3], sh[i0.y >> 16], acc[0]);
+  }
+}
+
+// k_pc_osem_rows for one image over the sheet form: one block per family
+// of the subset (blockIdx.y: the subset's k-th view, blockIdx.x: the
+// tangential bin) stages the family's sheet of `img` (x fastest) in LDS,
+// then reduces its rows as k_pc_osem_rows does.
+template <int F>
+__global__ __launch_bounds__(kSheetBlock) void k_pc_osem_rows_sheet(
+  int n_subsets, int subset, int n_tang, int n_fam,
+  const int* __restrict__ fam_row, const int* __restrict__ fam_off,
+  const int* __restrict__ fam_xy, const unsigned short* __restrict__ sidx,
+  RowData rd, int lanes, const float* __restrict__ img, int nxy, int nz,
+  OsemRowsIO io, float* __restrict__ w)
+{
+  __shared__ float sh[kSheetMaxFloats];
+  const int view = (int)blockIdx.y * n_subsets + subset;
+  const int f = view * n_tang + (int)blockIdx.x;
+  if (f >= n_fam) return;
+  const int r0 = fam_row[f], r1 = fam_row[f + 1];
+  if (r0 == r1) return;
+  const int p0 = fam_off[f];
+  const int L = fam_off[f + 1] - p0;
+  // Staging: eight loads in flight per thread before their LDS stores.
+  const int n_st = L * nz;
+  for (int i0 = threadIdx.x; i0 < n_st; i0 += 8 * kSheetBlock)
+  {
+    float val[8];
+    int at[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+    {
+      const int i = i0 + u * kSheetBlock;
+      at[u] = -1;
+      val[u] = 0.0f;
+      if (i < n_st)
+      {
+        const int z = i / L, p = i - z * L;
+        at[u] = p * nz + z;
+        val[u] = img[(long long)z * nxy + fam_xy[p0 + p]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (at[u] >= 0) sh[at[u]] = val[u];
+  }
+  const int lane = (int)threadIdx.x & (lanes - 1);
+  const int per_round = kSheetBlock / lanes;
+  __syncthreads();
+  for (int rb = r0; rb < r1; rb += per_round)
+  {
+    const int r = rb + (int)threadIdx.x / lanes;
+    const bool active = r < r1;
+    float acc[1] = {0.0f};
+    if (active)
+      sheet_sum<F>(sidx, rd.len, sh, rd.off[r] + 4LL * lane, rd.off[r + 1],
+                   lanes, acc);
+    lane_reduce<1>(acc, lanes);
+    if (active && lane == 0)
+    {
+      const float fct = rd.factor[r];
+      const int bin = rd.bin[r];
+      float out = 0.0f;
+      float line = acc[0] * fct;
+      if (io.bias[0]) line += io.bias[0][bin];
+      if (line > kEpsF) out = io.y[0][bin] / line * fct;
+      w[r] = out;
+    }
+  }
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -996,6 +1282,15 @@ struct RowMatrix
   int zq = 1;      // z-slices interleaved in the gather layouts
   int fmt = 0;
   int partial = 0; // holds the rows of some subsets only (not shared)
+  // Sheet form (tangential-major full matrices whose sheets fit in LDS).
+  int n_fam = 0;                    // n_views * n_tang
+  int n_pairs = 0;
+  int* d_fam_row = nullptr;         // [n_fam + 1] first row of each family
+  int* d_fam_off = nullptr;         // [n_fam + 1] first pixel of each sheet
+  int* d_fam_xy = nullptr;          // sheet pixels (x + y nx)
+  unsigned short* d_sidx = nullptr; // [n_slots] sheet index of each slot
+  long long sheet_bytes = 0;
+  int sheets_req = 0; // the context asked for the sheet form at build
   int n_views = 0;
   int n_rows = 0;
   long long n_elems = 0;            // path elements
@@ -1014,7 +1309,7 @@ struct RowMatrix
   long long bytes() const
   {
     return (long long)(n_views + 1) * (4 + 8) + (long long)n_rows * (4 + 4 + 1 + 8) +
-      8 + n_slots * (4 + (long long)len_bytes(fmt));
+      8 + n_slots * (4 + (long long)len_bytes(fmt)) + sheet_bytes;
   }
 
   ~RowMatrix()
@@ -1028,6 +1323,10 @@ struct RowMatrix
     (void)hipFree(d_off);
     (void)hipFree(d_idx);
     (void)hipFree(d_len);
+    (void)hipFree(d_fam_row);
+    (void)hipFree(d_fam_off);
+    (void)hipFree(d_fam_xy);
+    (void)hipFree(d_sidx);
   }
 };
 
@@ -1054,6 +1353,8 @@ struct fir_cache
   // Backprojections of one or two images walk the voxels z fastest (with
   // more images the scattered per-image stores cost more than it saves).
   int zfast = 1;
+  // One-image OSEM line integrals from the matrix's sheet form, if built.
+  int sheets = 1;
 
   ~fir_cache()
   {
@@ -1270,6 +1571,79 @@ int plan_subsets(fir_ctx* c, const int* count, long long n, long long per_view,
   return FIR_OK;
 }
 
+// The sheet form of a filled matrix whose family first rows are set
+// (d_fam_row). Sheets that do not fit in LDS, or a device too full for the
+// 2-byte indices, leave the matrix without it (the projectors then gather
+// from the image); only HIP failures are errors.
+int build_sheets(fir_ctx* c, RowMatrix* rm)
+{
+  const fir_vol_desc& v = rm->vol;
+  const int nxy = v.nx * v.ny, nz = v.nz;
+  const int words = (nxy + 63) / 64;
+  const int F = rm->n_fam;
+  if ((long long)(v.nx + v.ny) * nz > 65535 || rm->n_rows == 0) return FIR_OK;
+  size_t free_b = 0, total_b = 0;
+  FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  const long long temp = (long long)F * words * 10 + 8LL * F;
+  if ((long long)free_b < temp + rm->n_slots * 2 + (2LL << 30)) return FIR_OK;
+  DevBufs tmp;
+  unsigned long long* bits;
+  unsigned short* wpre;
+  int* n_px;
+  int rc;
+  if ((rc = tmp.get(&bits, (size_t)F * words, "sheet bitmaps")) ||
+      (rc = tmp.get(&wpre, (size_t)F * words, "sheet word ranks")) ||
+      (rc = tmp.get(&n_px, (size_t)F, "sheet sizes")))
+    return rc;
+  FIR_HIP_CHECK(hipMemsetAsync(bits, 0, sizeof(unsigned long long) * F * words,
+                               c->stream));
+  const XY xy{v.nx, v.ny, rm->zq};
+  const int lb = (int)len_bytes(rm->fmt);
+  const unsigned grid_rows = blocks_for((long long)rm->n_rows * kTrLanes);
+  hipLaunchKernelGGL(k_sh_mark, dim3(grid_rows), dim3(kBlock), 0, c->stream,
+                     rm->n_rows, rm->d_fam_row, F, rm->d_off, rm->d_idx,
+                     rm->d_len, lb, rm->d_yrow, xy, words, bits);
+  FIR_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sh_count, dim3(blocks_for(F)), dim3(kBlock), 0,
+                     c->stream, bits, F, words, wpre, n_px);
+  FIR_CHECK_LAUNCH();
+  std::vector<int> npx(F), off(F + 1, 0);
+  FIR_HIP_CHECK(hipMemcpyAsync(npx.data(), n_px, sizeof(int) * F,
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  int max_px = 0;
+  for (int f = 0; f < F; ++f)
+  {
+    off[f + 1] = off[f] + npx[f];
+    max_px = std::max(max_px, npx[f]);
+  }
+  if ((long long)max_px * nz > kSheetMaxFloats) return FIR_OK;
+  if ((rc = dev_alloc(&rm->d_fam_off, (size_t)F + 1, "sheet offsets")) ||
+      (rc = dev_alloc(&rm->d_fam_xy, (size_t)off[F], "sheet pixels")) ||
+      (rc = dev_alloc(&rm->d_sidx, (size_t)rm->n_slots, "sheet indices")))
+  {
+    (void)hipFree(rm->d_fam_off);
+    (void)hipFree(rm->d_fam_xy);
+    rm->d_fam_off = nullptr;
+    rm->d_fam_xy = nullptr;
+    rm->d_sidx = nullptr;
+    return rc == FIR_E_NOMEM ? FIR_OK : rc;
+  }
+  FIR_HIP_CHECK(hipMemcpyAsync(rm->d_fam_off, off.data(), sizeof(int) * (F + 1),
+                               hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_sh_pixels, dim3(blocks_for(F)), dim3(kBlock), 0,
+                     c->stream, bits, F, words, rm->d_fam_off, rm->d_fam_xy);
+  FIR_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sh_encode, dim3(grid_rows), dim3(kBlock), 0, c->stream,
+                     rm->n_rows, rm->d_fam_row, F, rm->d_off, rm->d_idx,
+                     rm->d_len, lb, rm->d_yrow, xy, nz, words, bits, wpre,
+                     rm->d_sidx);
+  FIR_CHECK_LAUNCH();
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  rm->sheet_bytes = 4LL * (F + 1) + 4LL * off[F] + 2LL * rm->n_slots;
+  return FIR_OK;
+}
+
 int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
                std::shared_ptr<RowMatrix>* out, PartialPlan* plan = nullptr)
 {
@@ -1338,6 +1712,7 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
   rm->zq = c->cache_zq;
   rm->fmt = fmt;
   rm->partial = plan && plan->n_cached < plan->S ? 1 : 0;
+  rm->sheets_req = c->cache_sheets;
   rm->n_views = c->n_views;
   if ((rc = dev_alloc(&rm->d_view_row, c->n_views + 1, "view rows"))) return rc;
   if ((rc = dev_alloc(&rm->d_view_elem, c->n_views + 1, "view elements")))
@@ -1366,6 +1741,19 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
                                hipMemcpyDeviceToHost, c->stream));
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
   rm->n_rows = last_row + (last_count > 0 ? 1 : 0);
+  if (c->tang_major && !plan && c->cache_sheets)
+  {
+    // Families (view, tangential bin) are contiguous in this row order.
+    rm->n_pairs = c->n_pairs;
+    rm->n_fam = c->n_views * c->n_tang;
+    if ((rc = dev_alloc(&rm->d_fam_row, (size_t)rm->n_fam + 1,
+                        "family rows")))
+      return rc;
+    hipLaunchKernelGGL(k_sh_fam_rows, dim3(blocks_for(rm->n_fam + 1)),
+                       dim3(kBlock), 0, c->stream, rowid, c->n_pairs,
+                       rm->n_fam, rm->n_rows, rm->d_fam_row);
+    FIR_CHECK_LAUNCH();
+  }
   rm->n_elems = real_total;
   rm->n_slots = last_eoff + ((last_count + 3) & ~3);
 
@@ -1427,6 +1815,7 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
                                sizeof(long long) * (c->n_views + 1),
                                hipMemcpyDeviceToHost, c->stream));
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (rm->d_fam_row && (rc = build_sheets(c, rm.get()))) return rc;
   *out = rm;
   return FIR_OK;
 }
@@ -1824,6 +2213,28 @@ void launch_rows_nb(fir_cache* cache, int s, const float* X,
                      row_data(rm), lanes, X, cache->d_xy, io, w);
 }
 
+// Whether one-image OSEM passes of this cache take the sheet form.
+bool use_sheets(const fir_cache* cache)
+{
+  return cache->sheets && cache->rm->d_sidx;
+}
+
+// k_pc_osem_rows over the sheet form for one image (img x fastest).
+template <int F>
+void launch_rows_sheet(fir_cache* cache, int s, const float* img,
+                       const OsemRowsIO& io, float* w)
+{
+  const RowMatrix* rm = cache->rm.get();
+  const int S = cache->n_subsets;
+  const int vpr = fir::views_in_subset(rm->n_views, S, s);
+  const int n_tang = rm->n_fam / rm->n_views;
+  hipLaunchKernelGGL((k_pc_osem_rows_sheet<F>), dim3(n_tang, vpr),
+                     dim3(kSheetBlock), 0, cache->ctx->stream, S, s, n_tang,
+                     rm->n_fam, rm->d_fam_row, rm->d_fam_off, rm->d_fam_xy,
+                     rm->d_sidx, row_data(rm), cache->fwd_lanes, img,
+                     cache->vol.nx * cache->vol.ny, cache->vol.nz, io, w);
+}
+
 template <int F>
 void launch_osem_rows(fir_cache* cache, int s, int nb, const float* X,
                       const OsemRowsIO& io, float* w)
@@ -1855,6 +2266,13 @@ int fir_set_cache_lanes(fir_ctx* c, int fwd_lanes, int bp_lanes)
                            "from 1 to 64");
   c->cache_lanes[0] = fwd_lanes;
   c->cache_lanes[1] = bp_lanes;
+  return FIR_OK;
+}
+
+int fir_set_cache_sheets(fir_ctx* c, int enable)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  c->cache_sheets = enable ? 1 : 0;
   return FIR_OK;
 }
 
@@ -1904,8 +2322,8 @@ static int cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
     if (sp && sp->generation == c->generation && same_vol(sp->vol, *v) &&
         sp->walk_mode == c->walk_mode && sp->pair_order == c->lor_order &&
         sp->tang_major == c->tang_major && sp->lanes == c->cache_lanes[0] &&
-        sp->zq == c->cache_zq &&
-        sp->fmt == len_format)
+        sp->zq == c->cache_zq && sp->fmt == len_format &&
+        sp->sheets_req == c->cache_sheets)
     {
       rm = sp;
       break;
@@ -2048,9 +2466,11 @@ int fir_cache_tune(fir_cache* cache, int fwd_lanes, int bp_lanes, int flags)
   if (!ok(fwd_lanes) || !ok(bp_lanes))
     return fail(FIR_E_ARG, "lanes per LOR / voxel must be a power of two "
                            "from 1 to 64");
-  if (flags & ~(FIR_CACHE_XCD_MAP | FIR_CACHE_STORED_VOXELS))
+  if (flags & ~(FIR_CACHE_XCD_MAP | FIR_CACHE_STORED_VOXELS |
+                FIR_CACHE_NO_SHEETS))
     return fail(FIR_E_ARG, "unknown cache flags 0x%x", flags);
   cache->zfast = (flags & FIR_CACHE_STORED_VOXELS) ? 0 : 1;
+  cache->sheets = (flags & FIR_CACHE_NO_SHEETS) ? 0 : 1;
   cache->fwd_lanes = fwd_lanes;
   cache->bp_lanes = bp_lanes;
   cache->xcd_map = (flags & FIR_CACHE_XCD_MAP) ? 1 : 0;
@@ -2178,6 +2598,30 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
       out.p[b] = bp[i0 + b];
     }
     io.n_images = nb;
+    if (wd == 1 && use_sheets(cache))
+    {
+      if (rm->fmt == FIR_LEN_SNORM16)
+      {
+        launch_rows_sheet<FIR_LEN_SNORM16>(cache, subset, x[i0], io,
+                                           cache->d_w);
+        FIR_CHECK_LAUNCH();
+        launch_backproject<FIR_LEN_SNORM16>(cache, subset, 1, cache->d_w, out);
+      }
+      else if (rm->fmt == FIR_LEN_FP16)
+      {
+        launch_rows_sheet<FIR_LEN_FP16>(cache, subset, x[i0], io, cache->d_w);
+        FIR_CHECK_LAUNCH();
+        launch_backproject<FIR_LEN_FP16>(cache, subset, 1, cache->d_w, out);
+      }
+      else
+      {
+        launch_rows_sheet<FIR_LEN_FP32>(cache, subset, x[i0], io, cache->d_w);
+        FIR_CHECK_LAUNCH();
+        launch_backproject<FIR_LEN_FP32>(cache, subset, 1, cache->d_w, out);
+      }
+      FIR_CHECK_LAUNCH();
+      continue;
+    }
     const float* X = wd > 1 || rm->zq > 1 ? cache->d_x : x[i0];
     pack_images(cache, wd, in, nb);
     FIR_CHECK_LAUNCH();

@@ -28,6 +28,7 @@ FIR_LEN_FP16 = 1
 FIR_LEN_FP32 = 2
 FIR_CACHE_XCD_MAP = 1
 FIR_CACHE_STORED_VOXELS = 2
+FIR_CACHE_NO_SHEETS = 4
 # Row order of path caches: per view, tangential bin then (segment, axial)
 # pair by centre slice -- the rows through one voxel are then close together.
 CACHE_ROW_ORDER = FIR_ORDER_VIEW_MAJOR | FIR_ORDER_TANG_MAJOR | FIR_ORDER_ZCENTER
@@ -46,6 +47,7 @@ EXPORTS = (
     "fir_path_count", "fir_poisson_nll_grad", "fir_osem_project_batch",
     "fir_set_uneven_subsets", "fir_set_lor_order", "fir_set_scatter_copies",
     "fir_set_cache_lanes", "fir_set_cache_zslices", "fir_set_cache_chunk",
+    "fir_set_cache_sheets",
     "fir_cache_create",
     "fir_cache_create_partial", "fir_cache_subsets", "fir_cache_destroy",
     "fir_cache_info",
@@ -137,6 +139,7 @@ def load() -> C.CDLL:
         "fir_set_cache_lanes": [vp, i, i],
         "fir_set_cache_zslices": [vp, i],
         "fir_set_cache_chunk": [vp, i64],
+        "fir_set_cache_sheets": [vp, i],
         "fir_cache_forward": [vp, vp, vp],
         "fir_cache_backward": [vp, vp, vp],
         "fir_cache_sensitivity": [vp, vp],
@@ -237,6 +240,10 @@ class Context:
         """z-slices interleaved in the gather layouts of the next caches."""
         _check(self.lib.fir_set_cache_zslices(self.h, zq))
         self._cache_zq = zq
+
+    def set_cache_sheets(self, enable: bool):
+        """Whether the next path matrices also get the sheet form."""
+        _check(self.lib.fir_set_cache_sheets(self.h, int(enable)))
 
     def set_cache_chunk(self, max_elements: int):
         """Largest number of path elements one transpose step of the next

@@ -348,6 +348,15 @@ int fir_set_cache_zslices(fir_ctx* ctx, int zslices);
  * voxel lists come out the same whatever the step. */
 int fir_set_cache_chunk(fir_ctx* ctx, int64_t max_elements);
 
+/* Whether the path matrices built afterwards on this context also get the
+ * sheet form (default 1): with rows in tangential-major order, the LORs of
+ * one (view, tangential bin) cross the same pixels, and each path element
+ * gets a 16-bit index within that family's sheet (its pixels x all
+ * slices), so that one-image OSEM passes stage the sheet in LDS and gather
+ * there (fir_cache_tune, FIR_CACHE_NO_SHEETS). It costs 2 bytes per
+ * element (7.8 GB for 3.9e9 elements); 0 saves that memory. */
+int fir_set_cache_sheets(fir_ctx* ctx, int enable);
+
 /* Build the cache of `vol` for `n_subsets` subsets (views v with
  * v % n_subsets == s, as LORCache.cc:49-51). Synchronizes the context's
  * stream; FIR_E_NOMEM if the matrices do not fit in device memory. */
@@ -389,11 +398,19 @@ int fir_cache_info(const fir_cache* cache, int64_t* n_rows,
  * Backprojections of one or two images walk the voxels z fastest (a wave's
  * voxels are then crossed by neighbouring rows). Flags: FIR_CACHE_XCD_MAP
  * deals contiguous eighths of the rows of a view (of the voxels) to the
- * eight XCDs; FIR_CACHE_STORED_VOXELS walks the voxels in storage order. */
+ * eight XCDs; FIR_CACHE_STORED_VOXELS walks the voxels in storage order.
+ * One-image OSEM passes of a cache whose rows are in tangential-major order
+ * (FIR_ORDER_VIEW_MAJOR | FIR_ORDER_TANG_MAJOR) compute their line
+ * integrals family by family: the LORs of one (view, tangential bin) cross
+ * the same pixels of the xy-plane, so that sheet of the image (pixels x
+ * slices, up to 64 KB) is staged in LDS and the paths index it with 16-bit
+ * entries (the same sums bit for bit); FIR_CACHE_NO_SHEETS gathers from
+ * the image instead. */
 enum
 {
   FIR_CACHE_XCD_MAP = 1,
-  FIR_CACHE_STORED_VOXELS = 2
+  FIR_CACHE_STORED_VOXELS = 2,
+  FIR_CACHE_NO_SHEETS = 4
 };
 int fir_cache_tune(fir_cache* cache, int fwd_lanes, int bp_lanes, int flags);
 

@@ -271,7 +271,7 @@ def test_cache_lanes_keep_results(c1, c1_golden):
     for s, b in outs[1:]:
         assert rel_max(s, outs[0][0]) < 1e-6
         assert rel_max(b, outs[0][1]) < 1e-6
-    for bad in [(0, 16, 0), (16, 3, 0), (128, 16, 0), (16, 16, 4)]:
+    for bad in [(0, 16, 0), (16, 3, 0), (128, 16, 0), (16, 16, 8)]:
         with pytest.raises(firlib.FirError):
             cache.tune(*bad)
 
@@ -370,3 +370,51 @@ def test_autotuner_picks_fastest_and_keeps_results(c1, c1_golden):
     assert pres.best_ms == min(t.ms for t in pres.trials)
     ctx.set_lor_order(0)
     ctx.set_scatter_copies(1)
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_sheet_form_equals_image_gathers(c1, c1_golden, fmt):
+    """One-image OSEM passes over the sheet form (each (view, tangential
+    bin) family's pixels x slices staged in LDS, 16-bit sheet indices) give
+    the same results bit for bit as the gathers from the image
+    (FIR_CACHE_NO_SHEETS): same elements, same order, same values."""
+    w, ctx, v, caches = c1
+    cache = ctx.path_cache(v, 4, fmt, row_order=firlib.CACHE_ROW_ORDER)
+    x = dev(c1_golden["act"].ravel() + 0.1)
+    y = dev(c1_golden["y"])
+    bias = dev(np.full_like(c1_golden["y"], 0.5))
+    for s in range(4):
+        for b in (None, [bias]):
+            a = torch.zeros(v.n_voxels, device="cuda")
+            cache.tune(8, 16, 0)
+            cache.osem_project([x], [y], b, [a], s)
+            g = torch.zeros(v.n_voxels, device="cuda")
+            cache.tune(8, 16, firlib.FIR_CACHE_NO_SHEETS)
+            cache.osem_project([x], [y], b, [g], s)
+            assert torch.equal(a, g) and float(a.abs().max()) > 0
+    cache.close()
+
+
+def test_cache_without_sheet_form(c1, c1_golden):
+    """fir_set_cache_sheets(0): the matrix has no sheet indices (less device
+    memory) and one-image passes gather from the image, with the same
+    results."""
+    w, ctx, v, caches = c1
+    with_s = ctx.path_cache(v, 4, firlib.FIR_LEN_SNORM16,
+                            row_order=firlib.CACHE_ROW_ORDER)
+    ctx.set_cache_sheets(False)
+    try:
+        without = ctx.path_cache(v, 4, firlib.FIR_LEN_SNORM16,
+                                 row_order=firlib.CACHE_ROW_ORDER)
+    finally:
+        ctx.set_cache_sheets(True)
+    assert without.info()["device_bytes"] < with_s.info()["device_bytes"]
+    x = dev(c1_golden["act"].ravel() + 0.1)
+    y = dev(c1_golden["y"])
+    a = torch.zeros(v.n_voxels, device="cuda")
+    b = torch.zeros(v.n_voxels, device="cuda")
+    with_s.osem_project([x], [y], None, [a], 1)
+    without.osem_project([x], [y], None, [b], 1)
+    assert torch.equal(a, b)
+    without.close()
+    with_s.close()
