@@ -944,9 +944,6 @@ struct ImgIn
   const float* p[kMaxCacheBatch];
 };
 
-// X[vx * NB + b] = x_b[v] (if X is not NULL) and XT[vy * NB + b] = x_b[v]
-// with vx, vy the indices of v in the x- and y-fastest layouts (0 for
-// b >= n_images).
 // The gather copies of the images of one launch, written in their own
 // order (so that each wave's stores are contiguous): blockIdx.y < nb_x
 // fills row b = blockIdx.y of the x-fastest copy (a = x), the others the
@@ -1218,6 +1215,7 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_rows_sheet(
   const int L = fam_off[f + 1] - p0;
   // Staging: eight loads in flight per thread before their LDS stores.
   const int n_st = L * nz;
+  if (n_st == 0 && threadIdx.x == 0) sh[0] = 0.0f; // padding slots read it
   for (int i0 = threadIdx.x; i0 < n_st; i0 += 8 * kSheetBlock)
   {
     float val[8];
