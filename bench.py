@@ -461,15 +461,18 @@ def main():
             yi = torch.poisson((mean_f * float(sc)), generator=g2)
             ctx.mul_clamped(yi, acf)
             ys.append(yi)
-        # batches of 8 interleaved images gather best from a layout with 2
-        # interleaved z-slices: build that cache in place of the
-        # single-image one (same row order and lanes)
+        # batches of 8 interleaved images gather from interleaved image
+        # copies, best with 2 interleaved z-slices and 8 lanes per LOR, and
+        # do not use the sheet form: build that cache in place of the
+        # single-image one (same row order)
         bz = 2
-        if tres.best["zslices"] != bz:
-            cache.close()
-            ctx.set_cache_zslices(bz)
-            cache = ctx.path_cache(v, n_sub, firlib.FIR_LEN_SNORM16,
-                                   row_order=tres.best["row_order"])
+        cache.close()
+        ctx.set_cache_zslices(bz)
+        ctx.set_cache_lanes(8, 16)
+        ctx.set_cache_sheets(False)
+        cache = ctx.path_cache(v, n_sub, firlib.FIR_LEN_SNORM16,
+                               row_order=tres.best["row_order"])
+        ctx.set_cache_sheets(True)
         grp = recon.OSEMGroup(ctx, v, n_sub, ys, sens=sens_c, fwhm=fwhm,
                               conv_interval=1, cache=cache)
         grp.capture()
@@ -485,6 +488,7 @@ def main():
             "n_images": args.batch,
             "path": "cached, interleaved images, one HIP graph per iteration",
             "cache_zslices": bz,
+            "cache_lanes": [8, 16],
             "osem_iter_ms": round(bms, 4),
             "osem_iter_ms_per_recon": round(bms / args.batch, 4),
             "mlors_s_per_recon_aggregate": round(
@@ -603,8 +607,9 @@ def main():
         "speedup_cached_vs_plain": round(plain_ms / ms_step, 3),
         "path_elements_per_iter": int(n_e_total),
         "roofline": {
-            "kernel": "k_pc_osem_rows + k_pc_backproject (one cached OSEM "
-                      "projector launch)",
+            "kernel": "k_pc_osem_rows_sheet + k_pc_backproject (one cached "
+                      "OSEM projector launch: line integrals from LDS-staged "
+                      "sheets, transposed backprojection)",
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
