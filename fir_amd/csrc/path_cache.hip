@@ -1088,7 +1088,7 @@ __global__ __launch_bounds__(kBlock) void k_sh_encode(
   const long long* __restrict__ off, const int* __restrict__ idx,
   const void* __restrict__ len, int lb, const unsigned char* __restrict__ yrow,
   XY xy, int nz, int words, const unsigned long long* __restrict__ bits,
-  const unsigned short* __restrict__ wpre, unsigned short* __restrict__ sidx)
+  const unsigned short* __restrict__ wpre, unsigned* __restrict__ sel)
 {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int r = (int)(t / kTrLanes);
@@ -1101,16 +1101,17 @@ __global__ __launch_bounds__(kBlock) void k_sh_encode(
   for (long long e = off[r] + j; e < end; e += kTrLanes)
   {
     const int v = tr_voxel(idx, len, lb, xy, yf, e, end);
-    unsigned short out = 0;
+    unsigned out = 0;
     if (v >= 0)
     {
       const int px = v % nxy, z = v / nxy;
       const long long wi = (long long)f * words + px / 64;
       const int rank =
         wpre[wi] + __popcll(bits[wi] & ((1ULL << (px % 64)) - 1ULL));
-      out = (unsigned short)(rank * nz + z);
+      out = (unsigned)(rank * nz + z) |
+        ((unsigned)((const unsigned short*)len)[e] << 16);
     }
-    sidx[e] = out;
+    sel[e] = out;
   }
 }
 
@@ -1118,78 +1119,68 @@ __global__ __launch_bounds__(kBlock) void k_sh_encode(
 // dec(len[i]) * sh[sidx[i]]: the quads and the order of gather_sum4's
 // single-image path (so the sums are the same bit for bit), gathering
 // from the staged sheet.
+// The four 16-bit lengths of a quad of sheet elements, as Quad<F> loads
+// them from the length array.
+__device__ __forceinline__ uint2 sheet_lengths(uint4 q)
+{
+  return make_uint2((q.x >> 16) | (q.y & 0xffff0000u),
+                    (q.z >> 16) | (q.w & 0xffff0000u));
+}
+
 template <int F>
-__device__ __forceinline__ void sheet_sum(const unsigned short* __restrict__ sidx,
-                                          const void* __restrict__ len,
+__device__ __forceinline__ void sheet_sum(const unsigned* __restrict__ sel,
                                           const float* sh, long long e,
                                           long long e1, int lanes, float* acc)
 {
-  using Raw = typename Quad<F>::Raw;
   constexpr int Q = kSheetQuads;
   const long long step = 4LL * lanes;
   if (e + (Q - 1) * step < e1)
   {
     // Q quads per trip, the next Q loaded before this trip's sums
-    uint2 i[Q];
-    Raw q[Q];
+    uint4 q[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k)
-    {
-      i[k] = *reinterpret_cast<const uint2*>(sidx + e + k * step);
-      q[k] = Quad<F>::load(len, e + k * step);
-    }
+      q[k] = *reinterpret_cast<const uint4*>(sel + e + k * step);
     for (;;)
     {
       float x[4 * Q];
 #pragma unroll
       for (int k = 0; k < Q; ++k)
       {
-        x[4 * k] = sh[i[k].x & 0xffffu];
-        x[4 * k + 1] = sh[i[k].x >> 16];
-        x[4 * k + 2] = sh[i[k].y & 0xffffu];
-        x[4 * k + 3] = sh[i[k].y >> 16];
+        x[4 * k] = sh[q[k].x & 0xffffu];
+        x[4 * k + 1] = sh[q[k].y & 0xffffu];
+        x[4 * k + 2] = sh[q[k].z & 0xffffu];
+        x[4 * k + 3] = sh[q[k].w & 0xffffu];
       }
       const long long en = e + Q * step;
       const bool more = en + (Q - 1) * step < e1;
-      uint2 ni[Q];
-      Raw nq[Q];
+      uint4 nq[Q];
 #pragma unroll
-      for (int k = 0; k < Q; ++k)
-      {
-        ni[k] = i[k];
-        nq[k] = q[k];
-      }
+      for (int k = 0; k < Q; ++k) nq[k] = q[k];
       if (more)
 #pragma unroll
         for (int k = 0; k < Q; ++k)
-        {
-          ni[k] = *reinterpret_cast<const uint2*>(sidx + en + k * step);
-          nq[k] = Quad<F>::load(len, en + k * step);
-        }
+          nq[k] = *reinterpret_cast<const uint4*>(sel + en + k * step);
       float l[4 * Q];
 #pragma unroll
-      for (int k = 0; k < Q; ++k) Quad<F>::dec(q[k], l + 4 * k);
+      for (int k = 0; k < Q; ++k) Quad<F>::dec(sheet_lengths(q[k]), l + 4 * k);
 #pragma unroll
       for (int u = 0; u < 4 * Q; ++u) acc[0] = fmaf(l[u], x[u], acc[0]);
       e = en;
       if (!more) break;
 #pragma unroll
-      for (int k = 0; k < Q; ++k)
-      {
-        i[k] = ni[k];
-        q[k] = nq[k];
-      }
+      for (int k = 0; k < Q; ++k) q[k] = nq[k];
     }
   }
   for (; e < e1; e += step)
   {
-    const uint2 i0 = *reinterpret_cast<const uint2*>(sidx + e);
+    const uint4 q0 = *reinterpret_cast<const uint4*>(sel + e);
     float l[4];
-    Quad<F>::dec(Quad<F>::load(len, e), l);
-    acc[0] = fmaf(l[0], sh[i0.x & 0xffffu], acc[0]);
-    acc[0] = fmaf(l[1], sh[i0.x >> 16], acc[0]);
-    acc[0] = fmaf(l[2], sh[i0.y & 0xffffu], acc[0]);
-    acc[0] = fmaf(l[3], sh[i0.y >> 16], acc[0]);
+    Quad<F>::dec(sheet_lengths(q0), l);
+    acc[0] = fmaf(l[0], sh[q0.x & 0xffffu], acc[0]);
+    acc[0] = fmaf(l[1], sh[q0.y & 0xffffu], acc[0]);
+    acc[0] = fmaf(l[2], sh[q0.z & 0xffffu], acc[0]);
+    acc[0] = fmaf(l[3], sh[q0.w & 0xffffu], acc[0]);
   }
 }
 
@@ -1201,7 +1192,7 @@ template <int F>
 __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_rows_sheet(
   int n_subsets, int subset, int n_tang, int n_fam,
   const int* __restrict__ fam_row, const int* __restrict__ fam_off,
-  const int* __restrict__ fam_xy, const unsigned short* __restrict__ sidx,
+  const int* __restrict__ fam_xy, const unsigned* __restrict__ sel,
   RowData rd, int lanes, const float* __restrict__ img, int nxy, int nz,
   OsemRowsIO io, float* __restrict__ w)
 {
@@ -1246,8 +1237,8 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_rows_sheet(
     const bool active = r < r1;
     float acc[1] = {0.0f};
     if (active)
-      sheet_sum<F>(sidx, rd.len, sh, rd.off[r] + 4LL * lane, rd.off[r + 1],
-                   lanes, acc);
+      sheet_sum<F>(sel, sh, rd.off[r] + 4LL * lane, rd.off[r + 1], lanes,
+                   acc);
     lane_reduce<1>(acc, lanes);
     if (active && lane == 0)
     {
@@ -1286,7 +1277,7 @@ struct RowMatrix
   int* d_fam_row = nullptr;         // [n_fam + 1] first row of each family
   int* d_fam_off = nullptr;         // [n_fam + 1] first pixel of each sheet
   int* d_fam_xy = nullptr;          // sheet pixels (x + y nx)
-  unsigned short* d_sidx = nullptr; // [n_slots] sheet index of each slot
+  unsigned* d_sel = nullptr;        // [n_slots] sheet index | length << 16
   long long sheet_bytes = 0;
   int sheets_req = 0; // the context asked for the sheet form at build
   int n_views = 0;
@@ -1324,7 +1315,7 @@ struct RowMatrix
     (void)hipFree(d_fam_row);
     (void)hipFree(d_fam_off);
     (void)hipFree(d_fam_xy);
-    (void)hipFree(d_sidx);
+    (void)hipFree(d_sel);
   }
 };
 
@@ -1579,11 +1570,15 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
   const int nxy = v.nx * v.ny, nz = v.nz;
   const int words = (nxy + 63) / 64;
   const int F = rm->n_fam;
-  if ((long long)(v.nx + v.ny) * nz > 65535 || rm->n_rows == 0) return FIR_OK;
+  // 16-bit lengths only: an element is a 16-bit sheet index and the
+  // length's 16 bits in one word.
+  if ((long long)(v.nx + v.ny) * nz > 65535 || rm->n_rows == 0 ||
+      len_bytes(rm->fmt) != 2)
+    return FIR_OK;
   size_t free_b = 0, total_b = 0;
   FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
   const long long temp = (long long)F * words * 10 + 8LL * F;
-  if ((long long)free_b < temp + rm->n_slots * 2 + (2LL << 30)) return FIR_OK;
+  if ((long long)free_b < temp + rm->n_slots * 4 + (2LL << 30)) return FIR_OK;
   DevBufs tmp;
   unsigned long long* bits;
   unsigned short* wpre;
@@ -1618,13 +1613,13 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
   if ((long long)max_px * nz > kSheetMaxFloats) return FIR_OK;
   if ((rc = dev_alloc(&rm->d_fam_off, (size_t)F + 1, "sheet offsets")) ||
       (rc = dev_alloc(&rm->d_fam_xy, (size_t)off[F], "sheet pixels")) ||
-      (rc = dev_alloc(&rm->d_sidx, (size_t)rm->n_slots, "sheet indices")))
+      (rc = dev_alloc(&rm->d_sel, (size_t)rm->n_slots, "sheet elements")))
   {
     (void)hipFree(rm->d_fam_off);
     (void)hipFree(rm->d_fam_xy);
     rm->d_fam_off = nullptr;
     rm->d_fam_xy = nullptr;
-    rm->d_sidx = nullptr;
+    rm->d_sel = nullptr;
     return rc == FIR_E_NOMEM ? FIR_OK : rc;
   }
   FIR_HIP_CHECK(hipMemcpyAsync(rm->d_fam_off, off.data(), sizeof(int) * (F + 1),
@@ -1635,10 +1630,10 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
   hipLaunchKernelGGL(k_sh_encode, dim3(grid_rows), dim3(kBlock), 0, c->stream,
                      rm->n_rows, rm->d_fam_row, F, rm->d_off, rm->d_idx,
                      rm->d_len, lb, rm->d_yrow, xy, nz, words, bits, wpre,
-                     rm->d_sidx);
+                     rm->d_sel);
   FIR_CHECK_LAUNCH();
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
-  rm->sheet_bytes = 4LL * (F + 1) + 4LL * off[F] + 2LL * rm->n_slots;
+  rm->sheet_bytes = 4LL * (F + 1) + 4LL * off[F] + 4LL * rm->n_slots;
   return FIR_OK;
 }
 
@@ -2214,7 +2209,7 @@ void launch_rows_nb(fir_cache* cache, int s, const float* X,
 // Whether one-image OSEM passes of this cache take the sheet form.
 bool use_sheets(const fir_cache* cache)
 {
-  return cache->sheets && cache->rm->d_sidx;
+  return cache->sheets && cache->rm->d_sel;
 }
 
 // k_pc_osem_rows over the sheet form for one image (img x fastest).
@@ -2229,7 +2224,7 @@ void launch_rows_sheet(fir_cache* cache, int s, const float* img,
   hipLaunchKernelGGL((k_pc_osem_rows_sheet<F>), dim3(n_tang, vpr),
                      dim3(kSheetBlock), 0, cache->ctx->stream, S, s, n_tang,
                      rm->n_fam, rm->d_fam_row, rm->d_fam_off, rm->d_fam_xy,
-                     rm->d_sidx, row_data(rm), cache->fwd_lanes, img,
+                     rm->d_sel, row_data(rm), cache->fwd_lanes, img,
                      cache->vol.nx * cache->vol.ny, cache->vol.nz, io, w);
 }
 
@@ -2596,6 +2591,7 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
       out.p[b] = bp[i0 + b];
     }
     io.n_images = nb;
+    // (the sheet form exists for the 16-bit length formats only)
     if (wd == 1 && use_sheets(cache))
     {
       if (rm->fmt == FIR_LEN_SNORM16)
@@ -2605,17 +2601,11 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
         FIR_CHECK_LAUNCH();
         launch_backproject<FIR_LEN_SNORM16>(cache, subset, 1, cache->d_w, out);
       }
-      else if (rm->fmt == FIR_LEN_FP16)
+      else
       {
         launch_rows_sheet<FIR_LEN_FP16>(cache, subset, x[i0], io, cache->d_w);
         FIR_CHECK_LAUNCH();
         launch_backproject<FIR_LEN_FP16>(cache, subset, 1, cache->d_w, out);
-      }
-      else
-      {
-        launch_rows_sheet<FIR_LEN_FP32>(cache, subset, x[i0], io, cache->d_w);
-        FIR_CHECK_LAUNCH();
-        launch_backproject<FIR_LEN_FP32>(cache, subset, 1, cache->d_w, out);
       }
       FIR_CHECK_LAUNCH();
       continue;

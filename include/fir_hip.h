@@ -352,9 +352,10 @@ int fir_set_cache_chunk(fir_ctx* ctx, int64_t max_elements);
  * sheet form (default 1): with rows in tangential-major order, the LORs of
  * one (view, tangential bin) cross the same pixels, and each path element
  * gets a 16-bit index within that family's sheet (its pixels x all
- * slices), so that one-image OSEM passes stage the sheet in LDS and gather
- * there (fir_cache_tune, FIR_CACHE_NO_SHEETS). It costs 2 bytes per
- * element (7.8 GB for 3.9e9 elements); 0 saves that memory. */
+ * slices), stored with its 16-bit length in one word, so that one-image
+ * OSEM passes stage the sheet in LDS and gather there (fir_cache_tune,
+ * FIR_CACHE_NO_SHEETS). 16-bit length formats only; it costs 4 bytes per
+ * element (15.6 GB for 3.9e9 elements); 0 saves that memory. */
 int fir_set_cache_sheets(fir_ctx* ctx, int enable);
 
 /* Build the cache of `vol` for `n_subsets` subsets (views v with
