@@ -343,6 +343,24 @@ def test_c2_cache_properties():
     rhs = float((x.double() * aty.double()).sum())
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
     cache.close()
+    # the sheet form at full size (rows in tangential-major order): one-image
+    # OSEM passes equal the image gathers bit for bit, and sum to the
+    # on-the-fly projector's within the snorm16 bar
+    cache = ctx.path_cache(v, 16, firlib.FIR_LEN_SNORM16,
+                           row_order=firlib.CACHE_ROW_ORDER)
+    yc = torch.poisson(b * 3.0, generator=g)
+    for s in (0, 7):
+        p1 = torch.zeros(v.n_voxels, device="cuda")
+        cache.tune(8, 16, 0)
+        cache.osem_project([x + 0.1], [yc], None, [p1], s)
+        p2 = torch.zeros(v.n_voxels, device="cuda")
+        cache.tune(8, 16, firlib.FIR_CACHE_NO_SHEETS)
+        cache.osem_project([x + 0.1], [yc], None, [p2], s)
+        assert torch.equal(p1, p2)
+        p3 = torch.zeros(v.n_voxels, device="cuda")
+        ctx.osem_project(v, x + 0.1, yc, None, p3, s, 16)
+        assert rel_l2(host(p1), host(p3)) < 2e-5
+    cache.close()
     ctx.close()
 
 
