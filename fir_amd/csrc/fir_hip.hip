@@ -307,6 +307,164 @@ __global__ __launch_bounds__(kBlock) void k_osem_project(
   });
 }
 
+// ---------------------------------------------------------------------------
+// Family form of the walked OSEM projector
+// ---------------------------------------------------------------------------
+// The LORs of one (view, tangential bin) -- every ring pair of one line of
+// the xy-plane -- cross the same pixels, in the same order: the family's
+// sheet is those pixels times all slices (at most (nx + ny - 1) * nz
+// voxels). One block per family stages the sheet of x in LDS, walks each
+// LOR (one per thread) gathering from LDS (the same values in the same
+// order: the line integrals are the same bit for bit), then reuses the LDS
+// as the family's backprojection accumulator (LDS atomics) and adds its
+// nonzero voxels to the image: one global atomic per touched sheet voxel
+// instead of one per path element.
+
+constexpr int kFamSheetFloats = 16384; // 64 KB of LDS
+constexpr int kFamBlock = 1024;
+constexpr int kFamMaxPx = 1024;
+
+__global__ __launch_bounds__(kFamBlock) void k_osem_project_fam(
+  Grid g, LorGeom lg, const float* __restrict__ x, const float* __restrict__ y,
+  const float* __restrict__ bias, fir::Scatter sc, int n_subsets, int subset,
+  int n_tang, int n_pairs, int nz)
+{
+  __shared__ float sh[kFamSheetFloats];
+  __shared__ int px[kFamMaxPx];
+  __shared__ int n_px;
+  const int view = (int)blockIdx.y * n_subsets + subset;
+  const int tang = (int)blockIdx.x;
+  const int nxy = g.slice;
+  if (threadIdx.x == 0)
+  {
+    // the family's pixels: the walk of its line at the volume's mid-depth
+    Lor L;
+    fir::lor_from_index(lg, (long long)tang * n_pairs, lg.n_views, view, L);
+    const double zc = 0.5 * (g.low[2] + g.high[2]);
+    const double e1[3] = {L.e1[0], L.e1[1], zc};
+    const double e2[3] = {L.e2[0], L.e2[1], zc};
+    Walk w;
+    int n = 0;
+    if (fir::walk_setup(g, e1, e2, w))
+    {
+      int last = -1;
+      fir::walk_run(g, w, [&](int idx, double) {
+        const int p = idx % nxy;
+        if (p != last && n < kFamMaxPx)
+        {
+          px[n++] = p;
+          last = p;
+        }
+      });
+    }
+    n_px = n;
+  }
+  __syncthreads();
+  const int L = n_px;
+  if (L == 0 || L * nz > kFamSheetFloats) return; // host checks the bound
+  const int n_st = L * nz;
+  for (int i = threadIdx.x; i < n_st; i += kFamBlock)
+  {
+    const int z = i / L, p = i - z * L;
+    sh[p * nz + z] = x[(long long)z * nxy + px[p]];
+  }
+  __syncthreads();
+
+  // Forward: one LOR per thread, the walk's pixels followed along the
+  // sheet (rank r), values from LDS.
+  const int pair = threadIdx.x;
+  float ratio = 0.0f;
+  Walk w;
+  bool hit = false;
+  if (pair < n_pairs)
+  {
+    Lor Lr;
+    fir::lor_from_index(lg, (long long)tang * n_pairs + pair, lg.n_views, view,
+                        Lr);
+    hit = fir::walk_setup(g, Lr.e1, Lr.e2, w);
+    if (hit)
+    {
+      float line = 0.0f;
+      int r = 0;
+      Walk v = w;
+      double prev = v.prev;
+      for (int it = 0; prev < v.a_hi && it < g.max_iter; ++it)
+      {
+        const double next =
+          fir::dmin(v.a_hi, fir::dmin(v.a[0], fir::dmin(v.a[1], v.a[2])));
+        if ((unsigned)v.pos[0] <= (unsigned)g.nm1[0] &&
+            (unsigned)v.pos[1] <= (unsigned)g.nm1[1] &&
+            (unsigned)v.pos[2] <= (unsigned)g.nm1[2])
+        {
+          const int P = v.pos[0] + v.pos[1] * g.row;
+          while (r < L && px[r] != P) ++r;
+          const float val = r < L ? sh[r * nz + v.pos[2]]
+                                  : x[P + (long long)v.pos[2] * nxy];
+          if (r >= L) r = 0; // not on the sheet (not expected): restart
+          line = (float)((double)line + (next - prev) * v.d12 * (double)val);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          if (fir::dabs(v.a[d] - next) < fir::kEps)
+          {
+            v.a[d] += v.da[d];
+            v.pos[d] += v.dir[d];
+          }
+        prev = next;
+      }
+      if (bias) line += bias[Lr.bin];
+      if (line > kEpsF) ratio = y[Lr.bin] / line;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_st; i += kFamBlock) sh[i] = 0.0f;
+  __syncthreads();
+
+  // Backprojection into the sheet.
+  float* bp = fir::scatter_target(sc);
+  if (hit && ratio != 0.0f)
+  {
+    const double dr = (double)ratio;
+    int r = 0;
+    double prev = w.prev;
+    for (int it = 0; prev < w.a_hi && it < g.max_iter; ++it)
+    {
+      const double next =
+        fir::dmin(w.a_hi, fir::dmin(w.a[0], fir::dmin(w.a[1], w.a[2])));
+      if ((unsigned)w.pos[0] <= (unsigned)g.nm1[0] &&
+          (unsigned)w.pos[1] <= (unsigned)g.nm1[1] &&
+          (unsigned)w.pos[2] <= (unsigned)g.nm1[2])
+      {
+        const int P = w.pos[0] + w.pos[1] * g.row;
+        while (r < L && px[r] != P) ++r;
+        const float add = (float)((next - prev) * w.d12 * dr);
+        if (r < L)
+          atomicAdd(sh + r * nz + w.pos[2], add);
+        else
+        {
+          unsafeAtomicAdd(bp + P + (long long)w.pos[2] * nxy, add);
+          r = 0;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        if (fir::dabs(w.a[d] - next) < fir::kEps)
+        {
+          w.a[d] += w.da[d];
+          w.pos[d] += w.dir[d];
+        }
+      prev = next;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_st; i += kFamBlock)
+  {
+    const int z = i / L, p = i - z * L;
+    const float a = sh[p * nz + z];
+    if (a != 0.0f) unsafeAtomicAdd(bp + (long long)z * nxy + px[p], a);
+  }
+}
+
 // Poisson negative log-likelihood L = sum_b (l_b - y_b log l_b), with
 // l = A x + r, and its gradient A^T (1 - y / l), fused per LOR: walk,
 // line integral, weight, replay of the walk for the scatter. Bins with
@@ -1239,9 +1397,23 @@ int fir_osem_project(fir_ctx* c, const fir_vol_desc* v, const float* x,
   const long long nvox = (long long)v->nx * v->ny * v->nz;
   fir::Scatter sc;
   if ((rc = scatter_for(c, bp, nvox, 1, &sc))) return rc;
-  hipLaunchKernelGGL(k_osem_project, dim3(blocks_for(n)), dim3(kBlock), 0,
-                     c->stream, g, lor_geom(c), x, y, bias, sc, n, n_subsets,
-                     subset);
+  if (c->walk_sheets && c->n_pairs <= kFamBlock &&
+      (long long)(v->nx + v->ny - 1) * v->nz <= kFamSheetFloats &&
+      v->nx + v->ny <= kFamMaxPx)
+  {
+    // one block per (view, tangential bin) family of the subset
+    LorGeom lg = lor_geom(c);
+    lg.row_mode = fir::ROWS_VIEW_MAJOR;
+    lg.tang_major = 1;
+    const int vpr = fir::views_in_subset(c->n_views, n_subsets, subset);
+    hipLaunchKernelGGL(k_osem_project_fam, dim3(c->n_tang, vpr),
+                       dim3(kFamBlock), 0, c->stream, g, lg, x, y, bias, sc,
+                       n_subsets, subset, c->n_tang, c->n_pairs, v->nz);
+  }
+  else
+    hipLaunchKernelGGL(k_osem_project, dim3(blocks_for(n)), dim3(kBlock), 0,
+                       c->stream, g, lor_geom(c), x, y, bias, sc, n, n_subsets,
+                       subset);
   FIR_CHECK_LAUNCH();
   return gather_copies(c, bp, nvox, 1, 0);
 }
@@ -1637,6 +1809,13 @@ int fir_set_lor_order(fir_ctx* c, int order)
   if (!c->proj_set) return FIR_OK;
   FIR_HIP_CHECK(hipSetDevice(c->device));
   return upload_pairs(c);
+}
+
+int fir_set_walk_sheets(fir_ctx* c, int enable)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  c->walk_sheets = enable ? 1 : 0;
+  return FIR_OK;
 }
 
 int fir_set_scatter_copies(fir_ctx* c, int copies)

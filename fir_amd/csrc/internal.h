@@ -88,6 +88,9 @@ struct fir_ctx
   long long cache_chunk = 1LL << 28;
   // Whether the path matrices built afterwards get the sheet form.
   int cache_sheets = 1;
+  // Whether the walked OSEM projector runs family by family with the
+  // family's sheet in LDS (fir_set_walk_sheets).
+  int walk_sheets = 1;
   // Path matrices of live caches, shared by caches of one volume, walk mode
   // and length format.
   std::vector<std::weak_ptr<RowMatrix>> row_matrices;

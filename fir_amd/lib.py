@@ -47,7 +47,7 @@ EXPORTS = (
     "fir_path_count", "fir_poisson_nll_grad", "fir_osem_project_batch",
     "fir_set_uneven_subsets", "fir_set_lor_order", "fir_set_scatter_copies",
     "fir_set_cache_lanes", "fir_set_cache_zslices", "fir_set_cache_chunk",
-    "fir_set_cache_sheets",
+    "fir_set_cache_sheets", "fir_set_walk_sheets",
     "fir_cache_create",
     "fir_cache_create_partial", "fir_cache_subsets", "fir_cache_destroy",
     "fir_cache_info",
@@ -140,6 +140,7 @@ def load() -> C.CDLL:
         "fir_set_cache_zslices": [vp, i],
         "fir_set_cache_chunk": [vp, i64],
         "fir_set_cache_sheets": [vp, i],
+        "fir_set_walk_sheets": [vp, i],
         "fir_cache_forward": [vp, vp, vp],
         "fir_cache_backward": [vp, vp, vp],
         "fir_cache_sensitivity": [vp, vp],
@@ -240,6 +241,10 @@ class Context:
         """z-slices interleaved in the gather layouts of the next caches."""
         _check(self.lib.fir_set_cache_zslices(self.h, zq))
         self._cache_zq = zq
+
+    def set_walk_sheets(self, enable: bool):
+        """Whether osem_project runs family by family with LDS sheets."""
+        _check(self.lib.fir_set_walk_sheets(self.h, int(enable)))
 
     def set_cache_sheets(self, enable: bool):
         """Whether the next path matrices also get the sheet form."""

@@ -137,6 +137,15 @@ int fir_set_uneven_subsets(fir_ctx* ctx, int allow);
 int fir_set_lor_order(fir_ctx* ctx, int order);
 int fir_set_scatter_copies(fir_ctx* ctx, int copies);
 
+/* Whether fir_osem_project (the walked OSEM projector) runs family by
+ * family (default 1, when the image's sheets fit in 64 KB of LDS): the
+ * LORs of one (view, tangential bin) cross the same pixels, so one block
+ * stages that sheet of x (pixels x slices) in LDS for the line integrals
+ * (the same sums bit for bit), then accumulates the family's
+ * backprojection there and adds each touched voxel to bp once. 0 walks
+ * LOR by LOR with one atomic per path element. */
+int fir_set_walk_sheets(fir_ctx* ctx, int enable);
+
 /* ---- device memory -------------------------------------------------- */
 
 int fir_malloc(fir_ctx* ctx, void** dptr, size_t bytes);
