@@ -324,7 +324,11 @@ constexpr int kFamSheetFloats = 16384; // 64 KB of LDS
 constexpr int kFamBlock = 1024;
 constexpr int kFamMaxPx = 1024;
 
-__global__ __launch_bounds__(kFamBlock) void k_osem_project_fam(
+// OSEM: the projector pass (line integrals, ratio y / (line + bias),
+// backprojection of the ratios). Otherwise the backprojection of sinogram y
+// (of ones if y is NULL), as k_backward.
+template <bool OSEM>
+__global__ __launch_bounds__(kFamBlock) void k_project_fam(
   Grid g, LorGeom lg, const float* __restrict__ x, const float* __restrict__ y,
   const float* __restrict__ bias, fir::Scatter sc, int n_subsets, int subset,
   int n_tang, int n_pairs, int nz)
@@ -363,12 +367,15 @@ __global__ __launch_bounds__(kFamBlock) void k_osem_project_fam(
   const int L = n_px;
   if (L == 0 || L * nz > kFamSheetFloats) return; // host checks the bound
   const int n_st = L * nz;
-  for (int i = threadIdx.x; i < n_st; i += kFamBlock)
+  if (OSEM)
   {
-    const int z = i / L, p = i - z * L;
-    sh[p * nz + z] = x[(long long)z * nxy + px[p]];
+    for (int i = threadIdx.x; i < n_st; i += kFamBlock)
+    {
+      const int z = i / L, p = i - z * L;
+      sh[p * nz + z] = x[(long long)z * nxy + px[p]];
+    }
+    __syncthreads();
   }
-  __syncthreads();
 
   // Forward: one LOR per thread, the walk's pixels followed along the
   // sheet (rank r), values from LDS.
@@ -382,7 +389,8 @@ __global__ __launch_bounds__(kFamBlock) void k_osem_project_fam(
     fir::lor_from_index(lg, (long long)tang * n_pairs + pair, lg.n_views, view,
                         Lr);
     hit = fir::walk_setup(g, Lr.e1, Lr.e2, w);
-    if (hit)
+    if (hit && !OSEM) ratio = y ? y[Lr.bin] : 1.0f;
+    if (hit && OSEM)
     {
       float line = 0.0f;
       int r = 0;
@@ -463,6 +471,32 @@ __global__ __launch_bounds__(kFamBlock) void k_osem_project_fam(
     const float a = sh[p * nz + z];
     if (a != 0.0f) unsafeAtomicAdd(bp + (long long)z * nxy + px[p], a);
   }
+}
+
+// Whether the walked projectors can run family by family on this geometry.
+bool family_ok(const fir_ctx* c, const fir_vol_desc* v)
+{
+  return c->walk_sheets && c->n_pairs <= kFamBlock &&
+    (long long)(v->nx + v->ny - 1) * v->nz <= kFamSheetFloats &&
+    v->nx + v->ny <= kFamMaxPx;
+}
+
+// One block per (view, tangential bin) family of the subset (n_subsets 0:
+// every view).
+template <bool OSEM>
+void launch_family(fir_ctx* c, const Grid& g, const fir_vol_desc* v,
+                   const float* x, const float* y, const float* bias,
+                   const fir::Scatter& sc, int n_subsets, int subset)
+{
+  LorGeom lg = lor_geom(c);
+  lg.row_mode = fir::ROWS_VIEW_MAJOR;
+  lg.tang_major = 1;
+  const int S = n_subsets > 0 ? n_subsets : 1;
+  const int s = n_subsets > 0 ? subset : 0;
+  const int vpr = fir::views_in_subset(c->n_views, S, s);
+  hipLaunchKernelGGL(k_project_fam<OSEM>, dim3(c->n_tang, vpr),
+                     dim3(kFamBlock), 0, c->stream, g, lg, x, y, bias, sc, S,
+                     s, c->n_tang, c->n_pairs, v->nz);
 }
 
 // Poisson negative log-likelihood L = sum_b (l_b - y_b log l_b), with
@@ -1350,9 +1384,13 @@ int fir_backward_subset(fir_ctx* c, const fir_vol_desc* v, const float* sino,
   const long long nvox = (long long)v->nx * v->ny * v->nz;
   fir::Scatter sc;
   if ((rc = scatter_for(c, img, nvox, 1, &sc))) return rc;
-  hipLaunchKernelGGL(k_backward, dim3(blocks_for(n)), dim3(kBlock), 0,
-                     c->stream, g, lor_geom(c), sino, sc, n, n_subsets,
-                     subset);
+  if (family_ok(c, v))
+    launch_family<false>(c, g, v, nullptr, sino, nullptr, sc, n_subsets,
+                         subset);
+  else
+    hipLaunchKernelGGL(k_backward, dim3(blocks_for(n)), dim3(kBlock), 0,
+                       c->stream, g, lor_geom(c), sino, sc, n, n_subsets,
+                       subset);
   FIR_CHECK_LAUNCH();
   return gather_copies(c, img, nvox, 1, 0);
 }
@@ -1397,19 +1435,8 @@ int fir_osem_project(fir_ctx* c, const fir_vol_desc* v, const float* x,
   const long long nvox = (long long)v->nx * v->ny * v->nz;
   fir::Scatter sc;
   if ((rc = scatter_for(c, bp, nvox, 1, &sc))) return rc;
-  if (c->walk_sheets && c->n_pairs <= kFamBlock &&
-      (long long)(v->nx + v->ny - 1) * v->nz <= kFamSheetFloats &&
-      v->nx + v->ny <= kFamMaxPx)
-  {
-    // one block per (view, tangential bin) family of the subset
-    LorGeom lg = lor_geom(c);
-    lg.row_mode = fir::ROWS_VIEW_MAJOR;
-    lg.tang_major = 1;
-    const int vpr = fir::views_in_subset(c->n_views, n_subsets, subset);
-    hipLaunchKernelGGL(k_osem_project_fam, dim3(c->n_tang, vpr),
-                       dim3(kFamBlock), 0, c->stream, g, lg, x, y, bias, sc,
-                       n_subsets, subset, c->n_tang, c->n_pairs, v->nz);
-  }
+  if (family_ok(c, v))
+    launch_family<true>(c, g, v, x, y, bias, sc, n_subsets, subset);
   else
     hipLaunchKernelGGL(k_osem_project, dim3(blocks_for(n)), dim3(kBlock), 0,
                        c->stream, g, lor_geom(c), x, y, bias, sc, n, n_subsets,

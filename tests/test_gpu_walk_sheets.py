@@ -65,3 +65,30 @@ def test_family_projector_matches_lor_walk_c2():
         fam, lor = project_both(w, x, y, None, (0, 9), 16, geometric)
         for a, c in zip(fam, lor):
             assert rel_l2(a, c) < 1e-6
+
+
+@pytest.mark.parametrize("with_sino", [True, False])
+def test_family_backprojection_matches_lor_walk(c1_golden, with_sino):
+    """fir_backward / sensitivity family by family against the LOR-by-LOR
+    scatter, C1 (all subsets) and C2 (one pass of every LOR)."""
+    if not torch.cuda.is_available():
+        pytest.fail("no HIP device visible")
+    for w, y in ((synth.c1(), c1_golden["y"]), (synth.c2(), None)):
+        ctx = firlib.context_for(w)
+        v = firlib.VolDesc.from_spec(w.vol)
+        if y is None:
+            y = np.random.default_rng(4).poisson(
+                3.0, ctx.n_bins).astype(np.float32)
+        yt = torch.from_numpy(y).cuda() if with_sino else None
+        out = []
+        for on in (True, False):
+            ctx.set_walk_sheets(on)
+            f = torch.zeros(4 * v.n_voxels, device="cuda")
+            if with_sino:
+                ctx.backward(v, yt, f, 4)
+            else:
+                ctx.sensitivity(v, f, 4)
+            out.append(f.cpu().numpy())
+        assert np.abs(out[0]).max() > 0
+        assert rel_l2(out[0], out[1]) < 1e-6
+        ctx.close()
