@@ -92,3 +92,33 @@ def test_family_backprojection_matches_lor_walk(c1_golden, with_sino):
         assert np.abs(out[0]).max() > 0
         assert rel_l2(out[0], out[1]) < 1e-6
         ctx.close()
+
+
+def test_forward_unaffected_by_walk_sheets(c1_golden):
+    """fir_forward walks LOR by LOR whatever the setting (a forward pass is
+    too light to repay a family's staging): bit for bit the reference's
+    forward projection at C1, and the same output either way at C2, every
+    bin written."""
+    if not torch.cuda.is_available():
+        pytest.fail("no HIP device visible")
+    w = synth.c1()
+    ctx = firlib.context_for(w)
+    v = firlib.VolDesc.from_spec(w.vol)
+    x = torch.from_numpy(c1_golden["act"].ravel()).cuda()
+    s = torch.zeros(ctx.n_bins, device="cuda")
+    ctx.forward(v, x, s)
+    assert np.array_equal(s.cpu().numpy(), c1_golden["y0"])
+    ctx.close()
+    w = synth.c2()
+    ctx = firlib.context_for(w)
+    v = firlib.VolDesc.from_spec(w.vol)
+    act, _ = synth.phantom(w)
+    x = torch.from_numpy(act.ravel()).cuda()
+    out = []
+    for on in (True, False):
+        ctx.set_walk_sheets(on)
+        s = torch.full((ctx.n_bins,), -1.0, device="cuda")
+        ctx.forward(v, x, s)
+        out.append(s.cpu().numpy())
+    assert np.array_equal(out[0], out[1]) and (out[0] > 0).any()
+    ctx.close()
