@@ -607,9 +607,11 @@ def main():
         "speedup_cached_vs_plain": round(plain_ms / ms_step, 3),
         "path_elements_per_iter": int(n_e_total),
         "roofline": {
-            "kernel": "k_pc_osem_rows_sheet + k_pc_backproject (one cached "
-                      "OSEM projector launch: line integrals from LDS-staged "
-                      "sheets, transposed backprojection)",
+            "kernel": "k_pc_osem_fam + k_pc_fam_reduce (one cached OSEM "
+                      "projector launch: line integrals from LDS-staged "
+                      "sheets, each family's voxel lists summed against its "
+                      "weights in LDS, then the families' partials summed "
+                      "per voxel)",
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,

@@ -991,6 +991,7 @@ __global__ void k_pc_pack(float* __restrict__ X, float* __restrict__ XT,
 constexpr int kSheetMaxFloats = 16384;
 constexpr int kSheetBlock = 1024;
 constexpr int kSheetQuads = 2; // quads in flight per lane (2 measured best)
+constexpr int kFamRowsMax = 2048; // rows of one family the fused pass holds
 
 // First row of every family: the rows before the family's first LOR.
 __global__ void k_sh_fam_rows(const int* __restrict__ rowid, int n_pairs,
@@ -1253,6 +1254,434 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_rows_sheet(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Family backprojection over the sheet form
+// ---------------------------------------------------------------------------
+// The block that computed a family's line integrals also backprojects its
+// weights, which it keeps in LDS: every sheet entry (pixel rank * nz + z)
+// has a list of the family's elements through that voxel (row in family |
+// length << 16, rows ascending, padded to whole quads with zeros), and the
+// block sums each list against the LDS weights into a partial sheet
+// (global entry (fam_off[f] + pixel rank) * nz + z). A reduction then sums,
+// for every voxel, the partials of the subset's families whose sheet holds
+// its pixel, in family order. No atomics: the sums are the same every run.
+
+// Lists, count pass: the padded quad total of each family's lists.
+constexpr int kFlBlock = 1024;
+
+__device__ __forceinline__ void fl_count(const int* __restrict__ fam_row,
+                                         const long long* __restrict__ off,
+                                         const unsigned* __restrict__ sel,
+                                         int f, int n_st, unsigned* cnt)
+{
+  for (int i = threadIdx.x; i < n_st; i += kFlBlock) cnt[i] = 0;
+  __syncthreads();
+  const int r0 = fam_row[f], r1 = fam_row[f + 1];
+  if (r0 < r1)
+    for (long long e = off[r0] + threadIdx.x; e < off[r1]; e += kFlBlock)
+    {
+      const unsigned u = sel[e];
+      if (u >> 16) atomicAdd(cnt + (u & 0xffffu), 1u);
+    }
+  __syncthreads();
+}
+
+// Exclusive scan of the padded quad counts of cnt[0 .. n_st) in place;
+// returns the total to every thread.
+__device__ __forceinline__ unsigned fl_scan(unsigned* cnt, int n_st,
+                                            unsigned* part)
+{
+  const int chunk = (n_st + kFlBlock - 1) / kFlBlock;
+  const int a = (int)threadIdx.x * chunk;
+  const int b = min(n_st, a + chunk);
+  unsigned t = 0;
+  for (int i = a; i < b; ++i) t += (cnt[i] + 3u) / 4u;
+  part[threadIdx.x] = t;
+  __syncthreads();
+  for (int d = 1; d < kFlBlock; d <<= 1) // inclusive Hillis-Steele scan
+  {
+    const unsigned v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  unsigned run = part[threadIdx.x] - t;
+  for (int i = a; i < b; ++i)
+  {
+    const unsigned c = cnt[i];
+    cnt[i] = run;
+    run += (c + 3u) / 4u;
+  }
+  const unsigned total = part[kFlBlock - 1];
+  __syncthreads();
+  return total;
+}
+
+__global__ __launch_bounds__(kFlBlock) void k_fl_count(
+  const int* __restrict__ fam_row, const int* __restrict__ fam_off, int nz,
+  const long long* __restrict__ off, const unsigned* __restrict__ sel,
+  unsigned* __restrict__ ftot)
+{
+  __shared__ unsigned cnt[kSheetMaxFloats];
+  __shared__ unsigned part[kFlBlock];
+  const int f = blockIdx.x;
+  const int n_st = (fam_off[f + 1] - fam_off[f]) * nz;
+  fl_count(fam_row, off, sel, f, n_st, cnt);
+  const unsigned total = fl_scan(cnt, n_st, part);
+  if (threadIdx.x == 0) ftot[f] = total;
+}
+
+// Lists, fill pass: first quads of the family's entries (fl_off), the
+// elements (fl, cleared beforehand) and each list sorted by row.
+__global__ __launch_bounds__(kFlBlock) void k_fl_fill(
+  const int* __restrict__ fam_row, const int* __restrict__ fam_off, int nz,
+  const long long* __restrict__ off, const unsigned* __restrict__ sel,
+  const long long* __restrict__ fbase, unsigned* __restrict__ fl_off,
+  unsigned* __restrict__ fl)
+{
+  __shared__ unsigned pos[kSheetMaxFloats];
+  __shared__ unsigned cur[kSheetMaxFloats];
+  __shared__ unsigned part[kFlBlock];
+  const int f = blockIdx.x;
+  const int n_st = (fam_off[f + 1] - fam_off[f]) * nz;
+  const long long g0 = (long long)fam_off[f] * nz;
+  const long long base = fbase[f];
+  fl_count(fam_row, off, sel, f, n_st, pos);
+  fl_scan(pos, n_st, part);
+  for (int i = threadIdx.x; i < n_st; i += kFlBlock)
+  {
+    fl_off[g0 + i] = (unsigned)(base + pos[i]);
+    cur[i] = 0;
+  }
+  __syncthreads();
+  const int r0 = fam_row[f], r1 = fam_row[f + 1];
+  const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  for (int r = r0 + wave; r < r1; r += kFlBlock / 64)
+    for (long long e = off[r] + lane; e < off[r + 1]; e += 64)
+    {
+      const unsigned u = sel[e];
+      if (!(u >> 16)) continue;
+      const unsigned k = atomicAdd(cur + (u & 0xffffu), 1u);
+      fl[(base + pos[u & 0xffffu]) * 4 + k] =
+        (unsigned)(r - r0) | (u & 0xffff0000u);
+    }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_st; i += kFlBlock)
+  {
+    unsigned* l = fl + (base + pos[i]) * 4;
+    const int n = (int)cur[i];
+    for (int a = 1; a < n; ++a)
+    {
+      const unsigned v = l[a];
+      int b = a - 1;
+      while (b >= 0 && (l[b] & 0xffffu) > (v & 0xffffu))
+      {
+        l[b + 1] = l[b];
+        --b;
+      }
+      l[b + 1] = v;
+    }
+  }
+}
+
+// The sheet entries (fam_off positions) of every pixel, per subset: counts.
+__global__ void k_fr_count(const int* __restrict__ fam_off,
+                           const int* __restrict__ fam_xy, int n_fam,
+                           int total, int n_tang, int S, int nxy,
+                           unsigned* __restrict__ cnt)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int lo = 0, hi = n_fam - 1; // family of entry i
+  while (lo < hi)
+  {
+    const int mid = (lo + hi + 1) >> 1;
+    if (fam_off[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const int s = (lo / n_tang) % S;
+  atomicAdd(cnt + (long long)s * (nxy + 1) + fam_xy[i], 1u);
+}
+
+__global__ void k_fr_fill(const int* __restrict__ fam_off,
+                          const int* __restrict__ fam_xy, int n_fam, int total,
+                          int n_tang, int S, int nxy,
+                          const long long* __restrict__ off,
+                          unsigned* __restrict__ cursor, int* __restrict__ ent)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int lo = 0, hi = n_fam - 1;
+  while (lo < hi)
+  {
+    const int mid = (lo + hi + 1) >> 1;
+    if (fam_off[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const int s = (lo / n_tang) % S;
+  const long long k = (long long)s * (nxy + 1) + fam_xy[i];
+  ent[off[k] + atomicAdd(cursor + k, 1u)] = i;
+}
+
+// Each pixel's entries in increasing order (family order).
+__global__ void k_fr_sort(const long long* __restrict__ off, long long n,
+                          int* __restrict__ ent)
+{
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const long long a = off[k], b = off[k + 1];
+  for (long long i = a + 1; i < b; ++i)
+  {
+    const int e = ent[i];
+    long long j = i - 1;
+    while (j >= a && ent[j] > e)
+    {
+      ent[j + 1] = ent[j];
+      --j;
+    }
+    ent[j + 1] = e;
+  }
+}
+
+// Lists, interleave pass: the family's entries in slots sorted by list
+// length (longest first), 64 slots to a group (one wave), and each group's
+// lists stored round by round: round m holds the m-th quad of every slot
+// whose list is longer than m, in slot order, so that a wave's loads of one
+// round are contiguous. Per slot: entry << 16 | list quads, and the group's
+// first quad.
+constexpr int kFlMaxQuads = kFamRowsMax / 4; // a voxel is on a row at most once
+
+__global__ __launch_bounds__(kFlBlock) void k_fl_interleave(
+  const int* __restrict__ fam_off, int nz, const long long* __restrict__ fbase,
+  const unsigned* __restrict__ cl_off, const unsigned* __restrict__ cl,
+  unsigned* __restrict__ fl_slot, unsigned* __restrict__ fl_gb,
+  unsigned* __restrict__ fl)
+{
+  __shared__ unsigned short lq[kSheetMaxFloats];
+  __shared__ unsigned short ord[kSheetMaxFloats];
+  __shared__ unsigned hist[kFlMaxQuads + 1];
+  __shared__ unsigned gpre[kSheetMaxFloats / 64 + 1];
+  const int f = blockIdx.x;
+  const int n_st = (fam_off[f + 1] - fam_off[f]) * nz;
+  if (n_st == 0) return;
+  const long long g0 = (long long)fam_off[f] * nz;
+  const long long base = fbase[f];
+  const int n_groups = (n_st + 63) / 64;
+  for (int i = threadIdx.x; i <= kFlMaxQuads; i += kFlBlock) hist[i] = 0;
+  for (int i = threadIdx.x; i <= n_groups; i += kFlBlock) gpre[i] = 0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < n_st; e += kFlBlock)
+  {
+    const unsigned q = cl_off[g0 + e + 1] - cl_off[g0 + e];
+    lq[e] = (unsigned short)q;
+    atomicAdd(hist + min(q, (unsigned)kFlMaxQuads), 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    unsigned run = 0;
+    for (int v = kFlMaxQuads; v >= 0; --v)
+    {
+      const unsigned c = hist[v];
+      hist[v] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n_st; e += kFlBlock)
+    ord[atomicAdd(hist + min((unsigned)lq[e], (unsigned)kFlMaxQuads), 1u)] =
+      (unsigned short)e;
+  __syncthreads();
+  for (int j = threadIdx.x; j < n_st; j += kFlBlock)
+    atomicAdd(gpre + j / 64, (unsigned)lq[ord[j]]);
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    unsigned run = 0;
+    for (int g = 0; g < n_groups; ++g)
+    {
+      const unsigned c = gpre[g];
+      gpre[g] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n_st; j += kFlBlock)
+  {
+    const int e = ord[j];
+    fl_slot[g0 + j] = ((unsigned)e << 16) | lq[e];
+    fl_gb[g0 + j] = (unsigned)(base + gpre[j / 64]);
+  }
+  const int wave = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
+  const unsigned long long below = (1ull << ln) - 1ull;
+  for (int g = wave; g < n_groups; g += kFlBlock / 64)
+  {
+    const int j = g * 64 + ln;
+    const int e = j < n_st ? ord[j] : 0;
+    const int q = j < n_st ? lq[e] : 0;
+    const unsigned src = j < n_st ? cl_off[g0 + e] : 0u;
+    const long long gb = base + gpre[g];
+    unsigned roff = 0;
+    for (int m = 0;; ++m)
+    {
+      const unsigned long long act = __ballot(m < q);
+      if (act == 0) break;
+      if (m < q)
+      {
+        const long long d = gb + roff + __popcll(act & below);
+        *reinterpret_cast<uint4*>(fl + 4 * d) =
+          *reinterpret_cast<const uint4*>(cl + 4LL * (src + m));
+      }
+      roff += (unsigned)__popcll(act);
+    }
+  }
+}
+
+// k_pc_osem_rows_sheet plus the family's backprojection into `part`.
+template <int F>
+__global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
+  int n_subsets, int subset, int n_tang, int n_fam,
+  const int* __restrict__ fam_row, const int* __restrict__ fam_off,
+  const int* __restrict__ fam_xy, const unsigned* __restrict__ sel,
+  RowData rd, int lanes, const float* __restrict__ img, int nxy, int nz,
+  OsemRowsIO io, float* __restrict__ w, const unsigned* __restrict__ fl_slot,
+  const unsigned* __restrict__ fl_gb, const unsigned* __restrict__ fl,
+  float* __restrict__ part)
+{
+  __shared__ float sh[kSheetMaxFloats];
+  __shared__ float wr[kFamRowsMax];
+  const int view = (int)blockIdx.y * n_subsets + subset;
+  const int f = view * n_tang + (int)blockIdx.x;
+  if (f >= n_fam) return;
+  const int r0 = fam_row[f], r1 = fam_row[f + 1];
+  const int p0 = fam_off[f];
+  const int L = fam_off[f + 1] - p0;
+  const int n_st = L * nz;
+  float* pf = part + (long long)p0 * nz;
+  if (r0 == r1 || r1 - r0 > kFamRowsMax)
+  {
+    // no rows: the family's partials are 0 (a family never exceeds
+    // kFamRowsMax rows: the host checks)
+    for (int i = threadIdx.x; i < n_st; i += kSheetBlock) pf[i] = 0.0f;
+    return;
+  }
+  if (n_st == 0 && threadIdx.x == 0) sh[0] = 0.0f;
+  for (int i0 = threadIdx.x; i0 < n_st; i0 += 8 * kSheetBlock)
+  {
+    float val[8];
+    int at[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+    {
+      const int i = i0 + u * kSheetBlock;
+      at[u] = -1;
+      val[u] = 0.0f;
+      if (i < n_st)
+      {
+        const int z = i / L, p = i - z * L;
+        at[u] = p * nz + z;
+        val[u] = img[(long long)z * nxy + fam_xy[p0 + p]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (at[u] >= 0) sh[at[u]] = val[u];
+  }
+  const int lane = (int)threadIdx.x & (lanes - 1);
+  const int per_round = kSheetBlock / lanes;
+  __syncthreads();
+  for (int rb = r0; rb < r1; rb += per_round)
+  {
+    const int r = rb + (int)threadIdx.x / lanes;
+    const bool active = r < r1;
+    float acc[1] = {0.0f};
+    if (active)
+      sheet_sum<F>(sel, sh, rd.off[r] + 4LL * lane, rd.off[r + 1], lanes, acc);
+    lane_reduce<1>(acc, lanes);
+    if (active && lane == 0)
+    {
+      const float fct = rd.factor[r];
+      const int bin = rd.bin[r];
+      float out = 0.0f;
+      float line = acc[0] * fct;
+      if (io.bias[0]) line += io.bias[0][bin];
+      if (line > kEpsF) out = io.y[0][bin] / line * fct;
+      w[r] = out;
+      wr[r - r0] = out;
+    }
+  }
+  __syncthreads();
+  // each sheet entry's list against the weights in LDS: a wave takes a
+  // group of 64 slots, a lane one entry, summed in row order (the same
+  // sums every run), four rounds of loads in flight
+  const long long g0 = (long long)p0 * nz;
+  const int wave = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
+  const unsigned long long below = (1ull << ln) - 1ull;
+  for (int g = wave; g * 64 < n_st; g += kSheetBlock / 64)
+  {
+    const int j = g * 64 + ln;
+    const unsigned sl = j < n_st ? fl_slot[g0 + j] : 0u;
+    const unsigned gb = j < n_st ? fl_gb[g0 + j] : 0u;
+    const int q = (int)(sl & 0xffffu);
+    float acc = 0.0f;
+    unsigned roff = 0;
+    for (int m = 0;; m += 4)
+    {
+      unsigned long long act[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) act[k] = __ballot(m + k < q);
+      if (act[0] == 0) break;
+      uint4 v[4];
+      unsigned o = roff;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+      {
+        v[k] = m + k < q ? *reinterpret_cast<const uint4*>(
+                             fl + 4LL * (gb + o + __popcll(act[k] & below)))
+                         : make_uint4(0u, 0u, 0u, 0u);
+        o += (unsigned)__popcll(act[k]);
+      }
+      roff = o;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+      {
+        float l[4];
+        Quad<F>::dec(sheet_lengths(v[k]), l);
+        acc = fmaf(l[0], wr[v[k].x & 0xffffu], acc);
+        acc = fmaf(l[1], wr[v[k].y & 0xffffu], acc);
+        acc = fmaf(l[2], wr[v[k].z & 0xffffu], acc);
+        acc = fmaf(l[3], wr[v[k].w & 0xffffu], acc);
+      }
+    }
+    if (j < n_st) pf[sl >> 16] = acc;
+  }
+}
+
+// out[v] = sum over the subset's families whose sheet holds v's pixel of
+// that family's partial (family order). One thread per voxel, slices
+// fastest (consecutive threads read consecutive partials).
+__global__ __launch_bounds__(kBlock) void k_pc_fam_reduce(
+  const long long* __restrict__ off, const int* __restrict__ ent,
+  const float* __restrict__ part, int nxy, int nz, float* __restrict__ out)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nxy * nz) return;
+  const int px = (int)(t / nz), z = (int)(t - (long long)px * nz);
+  const long long a = off[px], b = off[px + 1];
+  float acc = 0.0f;
+  long long k = a;
+  for (; k + 4 <= b; k += 4) // four loads in flight, added in order
+  {
+    float v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = part[(long long)ent[k + u] * nz + z];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += v[u];
+  }
+  for (; k < b; ++k) acc += part[(long long)ent[k] * nz + z];
+  out[(long long)z * nxy + px] = acc;
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -1279,6 +1708,14 @@ struct RowMatrix
   int* d_fam_xy = nullptr;          // sheet pixels (x + y nx)
   unsigned* d_sel = nullptr;        // [n_slots] sheet index | length << 16
   long long sheet_bytes = 0;
+  long long n_sheet_px = 0;         // sheet pixels over all families
+  // Per sheet entry (pixel, slice) of every family, the family's elements
+  // through that voxel (row in family | length << 16), whole quads, in the
+  // slot order and round-interleaved groups of k_fl_interleave.
+  unsigned* d_fl_slot = nullptr;    // [n_sheet_px * nz] entry << 16 | quads
+  unsigned* d_fl_gb = nullptr;      // [n_sheet_px * nz] group's first quad
+  unsigned* d_fl = nullptr;         // [4 * fl_quads]
+  long long fl_quads = 0;
   int sheets_req = 0; // the context asked for the sheet form at build
   int n_views = 0;
   int n_rows = 0;
@@ -1298,7 +1735,8 @@ struct RowMatrix
   long long bytes() const
   {
     return (long long)(n_views + 1) * (4 + 8) + (long long)n_rows * (4 + 4 + 1 + 8) +
-      8 + n_slots * (4 + (long long)len_bytes(fmt)) + sheet_bytes;
+      8 + n_slots * (4 + (long long)len_bytes(fmt)) + sheet_bytes +
+      (d_fl ? 8 * n_sheet_px * vol.nz + 16 * fl_quads : 0);
   }
 
   ~RowMatrix()
@@ -1316,6 +1754,9 @@ struct RowMatrix
     (void)hipFree(d_fam_off);
     (void)hipFree(d_fam_xy);
     (void)hipFree(d_sel);
+    (void)hipFree(d_fl_slot);
+    (void)hipFree(d_fl_gb);
+    (void)hipFree(d_fl);
   }
 };
 
@@ -1344,6 +1785,13 @@ struct fir_cache
   int zfast = 1;
   // One-image OSEM line integrals from the matrix's sheet form, if built.
   int sheets = 1;
+  // Family backprojection (with the sheet form): per subset and pixel the
+  // sheet entries holding it, and the partial sheets of one launch.
+  long long* d_fr_off = nullptr; // [n_subsets * (nxy + 1) + 1]
+  int* d_fr_ent = nullptr;       // [sheet entries]
+  float* d_part = nullptr;       // [sheet entries * nz]
+  long long fr_bytes = 0;
+  int family_bp = 1;
 
   ~fir_cache()
   {
@@ -1354,6 +1802,9 @@ struct fir_cache
     (void)hipFree(d_w);
     (void)hipFree(d_x);
     (void)hipFree(d_xy);
+    (void)hipFree(d_fr_off);
+    (void)hipFree(d_fr_ent);
+    (void)hipFree(d_part);
   }
 };
 
@@ -1634,6 +2085,143 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
   FIR_CHECK_LAUNCH();
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
   rm->sheet_bytes = 4LL * (F + 1) + 4LL * off[F] + 4LL * rm->n_slots;
+  rm->n_sheet_px = off[F];
+  return FIR_OK;
+}
+
+// The per-entry element lists of every family (the family backprojection's
+// input), for a matrix with the sheet form. Families with more rows than
+// the fused kernel holds, lists beyond 2^32 quads, or a device too full
+// leave the matrix without them; only HIP failures are errors.
+int build_family_lists(fir_ctx* c, RowMatrix* rm)
+{
+  const int F = rm->n_fam, nz = rm->vol.nz;
+  if (!rm->d_sel || rm->n_pairs > kFamRowsMax || rm->n_sheet_px <= 0)
+    return FIR_OK;
+  DevBufs tmp;
+  unsigned* ftot;
+  long long* fbase;
+  int rc;
+  if ((rc = tmp.get(&ftot, (size_t)F, "family list sizes")) ||
+      (rc = tmp.get(&fbase, (size_t)F, "family list bases")))
+    return rc;
+  hipLaunchKernelGGL(k_fl_count, dim3(F), dim3(kFlBlock), 0, c->stream,
+                     rm->d_fam_row, rm->d_fam_off, nz, rm->d_off, rm->d_sel,
+                     ftot);
+  FIR_CHECK_LAUNCH();
+  std::vector<unsigned> tot(F);
+  std::vector<long long> base(F);
+  FIR_HIP_CHECK(hipMemcpyAsync(tot.data(), ftot, 4LL * F,
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  long long quads = 0;
+  for (int f = 0; f < F; ++f)
+  {
+    base[f] = quads;
+    quads += tot[f];
+  }
+  const long long entries = rm->n_sheet_px * nz;
+  if (quads >= (1LL << 32)) return FIR_OK;
+  size_t free_b = 0, total_b = 0;
+  FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  if ((long long)free_b < 32 * quads + 12 * entries + (3LL << 30))
+    return FIR_OK;
+  // contiguous sorted lists first (temporary), then the interleaved layout
+  unsigned* cl_off;
+  unsigned* cl;
+  if ((rc = tmp.get(&cl_off, (size_t)entries + 1, "family list offsets")) ||
+      (rc = tmp.get(&cl, (size_t)(4 * std::max(quads, 1LL)), "family lists")))
+    return rc == FIR_E_NOMEM ? FIR_OK : rc;
+  if ((rc = dev_alloc(&rm->d_fl_slot, (size_t)entries, "family list slots")) ||
+      (rc = dev_alloc(&rm->d_fl_gb, (size_t)entries, "family list groups")) ||
+      (rc = dev_alloc(&rm->d_fl, (size_t)(4 * std::max(quads, 1LL)),
+                      "family lists")))
+  {
+    (void)hipFree(rm->d_fl_slot);
+    (void)hipFree(rm->d_fl_gb);
+    rm->d_fl_slot = nullptr;
+    rm->d_fl_gb = nullptr;
+    rm->d_fl = nullptr;
+    return rc == FIR_E_NOMEM ? FIR_OK : rc;
+  }
+  const unsigned end = (unsigned)quads;
+  FIR_HIP_CHECK(hipMemcpyAsync(fbase, base.data(), 8LL * F,
+                               hipMemcpyHostToDevice, c->stream));
+  FIR_HIP_CHECK(hipMemcpyAsync(cl_off + entries, &end, 4,
+                               hipMemcpyHostToDevice, c->stream));
+  FIR_HIP_CHECK(hipMemsetAsync(cl, 0, 16 * quads, c->stream));
+  hipLaunchKernelGGL(k_fl_fill, dim3(F), dim3(kFlBlock), 0, c->stream,
+                     rm->d_fam_row, rm->d_fam_off, nz, rm->d_off, rm->d_sel,
+                     fbase, cl_off, cl);
+  FIR_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_fl_interleave, dim3(F), dim3(kFlBlock), 0, c->stream,
+                     rm->d_fam_off, nz, fbase, cl_off, cl, rm->d_fl_slot,
+                     rm->d_fl_gb, rm->d_fl);
+  FIR_CHECK_LAUNCH();
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  rm->fl_quads = quads;
+  return FIR_OK;
+}
+
+// Tables of the family backprojection (k_pc_osem_fam + k_pc_fam_reduce) for
+// a cache over a matrix with the sheet form: per subset and pixel, the
+// sheet entries that hold it, in family order, and the partial sheets.
+// Families with too many rows, or a device too full, leave the cache on the
+// transposed-list backprojection; only HIP failures are errors.
+int build_family_bp(fir_ctx* c, fir_cache* cache)
+{
+  const RowMatrix* rm = cache->rm.get();
+  if (!rm->d_fl) return FIR_OK;
+  const int S = cache->n_subsets;
+  const int nxy = cache->vol.nx * cache->vol.ny, nz = cache->vol.nz;
+  const long long total = rm->n_sheet_px;
+  const long long n_keys = (long long)S * (nxy + 1);
+  const long long bytes = 8 * (n_keys + 1) + 4 * total + 4 * total * nz;
+  size_t free_b = 0, total_b = 0;
+  FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  if ((long long)free_b < bytes + 4 * n_keys + (2LL << 30)) return FIR_OK;
+  const int n_tang = rm->n_fam / rm->n_views;
+  DevBufs tmp;
+  unsigned* cnt;
+  int rc;
+  if ((rc = tmp.get(&cnt, (size_t)n_keys, "family table counts"))) return rc;
+  if ((rc = dev_alloc(&cache->d_fr_off, (size_t)n_keys + 1,
+                      "family table offsets")) ||
+      (rc = dev_alloc(&cache->d_fr_ent, (size_t)total, "family table")) ||
+      (rc = dev_alloc(&cache->d_part, (size_t)(total * nz),
+                      "family partial sheets")))
+  {
+    (void)hipFree(cache->d_fr_off);
+    (void)hipFree(cache->d_fr_ent);
+    cache->d_fr_off = nullptr;
+    cache->d_fr_ent = nullptr;
+    cache->d_part = nullptr;
+    return rc == FIR_E_NOMEM ? FIR_OK : rc;
+  }
+  FIR_HIP_CHECK(hipMemsetAsync(cnt, 0, 4 * n_keys, c->stream));
+  hipLaunchKernelGGL(k_fr_count, dim3(blocks_for(total)), dim3(kBlock), 0,
+                     c->stream, rm->d_fam_off, rm->d_fam_xy, rm->n_fam,
+                     (int)total, n_tang, S, nxy, cnt);
+  FIR_CHECK_LAUNCH();
+  std::vector<unsigned> h(n_keys);
+  std::vector<long long> off(n_keys + 1, 0);
+  FIR_HIP_CHECK(hipMemcpyAsync(h.data(), cnt, 4 * n_keys,
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (long long k = 0; k < n_keys; ++k) off[k + 1] = off[k] + h[k];
+  FIR_HIP_CHECK(hipMemcpyAsync(cache->d_fr_off, off.data(), 8 * (n_keys + 1),
+                               hipMemcpyHostToDevice, c->stream));
+  FIR_HIP_CHECK(hipMemsetAsync(cnt, 0, 4 * n_keys, c->stream));
+  hipLaunchKernelGGL(k_fr_fill, dim3(blocks_for(total)), dim3(kBlock), 0,
+                     c->stream, rm->d_fam_off, rm->d_fam_xy, rm->n_fam,
+                     (int)total, n_tang, S, nxy, cache->d_fr_off, cnt,
+                     cache->d_fr_ent);
+  FIR_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_fr_sort, dim3(blocks_for(n_keys)), dim3(kBlock), 0,
+                     c->stream, cache->d_fr_off, n_keys, cache->d_fr_ent);
+  FIR_CHECK_LAUNCH();
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  cache->fr_bytes = bytes;
   return FIR_OK;
 }
 
@@ -1808,7 +2396,9 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
                                sizeof(long long) * (c->n_views + 1),
                                hipMemcpyDeviceToHost, c->stream));
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
-  if (rm->d_fam_row && (rc = build_sheets(c, rm.get()))) return rc;
+  if (rm->d_fam_row && ((rc = build_sheets(c, rm.get())) ||
+                        (rc = build_family_lists(c, rm.get()))))
+    return rc;
   *out = rm;
   return FIR_OK;
 }
@@ -2228,6 +2818,35 @@ void launch_rows_sheet(fir_cache* cache, int s, const float* img,
                      cache->vol.nx * cache->vol.ny, cache->vol.nz, io, w);
 }
 
+// Whether one-image OSEM passes of this cache backproject family by family.
+bool use_family_bp(const fir_cache* cache)
+{
+  return use_sheets(cache) && cache->family_bp && cache->d_part;
+}
+
+// One-image OSEM pass family by family: line integrals and weights from the
+// LDS-staged sheets, each family's backprojection into its partial sheet,
+// then the per-voxel sums of the partials (overwriting bp).
+template <int F>
+void launch_osem_family(fir_cache* cache, int s, const float* img,
+                        const OsemRowsIO& io, float* w, float* bp)
+{
+  const RowMatrix* rm = cache->rm.get();
+  const int S = cache->n_subsets;
+  const int vpr = fir::views_in_subset(rm->n_views, S, s);
+  const int n_tang = rm->n_fam / rm->n_views;
+  const int nxy = cache->vol.nx * cache->vol.ny, nz = cache->vol.nz;
+  hipLaunchKernelGGL((k_pc_osem_fam<F>), dim3(n_tang, vpr), dim3(kSheetBlock),
+                     0, cache->ctx->stream, S, s, n_tang, rm->n_fam,
+                     rm->d_fam_row, rm->d_fam_off, rm->d_fam_xy, rm->d_sel,
+                     row_data(rm), cache->fwd_lanes, img, nxy, nz, io, w,
+                     rm->d_fl_slot, rm->d_fl_gb, rm->d_fl, cache->d_part);
+  hipLaunchKernelGGL(k_pc_fam_reduce, dim3(blocks_for((long long)nxy * nz)),
+                     dim3(kBlock), 0, cache->ctx->stream,
+                     cache->d_fr_off + (long long)s * (nxy + 1),
+                     cache->d_fr_ent, cache->d_part, nxy, nz, bp);
+}
+
 template <int F>
 void launch_osem_rows(fir_cache* cache, int s, int nb, const float* X,
                       const OsemRowsIO& io, float* w)
@@ -2372,6 +2991,7 @@ static int cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
   cache->x_floats = (size_t)layout_voxels(cache_xy(cache), v->nz) *
     kMaxCacheBatch;
   if ((rc = build_transpose(c, cache)) ||
+      (rc = build_family_bp(c, cache)) ||
       (rc = dev_alloc(&cache->d_w, cache->w_floats, "row weights")) ||
       (rc = dev_alloc(&cache->d_x, cache->x_floats, "interleaved images")) ||
       (rc = dev_alloc(&cache->d_xy, cache->x_floats, "y-fastest images")))
@@ -2447,7 +3067,8 @@ int fir_cache_info(const fir_cache* cache, int64_t* n_rows, int64_t* n_elements,
     *device_bytes = rm->bytes() +
       cache->t_slots * (4 + (long long)len_bytes(rm->fmt)) +
       (long long)cache->n_cached * (cache->nvox + 1) * 8 +
-      (long long)(cache->w_floats + 2 * cache->x_floats) * 4;
+      (long long)(cache->w_floats + 2 * cache->x_floats) * 4 +
+      cache->fr_bytes;
   return FIR_OK;
 }
 
@@ -2460,10 +3081,11 @@ int fir_cache_tune(fir_cache* cache, int fwd_lanes, int bp_lanes, int flags)
     return fail(FIR_E_ARG, "lanes per LOR / voxel must be a power of two "
                            "from 1 to 64");
   if (flags & ~(FIR_CACHE_XCD_MAP | FIR_CACHE_STORED_VOXELS |
-                FIR_CACHE_NO_SHEETS))
+                FIR_CACHE_NO_SHEETS | FIR_CACHE_NO_FAMILY_BP))
     return fail(FIR_E_ARG, "unknown cache flags 0x%x", flags);
   cache->zfast = (flags & FIR_CACHE_STORED_VOXELS) ? 0 : 1;
   cache->sheets = (flags & FIR_CACHE_NO_SHEETS) ? 0 : 1;
+  cache->family_bp = (flags & FIR_CACHE_NO_FAMILY_BP) ? 0 : 1;
   cache->fwd_lanes = fwd_lanes;
   cache->bp_lanes = bp_lanes;
   cache->xcd_map = (flags & FIR_CACHE_XCD_MAP) ? 1 : 0;
@@ -2592,6 +3214,17 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
     }
     io.n_images = nb;
     // (the sheet form exists for the 16-bit length formats only)
+    if (wd == 1 && use_family_bp(cache))
+    {
+      if (rm->fmt == FIR_LEN_SNORM16)
+        launch_osem_family<FIR_LEN_SNORM16>(cache, subset, x[i0], io,
+                                            cache->d_w, bp[i0]);
+      else
+        launch_osem_family<FIR_LEN_FP16>(cache, subset, x[i0], io, cache->d_w,
+                                         bp[i0]);
+      FIR_CHECK_LAUNCH();
+      continue;
+    }
     if (wd == 1 && use_sheets(cache))
     {
       if (rm->fmt == FIR_LEN_SNORM16)

@@ -29,6 +29,7 @@ FIR_LEN_FP32 = 2
 FIR_CACHE_XCD_MAP = 1
 FIR_CACHE_STORED_VOXELS = 2
 FIR_CACHE_NO_SHEETS = 4
+FIR_CACHE_NO_FAMILY_BP = 8
 # Row order of path caches: per view, tangential bin then (segment, axial)
 # pair by centre slice -- the rows through one voxel are then close together.
 CACHE_ROW_ORDER = FIR_ORDER_VIEW_MAJOR | FIR_ORDER_TANG_MAJOR | FIR_ORDER_ZCENTER

@@ -415,12 +415,22 @@ int fir_cache_info(const fir_cache* cache, int64_t* n_rows,
  * the same pixels of the xy-plane, so that sheet of the image (pixels x
  * slices, up to 64 KB) is staged in LDS and the paths index it with 16-bit
  * entries (the same sums bit for bit); FIR_CACHE_NO_SHEETS gathers from
- * the image instead. */
+ * the image instead. Such passes also backproject family by family when
+ * the matrix holds family lists (built with the sheet form when the device
+ * has room: 4 bytes per element plus 8 per sheet entry): every voxel of a
+ * family's sheet has the list of the family's elements through it, which
+ * the block sums against the family's weights, kept in LDS, into a partial
+ * sheet; a second kernel sums, for every voxel, the partials of the
+ * subset's families that hold its pixel, in family order. The sums are the
+ * same on every run. FIR_CACHE_NO_FAMILY_BP gathers the weights through
+ * the transposed voxel lists instead (sums in another order: results
+ * within FP32 rounding of each other). */
 enum
 {
   FIR_CACHE_XCD_MAP = 1,
   FIR_CACHE_STORED_VOXELS = 2,
-  FIR_CACHE_NO_SHEETS = 4
+  FIR_CACHE_NO_SHEETS = 4,
+  FIR_CACHE_NO_FAMILY_BP = 8
 };
 int fir_cache_tune(fir_cache* cache, int fwd_lanes, int bp_lanes, int flags);
 

@@ -344,22 +344,32 @@ def test_c2_cache_properties():
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
     cache.close()
     # the sheet form at full size (rows in tangential-major order): one-image
-    # OSEM passes equal the image gathers bit for bit, and sum to the
+    # OSEM passes equal the image gathers bit for bit (with the transposed
+    # lists' backprojection), the family backprojection agrees with that
+    # within FP32 summation order and repeats exactly, and both sum to the
     # on-the-fly projector's within the snorm16 bar
     cache = ctx.path_cache(v, 16, firlib.FIR_LEN_SNORM16,
                            row_order=firlib.CACHE_ROW_ORDER)
     yc = torch.poisson(b * 3.0, generator=g)
     for s in (0, 7):
         p1 = torch.zeros(v.n_voxels, device="cuda")
-        cache.tune(8, 16, 0)
+        cache.tune(8, 16, firlib.FIR_CACHE_NO_FAMILY_BP)
         cache.osem_project([x + 0.1], [yc], None, [p1], s)
         p2 = torch.zeros(v.n_voxels, device="cuda")
         cache.tune(8, 16, firlib.FIR_CACHE_NO_SHEETS)
         cache.osem_project([x + 0.1], [yc], None, [p2], s)
         assert torch.equal(p1, p2)
+        pf = torch.full((v.n_voxels,), -1.0, device="cuda")
+        pf2 = torch.zeros(v.n_voxels, device="cuda")
+        cache.tune(8, 16, 0)
+        cache.osem_project([x + 0.1], [yc], None, [pf], s)
+        cache.osem_project([x + 0.1], [yc], None, [pf2], s)
+        assert torch.equal(pf, pf2)
+        assert float((pf - p1).abs().max()) <= 1e-6 * float(p1.abs().max())
         p3 = torch.zeros(v.n_voxels, device="cuda")
         ctx.osem_project(v, x + 0.1, yc, None, p3, s, 16)
         assert rel_l2(host(p1), host(p3)) < 2e-5
+        assert rel_l2(host(pf), host(p3)) < 2e-5
     cache.close()
     ctx.close()
 
@@ -395,7 +405,8 @@ def test_sheet_form_equals_image_gathers(c1, c1_golden, fmt):
     """One-image OSEM passes over the sheet form (each (view, tangential
     bin) family's pixels x slices staged in LDS, 16-bit sheet indices) give
     the same results bit for bit as the gathers from the image
-    (FIR_CACHE_NO_SHEETS): same elements, same order, same values."""
+    (FIR_CACHE_NO_SHEETS): same elements, same order, same values (both
+    backprojecting from the transposed lists)."""
     w, ctx, v, caches = c1
     cache = ctx.path_cache(v, 4, fmt, row_order=firlib.CACHE_ROW_ORDER)
     x = dev(c1_golden["act"].ravel() + 0.1)
@@ -404,7 +415,7 @@ def test_sheet_form_equals_image_gathers(c1, c1_golden, fmt):
     for s in range(4):
         for b in (None, [bias]):
             a = torch.zeros(v.n_voxels, device="cuda")
-            cache.tune(8, 16, 0)
+            cache.tune(8, 16, firlib.FIR_CACHE_NO_FAMILY_BP)
             cache.osem_project([x], [y], b, [a], s)
             g = torch.zeros(v.n_voxels, device="cuda")
             cache.tune(8, 16, firlib.FIR_CACHE_NO_SHEETS)
@@ -431,8 +442,42 @@ def test_cache_without_sheet_form(c1, c1_golden):
     y = dev(c1_golden["y"])
     a = torch.zeros(v.n_voxels, device="cuda")
     b = torch.zeros(v.n_voxels, device="cuda")
+    with_s.tune(8, 16, firlib.FIR_CACHE_NO_FAMILY_BP)
     with_s.osem_project([x], [y], None, [a], 1)
     without.osem_project([x], [y], None, [b], 1)
     assert torch.equal(a, b)
     without.close()
     with_s.close()
+
+
+@pytest.mark.parametrize("fmt", [firlib.FIR_LEN_SNORM16, firlib.FIR_LEN_FP16])
+def test_family_backprojection(c1, c1_golden, fmt):
+    """One-image OSEM passes over the sheet form backproject family by
+    family (each (view, tangential bin) family's per-voxel element lists
+    summed against its weights in LDS, then the families' partial sheets
+    summed per voxel): within FP32 summation order of the transposed lists'
+    backprojection (1e-6 of the largest value), the same bits on every run,
+    every voxel written, and within the format's bar of the on-the-fly
+    projector."""
+    w, ctx, v, caches = c1
+    cache = ctx.path_cache(v, 4, fmt, row_order=firlib.CACHE_ROW_ORDER)
+    x = dev(c1_golden["act"].ravel() + 0.1)
+    y = dev(c1_golden["y"])
+    bias = dev(np.full_like(c1_golden["y"], 0.5))
+    for s in range(4):
+        for b in (None, [bias]):
+            f1 = torch.full((v.n_voxels,), -7.0, device="cuda")
+            f2 = torch.full((v.n_voxels,), 3.0, device="cuda")
+            cache.tune(8, 16, 0)
+            cache.osem_project([x], [y], b, [f1], s)
+            cache.osem_project([x], [y], b, [f2], s)
+            assert torch.equal(f1, f2) and float(f1.min()) >= 0
+            g = torch.zeros(v.n_voxels, device="cuda")
+            cache.tune(8, 16, firlib.FIR_CACHE_NO_FAMILY_BP)
+            cache.osem_project([x], [y], b, [g], s)
+            assert float((f1 - g).abs().max()) <= 1e-6 * float(g.abs().max())
+            p = torch.zeros(v.n_voxels, device="cuda")
+            ctx.osem_project(v, x, y, b[0] if b else None, p, s, 4)
+            assert rel_l2(host(f1), host(p)) < FWD_TOL[fmt]
+    cache.tune(8, 16, 0)
+    cache.close()
