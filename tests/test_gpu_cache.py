@@ -271,7 +271,7 @@ def test_cache_lanes_keep_results(c1, c1_golden):
     for s, b in outs[1:]:
         assert rel_max(s, outs[0][0]) < 1e-6
         assert rel_max(b, outs[0][1]) < 1e-6
-    for bad in [(0, 16, 0), (16, 3, 0), (128, 16, 0), (16, 16, 8)]:
+    for bad in [(0, 16, 0), (16, 3, 0), (128, 16, 0), (16, 16, 16)]:
         with pytest.raises(firlib.FirError):
             cache.tune(*bad)
 
