@@ -173,6 +173,50 @@ def test_cached_osem_vs_reference(c1, c1_golden, fmt):
     assert rel_l2(host(gc.x[0]), g["xc"]) < 1e-4
 
 
+def test_family_path_osem_vs_reference(c1, c1_golden):
+    """Runs A (2 x 4), B (ACF, Gaussian every sub-iteration, cut 55 mm) and
+    C (bias) of the reference through the one-image family path (sheet-form
+    line integrals, family-list backprojection), within 1e-4 of the
+    reference's images."""
+    w, ctx, v, _ = c1
+    g = c1_golden
+    cache = ctx.path_cache(v, 4, firlib.FIR_LEN_SNORM16,
+                           row_order=firlib.CACHE_ROW_ORDER)
+    y = dev(g["y"])
+    # the family path is the one taken (its sums differ in order from the
+    # transposed lists')
+    xa = dev(g["act"].ravel() + 0.1)
+    fam = torch.zeros(v.n_voxels, device="cuda")
+    gat = torch.zeros(v.n_voxels, device="cuda")
+    cache.osem_project([xa], [y], None, [fam], 0)
+    cache.tune(8, 16, firlib.FIR_CACHE_NO_FAMILY_BP)
+    cache.osem_project([xa], [y], None, [gat], 0)
+    cache.tune(8, 16, 0)
+    assert not torch.equal(fam, gat)
+    ga = recon.OSEMGroup(ctx, v, 4, [y], cache=cache)
+    ga.iteration_eager()
+    ga.iteration_eager()
+    assert rel_l2(host(ga.x[0]), g["xa"]) < 1e-4
+    mu = dev(g["hu"].ravel())
+    ctx.hu_to_mu(mu)
+    ctx.cut_circle(v, mu, 55.0)
+    acf = torch.zeros(ctx.n_bins, device="cuda")
+    ctx.forward(v, mu, acf)
+    ctx.exp_clamped(acf)
+    yb = dev(g["y"])
+    ctx.mul_clamped(yb, acf)
+    gb = recon.OSEMGroup(ctx, v, 4, [yb], fwhm=(4.0, 4.0, 4.0),
+                         conv_interval=1, cut=55.0, cache=cache)
+    gb.iteration_eager()
+    gb.iteration_eager()
+    assert rel_l2(host(gb.x[0]), g["xb"]) < 1e-4
+    bias = torch.full((ctx.n_bins,), float(g["bias_value"]), device="cuda")
+    gc = recon.OSEMGroup(ctx, v, 4, [y], biases=[bias], cache=cache)
+    gc.iteration_eager()
+    assert rel_l2(host(gc.x[0]), g["xc"]) < 1e-4
+    cache.close()
+
+
 @pytest.mark.parametrize("n_images", [2, 3, 5, 8, 11])
 def test_cached_batch_equals_single(c1, c1_golden, n_images):
     """Batched projector passes (interleaved images, 2/4/8 per matrix read)
