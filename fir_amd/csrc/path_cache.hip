@@ -1268,6 +1268,7 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_rows_sheet(
 
 // Lists, count pass: the padded quad total of each family's lists.
 constexpr int kFlBlock = 1024;
+constexpr long long kFlStageQuads = 1LL << 26; // 1 GiB of list staging
 
 __device__ __forceinline__ void fl_count(const int* __restrict__ fam_row,
                                          const long long* __restrict__ off,
@@ -1448,20 +1449,21 @@ __global__ void k_fr_sort(const long long* __restrict__ off, long long n,
 // lists stored round by round: round m holds the m-th quad of every slot
 // whose list is longer than m, in slot order, so that a wave's loads of one
 // round are contiguous. Per slot: entry << 16 | list quads, and the group's
-// first quad.
+// first quad. Families f_first + blockIdx.x; their lists are written to
+// fl[4 * (quad - dst_base)] (a staging buffer for a run of families).
 constexpr int kFlMaxQuads = kFamRowsMax / 4; // a voxel is on a row at most once
 
 __global__ __launch_bounds__(kFlBlock) void k_fl_interleave(
-  const int* __restrict__ fam_off, int nz, const long long* __restrict__ fbase,
-  const unsigned* __restrict__ cl_off, const unsigned* __restrict__ cl,
-  unsigned* __restrict__ fl_slot, unsigned* __restrict__ fl_gb,
-  unsigned* __restrict__ fl)
+  int f_first, const int* __restrict__ fam_off, int nz,
+  const long long* __restrict__ fbase, const unsigned* __restrict__ cl_off,
+  const unsigned* __restrict__ cl, unsigned* __restrict__ fl_slot,
+  unsigned* __restrict__ fl_gb, unsigned* __restrict__ fl, long long dst_base)
 {
   __shared__ unsigned short lq[kSheetMaxFloats];
   __shared__ unsigned short ord[kSheetMaxFloats];
   __shared__ unsigned hist[kFlMaxQuads + 1];
   __shared__ unsigned gpre[kSheetMaxFloats / 64 + 1];
-  const int f = blockIdx.x;
+  const int f = f_first + (int)blockIdx.x;
   const int n_st = (fam_off[f + 1] - fam_off[f]) * nz;
   if (n_st == 0) return;
   const long long g0 = (long long)fam_off[f] * nz;
@@ -1528,7 +1530,7 @@ __global__ __launch_bounds__(kFlBlock) void k_fl_interleave(
       if (act == 0) break;
       if (m < q)
       {
-        const long long d = gb + roff + __popcll(act & below);
+        const long long d = gb + roff + __popcll(act & below) - dst_base;
         *reinterpret_cast<uint4*>(fl + 4 * d) =
           *reinterpret_cast<const uint4*>(cl + 4LL * (src + m));
       }
@@ -2124,13 +2126,17 @@ int build_family_lists(fir_ctx* c, RowMatrix* rm)
   if (quads >= (1LL << 32)) return FIR_OK;
   size_t free_b = 0, total_b = 0;
   FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-  if ((long long)free_b < 32 * quads + 12 * entries + (3LL << 30))
+  // the contiguous sorted lists go into the final buffer first; runs of
+  // families are then interleaved through a staging buffer and copied back
+  const long long stage_q = std::min(quads, kFlStageQuads);
+  if ((long long)free_b <
+      16 * (quads + stage_q) + 12 * entries + (3LL << 30))
     return FIR_OK;
-  // contiguous sorted lists first (temporary), then the interleaved layout
   unsigned* cl_off;
-  unsigned* cl;
+  unsigned* stage;
   if ((rc = tmp.get(&cl_off, (size_t)entries + 1, "family list offsets")) ||
-      (rc = tmp.get(&cl, (size_t)(4 * std::max(quads, 1LL)), "family lists")))
+      (rc = tmp.get(&stage, (size_t)(4 * std::max(stage_q, 1LL)),
+                    "family list staging")))
     return rc == FIR_E_NOMEM ? FIR_OK : rc;
   if ((rc = dev_alloc(&rm->d_fl_slot, (size_t)entries, "family list slots")) ||
       (rc = dev_alloc(&rm->d_fl_gb, (size_t)entries, "family list groups")) ||
@@ -2149,15 +2155,29 @@ int build_family_lists(fir_ctx* c, RowMatrix* rm)
                                hipMemcpyHostToDevice, c->stream));
   FIR_HIP_CHECK(hipMemcpyAsync(cl_off + entries, &end, 4,
                                hipMemcpyHostToDevice, c->stream));
-  FIR_HIP_CHECK(hipMemsetAsync(cl, 0, 16 * quads, c->stream));
+  FIR_HIP_CHECK(hipMemsetAsync(rm->d_fl, 0, 16 * quads, c->stream));
   hipLaunchKernelGGL(k_fl_fill, dim3(F), dim3(kFlBlock), 0, c->stream,
                      rm->d_fam_row, rm->d_fam_off, nz, rm->d_off, rm->d_sel,
-                     fbase, cl_off, cl);
+                     fbase, cl_off, rm->d_fl);
   FIR_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_fl_interleave, dim3(F), dim3(kFlBlock), 0, c->stream,
-                     rm->d_fam_off, nz, fbase, cl_off, cl, rm->d_fl_slot,
-                     rm->d_fl_gb, rm->d_fl);
-  FIR_CHECK_LAUNCH();
+  // (a family holds far fewer quads than the staging buffer: n_st * 512)
+  for (int f0 = 0; f0 < F;)
+  {
+    int f1 = f0 + 1;
+    while (f1 < F && base[f1] + tot[f1] - base[f0] <= stage_q) ++f1;
+    const long long q0 = base[f0];
+    const long long nq = (f1 < F ? base[f1] : quads) - q0;
+    if (nq > stage_q)
+      return fail(FIR_E_NOMEM, "family lists: a family exceeds the staging");
+    hipLaunchKernelGGL(k_fl_interleave, dim3(f1 - f0), dim3(kFlBlock), 0,
+                       c->stream, f0, rm->d_fam_off, nz, fbase, cl_off,
+                       rm->d_fl, rm->d_fl_slot, rm->d_fl_gb, stage, q0);
+    FIR_CHECK_LAUNCH();
+    if (nq > 0)
+      FIR_HIP_CHECK(hipMemcpyAsync(rm->d_fl + 4 * q0, stage, 16 * nq,
+                                   hipMemcpyDeviceToDevice, c->stream));
+    f0 = f1;
+  }
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
   rm->fl_quads = quads;
   return FIR_OK;
