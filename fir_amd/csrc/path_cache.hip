@@ -1539,7 +1539,9 @@ __global__ __launch_bounds__(kFlBlock) void k_fl_interleave(
   }
 }
 
-// k_pc_osem_rows_sheet plus the family's backprojection into `part`.
+// k_pc_osem_rows_sheet plus the family's backprojection: the weights stay
+// in LDS, each lane sums one sheet voxel's list against them, and the sums
+// go to the family's partial sheet in `part`.
 template <int F>
 __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
   int n_subsets, int subset, int n_tang, int n_fam,
@@ -2184,9 +2186,9 @@ int build_family_lists(fir_ctx* c, RowMatrix* rm)
 }
 
 // Tables of the family backprojection (k_pc_osem_fam + k_pc_fam_reduce) for
-// a cache over a matrix with the sheet form: per subset and pixel, the
-// sheet entries that hold it, in family order, and the partial sheets.
-// Families with too many rows, or a device too full, leave the cache on the
+// a cache over a matrix with family lists: per subset and pixel, the sheet
+// entries that hold it, in family order, and the partial sheets. A matrix
+// without the lists, or a device too full, leaves the cache on the
 // transposed-list backprojection; only HIP failures are errors.
 int build_family_bp(fir_ctx* c, fir_cache* cache)
 {
