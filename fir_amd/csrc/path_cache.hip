@@ -3095,6 +3095,17 @@ int fir_cache_info(const fir_cache* cache, int64_t* n_rows, int64_t* n_elements,
   return FIR_OK;
 }
 
+int fir_cache_forms(const fir_cache* cache, int* forms)
+{
+  int rc = check_cache(cache);
+  if (rc) return rc;
+  if (!forms) return fail(FIR_E_ARG, "NULL output");
+  const RowMatrix* rm = cache->rm.get();
+  *forms = (rm->d_sel ? FIR_FORM_SHEETS : 0) |
+    (rm->d_sel && rm->d_fl && cache->d_part ? FIR_FORM_FAMILY_LISTS : 0);
+  return FIR_OK;
+}
+
 int fir_cache_tune(fir_cache* cache, int fwd_lanes, int bp_lanes, int flags)
 {
   int rc = check_cache(cache);

@@ -30,6 +30,8 @@ FIR_CACHE_XCD_MAP = 1
 FIR_CACHE_STORED_VOXELS = 2
 FIR_CACHE_NO_SHEETS = 4
 FIR_CACHE_NO_FAMILY_BP = 8
+FIR_FORM_SHEETS = 1
+FIR_FORM_FAMILY_LISTS = 2
 # Row order of path caches: per view, tangential bin then (segment, axial)
 # pair by centre slice -- the rows through one voxel are then close together.
 CACHE_ROW_ORDER = FIR_ORDER_VIEW_MAJOR | FIR_ORDER_TANG_MAJOR | FIR_ORDER_ZCENTER
@@ -51,7 +53,7 @@ EXPORTS = (
     "fir_set_cache_sheets", "fir_set_walk_sheets",
     "fir_cache_create",
     "fir_cache_create_partial", "fir_cache_subsets", "fir_cache_destroy",
-    "fir_cache_info",
+    "fir_cache_info", "fir_cache_forms",
     "fir_cache_tune", "fir_cache_forward", "fir_cache_backward",
     "fir_cache_sensitivity", "fir_cache_osem_project",
     "fir_histogram_events", "fir_rebin_weight",
@@ -130,6 +132,7 @@ def load() -> C.CDLL:
         "fir_poisson_nll_grad": [vp, P, vp, vp, vp, vp, vp],
         "fir_osem_project_batch": [vp, P, i, vp, vp, vp, vp, i, i],
         "fir_cache_create": [vp, P, i, i, C.POINTER(C.c_void_p)],
+        "fir_cache_forms": [vp, C.POINTER(C.c_int)],
         "fir_cache_create_partial": [vp, P, i, i, i, i64,
                                      C.POINTER(C.c_void_p)],
         "fir_cache_subsets": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
@@ -442,6 +445,14 @@ class PathCache:
                                        C.byref(b)))
         return dict(rows=int(r.value), elements=int(e.value),
                     device_bytes=int(b.value))
+
+    def forms(self) -> dict:
+        """Which forms of the path matrix the cache holds
+        (fir_cache_forms)."""
+        f = C.c_int()
+        _check(self.lib.fir_cache_forms(self.h, C.byref(f)))
+        return dict(sheets=bool(f.value & FIR_FORM_SHEETS),
+                    family_lists=bool(f.value & FIR_FORM_FAMILY_LISTS))
 
     def tune(self, fwd_lanes: int = 16, bp_lanes: int = 16, flags: int = 0):
         _check(self.lib.fir_cache_tune(self.h, fwd_lanes, bp_lanes, flags))

@@ -144,3 +144,36 @@ def poisson_counts(y0: np.ndarray, acf: np.ndarray, total: float, seed: int):
     s = mean.sum()
     mean = mean * (total / s) if s > 0 else mean
     return rng.poisson(mean).astype(np.float32)
+
+
+def attenuation(mu_line: np.ndarray) -> np.ndarray:
+    """exp(-m) of line integrals of mu, in float64: the reciprocal of
+    exp(m) from its Taylor polynomial of degree 30 (positive terms only),
+    evaluated with plain IEEE multiplies, divides and adds (Horner).
+
+    The data of the parity runs must come out bit for bit the same on every
+    machine, and vectorised libm exp may differ in the last place between
+    CPUs; these operations do not. For 0 <= m <= 4 the result is within
+    1e-15 relative of exp(-m)."""
+    m = np.asarray(mu_line, np.float64)
+    acc = np.ones_like(m)
+    for k in range(30, 0, -1):
+        acc = acc * m
+        acc = acc / float(k)
+        acc = acc + 1.0
+    return 1.0 / acc
+
+
+def parity_counts(w: Workload, y0: np.ndarray, mu_line: np.ndarray):
+    """Attenuated Poisson counts of a workload with bit-reproducible inputs:
+    y ~ Poisson(scale * y0 * exp(-m)), seeded with ``w.seed``, where y0 is
+    the forward projection of the phantom, m that of its mu map, and scale
+    makes the expected total ``w.counts`` (rounded to 6 significant digits,
+    so that summation order cannot change it). Returns (counts float32,
+    scale)."""
+    lam = y0.astype(np.float64) * attenuation(mu_line)
+    s = float(lam.sum())
+    scale = float(f"{w.counts / s:.6g}") if s > 0 else 0.0
+    lam *= scale
+    rng = np.random.default_rng(w.seed)
+    return rng.poisson(lam).astype(np.float32), scale

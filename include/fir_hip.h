@@ -399,6 +399,21 @@ int fir_cache_destroy(fir_cache* cache);
 int fir_cache_info(const fir_cache* cache, int64_t* n_rows,
                    int64_t* n_elements, int64_t* device_bytes);
 
+/* Which forms of the path matrix the cache holds, as bits of *forms:
+ * FIR_FORM_SHEETS, the sheet form (16-bit sheet indices of tangential-major
+ * rows), and FIR_FORM_FAMILY_LISTS, the family lists plus the partial
+ * sheets of the family backprojection. The build adds each only when the
+ * device has room for it (otherwise one-image passes fall back to the
+ * image gathers and the transposed lists, with results within FP32
+ * rounding); one-image OSEM passes use what is held unless fir_cache_tune
+ * switches it off. No counterpart in the reference. */
+enum
+{
+  FIR_FORM_SHEETS = 1,
+  FIR_FORM_FAMILY_LISTS = 2
+};
+int fir_cache_forms(const fir_cache* cache, int* forms);
+
 /* Kernel tuning (results change only by FP32 summation order): lanes
  * that share one LOR (forward) and one voxel (backprojection), powers of
  * two from 1 to 64 (default: the layout's, fir_set_cache_lanes; other

@@ -27,6 +27,12 @@ Outputs (all data: inputs and the reference's outputs):
   5 segments) sinogram, and ProjData::rebinWeight of sinograms of ones
   with one segment at spans 3 and 5 (``python make_golden.py bins`` writes
   this file alone).
+* ``c2_osem_golden.npz`` -- the 24-ring workload's OSEM run (3 iterations x
+  16 subsets, ACF from the HU map, 4 mm Gaussian every sub-iteration) of
+  the reference FIR_OSEM on bit-reproducible counts
+  (synth.parity_counts of the reference's own forward projections), the
+  same run through the all-FP64 oracle, and SHA-256 digests of the
+  forward projections and counts (``python make_golden.py c2osem``).
 * ``reso_golden.npz`` -- reconAlgos::OSEM_ResoReco of the compiled
   reference (called directly by ref_driver: no executable reaches it) on
   the C1 counts from an image of ones, 2 x 4 sub-iterations, 4 mm
@@ -348,6 +354,71 @@ def bins_fixture():
     print("bins_golden.npz written")
 
 
+def c2_osem_fixture(nthreads=8):
+    """c2_osem_golden.npz: reference FIR_OSEM and the all-FP64 oracle on the
+    24-ring workload (3 x 16, 4 mm Gaussian every sub-iteration, ACF from
+    the HU map, no cut)."""
+    import time
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    w = synth.c2()
+    fw = "{" + ",".join(f"{f:g}" for f in w.fwhm) + "}"
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        w.scanner.write(os.path.join(d, "scanner.scan"))
+        w.proj.write(os.path.join(d, "tmpl.hs"))
+        w.vol.write(os.path.join(d, "tmpl.h33"))
+        act, hu = synth.phantom(w)
+        mu = O.hu_to_mu(hu.ravel())
+        w.vol.write(os.path.join(d, "act.h33"), act)
+        w.vol.write(os.path.join(d, "hu.h33"), hu)
+        w.vol.write(os.path.join(d, "mu.h33"), mu)
+        for src, out in (("act.h33", "y0"), ("mu.h33", "m")):
+            io.write_params(os.path.join(d, f"{out}.params"),
+                            "FORWARD PROJECTION PARAMETERS", [
+                                ("input volume file", src),
+                                ("scanner file", "scanner.scan"),
+                                ("output projection header", "tmpl.hs"),
+                                ("output projection file name", out)])
+            run([os.path.join(REF, "FIR_ForwardProj"), f"{out}.params"], d)
+        y0 = io.read_proj(os.path.join(d, "y0.hs"))
+        m = io.read_proj(os.path.join(d, "m.hs"))
+        counts, scale = synth.parity_counts(w, y0, m)
+        io.write_proj(os.path.join(d, "y.hs"), w.proj, counts)
+        t0 = time.perf_counter()
+        osem_params(os.path.join(d, "x.params"), "y.hs", "x",
+                    w.n_iterations, w.n_subsets,
+                    attenuation_volume_in_HU="hu.h33",
+                    convolution_interval="1",
+                    convolution_FHWM_XYZ_in_mm=fw)
+        run([os.path.join(REF, "FIR_OSEM"), "x.params"], d)
+        print(f"reference FIR_OSEM {time.perf_counter() - t0:.1f} s")
+        x_ref = io.read_vol(os.path.join(d, "x.h33"))
+
+    # all-FP64 oracle: mu, line integrals, ACF, sensitivity, OSEM in double
+    t0 = time.perf_counter()
+    xy, z = O.scanner_tables(w.scanner)
+    mu64 = np.where(hu.ravel() <= -1000.0, 0.0,
+                    0.0096 + hu.ravel().astype(np.float64) * 0.0096 / 1000.0)
+    m64 = O.forward(w.vol, w.proj, xy, z, mu64, nthreads=nthreads)
+    acf64 = np.where(m64 > 1e-5, np.exp(m64), 1.0)
+    c64 = counts.astype(np.float64)
+    y64 = np.where((c64 > 1e-5) & (acf64 > 1e-5), c64 * acf64, 0.0)
+    sens64 = O.sensitivity(w.vol, w.proj, xy, z, w.n_subsets,
+                           dtype=np.float64, nthreads=nthreads)
+    x64 = O.osem(w.vol, w.proj, xy, z, y64, sens64,
+                 np.ones(w.vol.n_voxels), w.n_iterations, w.n_subsets,
+                 conv_interval=1, fwhm=w.fwhm, nthreads=nthreads)
+    print(f"FP64 oracle {time.perf_counter() - t0:.1f} s")
+    np.savez_compressed(
+        os.path.join(HERE, "c2_osem_golden.npz"),
+        x_ref=x_ref.astype(np.float32), x_f64=x64.astype(np.float64),
+        y0_sha256=sha(y0), m_sha256=sha(m), counts_sha256=sha(counts),
+        scale=np.float64(scale), total_counts=np.float64(counts.sum()),
+        schedule=np.array([w.n_iterations, w.n_subsets, 1], np.int32),
+        fwhm=np.asarray(w.fwhm, np.float32))
+    print("c2_osem_golden.npz written")
+
+
 RESO_RUNS = {"xd": (2, 4, 4.0, 1, 0.0), "xe": (2, 4, 4.0, 0, 55.0)}
 
 
@@ -376,6 +447,8 @@ if __name__ == "__main__":
         bins_fixture()
     elif len(sys.argv) > 1 and sys.argv[1] == "reso":
         reso_fixture()
+    elif len(sys.argv) > 1 and sys.argv[1] == "c2osem":
+        c2_osem_fixture()
     else:
         main()
         bins_fixture()
