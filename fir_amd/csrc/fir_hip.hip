@@ -22,6 +22,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -853,6 +854,7 @@ __global__ void k_osem_update(float* __restrict__ x, float* __restrict__ bp,
 // cut > 0 the circular cut follows (operations.cc:194-220).
 constexpr int kGaussX = 32, kGaussY = kBlock / kGaussX;
 constexpr int kGaussMaxH = 7; // taps per side loaded up front (15 taps)
+constexpr int kGaussTapFloats = 192; // X, Y, Z taps, 64 floats each
 
 // Sum over the in-bounds taps of one axis, in tap order: s = sum(k * v),
 // w = sum(k) (operations.cc:83-156). The taps' loads come first (clamped to
@@ -940,6 +942,68 @@ __global__ __launch_bounds__(kBlock) void k_gauss(const float* __restrict__ in,
   }
 }
 
+// Taps passed by value (kernel arguments), so that no host-to-device copy
+// and no device allocation is needed to place them.
+struct GaussTaps
+{
+  float v[kGaussTapFloats];
+};
+
+__global__ void k_store_taps(GaussTaps t, float* __restrict__ dst)
+{
+  const int i = threadIdx.x;
+  if (i < kGaussTapFloats) dst[i] = t.v[i];
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Static scratch of fir_convolve_noalloc: the taps, then the two intermediate
+// images of the separable Gaussian, for frames of up to kNoAllocMaxVoxels
+// voxels. Every device the library runs on holds one copy (about 64 MB) from
+// the module's load until the process ends. Host code never passes this
+// name to a kernel or a copy: it takes the copy's device address from
+// hipGetSymbolAddress once per device (noalloc_setup).
+// ---------------------------------------------------------------------------
+namespace
+{
+constexpr long long kNoAllocMaxVoxels = 1LL << 23;
+constexpr long long kNoAllocTapSlot = 256; // floats before the images
+constexpr long long kNoAllocFloats = kNoAllocTapSlot + 2 * kNoAllocMaxVoxels;
+constexpr int kNoAllocMaxDevices = 64;
+} // namespace
+
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+namespace
+{
+std::mutex g_noalloc_mu;
+float* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of this device's copy of `scratch`, looked up once per
+// device; `device` must be current.
+int noalloc_setup(int device)
+{
+  if (device < 0 || device >= kNoAllocMaxDevices)
+    return fail(FIR_E_ARG, "device %d beyond the %d the static scratch "
+                           "tracks", device, kNoAllocMaxDevices);
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return FIR_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(FIR_E_HIP, "hipGetSymbolAddress of the static scratch on "
+                           "device %d: %s", device, hipGetErrorString(e));
+  g_noalloc_ptr[device] = static_cast<float*>(p);
+  return FIR_OK;
+}
+
+float* noalloc_ptr(int device)
+{
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  return device >= 0 && device < kNoAllocMaxDevices ? g_noalloc_ptr[device]
+                                                    : nullptr;
+}
 } // namespace
 
 namespace
@@ -1018,6 +1082,13 @@ int fir_ctx_create(fir_ctx** out, int device)
     return fail(FIR_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
   }
   c->stream = c->own_stream;
+  const int rc = noalloc_setup(device);
+  if (rc)
+  {
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return rc;
+  }
   *out = c;
   return FIR_OK;
 }
@@ -1465,9 +1536,34 @@ int fir_osem_update(fir_ctx* c, const fir_vol_desc* v, float* x, float* bp,
 namespace
 {
 // Gaussian taps of operations::convolve (operations.cc:31-75) for (fwhm,
-// voxel size): the context's set for that key, uploaded on first use. X and
-// Y use a float sigma, Z a double sigma; taps are computed in double, stored
-// as float.
+// voxel size), computed on the host: X taps at [0, 64), Y at [64, 128), Z at
+// [128, 192), h[d] the half-width per axis. X and Y use a float sigma, Z a
+// double sigma; taps are computed in double, stored as float.
+int gauss_taps_host(const fir_vol_desc* v, const float* fwhm, float* taps,
+                    int* h)
+{
+  const float sx = fwhm[0] / 2.3548f;
+  const float sy = fwhm[1] / 2.3548f;
+  const double sz = fwhm[2] / 2.3548;
+  h[0] = ((int)(6.0 * sx / v->dx) + 1) / 2;
+  h[1] = ((int)(6.0 * sy / v->dy) + 1) / 2;
+  h[2] = ((int)(6.0 * sz / v->dz) + 1) / 2;
+  for (int d = 0; d < 3; ++d)
+    if (h[d] > 31 || h[d] < 0)
+      return fail(FIR_E_ARG, "Gaussian kernel wider than 63 taps");
+  std::memset(taps, 0, sizeof(float) * kGaussTapFloats);
+  for (int k = -h[0]; k <= h[0]; ++k)
+    taps[h[0] + k] = (float)std::exp(-(k * k * v->dx * v->dx) / (2 * sx * sx));
+  for (int k = -h[1]; k <= h[1]; ++k)
+    taps[64 + h[1] + k] =
+      (float)std::exp(-(k * k * v->dy * v->dy) / (2 * sy * sy));
+  for (int k = -h[2]; k <= h[2]; ++k)
+    taps[128 + h[2] + k] =
+      (float)std::exp(-(k * k * v->dz * v->dz) / (2 * sz * sz));
+  return FIR_OK;
+}
+
+// The context's tap set for (fwhm, voxel size), uploaded on first use.
 int prepare_taps(fir_ctx* c, const fir_vol_desc* v, const float* fwhm,
                  const fir_ctx::TapSet** out)
 {
@@ -1478,27 +1574,10 @@ int prepare_taps(fir_ctx* c, const fir_vol_desc* v, const float* fwhm,
       *out = &t;
       return FIR_OK;
     }
-  const float sx = fwhm[0] / 2.3548f;
-  const float sy = fwhm[1] / 2.3548f;
-  const double sz = fwhm[2] / 2.3548;
-  int h[3];
-  h[0] = ((int)(6.0 * sx / v->dx) + 1) / 2;
-  h[1] = ((int)(6.0 * sy / v->dy) + 1) / 2;
-  h[2] = ((int)(6.0 * sz / v->dz) + 1) / 2;
-  for (int d = 0; d < 3; ++d)
-    if (h[d] > 31 || h[d] < 0)
-      return fail(FIR_E_ARG, "Gaussian kernel wider than 63 taps");
-  float taps[192];
-  std::memset(taps, 0, sizeof(taps));
-  for (int k = -h[0]; k <= h[0]; ++k)
-    taps[h[0] + k] = (float)std::exp(-(k * k * v->dx * v->dx) / (2 * sx * sx));
-  for (int k = -h[1]; k <= h[1]; ++k)
-    taps[64 + h[1] + k] =
-      (float)std::exp(-(k * k * v->dy * v->dy) / (2 * sy * sy));
-  for (int k = -h[2]; k <= h[2]; ++k)
-    taps[128 + h[2] + k] =
-      (float)std::exp(-(k * k * v->dz * v->dz) / (2 * sz * sz));
+  float taps[kGaussTapFloats];
   fir_ctx::TapSet t;
+  int rc = gauss_taps_host(v, fwhm, taps, t.h);
+  if (rc) return rc;
   FIR_HIP_CHECK(hipMalloc(&t.d, sizeof(taps)));
   if (hipMemcpyAsync(t.d, taps, sizeof(taps), hipMemcpyHostToDevice,
                      c->stream) != hipSuccess ||
@@ -1507,11 +1586,7 @@ int prepare_taps(fir_ctx* c, const fir_vol_desc* v, const float* fwhm,
     (void)hipFree(t.d);
     return fail(FIR_E_HIP, "upload of the Gaussian taps failed");
   }
-  for (int d = 0; d < 3; ++d)
-  {
-    t.key[d] = fwhm[d];
-    t.h[d] = h[d];
-  }
+  for (int d = 0; d < 3; ++d) t.key[d] = fwhm[d];
   t.vox[0] = v->dx;
   t.vox[1] = v->dy;
   t.vox[2] = v->dz;
@@ -1534,14 +1609,14 @@ int ensure_scratch(fir_ctx* c, size_t floats)
   return FIR_OK;
 }
 
-// Gaussian of one frame plus edge restore and cut (fused in the Z pass).
+// Gaussian of one frame plus edge restore and cut (fused in the Z pass):
+// X pass img -> t1, Y pass t1 -> t2, Z pass t2 -> img. Every voxel of t1
+// and t2 is written before it is read, so they need no clearing. taps is a
+// device array laid out as gauss_taps_host's, h its half-widths.
 int convolve_frame(fir_ctx* c, const fir_vol_desc* v, float* img,
-                   const fir_ctx::TapSet* ts, const float* fwhm,
-                   float restore_cut, float final_cut)
+                   const float* taps, const int* h, float* t1, float* t2,
+                   const float* fwhm, float restore_cut, float final_cut)
 {
-  const long long n = (long long)v->nx * v->ny * v->nz;
-  float* t1 = c->d_scratch;
-  float* t2 = c->d_scratch + n;
   const CutGeom cg = cut_geom(v);
   const float fw = fwhm[0] > fwhm[1] ? fwhm[0] : fwhm[1];
   const int restore = restore_cut > 0.0f ? 1 : 0;
@@ -1549,17 +1624,35 @@ int convolve_frame(fir_ctx* c, const fir_vol_desc* v, float* img,
   const dim3 nb((v->nx + kGaussX - 1) / kGaussX,
                 (v->ny + kGaussY - 1) / kGaussY, v->nz);
   hipLaunchKernelGGL((k_gauss<0, false>), nb, dim3(kBlock), 0,
-                     c->stream, img, t1, ts->d, ts->h[0], v->nx,
+                     c->stream, img, t1, taps, h[0], v->nx,
                      v->ny, v->nz, cg, 0, 0.0, 0.0f);
   hipLaunchKernelGGL((k_gauss<1, false>), nb, dim3(kBlock), 0,
-                     c->stream, t1, t2, ts->d + 64, ts->h[1], v->nx,
+                     c->stream, t1, t2, taps + 64, h[1], v->nx,
                      v->ny, v->nz, cg, 0, 0.0, 0.0f);
   // Z pass writes img in place: each voxel reads its own img value (the
   // pre-convolution copy) only for the edge restore.
   hipLaunchKernelGGL((k_gauss<2, true>), nb, dim3(kBlock), 0, c->stream,
-                     t2, img, ts->d + 128, ts->h[2], v->nx, v->ny,
+                     t2, img, taps + 128, h[2], v->nx, v->ny,
                      v->nz, cg, restore, rr, final_cut);
   FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+// Gaussian of n_frames frames through the context's tap set and scratch.
+int convolve_ctx(fir_ctx* c, const fir_vol_desc* v, float* img, int n_frames,
+                 const float* fwhm, float restore_cut, float final_cut)
+{
+  const long long n = (long long)v->nx * v->ny * v->nz;
+  const fir_ctx::TapSet* ts = nullptr;
+  int rc;
+  if ((rc = prepare_taps(c, v, fwhm, &ts))) return rc;
+  const float* taps = ts->d;
+  const int h[3] = {ts->h[0], ts->h[1], ts->h[2]};
+  if ((rc = ensure_scratch(c, 2 * (size_t)n))) return rc;
+  for (int f = 0; f < n_frames; ++f)
+    if ((rc = convolve_frame(c, v, img + n * f, taps, h, c->d_scratch,
+                             c->d_scratch + n, fwhm, restore_cut, final_cut)))
+      return rc;
   return FIR_OK;
 }
 } // namespace
@@ -1574,12 +1667,39 @@ int fir_convolve(fir_ctx* c, const fir_vol_desc* v, float* img, int n_frames,
   if ((rc = make_grid(c, v, &g))) return rc;
   if (!(fwhm[0] > 0.0 && fwhm[1] > 0.0 && fwhm[2] > 0.0)) return FIR_OK;
   FIR_HIP_CHECK(hipSetDevice(c->device));
+  return convolve_ctx(c, v, img, n_frames, fwhm, cut, 0.0f);
+}
+
+int fir_convolve_noalloc(fir_ctx* c, const fir_vol_desc* v, float* img,
+                         int n_frames, const float* fwhm, float cut)
+{
+  if (!c || !fwhm) return fail(FIR_E_ARG, "NULL argument");
+  if (!img) return fail(FIR_E_ARG, "NULL image");
+  Grid g;
+  int rc;
+  if ((rc = make_grid(c, v, &g))) return rc;
+  if (!(fwhm[0] > 0.0 && fwhm[1] > 0.0 && fwhm[2] > 0.0)) return FIR_OK;
   const long long n = (long long)v->nx * v->ny * v->nz;
-  const fir_ctx::TapSet* ts = nullptr;
-  if ((rc = prepare_taps(c, v, fwhm, &ts))) return rc;
-  if ((rc = ensure_scratch(c, 2 * (size_t)n))) return rc;
+  if (n > kNoAllocMaxVoxels)
+    return fail(FIR_E_ARG, "fir_convolve_noalloc: %lld voxels per frame "
+                           "exceed the static scratch's %lld",
+                n, kNoAllocMaxVoxels);
+  GaussTaps t;
+  int h[3];
+  if ((rc = gauss_taps_host(v, fwhm, t.v, h))) return rc;
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  float* base = noalloc_ptr(c->device);
+  if (!base)
+    return fail(FIR_E_STATE, "static scratch of device %d not set up",
+                c->device);
+  float* t1 = base + kNoAllocTapSlot;
+  float* t2 = t1 + n;
+  hipLaunchKernelGGL(k_store_taps, dim3(1), dim3(kGaussTapFloats), 0,
+                     c->stream, t, base);
+  FIR_CHECK_LAUNCH();
   for (int f = 0; f < n_frames; ++f)
-    if ((rc = convolve_frame(c, v, img + n * f, ts, fwhm, cut, 0.0f)))
+    if ((rc = convolve_frame(c, v, img + n * f, base, h, t1, t2, fwhm, cut,
+                             0.0f)))
       return rc;
   return FIR_OK;
 }
@@ -1615,11 +1735,8 @@ int fir_osem_subiter(fir_ctx* c, const fir_vol_desc* v, float* x,
     return rc;
   if (conv)
   {
-    const long long n = (long long)v->nx * v->ny * v->nz;
-    const fir_ctx::TapSet* ts = nullptr;
-    if ((rc = prepare_taps(c, v, fwhm, &ts))) return rc;
-    if ((rc = ensure_scratch(c, 2 * (size_t)n))) return rc;
-    if ((rc = convolve_frame(c, v, x, ts, fwhm, cut, cut))) return rc;
+    FIR_HIP_CHECK(hipSetDevice(c->device));
+    if ((rc = convolve_ctx(c, v, x, 1, fwhm, cut, cut))) return rc;
   }
   return FIR_OK;
 }

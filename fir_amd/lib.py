@@ -45,7 +45,8 @@ EXPORTS = (
     "fir_fill", "fir_forward",
     "fir_forward_subset", "fir_backward", "fir_backward_subset",
     "fir_sensitivity", "fir_osem_project", "fir_osem_update",
-    "fir_osem_subiter", "fir_convolve", "fir_cut_circle", "fir_mul_clamped",
+    "fir_osem_subiter", "fir_convolve", "fir_convolve_noalloc",
+    "fir_cut_circle", "fir_mul_clamped",
     "fir_div_clamped", "fir_exp_clamped", "fir_hu_to_mu", "fir_path_dump",
     "fir_path_count", "fir_poisson_nll_grad", "fir_osem_project_batch",
     "fir_set_uneven_subsets", "fir_set_lor_order", "fir_set_scatter_copies",
@@ -120,6 +121,7 @@ def load() -> C.CDLL:
         "fir_osem_update": [vp, P, vp, vp, vp, f, i, i],
         "fir_osem_subiter": [vp, P, vp, vp, vp, vp, vp, i, i, i, vp, f, i],
         "fir_convolve": [vp, P, vp, i, vp, f],
+        "fir_convolve_noalloc": [vp, P, vp, i, vp, f],
         "fir_cut_circle": [vp, P, vp, i, f],
         "fir_mul_clamped": [vp, vp, vp, i64],
         "fir_div_clamped": [vp, vp, vp, i64],
@@ -322,6 +324,13 @@ class Context:
         fw = (C.c_float * 3)(*fwhm)
         _check(self.lib.fir_convolve(self.h, C.byref(vol), _ptr(img),
                                      n_frames, fw, cut))
+
+    def convolve_noalloc(self, vol, img, fwhm, cut=0.0, n_frames=1):
+        """fir_convolve through the library's static scratch: allocates no
+        device memory (frames of up to 2^23 voxels)."""
+        fw = (C.c_float * 3)(*fwhm)
+        _check(self.lib.fir_convolve_noalloc(self.h, C.byref(vol), _ptr(img),
+                                             n_frames, fw, cut))
 
     def cut_circle(self, vol, img, cut, n_frames=1):
         _check(self.lib.fir_cut_circle(self.h, C.byref(vol), _ptr(img),

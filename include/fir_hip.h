@@ -251,6 +251,21 @@ int fir_poisson_nll_grad(fir_ctx* ctx, const fir_vol_desc* vol,
 int fir_convolve(fir_ctx* ctx, const fir_vol_desc* vol, float* img,
                  int n_frames, const float* fwhm_xyz, float cut_radius);
 
+/* fir_convolve without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. Same kernels and the
+ * same result bit for bit as fir_convolve; its taps and two intermediate
+ * images live in a static device buffer that the library holds on every
+ * device it runs on (about 64 MB, until the process ends) instead of the
+ * context's scratch, and the taps travel as kernel arguments. Frames of
+ * at most 8 388 608 voxels (2^23, e.g. 256x256x128); larger ones fail
+ * with FIR_E_ARG. All calls on one device share that buffer: calls that
+ * may overlap on one device (two streams, two contexts, two threads) race
+ * and must be serialised by the caller. */
+int fir_convolve_noalloc(fir_ctx* ctx, const fir_vol_desc* vol, float* img,
+                         int n_frames, const float* fwhm_xyz,
+                         float cut_radius);
+
 /* operations::cutCircle (operations.cc:194-220) on n_frames frames. */
 int fir_cut_circle(fir_ctx* ctx, const fir_vol_desc* vol, float* img,
                    int n_frames, float cut_radius);
