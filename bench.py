@@ -11,7 +11,11 @@ back-projected once per iteration, plus the per-sub-iteration update and
 Gaussian) with the projector streaming the path cache (the Siddon system
 matrix kept in HBM with snorm16 lengths; OSEM images within 1e-4 relative L2
 of the reference). Inputs and the cache are resident in HBM before timing
-starts; the cache build is timed separately.
+starts; the cache build is timed separately. The counts are those of the
+parity fixture (synth.parity_counts), so before timing, the golden schedule
+(3 iterations) is run on the timed path and on the plain path and compared
+with the reference's FIR_OSEM image of the same counts and with the FP64
+oracle's (``osem_parity``).
 
     value = LORs per step / step time  [M LORs/s, fwd+backproj]
 
@@ -50,14 +54,76 @@ def traffic_from_profiles():
     committed rocprofv3 summary profiles/osem_cached_c2_traffic.md."""
     p = os.path.join(ROOT, "profiles", "osem_cached_c2_traffic.md")
     if not os.path.exists(p):
-        return None
+        return None, None
+    val, commit = None, None
     for line in open(p):
         if line.startswith("bytes_per_launch:"):
             try:
-                return int(line.split(":", 1)[1])
+                val = int(line.split(":", 1)[1])
             except ValueError:
-                return None
-    return None
+                val = None
+        elif line.startswith("commit:"):
+            commit = line.split(":", 1)[1].strip()
+    return val, commit
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def parity_golden(w):
+    """The reference's OSEM image of this workload's counts, with the FP64
+    oracle's (tests/golden/c2_osem_golden.npz), or None for workloads it
+    does not cover."""
+    p = os.path.join(ROOT, "tests", "golden", "c2_osem_golden.npz")
+    if w.name != "c2_24x448_160x160x47" or not os.path.exists(p):
+        return None
+    return dict(np.load(p))
+
+
+def copy_bandwidth(reps=10):
+    """Achievable HBM bandwidth: a 1 GiB device-to-device copy (1 GiB read
+    and 1 GiB written per copy), mean of reps copies timed with HIP
+    events on torch's current stream."""
+    import torch
+    src = torch.ones(1 << 28, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2 * (1 << 30) / (ms * 1e-3) / 1e9
+
+
+def host_cpu():
+    """CPU model, sockets and logical CPUs of this host (lscpu)."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True,
+                             timeout=30).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    for line in out.splitlines():
+        k, _, v = line.partition(":")
+        k, v = k.strip(), v.strip()
+        if k == "Model name":
+            info["model"] = v
+        elif k == "Socket(s)":
+            info["sockets"] = v
+        elif k == "Core(s) per socket":
+            info["cores_per_socket"] = v
+        elif k == "Thread(s) per core":
+            info["threads_per_core"] = v
+    return info
 
 
 def cpu_baseline(w, n_subsets):
@@ -67,15 +133,19 @@ def cpu_baseline(w, n_subsets):
     restatement if the reference build is absent."""
     drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    host = host_cpu()
+    reps = 3
     if os.path.exists(drv):
         with tempfile.TemporaryDirectory() as d:
             w.scanner.write(os.path.join(d, "s.scan"))
             w.proj.write(os.path.join(d, "p.hs"))
             w.vol.write(os.path.join(d, "v.h33"))
-            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads),
+                       OMP_PROC_BIND="close", OMP_PLACES="cores")
             r = subprocess.run([drv, "time", "subiter", "v.h33", "s.scan",
-                                "p.hs", str(n_subsets), "1"], cwd=d, env=env,
-                               capture_output=True, text=True, timeout=900)
+                                "p.hs", str(n_subsets), str(reps)], cwd=d,
+                               env=env, capture_output=True, text=True,
+                               timeout=900)
             if r.returncode != 0:
                 log("reference driver failed:", r.stderr[-500:])
                 return None
@@ -83,11 +153,13 @@ def cpu_baseline(w, n_subsets):
         t = res["median_s"]
         return {"value": w.n_lors / t / 1e6, "unit": "M LORs/s",
                 "cores": res["threads"], "kind": "reference",
-                "osem_iter_s": t,
+                "osem_iter_s": t, "min_s": res["min_s"], "reps": reps,
+                "omp": "OMP_PROC_BIND=close OMP_PLACES=cores",
+                "host": host,
                 "sample": (f"one {n_subsets}-subset OSEM iteration of the "
                            "reference FIR built from source (oracle/_ref), "
                            "same geometry, uniform image and sinogram, "
-                           "1 warm-up + 1 timed run")}
+                           f"median of {reps} timed runs after 1 warm-up")}
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     xy, z = O.scanner_tables(w.scanner)
@@ -100,6 +172,7 @@ def cpu_baseline(w, n_subsets):
     t = time.perf_counter() - t0
     return {"value": w.n_lors / t / 1e6, "unit": "M LORs/s",
             "cores": threads, "kind": "port", "osem_iter_s": t,
+            "host": host,
             "sample": f"one {n_subsets}-subset OSEM iteration of the CPU "
                       "oracle, uniform image and sinogram"}
 
@@ -305,6 +378,9 @@ def main():
     nb = ctx.n_bins
 
     # --- synthetic data, resident in HBM ---------------------------------
+    # The counts are those of the parity fixture: Poisson draws from the
+    # forward projections of the phantom and of its mu map, reproducible bit
+    # for bit on any host (synth.parity_counts).
     act, hu = synth.phantom(w)
     act_d = torch.from_numpy(act.ravel()).cuda()
     mu = torch.from_numpy(hu.ravel()).cuda()
@@ -313,20 +389,23 @@ def main():
     ctx.hu_to_mu(mu)
     acf = torch.zeros(nb, device="cuda")
     ctx.forward(v, mu, acf)
+    y0h, mh = y0.cpu().numpy(), acf.cpu().numpy()
+    counts_h, scale = synth.parity_counts(w, y0h, mh)
+    mean_f = torch.from_numpy(
+        (y0h * synth.attenuation(mh) * scale).astype(np.float32)).cuda()
+    del y0h, mh
     ctx.exp_clamped(acf)
-    mean = y0.double() / acf.double()
-    mean = mean * (w.counts / mean.sum())
-    g = torch.Generator(device="cuda").manual_seed(w.seed)
-    y = torch.poisson(mean.float(), generator=g)
-    counts = None if args.no_cli or args.traffic_probe else y.cpu().numpy()
+    y = torch.from_numpy(counts_h).cuda()
+    counts = None if args.no_cli or args.traffic_probe else counts_h
+    del counts_h
     ctx.mul_clamped(y, acf)  # FIR_OSEM: inputProj *= ACF (OSEM.cc:336-337)
-    mean_f = mean.float()
     sens = torch.zeros(n_sub * nvox, device="cuda")
     ctx.sensitivity(v, sens, n_sub)
     x = torch.ones(nvox, device="cuda")
     bp = torch.zeros(nvox, device="cuda")
     torch.cuda.synchronize()
-    del mean, act_d, y0
+    del act_d, y0
+    golden = parity_golden(w)
 
     stream = torch.cuda.current_stream()
 
@@ -413,6 +492,31 @@ def main():
                               for (a, b) in row]))
         return ms, proj, wall / steps * 1e3
 
+    def parity(project, sens_v):
+        """The golden schedule from x = 1 on this path, against the
+        reference's FIR_OSEM image and the FP64 oracle's (untimed)."""
+        if golden is None:
+            return None
+        x.fill_(1.0)
+        bp.zero_()
+        for _ in range(int(golden["schedule"][0])):
+            osem_iteration(project, sens_v)
+        torch.cuda.synchronize()
+        xh = x.cpu().numpy()
+        return {"rel_l2_vs_reference": float(f"{rel_l2(xh, golden['x_ref']):.3e}"),
+                "rel_l2_vs_fp64_oracle":
+                    float(f"{rel_l2(xh, golden['x_f64']):.3e}"),
+                "schedule": (f"{int(golden['schedule'][0])} iterations x "
+                             f"{int(golden['schedule'][1])} subsets, ACF, "
+                             "4 mm Gaussian every sub-iteration, from x = 1"),
+                "reference": "FIR_OSEM of the reference built from source, "
+                             "same counts (tests/golden/c2_osem_golden.npz)"}
+
+    cached_parity = parity(cached_project, sens_c)
+    if cached_parity:
+        log(f"cached path vs reference: rel L2 "
+            f"{cached_parity['rel_l2_vs_reference']:.2e}")
+
     # --- the measured step: cached OSEM iterations ------------------------
     ms_step, proj_avg_ms, wall_ms = timed_iterations(
         cached_project, sens_c, args.steps, args.warmup)
@@ -422,6 +526,7 @@ def main():
     # --- the plain path (FP64 walk per LOR), same schedule ----------------
     pres = autotune.tune_plain(ctx, v, n_sub, x, y, reps=2)
     log(f"plain projector autotuned: {pres.best}")
+    plain_parity = parity(plain_project, sens)
     plain_ms, plain_proj_ms, _ = timed_iterations(
         plain_project, sens, max(1, min(args.steps, 2)), 1)
     log(f"plain OSEM iteration {plain_ms:.3f} ms, projector launch avg "
@@ -546,14 +651,16 @@ def main():
     bytes_launch = 2 * (12 * lors_per_sub + 6 * n_e_per_sub) + 12 * nvox
     achieved = bytes_launch / (proj_avg_ms * 1e-3) / 1e9
     if batched is not None:
-        # The batch reads the matrix once for all its images: per
-        # reconstruction the same algorithmic bytes as one launch, so the
-        # equivalent rate (sum over reconstructions) can exceed HBM's peak.
+        # The batch reads the matrix once for all its images: summed over
+        # its reconstructions, the algorithmic bytes of one single-image
+        # launch each per sub-iteration. A throughput figure per
+        # reconstruction, not a fraction of HBM's bandwidth.
         per_sub_ms = batched["osem_iter_ms"] / n_sub
         eq = args.batch * bytes_launch / (per_sub_ms * 1e-3) / 1e9
-        batched["equivalent_gb_s"] = round(eq, 1)
-        batched["equivalent_frac_of_hbm_peak"] = round(eq / HBM_PEAK_GBS, 3)
-    traffic = traffic_from_profiles()
+        batched["single_image_bytes_rate_gb_s"] = round(eq, 1)
+    traffic, traffic_commit = traffic_from_profiles()
+    copy_gbs = copy_bandwidth()
+    log(f"device copy: {copy_gbs:.0f} GB/s (1 GiB read + 1 GiB written)")
 
     cpu = None
     if not args.no_cpu:
@@ -602,9 +709,11 @@ def main():
                   "mlors_s": round(w.n_lors / (plain_ms * 1e-3) / 1e6, 3),
                   "fwd_ms": round(fwd_ms, 4),
                   "bp_ms": round(bp_ms, 4),
+                  "osem_parity": plain_parity,
                   "fwd_mlors_s": round(w.n_lors / (fwd_ms * 1e-3) / 1e6, 3),
                   "bp_mlors_s": round(w.n_lors / (bp_ms * 1e-3) / 1e6, 3)},
         "speedup_cached_vs_plain": round(plain_ms / ms_step, 3),
+        "osem_parity": cached_parity,
         "path_elements_per_iter": int(n_e_total),
         "roofline": {
             "kernel": "k_pc_osem_fam + k_pc_fam_reduce (one cached OSEM "
@@ -618,6 +727,11 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, "
+                               "profiles/osem_cached_c2_traffic.md at "
+                               f"commit {traffic_commit}"),
+            "achievable_copy_gb_s": round(copy_gbs, 1),
+            "frac_of_copy": round(achieved / copy_gbs, 4),
             "avg_launch_ms": round(proj_avg_ms, 4),
             "algorithmic_bytes_per_launch": int(bytes_launch),
         },
