@@ -11,6 +11,7 @@
 #include <vector>
 
 class ProjData;
+class VolData;
 
 class ScannerData
 {
@@ -36,6 +37,20 @@ public:
 
   void printContent() const;
   void printCrystalPositions() const;
+  // Bin, crystal pair, crystal positions and, if given, the path (voxel
+  // indices, lengths, the values of vol along it) and its line integral:
+  // ScannerData::printBinInfo (ScannerData.cc:113-200), same text.
+  void printBinInfo(int seg, int view, int axialCoord, int tangCoord,
+                    int crystal1, int slice1, int crystal2, int slice2,
+                    const VolData& vol,
+                    types::PathElement* pathElementsArray = 0,
+                    types::VoxelValue line = 0) const;
+
+  // Ring and in-ring crystal coordinates of crystal crystalID of module
+  // moduleID of rSector rSectorID: ScannerData::getCrystalCoordinates
+  // (ScannerData.cc:202-231).
+  void getCrystalCoordinates(int rSectorID, int moduleID, int crystalID,
+                             int* ring, int* crystal) const;
 
 private:
   void computeTables();

@@ -58,6 +58,19 @@ public:
   VolData& operator*=(const VolData& inputVol);
   VolData& operator/=(const VolData& inputVol);
 
+  // Line integral of the active frame along a path (elements up to the
+  // terminator coord == -1), accumulated in path order into a float as
+  // float(line + length * value): VolData::computeLineIntegral
+  // (VolData.cc:357-371). Host-side, for single paths; the projectors run
+  // on the GPU.
+  types::VoxelValue computeLineIntegral(
+    types::PathElement* pathElementsArray) const;
+  // Adds length * line to every voxel of the path in the active frame, each
+  // update atomic (safe from concurrent threads):
+  // VolData::projectLineIntegral (VolData.cc:373-385).
+  void projectLineIntegral(types::PathElement* pathElementsArray,
+                           types::VoxelValue line);
+
   void setActiveFrame(int frame) const;
   int getActiveFrame() const { return mActiveFrame; }
 

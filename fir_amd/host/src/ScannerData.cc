@@ -1,11 +1,13 @@
 #include <ProjData.h>
 #include <ScannerData.h>
 #include <ScannerInterfileReader.h>
+#include <VolData.h>
 #include <console.h>
 #include <macros.h>
 
 #include <cmath>
 #include <cstdio>
+#include <iostream>
 
 ScannerData::ScannerData(const std::string& file)
 {
@@ -112,4 +114,70 @@ void ScannerData::printCrystalPositions() const
     std::printf("crystal %zu: x = %.6f, y = %.6f\n", i, mXY[i].x, mXY[i].y);
   for (size_t s = 0; s < mZ.size(); ++s)
     std::printf("slice %zu: z = %.6f\n", s, mZ[s]);
+}
+
+namespace
+{
+// "{a,b,c}" of one field of the path elements, up to the terminator.
+template<typename F>
+void printPathField(const types::PathElement* path, F field)
+{
+  std::cout << "{";
+  for (const types::PathElement* e = path; e->coord != -1; ++e)
+    std::cout << (e == path ? "" : ",") << field(*e);
+  std::cout << "}" << std::endl;
+}
+}
+
+void ScannerData::printBinInfo(int seg, int view, int axialCoord,
+                               int tangCoord, int crystal1, int slice1,
+                               int crystal2, int slice2, const VolData& vol,
+                               types::PathElement* path,
+                               types::VoxelValue line) const
+{
+  printValue("seg", seg);
+  printValue("view", view);
+  printValue("axialCoord", axialCoord);
+  printValue("tangCoord", tangCoord);
+  printEmptyLine();
+  printValue("crystal1", crystal1);
+  printValue("slice1", slice1);
+  printValue("crystal2", crystal2);
+  printValue("slice2", slice2);
+  printEmptyLine();
+  std::cout << "X: (" << mXY[crystal1].x << "," << mXY[crystal2].x << ")"
+            << std::endl;
+  std::cout << "Y: (" << mXY[crystal1].y << "," << mXY[crystal2].y << ")"
+            << std::endl;
+  std::cout << "Z: (" << mZ[slice1] << "," << mZ[slice2] << ")" << std::endl;
+  printEmptyLine();
+  if (path)
+  {
+    const types::VoxelValue* data = vol.getDataArray();
+    printPathField(path, [](const types::PathElement& e) { return e.coord; });
+    printPathField(path, [](const types::PathElement& e) { return e.length; });
+    std::cout << std::endl;
+    printPathField(path,
+                   [data](const types::PathElement& e) { return data[e.coord]; });
+    std::cout << std::endl;
+  }
+  printValue("line", line);
+  printEmptyLine();
+}
+
+void ScannerData::getCrystalCoordinates(int rSectorID, int moduleID,
+                                        int crystalID, int* ring,
+                                        int* crystal) const
+{
+  // rSectors hold moduleRepeat[0] x moduleRepeat[1] modules, each
+  // crystalRepeat[0] x crystalRepeat[1] crystals (y: around the ring, z:
+  // along the axis); in-ring index 0 is the middle crystal of rSector 0
+  const int my = mHeader.moduleRepeatNumbersYZ[0];
+  const int cy = mHeader.crystalRepeatNumbersYZ[0];
+  const int cz = mHeader.crystalRepeatNumbersYZ[1];
+  *ring = (moduleID / my) * cz + crystalID / cy;
+  int c = (rSectorID * my + moduleID % my) * cy + crystalID % cy -
+    mGeometry.crystalOffset;
+  if (c >= mGeometry.nCrystalsPerRing) c -= mGeometry.nCrystalsPerRing;
+  *crystal = c;
 }

@@ -235,6 +235,32 @@ VolData& VolData::operator/=(const VolData& in)
   return *this;
 }
 
+types::VoxelValue VolData::computeLineIntegral(
+  types::PathElement* path) const
+{
+  types::VoxelValue line = 0.0f;
+  for (const types::PathElement* e = path; e->coord != -1; ++e)
+    line = static_cast<types::VoxelValue>(
+      line + e->length * mDataArray[e->coord]);
+  return line;
+}
+
+void VolData::projectLineIntegral(types::PathElement* path,
+                                  types::VoxelValue line)
+{
+  for (const types::PathElement* e = path; e->coord != -1; ++e)
+  {
+    types::VoxelValue* v = mDataArray + e->coord;
+    types::VoxelValue seen;
+    __atomic_load(v, &seen, __ATOMIC_RELAXED);
+    types::VoxelValue next;
+    do
+      next = static_cast<types::VoxelValue>(seen + e->length * line);
+    while (!__atomic_compare_exchange(v, &seen, &next, false,
+                                      __ATOMIC_RELAXED, __ATOMIC_RELAXED));
+  }
+}
+
 void VolData::setActiveFrame(int frame) const
 {
   if (!(frame >= 0 && frame < mHeader.nFrames)) error("Invalid frame");

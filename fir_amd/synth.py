@@ -177,3 +177,30 @@ def parity_counts(w: Workload, y0: np.ndarray, mu_line: np.ndarray):
     lam *= scale
     rng = np.random.default_rng(w.seed)
     return rng.poisson(lam).astype(np.float32), scale
+
+
+def hash_counts(n_bins: int) -> np.ndarray:
+    """Deterministic count sinogram for the precision runs of scanners
+    whose data is too large to ship: y[b] = (b * 2654435761 mod 2^32) >> 27,
+    integers 0-31, identical on every host."""
+    y = np.empty(n_bins, np.float32)
+    step = 1 << 26
+    for a in range(0, n_bins, step):
+        b = np.arange(a, min(a + step, n_bins), dtype=np.uint64)
+        y[a:a + b.size] = (((b * np.uint64(2654435761)) & np.uint64(0xffffffff))
+                           >> np.uint64(27)).astype(np.float32)
+    return y
+
+
+def image_summary(img: np.ndarray, shape, block: int = 8) -> dict:
+    """Compact view of an image for fixtures of large grids: sums over
+    block^3 voxel blocks and the three central planes (z, y and x through
+    the grid centre), as float64. shape = (nz, ny, nx)."""
+    a = np.asarray(img, np.float64).reshape(shape)
+    nz, ny, nx = shape
+    bz, by, bx = nz // block, ny // block, nx // block
+    blocks = a[:bz * block, :by * block, :bx * block].reshape(
+        bz, block, by, block, bx, block).sum(axis=(1, 3, 5))
+    return {"blocks": blocks, "plane_z": a[nz // 2].copy(),
+            "plane_y": a[:, ny // 2, :].copy(),
+            "plane_x": a[:, :, nx // 2].copy()}

@@ -45,6 +45,73 @@ void ORC_SFX(orc_forward)(const orc_vol* v, int geometric, const orc_proj* p,
   }
 }
 
+/* Forward projection of the listed bins only (all-bins order): out[i] is
+ * what orc_forward writes for bin bins[i], with the same accumulation. */
+void ORC_SFX(orc_forward_bins)(const orc_vol* v, int geometric,
+                               const orc_proj* p, const double* xy,
+                               const double* z, const ORC_REAL* img,
+                               const long long* bins, long long n,
+                               ORC_REAL* out, int nthreads)
+{
+  orc_grid g;
+  grid_init(&g, v, geometric);
+  set_threads(nthreads);
+#pragma omp parallel
+  {
+    int* ti = (int*)malloc(sizeof(int) * (g.max_path + 16));
+    double* tl = (double*)malloc(sizeof(double) * (g.max_path + 16));
+#pragma omp for schedule(dynamic, 256)
+    for (long long i = 0; i < n; ++i)
+    {
+      int seg, view, axial, tang;
+      double e1[3], e2[3];
+      bin_decode(p, bins[i], &seg, &view, &axial, &tang);
+      lor_endpoints(p, xy, z, seg, view, axial, tang, e1, e2);
+      const int m = grid_path(&g, e1, e2, ti, tl, g.max_path + 16);
+      ORC_REAL line = 0;
+      for (int e = 0; e < m; ++e) line += tl[e] * img[ti[e]];
+      out[i] = line;
+    }
+    free(ti);
+    free(tl);
+  }
+}
+
+/* Backprojection of the listed bins only, added into img (not cleared):
+ * each bin's path scattered with vals[i], as orc_backward scatters it. */
+void ORC_SFX(orc_backward_bins)(const orc_vol* v, int geometric,
+                                const orc_proj* p, const double* xy,
+                                const double* z, const long long* bins,
+                                const ORC_REAL* vals, long long n,
+                                ORC_REAL* img, int nthreads)
+{
+  orc_grid g;
+  grid_init(&g, v, geometric);
+  set_threads(nthreads);
+#pragma omp parallel
+  {
+    int* ti = (int*)malloc(sizeof(int) * (g.max_path + 16));
+    double* tl = (double*)malloc(sizeof(double) * (g.max_path + 16));
+#pragma omp for schedule(dynamic, 256)
+    for (long long i = 0; i < n; ++i)
+    {
+      int seg, view, axial, tang;
+      double e1[3], e2[3];
+      bin_decode(p, bins[i], &seg, &view, &axial, &tang);
+      lor_endpoints(p, xy, z, seg, view, axial, tang, e1, e2);
+      const int m = grid_path(&g, e1, e2, ti, tl, g.max_path + 16);
+      const ORC_REAL val = vals[i];
+      for (int e = 0; e < m; ++e)
+      {
+#pragma omp atomic
+        img[ti[e]] += tl[e] * val;
+      }
+    }
+    free(ti);
+    free(tl);
+  }
+}
+
 /* Visit the LORs of (subset, seg) in the reference LOR-cache order
  * (LORCache.cc:49-51,118-133): index -> (subview, axial, tang) with
  * view = subview * nSubsets + subset. */

@@ -185,6 +185,32 @@ def forward(vol, proj, xy, z, img, geometric=False, nthreads=0):
     return sino
 
 
+def forward_bins(vol, proj, xy, z, img, bins, geometric=False, nthreads=0):
+    """Line integrals of the listed bins (all-bins order), as forward()
+    computes them; dtype of img (float32 or float64)."""
+    v, p = vol_desc(vol), proj_desc(proj)
+    img = np.ascontiguousarray(img).ravel()
+    b = np.ascontiguousarray(bins, np.int64)
+    out = np.zeros(b.size, img.dtype)
+    getattr(lib(), "orc_forward_bins" + _sfx(img.dtype))(
+        C.byref(v), int(geometric), C.byref(p), _p(xy), _p(z), _p(img),
+        _p(b), C.c_longlong(b.size), _p(out), nthreads)
+    return out
+
+
+def backward_bins(vol, proj, xy, z, bins, vals, geometric=False,
+                  nthreads=0):
+    """Backprojection of the listed bins only (values vals), one frame."""
+    v, p = vol_desc(vol), proj_desc(proj)
+    vals = np.ascontiguousarray(vals).ravel()
+    b = np.ascontiguousarray(bins, np.int64)
+    img = np.zeros(v.nx * v.ny * v.nz, vals.dtype)
+    getattr(lib(), "orc_backward_bins" + _sfx(vals.dtype))(
+        C.byref(v), int(geometric), C.byref(p), _p(xy), _p(z), _p(b),
+        _p(vals), C.c_longlong(b.size), _p(img), nthreads)
+    return img
+
+
 def backward(vol, proj, xy, z, sino, n_subsets=1, geometric=False,
              nthreads=0):
     v, p = vol_desc(vol), proj_desc(proj)

@@ -33,6 +33,18 @@ Outputs (all data: inputs and the reference's outputs):
   (synth.parity_counts of the reference's own forward projections), the
   same run through the all-FP64 oracle, and SHA-256 digests of the
   forward projections and counts (``python make_golden.py c2osem``).
+* ``c5_fp64_golden.npz`` -- the 64-ring scanner (310 M LORs, 256x256x128
+  image at 1.6 mm, geometric walk, where the reference itself is invalid:
+  int-truncated steps and 16-bit crystal ids) through the all-FP64 oracle
+  on the deterministic count sinogram synth.hash_counts: backprojection,
+  the 24-subset sensitivity (summed frames) and OSEM 2 x 24 from a uniform
+  image, each as synth.image_summary (block sums and central planes)
+  (``python make_golden.py c5fp64``; about 20 minutes on 8 threads).
+* ``host_api_golden.json`` -- the output of tests/host_api/caller.cc (a
+  program using only the reference's public ScannerData/VolData API:
+  getCrystalCoordinates, computeLineIntegral, projectLineIntegral,
+  printBinInfo) compiled against the reference, on the multi-module
+  scanner with gaps and the C1 phantom (``python make_golden.py hostapi``).
 * ``reso_golden.npz`` -- reconAlgos::OSEM_ResoReco of the compiled
   reference (called directly by ref_driver: no executable reaches it) on
   the C1 counts from an image of ones, 2 x 4 sub-iterations, 4 mm
@@ -419,6 +431,76 @@ def c2_osem_fixture(nthreads=8):
     print("c2_osem_golden.npz written")
 
 
+def c5_fp64_fixture(nthreads=8):
+    """c5_fp64_golden.npz (all-FP64 oracle; no reference run exists for this
+    scanner)."""
+    import time
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    w = synth.c5()
+    xy, z = O.scanner_tables(w.scanner)
+    shape = tuple(reversed(w.vol.size))
+    y = synth.hash_counts(w.n_lors).astype(np.float64)
+    arrays = {}
+    t = time.perf_counter()
+    bp = O.backward(w.vol, w.proj, xy, z, y, 1, geometric=True,
+                    nthreads=nthreads)
+    for k, a in synth.image_summary(bp, shape).items():
+        arrays["bp_" + k] = a
+    del bp
+    print(f"backprojection {time.perf_counter() - t:.0f} s", flush=True)
+    t = time.perf_counter()
+    sens = O.sensitivity(w.vol, w.proj, xy, z, w.n_subsets,
+                         dtype=np.float64, geometric=True, nthreads=nthreads)
+    ssum = sens.reshape(w.n_subsets, -1).sum(0)
+    for k, a in synth.image_summary(ssum, shape).items():
+        arrays["sens_" + k] = a
+    print(f"sensitivity {time.perf_counter() - t:.0f} s", flush=True)
+    t = time.perf_counter()
+    x = O.osem(w.vol, w.proj, xy, z, y, sens,
+               np.ones(w.vol.n_voxels, np.float64), w.n_iterations,
+               w.n_subsets, geometric=True, nthreads=nthreads)
+    for k, a in synth.image_summary(x, shape).items():
+        arrays["osem_" + k] = a
+    print(f"OSEM {time.perf_counter() - t:.0f} s", flush=True)
+    arrays["schedule"] = np.array([w.n_iterations, w.n_subsets], np.int32)
+    np.savez_compressed(os.path.join(HERE, "c5_fp64_golden.npz"), **arrays)
+    print("c5_fp64_golden.npz written")
+
+
+HOST_API_SCANNER = dict(
+    rsector_repeat_number=16, rsector_inner_radius=300.0,
+    crystal_dimensions=(20.0, 4.0, 4.0), crystal_repeat_numbers=(4, 2),
+    inter_crystal_distance=(0.1, 0.2), module_repeat_numbers=(2, 3),
+    inter_module_distance=(1.0, 2.0))
+
+
+def host_api_fixture():
+    """host_api_golden.json: tests/host_api/caller.cc built against the
+    reference library (oracle/_ref) and run on the multi-module scanner and
+    the C1 phantom."""
+    src = os.path.join(ROOT, "tests", "host_api", "caller.cc")
+    refsrc = "/root/reference/src_lib"
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "caller")
+        run(["g++", "-O2", "-std=c++17", "-fopenmp", f"-I{refsrc}", "-o",
+             exe, src, f"-L{REF}", "-lFIRref", f"-Wl,-rpath,{REF}"], d)
+        sc = io.ScannerSpec(**HOST_API_SCANNER)
+        sc.write(os.path.join(d, "s.scan"))
+        w = synth.c1()
+        act, _ = synth.phantom(w)
+        w.vol.write(os.path.join(d, "v.h33"), act)
+        m = HOST_API_SCANNER["module_repeat_numbers"]
+        c = HOST_API_SCANNER["crystal_repeat_numbers"]
+        args = [str(HOST_API_SCANNER["rsector_repeat_number"]),
+                str(m[0] * m[1]), str(c[0] * c[1])]
+        out = run([exe, "s.scan", "v.h33", *args], d)
+    with open(os.path.join(HERE, "host_api_golden.json"), "w") as f:
+        json.dump({"scanner": HOST_API_SCANNER, "args": args,
+                   "stdout": out}, f, indent=1)
+    print("host_api_golden.json written")
+
+
 RESO_RUNS = {"xd": (2, 4, 4.0, 1, 0.0), "xe": (2, 4, 4.0, 0, 55.0)}
 
 
@@ -449,6 +531,10 @@ if __name__ == "__main__":
         reso_fixture()
     elif len(sys.argv) > 1 and sys.argv[1] == "c2osem":
         c2_osem_fixture()
+    elif len(sys.argv) > 1 and sys.argv[1] == "hostapi":
+        host_api_fixture()
+    elif len(sys.argv) > 1 and sys.argv[1] == "c5fp64":
+        c5_fp64_fixture()
     else:
         main()
         bins_fixture()
