@@ -211,11 +211,14 @@ def run_uneven(args):
         y = torch.poisson(y0 * float(total), generator=g)
         solves.append(dict(vol=v128 if i % 2 else v160,
                            n_subsets=(8, 16, 24)[i % 3], y=y))
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
     t0 = time.perf_counter()
     b = recon.BatchedOSEM(w, solves, fwhm=w.fwhm, conv_interval=1,
                           cache_format=firlib.FIR_LEN_SNORM16)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
+    used = free0 - torch.cuda.mem_get_info()[0]
     log(f"{len(b.groups)} groups, caches built in {setup_s:.1f} s")
     b.capture()
     for _ in range(args.warmup):
@@ -240,7 +243,6 @@ def run_uneven(args):
     e1.record(stream)
     torch.cuda.synchronize()
     plain_ms = e0.elapsed_time(e1)
-    mem = sum(c.info()["device_bytes"] for c in b.caches)
     out = {
         "metric": "M LORs/s fwd+backproj & OSEM iter ms: plain vs batched "
                   "GPU at rel err<=1e-4",
@@ -261,7 +263,7 @@ def run_uneven(args):
         "plain_single_solve_iteration_ms": round(plain_ms, 3),
         "speedup_per_solve_vs_plain": round(plain_ms * 64 / ms, 3),
         "cache_setup_s": round(setup_s, 2),
-        "cache_device_gb": round(mem / 1e9, 2),
+        "cache_device_gb": round(used / 1e9, 2),
     }
     print(json.dumps(out), flush=True)
     b.close()
@@ -566,18 +568,17 @@ def main():
             yi = torch.poisson((mean_f * float(sc)), generator=g2)
             ctx.mul_clamped(yi, acf)
             ys.append(yi)
-        # batches of 8 interleaved images gather from interleaved image
-        # copies, best with 2 interleaved z-slices and 8 lanes per LOR, and
-        # do not use the sheet form: build that cache in place of the
-        # single-image one (same row order)
+        # batches of 8 interleaved images gather their line integrals from
+        # interleaved image copies, best with 2 interleaved z-slices and 8
+        # lanes per LOR, and backproject family by family from the sheet
+        # form's lists: build that cache in place of the single-image one
+        # (same row order)
         bz = 2
         cache.close()
         ctx.set_cache_zslices(bz)
         ctx.set_cache_lanes(8, 16)
-        ctx.set_cache_sheets(False)
         cache = ctx.path_cache(v, n_sub, firlib.FIR_LEN_SNORM16,
                                row_order=tres.best["row_order"])
-        ctx.set_cache_sheets(True)
         grp = recon.OSEMGroup(ctx, v, n_sub, ys, sens=sens_c, fwhm=fwhm,
                               conv_interval=1, cache=cache)
         grp.capture()
@@ -591,7 +592,9 @@ def main():
         bms = e0.elapsed_time(e1) / args.steps
         batched = {
             "n_images": args.batch,
-            "path": "cached, interleaved images, one HIP graph per iteration",
+            "path": "cached: line integrals gathered from interleaved image "
+                    "copies, family backprojection of all images from the "
+                    "sheet form's lists; one HIP graph per iteration",
             "cache_zslices": bz,
             "cache_lanes": [8, 16],
             "osem_iter_ms": round(bms, 4),

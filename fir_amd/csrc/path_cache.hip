@@ -1405,9 +1405,13 @@ __global__ void k_fr_count(const int* __restrict__ fam_off,
   atomicAdd(cnt + (long long)s * (nxy + 1) + fam_xy[i], 1u);
 }
 
+// ... and the entries themselves, as positions among the sheet pixels of
+// their subset's families (fam_loc: a family's first position), which is
+// where the launches over that subset keep the family's partial sheet.
 __global__ void k_fr_fill(const int* __restrict__ fam_off,
-                          const int* __restrict__ fam_xy, int n_fam, int total,
-                          int n_tang, int S, int nxy,
+                          const int* __restrict__ fam_xy,
+                          const int* __restrict__ fam_loc, int n_fam,
+                          int total, int n_tang, int S, int nxy,
                           const long long* __restrict__ off,
                           unsigned* __restrict__ cursor, int* __restrict__ ent)
 {
@@ -1422,7 +1426,7 @@ __global__ void k_fr_fill(const int* __restrict__ fam_off,
   }
   const int s = (lo / n_tang) % S;
   const long long k = (long long)s * (nxy + 1) + fam_xy[i];
-  ent[off[k] + atomicAdd(cursor + k, 1u)] = i;
+  ent[off[k] + atomicAdd(cursor + k, 1u)] = fam_loc[lo] + (i - fam_off[lo]);
 }
 
 // Each pixel's entries in increasing order (family order).
@@ -1551,7 +1555,7 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
   RowData rd, int lanes, const float* __restrict__ img, int nxy, int nz,
   OsemRowsIO io, float* __restrict__ w, const unsigned* __restrict__ fl_slot,
   const unsigned* __restrict__ fl_gb, const unsigned* __restrict__ fl,
-  float* __restrict__ part)
+  const int* __restrict__ fam_loc, float* __restrict__ part)
 {
   __shared__ float sh[kSheetMaxFloats];
   __shared__ float wr[kFamRowsMax];
@@ -1562,7 +1566,7 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
   const int p0 = fam_off[f];
   const int L = fam_off[f + 1] - p0;
   const int n_st = L * nz;
-  float* pf = part + (long long)p0 * nz;
+  float* pf = part + (long long)fam_loc[f] * nz;
   if (r0 == r1 || r1 - r0 > kFamRowsMax)
   {
     // no rows: the family's partials are 0 (a family never exceeds
@@ -1662,29 +1666,128 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
   }
 }
 
-// out[v] = sum over the subset's families whose sheet holds v's pixel of
-// that family's partial (family order). One thread per voxel, slices
-// fastest (consecutive threads read consecutive partials).
+// Family backprojection of NB images with given per-row weights
+// w[r * NB + b] (a batch's line integrals, or a sinogram times the length
+// factors): one block per family of the subset copies the family's weights
+// to LDS and sums every sheet entry's list against them, as the second
+// half of k_pc_osem_fam does for one image, into the partial sheets
+// part[(entry * NB + b)], entry = fam_loc[f] * nz + sheet index.
+template <int F, int NB>
+__global__ __launch_bounds__(kSheetBlock) void k_pc_fam_bp(
+  int n_subsets, int subset, int n_tang, int n_fam,
+  const int* __restrict__ fam_row, const int* __restrict__ fam_off,
+  const int* __restrict__ fam_loc, int nz, const float* __restrict__ w,
+  const unsigned* __restrict__ fl_slot, const unsigned* __restrict__ fl_gb,
+  const unsigned* __restrict__ fl, float* __restrict__ part)
+{
+  __shared__ float wr[kFamRowsMax * NB];
+  const int view = (int)blockIdx.y * n_subsets + subset;
+  const int f = view * n_tang + (int)blockIdx.x;
+  if (f >= n_fam) return;
+  const int r0 = fam_row[f], r1 = fam_row[f + 1];
+  const int p0 = fam_off[f];
+  const int n_st = (fam_off[f + 1] - p0) * nz;
+  float* pf = part + (long long)fam_loc[f] * nz * NB;
+  if (r0 == r1 || r1 - r0 > kFamRowsMax)
+  {
+    for (int i = threadIdx.x; i < n_st * NB; i += kSheetBlock) pf[i] = 0.0f;
+    return;
+  }
+  const int nw = (r1 - r0) * NB;
+  for (int i = threadIdx.x; i < nw; i += kSheetBlock)
+    wr[i] = w[(long long)r0 * NB + i];
+  __syncthreads();
+  const long long g0 = (long long)p0 * nz;
+  const int wave = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
+  const unsigned long long below = (1ull << ln) - 1ull;
+  for (int g = wave; g * 64 < n_st; g += kSheetBlock / 64)
+  {
+    const int j = g * 64 + ln;
+    const unsigned sl = j < n_st ? fl_slot[g0 + j] : 0u;
+    const unsigned gb = j < n_st ? fl_gb[g0 + j] : 0u;
+    const int q = (int)(sl & 0xffffu);
+    float acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = 0.0f;
+    unsigned roff = 0;
+    for (int m = 0;; m += kFlRounds)
+    {
+      unsigned long long act[kFlRounds];
+#pragma unroll
+      for (int k = 0; k < kFlRounds; ++k) act[k] = __ballot(m + k < q);
+      if (act[0] == 0) break;
+      uint4 v[kFlRounds];
+      unsigned o = roff;
+#pragma unroll
+      for (int k = 0; k < kFlRounds; ++k)
+      {
+        v[k] = m + k < q ? *reinterpret_cast<const uint4*>(
+                             fl + 4LL * (gb + o + __popcll(act[k] & below)))
+                         : make_uint4(0u, 0u, 0u, 0u);
+        o += (unsigned)__popcll(act[k]);
+      }
+      roff = o;
+#pragma unroll
+      for (int k = 0; k < kFlRounds; ++k)
+      {
+        float l[4];
+        Quad<F>::dec(sheet_lengths(v[k]), l);
+        const unsigned rw[4] = {v[k].x & 0xffffu, v[k].y & 0xffffu,
+                                v[k].z & 0xffffu, v[k].w & 0xffffu};
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+          const float* wp = wr + rw[u] * NB;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) acc[b] = fmaf(l[u], wp[b], acc[b]);
+        }
+      }
+    }
+    if (j < n_st)
+    {
+      float* o = pf + (long long)(sl >> 16) * NB;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) o[b] = acc[b];
+    }
+  }
+}
+
+// out_b[v] = sum over the subset's families whose sheet holds v's pixel of
+// that family's partial of image b (family order). One thread per voxel,
+// slices fastest (consecutive threads read consecutive partials).
+template <int NB>
 __global__ __launch_bounds__(kBlock) void k_pc_fam_reduce(
   const long long* __restrict__ off, const int* __restrict__ ent,
-  const float* __restrict__ part, int nxy, int nz, float* __restrict__ out)
+  const float* __restrict__ part, int nxy, int nz, VoxOut out)
 {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nxy * nz) return;
   const int px = (int)(t / nz), z = (int)(t - (long long)px * nz);
   const long long a = off[px], b = off[px + 1];
-  float acc = 0.0f;
+  float acc[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) acc[i] = 0.0f;
   long long k = a;
   for (; k + 4 <= b; k += 4) // four loads in flight, added in order
   {
-    float v[4];
+    Vec<NB> v[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = part[(long long)ent[k + u] * nz + z];
+    for (int u = 0; u < 4; ++u)
+      v[u] = Vec<NB>::load(part, (long long)ent[k + u] * nz + z);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc += v[u];
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int i = 0; i < NB; ++i) acc[i] += v[u].v[i];
   }
-  for (; k < b; ++k) acc += part[(long long)ent[k] * nz + z];
-  out[(long long)z * nxy + px] = acc;
+  for (; k < b; ++k)
+  {
+    const Vec<NB> v = Vec<NB>::load(part, (long long)ent[k] * nz + z);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) acc[i] += v.v[i];
+  }
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+    if (out.p[i]) out.p[i][(long long)z * nxy + px] = acc[i];
 }
 
 } // namespace
@@ -1793,8 +1896,10 @@ struct fir_cache
   // Family backprojection (with the sheet form): per subset and pixel the
   // sheet entries holding it, and the partial sheets of one launch.
   long long* d_fr_off = nullptr; // [n_subsets * (nxy + 1) + 1]
-  int* d_fr_ent = nullptr;       // [sheet entries]
-  float* d_part = nullptr;       // [sheet entries * nz]
+  int* d_fr_ent = nullptr;       // [sheet entries], positions in the subset
+  int* d_fam_loc = nullptr;      // [n_fam] first position of each family
+  // partial sheets of one launch, up to 8 images: [subset entries * nz * 8]
+  float* d_part = nullptr;
   long long fr_bytes = 0;
   int family_bp = 1;
 
@@ -1809,6 +1914,7 @@ struct fir_cache
     (void)hipFree(d_xy);
     (void)hipFree(d_fr_off);
     (void)hipFree(d_fr_ent);
+    (void)hipFree(d_fam_loc);
     (void)hipFree(d_part);
   }
 };
@@ -2199,28 +2305,59 @@ int build_family_bp(fir_ctx* c, fir_cache* cache)
   const int nxy = cache->vol.nx * cache->vol.ny, nz = cache->vol.nz;
   const long long total = rm->n_sheet_px;
   const long long n_keys = (long long)S * (nxy + 1);
-  const long long bytes = 8 * (n_keys + 1) + 4 * total + 4 * total * nz;
+  const int n_tang = rm->n_fam / rm->n_views;
+  // each family's first position among its subset's sheet pixels (views
+  // v = k * S + s in order, tangential bins in order), and the largest
+  // subset: the partial sheets of a launch hold one subset
+  std::vector<int> foff(rm->n_fam + 1), floc(rm->n_fam);
+  FIR_HIP_CHECK(hipMemcpyAsync(foff.data(), rm->d_fam_off,
+                               4LL * (rm->n_fam + 1), hipMemcpyDeviceToHost,
+                               c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  long long max_loc = 0;
+  for (int s0 = 0; s0 < S; ++s0)
+  {
+    long long loc = 0;
+    for (int view = s0; view < rm->n_views; view += S)
+      for (int t = 0; t < n_tang; ++t)
+      {
+        const int f = view * n_tang + t;
+        floc[f] = (int)loc;
+        loc += foff[f + 1] - foff[f];
+      }
+    max_loc = std::max(max_loc, loc);
+  }
+  const long long part_floats = max_loc * nz * kMaxCacheBatch;
+  const long long bytes =
+    8 * (n_keys + 1) + 4 * total + 4LL * rm->n_fam + 4 * part_floats;
   size_t free_b = 0, total_b = 0;
   FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
   if ((long long)free_b < bytes + 4 * n_keys + (2LL << 30)) return FIR_OK;
-  const int n_tang = rm->n_fam / rm->n_views;
   DevBufs tmp;
   unsigned* cnt;
   int rc;
   if ((rc = tmp.get(&cnt, (size_t)n_keys, "family table counts"))) return rc;
   if ((rc = dev_alloc(&cache->d_fr_off, (size_t)n_keys + 1,
                       "family table offsets")) ||
-      (rc = dev_alloc(&cache->d_fr_ent, (size_t)total, "family table")) ||
-      (rc = dev_alloc(&cache->d_part, (size_t)(total * nz),
+      (rc = dev_alloc(&cache->d_fr_ent, (size_t)std::max(total, 1LL),
+                      "family table")) ||
+      (rc = dev_alloc(&cache->d_fam_loc, (size_t)rm->n_fam,
+                      "family positions")) ||
+      (rc = dev_alloc(&cache->d_part, (size_t)std::max(part_floats, 1LL),
                       "family partial sheets")))
   {
     (void)hipFree(cache->d_fr_off);
     (void)hipFree(cache->d_fr_ent);
+    (void)hipFree(cache->d_fam_loc);
     cache->d_fr_off = nullptr;
     cache->d_fr_ent = nullptr;
+    cache->d_fam_loc = nullptr;
     cache->d_part = nullptr;
     return rc == FIR_E_NOMEM ? FIR_OK : rc;
   }
+  FIR_HIP_CHECK(hipMemcpyAsync(cache->d_fam_loc, floc.data(),
+                               4LL * rm->n_fam, hipMemcpyHostToDevice,
+                               c->stream));
   FIR_HIP_CHECK(hipMemsetAsync(cnt, 0, 4 * n_keys, c->stream));
   hipLaunchKernelGGL(k_fr_count, dim3(blocks_for(total)), dim3(kBlock), 0,
                      c->stream, rm->d_fam_off, rm->d_fam_xy, rm->n_fam,
@@ -2236,9 +2373,9 @@ int build_family_bp(fir_ctx* c, fir_cache* cache)
                                hipMemcpyHostToDevice, c->stream));
   FIR_HIP_CHECK(hipMemsetAsync(cnt, 0, 4 * n_keys, c->stream));
   hipLaunchKernelGGL(k_fr_fill, dim3(blocks_for(total)), dim3(kBlock), 0,
-                     c->stream, rm->d_fam_off, rm->d_fam_xy, rm->n_fam,
-                     (int)total, n_tang, S, nxy, cache->d_fr_off, cnt,
-                     cache->d_fr_ent);
+                     c->stream, rm->d_fam_off, rm->d_fam_xy, cache->d_fam_loc,
+                     rm->n_fam, (int)total, n_tang, S, nxy, cache->d_fr_off,
+                     cnt, cache->d_fr_ent);
   FIR_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_fr_sort, dim3(blocks_for(n_keys)), dim3(kBlock), 0,
                      c->stream, cache->d_fr_off, n_keys, cache->d_fr_ent);
@@ -2863,11 +3000,52 @@ void launch_osem_family(fir_cache* cache, int s, const float* img,
                      0, cache->ctx->stream, S, s, n_tang, rm->n_fam,
                      rm->d_fam_row, rm->d_fam_off, rm->d_fam_xy, rm->d_sel,
                      row_data(rm), cache->fwd_lanes, img, nxy, nz, io, w,
-                     rm->d_fl_slot, rm->d_fl_gb, rm->d_fl, cache->d_part);
-  hipLaunchKernelGGL(k_pc_fam_reduce, dim3(blocks_for((long long)nxy * nz)),
-                     dim3(kBlock), 0, cache->ctx->stream,
+                     rm->d_fl_slot, rm->d_fl_gb, rm->d_fl, cache->d_fam_loc,
+                     cache->d_part);
+  VoxOut out;
+  std::memset(&out, 0, sizeof(out));
+  out.p[0] = bp;
+  hipLaunchKernelGGL(k_pc_fam_reduce<1>,
+                     dim3(blocks_for((long long)nxy * nz)), dim3(kBlock), 0,
+                     cache->ctx->stream,
                      cache->d_fr_off + (long long)s * (nxy + 1),
-                     cache->d_fr_ent, cache->d_part, nxy, nz, bp);
+                     cache->d_fr_ent, cache->d_part, nxy, nz, out);
+}
+
+// Family backprojection of nb images (nb = 1, 2, 4 or 8) of subset s with
+// per-row weights w[r * nb + b], into out.p[b] (overwritten).
+template <int F, int NB>
+void launch_fam_bp_nb(fir_cache* cache, int s, const float* w,
+                      const VoxOut& out)
+{
+  const RowMatrix* rm = cache->rm.get();
+  const int S = cache->n_subsets;
+  const int vpr = fir::views_in_subset(rm->n_views, S, s);
+  const int n_tang = rm->n_fam / rm->n_views;
+  const int nxy = cache->vol.nx * cache->vol.ny, nz = cache->vol.nz;
+  hipLaunchKernelGGL((k_pc_fam_bp<F, NB>), dim3(n_tang, vpr),
+                     dim3(kSheetBlock), 0, cache->ctx->stream, S, s, n_tang,
+                     rm->n_fam, rm->d_fam_row, rm->d_fam_off,
+                     cache->d_fam_loc, nz, w, rm->d_fl_slot, rm->d_fl_gb,
+                     rm->d_fl, cache->d_part);
+  hipLaunchKernelGGL(k_pc_fam_reduce<NB>,
+                     dim3(blocks_for((long long)nxy * nz)), dim3(kBlock), 0,
+                     cache->ctx->stream,
+                     cache->d_fr_off + (long long)s * (nxy + 1),
+                     cache->d_fr_ent, cache->d_part, nxy, nz, out);
+}
+
+template <int F>
+void launch_fam_bp(fir_cache* cache, int s, int nb, const float* w,
+                   const VoxOut& out)
+{
+  switch (nb)
+  {
+  case 1: launch_fam_bp_nb<F, 1>(cache, s, w, out); break;
+  case 2: launch_fam_bp_nb<F, 2>(cache, s, w, out); break;
+  case 4: launch_fam_bp_nb<F, 4>(cache, s, w, out); break;
+  default: launch_fam_bp_nb<F, 8>(cache, s, w, out); break;
+  }
 }
 
 template <int F>
@@ -2995,10 +3173,15 @@ static int cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
   cache->bp_lanes = c->cache_lanes[1];
   size_t free_b = 0, total_b = 0;
   FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  // With family lists every backprojection runs family by family and the
+  // voxel-major transpose is built only if tuning asks for the other
+  // paths (fir_cache_tune); without them it is built now.
+  const bool lists = rm->d_fl != nullptr;
   const long long need =
-    (rm->n_elems + 3LL * cache->n_cached * cache->nvox) *
-      (4 + (long long)len_bytes(len_format)) +
-    (long long)cache->n_cached * (cache->nvox + 1) * 8 +
+    (lists ? 0
+           : (rm->n_elems + 3LL * cache->n_cached * cache->nvox) *
+               (4 + (long long)len_bytes(len_format)) +
+             (long long)cache->n_cached * (cache->nvox + 1) * 8) +
     ((long long)rm->n_rows + 2 * cache->nvox) * kMaxCacheBatch * 4;
   if ((long long)free_b < need + (1LL << 30))
   {
@@ -3013,8 +3196,8 @@ static int cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
   cache->w_floats = (size_t)rm->n_rows * kMaxCacheBatch;
   cache->x_floats = (size_t)layout_voxels(cache_xy(cache), v->nz) *
     kMaxCacheBatch;
-  if ((rc = build_transpose(c, cache)) ||
-      (rc = build_family_bp(c, cache)) ||
+  if ((rc = build_family_bp(c, cache)) ||
+      (!use_family_bp(cache) && (rc = build_transpose(c, cache))) ||
       (rc = dev_alloc(&cache->d_w, cache->w_floats, "row weights")) ||
       (rc = dev_alloc(&cache->d_x, cache->x_floats, "interleaved images")) ||
       (rc = dev_alloc(&cache->d_xy, cache->x_floats, "y-fastest images")))
@@ -3117,9 +3300,30 @@ int fir_cache_tune(fir_cache* cache, int fwd_lanes, int bp_lanes, int flags)
   if (flags & ~(FIR_CACHE_XCD_MAP | FIR_CACHE_STORED_VOXELS |
                 FIR_CACHE_NO_SHEETS | FIR_CACHE_NO_FAMILY_BP))
     return fail(FIR_E_ARG, "unknown cache flags 0x%x", flags);
-  cache->zfast = (flags & FIR_CACHE_STORED_VOXELS) ? 0 : 1;
+  const int sheets = cache->sheets, family_bp = cache->family_bp;
   cache->sheets = (flags & FIR_CACHE_NO_SHEETS) ? 0 : 1;
   cache->family_bp = (flags & FIR_CACHE_NO_FAMILY_BP) ? 0 : 1;
+  if (!use_family_bp(cache) && !cache->d_voff)
+  {
+    // the transposed voxel lists, skipped at creation, are needed now; on
+    // failure the cache keeps its previous settings
+    FIR_HIP_CHECK(hipSetDevice(cache->ctx->device));
+    if ((rc = build_transpose(cache->ctx, cache)))
+    {
+      (void)hipStreamSynchronize(cache->ctx->stream);
+      (void)hipFree(cache->d_voff);
+      (void)hipFree(cache->d_trow);
+      (void)hipFree(cache->d_tlen);
+      cache->d_voff = nullptr;
+      cache->d_trow = nullptr;
+      cache->d_tlen = nullptr;
+      cache->t_slots = 0;
+      cache->sheets = sheets;
+      cache->family_bp = family_bp;
+      return rc;
+    }
+  }
+  cache->zfast = (flags & FIR_CACHE_STORED_VOXELS) ? 0 : 1;
   cache->fwd_lanes = fwd_lanes;
   cache->bp_lanes = bp_lanes;
   cache->xcd_map = (flags & FIR_CACHE_XCD_MAP) ? 1 : 0;
@@ -3186,7 +3390,14 @@ static int cache_backward_impl(fir_cache* cache, const float* sino,
     VoxOut out;
     std::memset(&out, 0, sizeof(out));
     out.p[0] = frames + (long long)s * cache->nvox;
-    if (rm->fmt == FIR_LEN_SNORM16)
+    if (use_family_bp(cache))
+    {
+      if (rm->fmt == FIR_LEN_SNORM16)
+        launch_fam_bp<FIR_LEN_SNORM16>(cache, s, 1, cache->d_w, out);
+      else
+        launch_fam_bp<FIR_LEN_FP16>(cache, s, 1, cache->d_w, out);
+    }
+    else if (rm->fmt == FIR_LEN_SNORM16)
       launch_backproject<FIR_LEN_SNORM16>(cache, s, 1, cache->d_w, out);
     else if (rm->fmt == FIR_LEN_FP16)
       launch_backproject<FIR_LEN_FP16>(cache, s, 1, cache->d_w, out);
@@ -3280,6 +3491,26 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
     const float* X = wd > 1 || rm->zq > 1 ? cache->d_x : x[i0];
     pack_images(cache, wd, in, nb);
     FIR_CHECK_LAUNCH();
+    if (wd > 1 && use_family_bp(cache))
+    {
+      // batch line integrals from the image copies, then the family
+      // backprojection of all its images from the weights
+      if (rm->fmt == FIR_LEN_SNORM16)
+      {
+        launch_osem_rows<FIR_LEN_SNORM16>(cache, subset, wd, X, io,
+                                          cache->d_w);
+        FIR_CHECK_LAUNCH();
+        launch_fam_bp<FIR_LEN_SNORM16>(cache, subset, wd, cache->d_w, out);
+      }
+      else
+      {
+        launch_osem_rows<FIR_LEN_FP16>(cache, subset, wd, X, io, cache->d_w);
+        FIR_CHECK_LAUNCH();
+        launch_fam_bp<FIR_LEN_FP16>(cache, subset, wd, cache->d_w, out);
+      }
+      FIR_CHECK_LAUNCH();
+      continue;
+    }
     if (rm->fmt == FIR_LEN_SNORM16)
     {
       launch_osem_rows<FIR_LEN_SNORM16>(cache, subset, wd, X, io, cache->d_w);

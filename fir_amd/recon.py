@@ -154,17 +154,11 @@ class BatchedOSEM:
             shared = firlib.context_for(workload, device)
             shared.set_uneven_subsets(True)
             # groups project several images per launch, for which 2
-            # interleaved z-slices suit the gathers best
+            # interleaved z-slices suit the gathers best; the sheet form's
+            # family lists serve their backprojections (and make the
+            # voxel-major transposes unnecessary)
             shared.set_cache_zslices(cache_zslices)
-            # the sheet form serves one-image passes only: skip its memory
-            # when every group projects several images at once
-            sizes = {}
-            for sv in solves:
-                v = sv["vol"]
-                k = (v.nx, v.ny, v.nz, v.dx, v.dy, v.dz, v.ox, v.oy, v.oz,
-                     sv["n_subsets"])
-                sizes[k] = sizes.get(k, 0) + 1
-            shared.set_cache_sheets(min(sizes.values()) == 1)
+            shared.set_cache_sheets(True)
         self.ctx = shared
         for k in keys:
             idx = [i for i, s in enumerate(solves)

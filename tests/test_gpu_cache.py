@@ -453,6 +453,8 @@ def test_sheet_form_equals_image_gathers(c1, c1_golden, fmt):
     backprojecting from the transposed lists)."""
     w, ctx, v, caches = c1
     cache = ctx.path_cache(v, 4, fmt, row_order=firlib.CACHE_ROW_ORDER)
+    # the sheet form exists for the 16-bit length formats only
+    assert cache.forms()["sheets"] == (fmt != firlib.FIR_LEN_FP32)
     x = dev(c1_golden["act"].ravel() + 0.1)
     y = dev(c1_golden["y"])
     bias = dev(np.full_like(c1_golden["y"], 0.5))
@@ -505,6 +507,7 @@ def test_family_backprojection(c1, c1_golden, fmt):
     projector."""
     w, ctx, v, caches = c1
     cache = ctx.path_cache(v, 4, fmt, row_order=firlib.CACHE_ROW_ORDER)
+    assert cache.forms() == {"sheets": True, "family_lists": True}
     x = dev(c1_golden["act"].ravel() + 0.1)
     y = dev(c1_golden["y"])
     bias = dev(np.full_like(c1_golden["y"], 0.5))
@@ -524,4 +527,50 @@ def test_family_backprojection(c1, c1_golden, fmt):
             ctx.osem_project(v, x, y, b[0] if b else None, p, s, 4)
             assert rel_l2(host(f1), host(p)) < FWD_TOL[fmt]
     cache.tune(8, 16, 0)
+    cache.close()
+
+
+@pytest.mark.parametrize("n_images", [2, 3, 8, 11])
+def test_family_batch_backprojection(c1, c1_golden, n_images):
+    """Batches over a cache with family lists gather their line integrals
+    from the image copies and backproject all their images family by
+    family from the lists (one read of the lists per batch, weights of
+    every image in LDS): each image gets exactly what its one-image family
+    pass gives (the same weights, lists and summation order), within FP32
+    summation order of the transposed-list backprojection (which the cache
+    builds only when tuning asks for it), and the full backprojection and
+    sensitivity through the lists match the transposed ones the same way."""
+    w, ctx, v, caches = c1
+    cache = ctx.path_cache(v, 4, firlib.FIR_LEN_SNORM16,
+                           row_order=firlib.CACHE_ROW_ORDER)
+    assert cache.forms()["family_lists"]
+    rng = np.random.default_rng(40 + n_images)
+    xs = [dev(rng.uniform(0.5, 2.0, v.n_voxels).astype(np.float32))
+          for _ in range(n_images)]
+    ys = [dev(rng.poisson(c1_golden["y0"] * s).astype(np.float32))
+          for s in rng.uniform(0.1, 4.0, n_images)]
+    bps = [torch.full((v.n_voxels,), -1.0, device="cuda")
+           for _ in range(n_images)]
+    bytes_before = cache.info()["device_bytes"]
+    cache.osem_project(xs, ys, None, bps, 2)
+    sens_f = torch.zeros(4 * v.n_voxels, device="cuda")
+    cache.sensitivity(sens_f)
+    back_f = torch.zeros(4 * v.n_voxels, device="cuda")
+    cache.backward(ys[0], back_f)
+    for x, y, bp in zip(xs, ys, bps):
+        one = torch.zeros(v.n_voxels, device="cuda")
+        cache.osem_project([x], [y], None, [one], 2)
+        assert np.array_equal(host(bp), host(one))
+    cache.tune(8, 16, firlib.FIR_CACHE_NO_FAMILY_BP)
+    assert cache.info()["device_bytes"] > bytes_before  # transposes built
+    for x, y, bp in zip(xs, ys, bps):
+        g = torch.zeros(v.n_voxels, device="cuda")
+        cache.osem_project([x], [y], None, [g], 2)
+        assert float((bp - g).abs().max()) <= 1e-6 * float(g.abs().max())
+    sens_t = torch.zeros(4 * v.n_voxels, device="cuda")
+    cache.sensitivity(sens_t)
+    back_t = torch.zeros(4 * v.n_voxels, device="cuda")
+    cache.backward(ys[0], back_t)
+    assert rel_max(host(sens_f), host(sens_t)) <= 1e-6
+    assert rel_max(host(back_f), host(back_t)) <= 1e-6
     cache.close()
