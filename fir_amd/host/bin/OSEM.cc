@@ -92,6 +92,8 @@ int main(int argc, char** argv)
       recomputeSens = !sensNamed || !fileExists(params.sensVolFile);
     const bool acfExists = acfNamed && fileExists(params.attenCorrFactorsFile);
 
+    PhaseTimer total("total");
+    PhaseTimer tRead("read inputs");
     ProjData inputProj(params.inputProjFile,
                        ProjData::ConstructionMode::READ_DATA);
     inputProj.checkNSubsets(params.algo.nSubsets);
@@ -101,8 +103,10 @@ int main(int argc, char** argv)
     VolData outputVol(params.outputVolHeader,
                       VolData::ConstructionMode::READ_DATA_IF_PROVIDED, 1.0);
     printEmptyLine();
+    tRead.stop();
 
     VolData sensVol;
+    PhaseTimer tSens("sensitivity");
     if (recomputeSens)
     {
       echo("Computing sensitivity map");
@@ -128,6 +132,8 @@ int main(int argc, char** argv)
               "provided");
     }
 
+    tSens.stop();
+
     std::optional<ProjData> biasProj;
     if (!params.biasProjFile.empty())
     {
@@ -138,6 +144,7 @@ int main(int argc, char** argv)
                        ProjData::ConstructionMode::READ_DATA);
     }
 
+    PhaseTimer tAcf("attenuation");
     if (huNamed || acfExists)
     {
       if (!recomputeAcf && huNamed && !acfExists) recomputeAcf = true;
@@ -173,6 +180,9 @@ int main(int argc, char** argv)
       inputProj *= acf;
     }
 
+    tAcf.stop();
+
+    PhaseTimer tOsem("OSEM");
     // The reference selects OSEM_ResoReco with a constant that is always
     // false (OSEM.cc:340-341); here FIR_RESO_RECO=1 selects it.
     const char* reso = std::getenv("FIR_RESO_RECO");
@@ -185,6 +195,9 @@ int main(int argc, char** argv)
                        params.outputVolFileName, params.algo, sensVol,
                        biasProj);
 
+    tOsem.stop();
+
+    PhaseTimer tWrite("write output");
     printQuotedValue("Saving reconstructed volume to file",
                      params.outputVolFileName);
     printEmptyLine();

@@ -4,6 +4,9 @@
 // executables catch, print to stderr and return EXIT_FAILURE.
 #pragma once
 
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
@@ -67,3 +70,43 @@ template<typename... Args>
   s << '\n';
   throw std::runtime_error(s.str());
 }
+
+// Wall-clock time of one phase of an executable, printed to stderr as
+// "timing: <name> <ms> ms" when the phase ends, if FIR_TIMINGS is set (and
+// not "0"). Device work still queued at the end of a phase is counted in
+// the phase that waits for it.
+class PhaseTimer
+{
+public:
+  explicit PhaseTimer(const char* name)
+    : mName(name), mStart(std::chrono::steady_clock::now())
+  {
+  }
+  ~PhaseTimer() { stop(); }
+  void stop()
+  {
+    if (!mName) return;
+    if (enabled())
+    {
+      const double ms = std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - mStart)
+                          .count();
+      std::ostringstream s;
+      s << "timing: " << mName << " " << ms << " ms\n";
+      std::cerr << s.str() << std::flush;
+    }
+    mName = nullptr;
+  }
+  static bool enabled()
+  {
+    static const bool on = [] {
+      const char* e = std::getenv("FIR_TIMINGS");
+      return e && *e && std::strcmp(e, "0") != 0;
+    }();
+    return on;
+  }
+
+private:
+  const char* mName;
+  std::chrono::steady_clock::time_point mStart;
+};

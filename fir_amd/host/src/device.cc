@@ -42,6 +42,7 @@ fir_ctx* context()
   State& s = state();
   if (!s.ctx)
   {
+    PhaseTimer t("device context");
     const char* env = std::getenv("FIR_DEVICE");
     const int dev = env ? std::atoi(env) : 0;
     check(fir_ctx_create(&s.ctx, dev));

@@ -106,6 +106,7 @@ void osemImpl(const ProjData& inputProj, const ScannerData& scanner,
   if (cacheFormat >= 0)
   {
     print("Building the path cache");
+    PhaseTimer tBuild("path cache build");
     // Cache rows per view: tangential bin, then pair by centre slice.
     device::check(fir_set_lor_order(
       ctx, FIR_ORDER_VIEW_MAJOR | FIR_ORDER_TANG_MAJOR | FIR_ORDER_ZCENTER));
@@ -120,6 +121,8 @@ void osemImpl(const ProjData& inputProj, const ScannerData& scanner,
                                     &cached.cache);
     device::check(fir_set_lor_order(ctx, FIR_ORDER_SEGMENT));
     device::check(rc);
+    device::check(fir_synchronize(ctx));
+    tBuild.stop();
     device::check(fir_cache_subsets(cached.cache, nullptr, &nCached));
     if (nCached < p.nSubsets)
       print("The path cache holds subsets 1 to ", nCached, " of ", p.nSubsets,
