@@ -1396,6 +1396,30 @@ int fir_memcpy_d2h(fir_ctx* c, void* dst, const void* src, size_t bytes)
   return FIR_OK;
 }
 
+int fir_memcpy_h2d_async(fir_ctx* c, void* dst, const void* src,
+                         size_t bytes)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (bytes == 0) return FIR_OK;
+  if (!dst || !src) return fail(FIR_E_ARG, "NULL buffer");
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  FIR_HIP_CHECK(
+    hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return FIR_OK;
+}
+
+int fir_memcpy_d2h_async(fir_ctx* c, void* dst, const void* src,
+                         size_t bytes)
+{
+  if (!c) return fail(FIR_E_ARG, "context is NULL");
+  if (bytes == 0) return FIR_OK;
+  if (!dst || !src) return fail(FIR_E_ARG, "NULL buffer");
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  FIR_HIP_CHECK(
+    hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  return FIR_OK;
+}
+
 int fir_fill(fir_ctx* c, float* dst, int64_t n, float value)
 {
   if (!c) return fail(FIR_E_ARG, "context is NULL");

@@ -6,8 +6,8 @@
 #include <ScannerData.h>
 #include <VolData.h>
 #include <console.h>
+#include <device.h>
 #include <operations.h>
-#include <projections.h>
 
 #include <cstdlib>
 #include <iostream>
@@ -82,14 +82,25 @@ int main(int argc, char** argv)
     ScannerData scanner(params.scannerFile);
     ProjData out(params.outputProjHeader, ProjData::ConstructionMode::ALLOCATE);
 
+    // projections::forward (projections.cc:15-92) on the device; the
+    // sinogram is downloaded chunk by chunk while earlier chunks are written
+    scanner.checkProjData(out);
+    device::useGeometry(scanner, out.getHeader());
+    const fir_vol_desc v = device::describe(inputVol.getHeader());
+    const size_t nv = static_cast<size_t>(inputVol.getNVoxelsPerFrame());
+    const size_t nb = static_cast<size_t>(out.getGeometry().nBins);
+    device::Buffer img(nv * sizeof(float)), sino(nb * sizeof(float));
     for (int frame : params.frames)
     {
       inputVol.setActiveFrame(frame);
       if (mask) operations::applyMask(inputVol, *mask);
-      projections::forward(inputVol, scanner, out);
-      out.write(single ? params.outputProjFileName
-                       : params.outputProjFileName + "_frame_" +
-                           std::to_string(frame));
+      img.upload(inputVol.getDataArray(), nv * sizeof(float));
+      echo("Forward projection (GPU)");
+      device::check(fir_forward(device::context(), &v, img.f(), sino.f()));
+      device::writeProj(sino, out,
+                        single ? params.outputProjFileName
+                               : params.outputProjFileName + "_frame_" +
+                                   std::to_string(frame));
     }
   }
   catch (const std::exception& ex)

@@ -1,11 +1,14 @@
 // FIR_OSEM params [recomputeSens] [recomputeAttn] -- OSEM reconstruction
 // with the reference's parameter file, flags, sensitivity/ACF caching and
-// outputs (src_bin/OSEM.cc:16-502); all projector work runs on the GPU.
+// outputs (src_bin/OSEM.cc:16-502); all projector work runs on the GPU,
+// and the sinograms stay in device memory from the moment they are read
+// (each chunk's upload overlaps the reading of the next).
 //
 // Flags: anything not starting with '0' means 1; both default to 1.
 //   recomputeSens: recompute the sensitivity map (always recomputed when no
 //     sensitivity file is named or the named file does not exist; saved when
-//     a file is named).
+//     a file is named). It is the sensitivity of the projector the
+//     iterations use: with FIR_PATH_CACHE, that of the cached paths.
 //   recomputeAttn: recompute the attenuation correction factors from the HU
 //     volume (falls back to reading the ACF file when no HU volume is
 //     given, and to recomputing when the ACF file does not exist).
@@ -14,15 +17,13 @@
 #include <ScannerData.h>
 #include <VolData.h>
 #include <console.h>
-#include <operations.h>
-#include <projections.h>
+#include <device.h>
 #include <reconAlgos.h>
 
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
-#include <optional>
 #include <string>
 
 namespace
@@ -94,31 +95,42 @@ int main(int argc, char** argv)
 
     PhaseTimer total("total");
     PhaseTimer tRead("read inputs");
-    ProjData inputProj(params.inputProjFile,
-                       ProjData::ConstructionMode::READ_DATA);
+    // the sinogram goes to the device chunk by chunk as it is read; every
+    // step below keeps its data there
+    ProjData inputProj;
+    device::Buffer y = device::readProj(params.inputProjFile, inputProj);
     inputProj.checkNSubsets(params.algo.nSubsets);
     ScannerData scanner(params.scannerFile);
+    scanner.checkProjData(inputProj);
 
     // initial estimate: the template's data, or ones
     VolData outputVol(params.outputVolHeader,
                       VolData::ConstructionMode::READ_DATA_IF_PROVIDED, 1.0);
     printEmptyLine();
     tRead.stop();
+    device::useGeometry(scanner, inputProj.getHeader());
+    fir_ctx* ctx = device::context();
+    const int nSub = params.algo.nSubsets;
+    const size_t vb =
+      static_cast<size_t>(outputVol.getNVoxelsPerFrame()) * sizeof(float);
+    const size_t nb = static_cast<size_t>(inputProj.getGeometry().nBins);
 
+    device::Buffer sens(vb * nSub);
     VolData sensVol;
-    PhaseTimer tSens("sensitivity");
+    DeviceOSEMInputs in;
+    in.y = y.f();
+    in.sens = sens.f();
     if (recomputeSens)
     {
+      // computed on the device with the projector the iterations use,
+      // right before them (after the path cache, when one is built)
       echo("Computing sensitivity map");
       printEmptyLine();
-      sensVol.allocateAsMultiVol(outputVol, params.algo.nSubsets);
-      projections::computeSensitivityVol(inputProj, scanner, sensVol,
-                                         params.algo.nSubsets);
+      in.computeSens = true;
       if (sensNamed)
       {
-        printQuotedValue("Saving sensitivity map to file", params.sensVolFile);
-        printEmptyLine();
-        sensVol.write(params.sensVolFile);
+        sensVol.allocateAsMultiVol(outputVol, nSub);
+        in.sensOut = &sensVol;
       }
     }
     else
@@ -130,18 +142,22 @@ int main(int argc, char** argv)
       if (sensVol.getHeader() != outputVol.getHeader())
         error("Sensitivity volume provided doesn't fit with output volume "
               "provided");
+      if (sensVol.getNFrames() != nSub)
+        error("Wrong number of frames (expected ", nSub, ")");
+      sens.upload(sensVol.getAllFrames(), vb * nSub);
     }
 
-    tSens.stop();
-
-    std::optional<ProjData> biasProj;
+    ProjData biasProj;
+    device::Buffer bias;
     if (!params.biasProjFile.empty())
     {
       printQuotedValue("Reading bias projection from file",
                        params.biasProjFile);
       printEmptyLine();
-      biasProj.emplace(params.biasProjFile,
-                       ProjData::ConstructionMode::READ_DATA);
+      bias = device::readProj(params.biasProjFile, biasProj);
+      if (biasProj.getHeader() != inputProj.getHeader())
+        error("Bias projection doesn't fit with input projection");
+      in.bias = bias.f();
     }
 
     PhaseTimer tAcf("attenuation");
@@ -150,23 +166,34 @@ int main(int argc, char** argv)
       if (!recomputeAcf && huNamed && !acfExists) recomputeAcf = true;
       if (recomputeAcf && !huNamed && acfExists) recomputeAcf = false;
       ProjData acf;
+      device::Buffer acfDev;
       if (recomputeAcf)
       {
+        // the reference's chain (OSEM.cc:294-337): HU -> mu, cut, forward
+        // projection, clamped exponential; all on the device
         echo("Computing attenuation correction factors");
         printEmptyLine();
         VolData muMap(params.attenVolHUFile,
                       VolData::ConstructionMode::READ_DATA);
-        operations::HounsfieldToMuMap(muMap);
-        operations::cutCircle(muMap, params.algo.cutRadius);
-        acf.copy(inputProj, ProjData::ConstructionMode::INITIALIZE);
-        projections::forward(muMap, scanner, acf);
-        acf.exponential();
+        const fir_vol_desc mv = device::describe(muMap.getHeader());
+        const size_t mn = static_cast<size_t>(muMap.getNVoxelsPerFrame());
+        device::Buffer mu(mn * sizeof(float));
+        mu.upload(muMap.getDataArray(), mn * sizeof(float));
+        device::check(fir_hu_to_mu(ctx, mu.f(), static_cast<int64_t>(mn)));
+        device::check(
+          fir_cut_circle(ctx, &mv, mu.f(), 1, params.algo.cutRadius));
+        acfDev = device::Buffer(nb * sizeof(float));
+        echo("Forward projection (GPU)");
+        device::check(fir_forward(ctx, &mv, mu.f(), acfDev.f()));
+        device::check(
+          fir_exp_clamped(ctx, acfDev.f(), static_cast<int64_t>(nb)));
         if (acfNamed)
         {
           printQuotedValue("Saving attenuation correction factors to file",
                            params.attenCorrFactorsFile);
           printEmptyLine();
-          acf.write(params.attenCorrFactorsFile);
+          acf.copy(inputProj, ProjData::ConstructionMode::ALLOCATE);
+          device::writeProj(acfDev, acf, params.attenCorrFactorsFile);
         }
       }
       else
@@ -174,27 +201,31 @@ int main(int argc, char** argv)
         printQuotedValue("Reading attenuation correction factors from file",
                          params.attenCorrFactorsFile);
         printEmptyLine();
-        acf.read(params.attenCorrFactorsFile,
-                 ProjData::ConstructionMode::READ_DATA);
+        acfDev = device::readProj(params.attenCorrFactorsFile, acf);
+        if (acf.getHeader() != inputProj.getHeader())
+          error("Projections do not have the same size");
       }
-      inputProj *= acf;
+      // inputProj *= ACF (ProjData.cc:481-512), on the device copy
+      device::check(fir_mul_clamped(ctx, y.f(), acfDev.f(),
+                                    static_cast<int64_t>(nb)));
+      device::check(fir_synchronize(ctx));
     }
-
     tAcf.stop();
 
     PhaseTimer tOsem("OSEM");
     // The reference selects OSEM_ResoReco with a constant that is always
     // false (OSEM.cc:340-341); here FIR_RESO_RECO=1 selects it.
     const char* reso = std::getenv("FIR_RESO_RECO");
-    if (reso && *reso && std::strcmp(reso, "0") != 0)
-      reconAlgos::OSEM_ResoReco(inputProj, scanner, outputVol,
-                                params.outputVolFileName, params.algo, sensVol,
-                                biasProj);
-    else
-      reconAlgos::OSEM(inputProj, scanner, outputVol,
-                       params.outputVolFileName, params.algo, sensVol,
-                       biasProj);
-
+    const bool resoReco = reso && *reso && std::strcmp(reso, "0") != 0;
+    reconAlgos::OSEMOnDevice(inputProj, scanner, outputVol,
+                             params.outputVolFileName, params.algo, in,
+                             resoReco);
+    if (in.sensOut)
+    {
+      printQuotedValue("Saving sensitivity map to file", params.sensVolFile);
+      printEmptyLine();
+      sensVol.write(params.sensVolFile);
+    }
     tOsem.stop();
 
     PhaseTimer tWrite("write output");

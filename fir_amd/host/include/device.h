@@ -7,8 +7,10 @@
 
 #include <cstddef>
 #include <fir_hip.h>
+#include <string>
 
 
+class ProjData;
 class ScannerData;
 struct ProjHeader;
 
@@ -43,6 +45,18 @@ private:
 };
 
 fir_vol_desc describe(const VolHeader& header);
+
+// Reads the sinogram of `headerFile` into `proj` (as ProjData::read with
+// READ_DATA) and into a new device buffer: each chunk's copy to the device
+// is enqueued as soon as the chunk has been read, so that the copies run
+// while the rest of the file is read (proj's storage is page-locked).
+Buffer readProj(const std::string& headerFile, ProjData& proj);
+
+// Writes `proj`'s header and the device sinogram `dev` to outputProjFile
+// (as ProjData::write): each chunk is downloaded into proj's storage just
+// before it is written, while earlier chunks are still being written.
+void writeProj(const Buffer& dev, ProjData& proj,
+               const std::string& outputProjFile);
 
 // Make the context's scanner tables and sinogram geometry those of
 // (scanner, proj); no-op if they already are.

@@ -20,6 +20,21 @@ struct OSEMCoreParams
   std::vector<float> fwhmXYZ{0.0, 0.0, 0.0};
 };
 
+// Device-resident data of one OSEM run: device pointers (nBins floats for
+// y and bias, nSubsets frames for sens). No counterpart in the reference
+// API: FIR_OSEM keeps its inputs in HBM from the moment they are read.
+struct DeviceOSEMInputs
+{
+  const float* y = nullptr;    // measured sinogram (attenuation applied)
+  const float* bias = nullptr; // additive term, or none
+  float* sens = nullptr;       // sensitivity frames
+  // fill sens first with the sensitivity of the projector the iterations
+  // use (the path cache's for its subsets, else the walk's), copying it to
+  // sensOut (host frames) when given
+  bool computeSens = false;
+  VolData* sensOut = nullptr;
+};
+
 namespace reconAlgos
 {
 // OSEM on the active frame of outputVol (the initial estimate). Per
@@ -39,6 +54,14 @@ void OSEM(const ProjData& inputProj, const ScannerData& scanner,
 // x /= sensitivity frame, x *= bp, Gaussian every convolutionInterval
 // sub-iterations, circular cut. The forward projection uses x itself, as
 // the reference's does.
+// OSEM (reso false) or OSEM_ResoReco (reso true, in.sens convolved in
+// place) on device-resident inputs; inputProj gives the sinogram geometry
+// only (its bins are not read). An extension of the reference API.
+void OSEMOnDevice(const ProjData& inputProj, const ScannerData& scanner,
+                  VolData& outputVol, const std::string& outputVolFileName,
+                  const OSEMCoreParams& params, const DeviceOSEMInputs& in,
+                  bool reso = false);
+
 void OSEM_ResoReco(const ProjData& inputProj, const ScannerData& scanner,
                    VolData& outputVol, const std::string& outputVolFileName,
                    const OSEMCoreParams& params, VolData& sensitivityMap,

@@ -26,27 +26,61 @@ ProjInterfileReader::ProjInterfileReader(const std::string& headerFileName)
   mGeometry.fill(mHeader);
 }
 
-void ProjInterfileReader::readData(types::BinValue** data)
+namespace
 {
-  std::ifstream in(mDataFileName, std::ios::binary);
-  if (!in.is_open()) error("Couldn't open file ", mDataFileName);
-  for (int seg = -mGeometry.segOffset; seg <= mGeometry.segOffset; ++seg)
+// Bins of each segment, and whether the segment arrays are one contiguous
+// block in file order.
+bool contiguous(const ProjGeometry& g, int nTang, types::BinValue** data,
+                std::vector<long long>* counts)
+{
+  counts->clear();
+  bool one = true;
+  for (int seg = -g.segOffset; seg <= g.segOffset; ++seg)
   {
-    const long long n = static_cast<long long>(mGeometry.nViews) *
-      mGeometry.getNAxialCoords(seg) * mHeader.nTangCoords;
-    in.read(reinterpret_cast<char*>(data[seg + mGeometry.segOffset]),
-            static_cast<std::streamsize>(n * sizeof(types::BinValue)));
-    if (!in)
-      error("The number of bins that were read from the data file (",
-            in.gcount(), ") is inferior to that expected from the header "
-            "file (",
-            mGeometry.nBins, ")");
+    const long long n =
+      static_cast<long long>(g.nViews) * g.getNAxialCoords(seg) * nTang;
+    const int k = seg + g.segOffset;
+    if (k > 0 && data[k] != data[k - 1] + counts->back()) one = false;
+    counts->push_back(n);
+  }
+  return one;
+}
+
+void shortRead(long long got, long long expected)
+{
+  error("The number of bins that were read from the data file (", got,
+        ") is inferior to that expected from the header file (", expected,
+        ")");
+}
+}
+
+void ProjInterfileReader::readData(types::BinValue** data,
+                                   const streamio::ChunkHook& landed)
+{
+  std::vector<long long> counts;
+  const long long nBins = mGeometry.nBins;
+  const size_t fb = sizeof(types::BinValue);
+  if (contiguous(mGeometry, mHeader.nTangCoords, data, &counts))
+  {
+    const size_t got =
+      streamio::readFile(mDataFileName, 0, data[0], nBins * fb, landed);
+    if (got < nBins * fb) shortRead(static_cast<long long>(got / fb), nBins);
+    return;
+  }
+  size_t at = 0;
+  for (size_t k = 0; k < counts.size(); ++k)
+  {
+    const size_t bytes = counts[k] * fb;
+    const size_t got = streamio::readFile(mDataFileName, at, data[k], bytes);
+    if (got < bytes) shortRead(static_cast<long long>((at + got) / fb), nBins);
+    at += bytes;
   }
 }
 
 void ProjInterfileReader::writeProjInterfile(const std::string& outputProjFile,
                                              const ProjHeader& h,
-                                             types::BinValue** data)
+                                             types::BinValue** data,
+                                             const streamio::ChunkHook& ready)
 {
   ProjGeometry g;
   g.fill(h);
@@ -66,13 +100,16 @@ void ProjInterfileReader::writeProjInterfile(const std::string& outputProjFile,
     writeKey(os, "number of tangential coordinates", h.nTangCoords);
     writeKey(os, "!END OF PROJECTION DATA PARAMETERS");
   }
+  std::vector<long long> counts;
+  const size_t fb = sizeof(types::BinValue);
+  if (contiguous(g, h.nTangCoords, data, &counts))
+  {
+    streamio::writeFile(dataPath.string(), data[0], g.nBins * fb, ready);
+    return;
+  }
   std::ofstream os(dataPath, std::ios::binary);
   if (!os.is_open()) error("Couldn't create file ", dataPath.string());
-  for (int seg = -g.segOffset; seg <= g.segOffset; ++seg)
-  {
-    const long long n =
-      static_cast<long long>(g.nViews) * g.getNAxialCoords(seg) * h.nTangCoords;
-    os.write(reinterpret_cast<const char*>(data[seg + g.segOffset]),
-             static_cast<std::streamsize>(n * sizeof(types::BinValue)));
-  }
+  for (size_t k = 0; k < counts.size(); ++k)
+    os.write(reinterpret_cast<const char*>(data[k]),
+             static_cast<std::streamsize>(counts[k] * fb));
 }

@@ -2,6 +2,8 @@
 #include <ScannerData.h>
 #include <console.h>
 #include <device.h>
+#include <ProjData.h>
+#include <ProjInterfileReader.h>
 
 #include <cstdlib>
 #include <cstring>
@@ -89,6 +91,40 @@ void Buffer::download(void* host, size_t bytes) const
 {
   if (bytes > mBytes) error("device buffer too small for download");
   if (bytes) check(fir_memcpy_d2h(context(), host, mPtr, bytes));
+}
+
+Buffer readProj(const std::string& headerFile, ProjData& proj)
+{
+  proj.read(headerFile, ProjData::ConstructionMode::ALLOCATE);
+  const size_t bytes =
+    static_cast<size_t>(proj.getGeometry().nBins) * sizeof(types::BinValue);
+  Buffer dev(bytes);
+  fir_ctx* c = context();
+  char* host = reinterpret_cast<char*>(proj.getContiguousData());
+  char* d = static_cast<char*>(dev.get());
+  ProjInterfileReader reader(headerFile);
+  reader.readData(proj.getDataArray(), [&](size_t off, size_t n) {
+    check(fir_memcpy_h2d_async(c, d + off, host + off, n));
+  });
+  check(fir_synchronize(c));
+  return dev;
+}
+
+void writeProj(const Buffer& dev, ProjData& proj,
+               const std::string& outputProjFile)
+{
+  const size_t bytes =
+    static_cast<size_t>(proj.getGeometry().nBins) * sizeof(types::BinValue);
+  if (dev.size() < bytes) error("device sinogram too small for the header");
+  fir_ctx* c = context();
+  char* host = reinterpret_cast<char*>(proj.getContiguousData());
+  const char* d = static_cast<const char*>(dev.get());
+  ProjInterfileReader::writeProjInterfile(
+    outputProjFile, proj.getHeader(), proj.getDataArray(),
+    [&](size_t off, size_t n) {
+      check(fir_memcpy_d2h_async(c, host + off, d + off, n));
+      check(fir_synchronize(c));
+    });
 }
 
 fir_vol_desc describe(const VolHeader& h)

@@ -49,27 +49,20 @@ struct CacheHolder
   }
 };
 
-// Shared body of OSEM and OSEM_ResoReco. With reso (reconAlgos.cc:203-367):
-// every sensitivity frame is convolved once (and written back to sensOut),
-// and each sub-iteration convolves the backprojection and updates
-// x /= sens_s, x *= bp (clamped) instead of bp /= sens_s, x *= bp.
-void osemImpl(const ProjData& inputProj, const ScannerData& scanner,
+// Shared body of OSEM and OSEM_ResoReco on device-resident data. With reso
+// (reconAlgos.cc:203-367): every sensitivity frame is convolved once (in
+// place, in.sens), and each sub-iteration convolves the backprojection and
+// updates x /= sens_s, x *= bp (clamped) instead of bp /= sens_s,
+// x *= bp.
+void osemCore(const ProjData& inputProj, const ScannerData& scanner,
               VolData& outputVol, const std::string& outputVolFileName,
-              const OSEMCoreParams& p, const VolData& sens,
-              const std::optional<ProjData>& biasProj, bool reso,
-              VolData* sensOut)
+              const OSEMCoreParams& p, const DeviceOSEMInputs& in, bool reso)
 {
   scanner.checkProjData(inputProj);
   inputProj.checkNSubsets(p.nSubsets);
-  if (sens.getNFrames() != p.nSubsets)
-    error("Wrong number of frames (expected ", p.nSubsets, ")");
-  if (sens.getHeader() != outputVol.getHeader())
-    error("Sensitivity volume provided doesn't fit with output volume "
-          "provided");
-  if (biasProj && biasProj->getHeader() != inputProj.getHeader())
-    error("Bias projection doesn't fit with input projection");
   if (p.fwhmXYZ.size() != 3)
     error("\"convolution FHWM XYZ in mm\" must contain three elements");
+  if (!in.y || !in.sens) error("OSEM: device sinogram or sensitivity missing");
 
   const bool convolveFlag = p.convolutionInterval > 0 && p.fwhmXYZ[0] > 0.0 &&
     p.fwhmXYZ[1] > 0.0 && p.fwhmXYZ[2] > 0.0;
@@ -78,27 +71,23 @@ void osemImpl(const ProjData& inputProj, const ScannerData& scanner,
   fir_ctx* ctx = device::context();
   const fir_vol_desc v = device::describe(outputVol.getHeader());
   const size_t nv = static_cast<size_t>(outputVol.getNVoxelsPerFrame());
-  const size_t nb = static_cast<size_t>(inputProj.getGeometry().nBins);
   const size_t vb = nv * sizeof(float);
 
-  device::Buffer x(vb), bp(vb), y(nb * sizeof(float));
-  device::Buffer s(vb * p.nSubsets), bias;
-  x.upload(outputVol.getDataArray(), vb);
-  y.upload(inputProj.getContiguousData(), nb * sizeof(float));
-  s.upload(sens.getAllFrames(), vb * p.nSubsets);
-  if (biasProj)
+  struct DevView
   {
-    bias = device::Buffer(nb * sizeof(float));
-    bias.upload(biasProj->getContiguousData(), nb * sizeof(float));
-  }
+    const float* p;
+    const float* f() const { return p; }
+  } y{in.y}, bias{in.bias};
+  struct SensView
+  {
+    float* p;
+    float* f() const { return p; }
+  } s{in.sens};
+  const bool hasBias = in.bias != nullptr;
+  device::Buffer x(vb), bp(vb);
+  x.upload(outputVol.getDataArray(), vb);
   device::check(fir_fill(ctx, bp.f(), static_cast<int64_t>(nv), 0.0f));
   device::check(fir_cut_circle(ctx, &v, x.f(), 1, p.cutRadius));
-  if (reso)
-  {
-    device::check(fir_convolve(ctx, &v, s.f(), p.nSubsets, p.fwhmXYZ.data(),
-                               p.cutRadius));
-    if (sensOut) s.download(sensOut->getAllFrames(), vb * p.nSubsets);
-  }
 
   CacheHolder cached;
   int nCached = 0;
@@ -128,19 +117,37 @@ void osemImpl(const ProjData& inputProj, const ScannerData& scanner,
       print("The path cache holds subsets 1 to ", nCached, " of ", p.nSubsets,
             "; the others are walked on the fly");
   }
+  if (in.computeSens)
+  {
+    // the sensitivity of the projector the iterations use: the cached
+    // subsets' frames from the cache, the others from the walk
+    PhaseTimer tSens("sensitivity");
+    echo("Sensitivity (GPU)");
+    if (nCached < p.nSubsets)
+      device::check(fir_sensitivity(ctx, &v, s.f(), p.nSubsets));
+    if (cached.cache)
+      device::check(fir_cache_sensitivity(cached.cache, s.f()));
+    if (in.sensOut)
+      device::check(fir_memcpy_d2h(ctx, in.sensOut->getAllFrames(), s.f(),
+                                   vb * p.nSubsets));
+    device::check(fir_synchronize(ctx));
+  }
+  if (reso)
+    device::check(fir_convolve(ctx, &v, s.f(), p.nSubsets, p.fwhmXYZ.data(),
+                               p.cutRadius));
   auto projector = [&](int subset) {
     if (cached.cache && subset < nCached)
     {
       const float* xs[1] = {x.f()};
       const float* ys[1] = {y.f()};
-      const float* bs[1] = {biasProj ? bias.f() : nullptr};
+      const float* bs[1] = {hasBias ? bias.f() : nullptr};
       float* bps[1] = {bp.f()};
       device::check(
         fir_cache_osem_project(cached.cache, 1, xs, ys, bs, bps, subset));
     }
     else
       device::check(fir_osem_project(ctx, &v, x.f(), y.f(),
-                                     biasProj ? bias.f() : nullptr, bp.f(),
+                                     hasBias ? bias.f() : nullptr, bp.f(),
                                      subset, p.nSubsets));
   };
 
@@ -186,7 +193,7 @@ void osemImpl(const ProjData& inputProj, const ScannerData& scanner,
       }
       else
         device::check(fir_osem_subiter(
-          ctx, &v, x.f(), y.f(), biasProj ? bias.f() : nullptr, sens_s,
+          ctx, &v, x.f(), y.f(), hasBias ? bias.f() : nullptr, sens_s,
           bp.f(), subset, p.nSubsets, conv, p.fwhmXYZ.data(), p.cutRadius,
           reset));
       if (p.saveInterval > 0 && subiter % p.saveInterval == 0 &&
@@ -200,6 +207,43 @@ void osemImpl(const ProjData& inputProj, const ScannerData& scanner,
   }
   x.download(outputVol.getDataArray(), vb);
 }
+
+// The reference API's entry: host sensitivity and sinograms, uploaded; the
+// convolved sensitivity of OSEM_ResoReco is written back to sensOut.
+void osemImpl(const ProjData& inputProj, const ScannerData& scanner,
+              VolData& outputVol, const std::string& outputVolFileName,
+              const OSEMCoreParams& p, const VolData& sens,
+              const std::optional<ProjData>& biasProj, bool reso,
+              VolData* sensOut)
+{
+  scanner.checkProjData(inputProj);
+  inputProj.checkNSubsets(p.nSubsets);
+  if (sens.getNFrames() != p.nSubsets)
+    error("Wrong number of frames (expected ", p.nSubsets, ")");
+  if (sens.getHeader() != outputVol.getHeader())
+    error("Sensitivity volume provided doesn't fit with output volume "
+          "provided");
+  if (biasProj && biasProj->getHeader() != inputProj.getHeader())
+    error("Bias projection doesn't fit with input projection");
+  const size_t vb =
+    static_cast<size_t>(outputVol.getNVoxelsPerFrame()) * sizeof(float);
+  const size_t yb =
+    static_cast<size_t>(inputProj.getGeometry().nBins) * sizeof(float);
+  device::Buffer y(yb), s(vb * p.nSubsets), bias;
+  y.upload(inputProj.getContiguousData(), yb);
+  s.upload(sens.getAllFrames(), vb * p.nSubsets);
+  if (biasProj)
+  {
+    bias = device::Buffer(yb);
+    bias.upload(biasProj->getContiguousData(), yb);
+  }
+  DeviceOSEMInputs in;
+  in.y = y.f();
+  in.bias = biasProj ? bias.f() : nullptr;
+  in.sens = s.f();
+  osemCore(inputProj, scanner, outputVol, outputVolFileName, p, in, reso);
+  if (sensOut) s.download(sensOut->getAllFrames(), vb * p.nSubsets);
+}
 }
 
 void OSEM(const ProjData& inputProj, const ScannerData& scanner,
@@ -210,6 +254,15 @@ void OSEM(const ProjData& inputProj, const ScannerData& scanner,
   echo("OSEM:");
   osemImpl(inputProj, scanner, outputVol, outputVolFileName, p, sens,
            biasProj, false, nullptr);
+}
+
+void OSEMOnDevice(const ProjData& inputProj, const ScannerData& scanner,
+                  VolData& outputVol, const std::string& outputVolFileName,
+                  const OSEMCoreParams& p, const DeviceOSEMInputs& in,
+                  bool reso)
+{
+  echo(reso ? "OSEM_ResoReco:" : "OSEM:");
+  osemCore(inputProj, scanner, outputVol, outputVolFileName, p, in, reso);
 }
 
 void OSEM_ResoReco(const ProjData& inputProj, const ScannerData& scanner,

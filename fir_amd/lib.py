@@ -41,7 +41,8 @@ EXPORTS = (
     "fir_last_error", "fir_abi_version", "fir_ctx_create", "fir_ctx_destroy",
     "fir_set_stream", "fir_synchronize", "fir_set_scanner", "fir_set_proj",
     "fir_proj_n_bins", "fir_set_walk_mode", "fir_malloc", "fir_free",
-    "fir_memcpy_h2d", "fir_memcpy_d2h", "fir_host_alloc", "fir_host_free",
+    "fir_memcpy_h2d", "fir_memcpy_d2h", "fir_memcpy_h2d_async",
+    "fir_memcpy_d2h_async", "fir_host_alloc", "fir_host_free",
     "fir_fill", "fir_forward",
     "fir_forward_subset", "fir_backward", "fir_backward_subset",
     "fir_sensitivity", "fir_osem_project", "fir_osem_update",

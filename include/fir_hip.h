@@ -152,6 +152,15 @@ int fir_malloc(fir_ctx* ctx, void** dptr, size_t bytes);
 int fir_free(fir_ctx* ctx, void* dptr);
 int fir_memcpy_h2d(fir_ctx* ctx, void* dst, const void* src, size_t bytes);
 int fir_memcpy_d2h(fir_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* The same copies enqueued on the context's stream without waiting for
+ * them (fir_synchronize waits). The host memory must stay valid and
+ * unchanged (h2d) or unread (d2h) until then; from page-locked memory
+ * (fir_host_alloc) the copy runs as DMA while the host goes on, e.g.
+ * reading the next chunk of a file. No counterpart in the reference. */
+int fir_memcpy_h2d_async(fir_ctx* ctx, void* dst, const void* src,
+                         size_t bytes);
+int fir_memcpy_d2h_async(fir_ctx* ctx, void* dst, const void* src,
+                         size_t bytes);
 /* Page-locked host memory (hipHostMalloc): copies to and from it run as
  * direct DMA instead of through a pageable staging copy. No counterpart in
  * the reference (its containers malloc their data, ProjData.cc:623-651,
