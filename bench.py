@@ -351,8 +351,9 @@ def main():
     ap.add_argument("--batch", type=int, default=8,
                     help="reconstructions in the batched measurement (0: skip)")
     ap.add_argument("--traffic-probe", action="store_true",
-                    help="run only one projector launch of each kind and a "
-                         "1 GiB device copy (for rocprofv3 --pmc runs)")
+                    help="run only the cached projector launch (twice), "
+                         "one walked projector launch and a 1 GiB device "
+                         "copy (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.workload == "uneven":
         if args.gpus != 1:
@@ -446,6 +447,9 @@ def main():
         cache.osem_project([x], [y], None, [bp], 3)
         torch.cuda.synchronize()
         cache.osem_project([x], [y], None, [bp], 3)
+        # the walked projector's launch of the same subset (FP64 counters)
+        bp.zero_()
+        ctx.osem_project(v, x, y, None, bp, 3, n_sub)
         src = torch.ones(1 << 28, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
