@@ -572,15 +572,15 @@ def main():
             yi = torch.poisson((mean_f * float(sc)), generator=g2)
             ctx.mul_clamped(yi, acf)
             ys.append(yi)
-        # batches of 8 interleaved images gather their line integrals from
-        # interleaved image copies, best with 2 interleaved z-slices and 8
-        # lanes per LOR, and backproject family by family from the sheet
-        # form's lists: build that cache in place of the single-image one
-        # (same row order)
+        # batches of 8 images take their line integrals from slab-staged
+        # sheets, best with 4 lanes per LOR, and backproject family by
+        # family from the sheet form's lists: build that cache in place of
+        # the single-image one (same row order)
         bz = 2
+        blanes = (4, 16)
         cache.close()
         ctx.set_cache_zslices(bz)
-        ctx.set_cache_lanes(8, 16)
+        ctx.set_cache_lanes(*blanes)
         cache = ctx.path_cache(v, n_sub, firlib.FIR_LEN_SNORM16,
                                row_order=tres.best["row_order"])
         grp = recon.OSEMGroup(ctx, v, n_sub, ys, sens=sens_c, fwhm=fwhm,
@@ -596,11 +596,12 @@ def main():
         bms = e0.elapsed_time(e1) / args.steps
         batched = {
             "n_images": args.batch,
-            "path": "cached: line integrals gathered from interleaved image "
-                    "copies, family backprojection of all images from the "
-                    "sheet form's lists; one HIP graph per iteration",
-            "cache_zslices": bz,
-            "cache_lanes": [8, 16],
+            "path": "cached: line integrals from each family's sheet of "
+                    "all images staged in LDS slab by slab, family "
+                    "backprojection of all images from the sheet form's "
+                    "lists; one HIP graph per iteration",
+            "cache_forms": cache.forms(),
+            "cache_lanes": list(blanes),
             "osem_iter_ms": round(bms, 4),
             "osem_iter_ms_per_recon": round(bms / args.batch, 4),
             "mlors_s_per_recon_aggregate": round(

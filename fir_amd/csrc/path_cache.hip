@@ -1082,14 +1082,91 @@ __global__ void k_sh_pixels(const unsigned long long* __restrict__ bits,
   }
 }
 
+// The sheet pixels of every family in the order its rows cross them: by
+// the position of the pixel's centre along the family's line, from the
+// first crystal to the second (every row of the family has the same two
+// crystals in the xy-plane and walks from the first, Siddon.cc:112-412).
+// A walk steps to a neighbouring pixel in the direction of the line, which
+// moves the centre forward along it, so every row visits its pixels in
+// rising rank, and each lane's elements (the quad interleave keeps a
+// lane's elements in path order) rise too. One block per family: pix_rank
+// gets, for the k-th pixel in pixel order, its rank along the line, and
+// fam_xy is rewritten in line order. A family holds at most max_px pixels
+// (the dynamic LDS: a double and an int each).
+__global__ __launch_bounds__(kBlock) void k_sh_order(
+  LorGeom lg, Grid g, int n_tang, const int* __restrict__ fam_off,
+  int* __restrict__ fam_xy, unsigned short* __restrict__ pix_rank)
+{
+  extern __shared__ double key[];
+  const int f = blockIdx.x;
+  const int p0 = fam_off[f], L = fam_off[f + 1] - p0;
+  int* px = reinterpret_cast<int*>(key + L);
+  const int view = f / n_tang, tt = f - view * n_tang;
+  const int2 c = fir::view_crystals(lg, view, tt);
+  const double2 a = lg.xy[c.x], b = lg.xy[c.y];
+  const double dx = b.x - a.x, dy = b.y - a.y;
+  for (int i = threadIdx.x; i < L; i += kBlock)
+  {
+    const int v = fam_xy[p0 + i];
+    const int x = v % g.row, y = v / g.row;
+    const double cx = g.low[0] + (x + 0.5) * g.step[0];
+    const double cy = g.low[1] + (y + 0.5) * g.step[1];
+    key[i] = (cx - a.x) * dx + (cy - a.y) * dy;
+    px[i] = v;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < L; i += kBlock)
+  {
+    const double ki = key[i];
+    int k = 0;
+    for (int j = 0; j < L; ++j)
+      k += (key[j] < ki || (key[j] == ki && j < i)) ? 1 : 0;
+    pix_rank[p0 + i] = (unsigned short)k;
+    fam_xy[p0 + k] = px[i];
+  }
+}
+
+// Whether every lane's contributing elements (lanes = the quad layout's)
+// have non-decreasing sheet pixels: clears *ok otherwise.
+__global__ __launch_bounds__(kBlock) void k_sh_check_order(
+  int n_rows, const long long* __restrict__ off,
+  const unsigned* __restrict__ sel, int lanes, int nz, int* __restrict__ ok)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(t / lanes);
+  if (r >= n_rows) return;
+  const int j = (int)(t % lanes);
+  int prev = 0;
+  const long long end = off[r + 1];
+  for (long long e = off[r] + 4LL * j; e < end; e += 4LL * lanes)
+  {
+    const uint4 q = *reinterpret_cast<const uint4*>(sel + e);
+    const unsigned u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+      if ((u[k] >> 16) == 0) continue;
+      const int p = (int)(u[k] & 0xffffu) / nz;
+      if (p < prev)
+      {
+        atomicAnd(ok, 0);
+        return;
+      }
+      prev = p;
+    }
+  }
+}
+
 // Sheet index of every slot: pixel rank * nz + z (0 for the padding and
-// zero-length elements, which contribute nothing).
+// zero-length elements, which contribute nothing). The rank is the pixel's
+// position in pixel order, or, with pix_rank, its rank along the line.
 __global__ __launch_bounds__(kBlock) void k_sh_encode(
   int n_rows, const int* __restrict__ fam_row, int n_fam,
   const long long* __restrict__ off, const int* __restrict__ idx,
   const void* __restrict__ len, int lb, const unsigned char* __restrict__ yrow,
   XY xy, int nz, int words, const unsigned long long* __restrict__ bits,
-  const unsigned short* __restrict__ wpre, unsigned* __restrict__ sel)
+  const unsigned short* __restrict__ wpre, const int* __restrict__ fam_off,
+  const unsigned short* __restrict__ pix_rank, unsigned* __restrict__ sel)
 {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int r = (int)(t / kTrLanes);
@@ -1107,8 +1184,8 @@ __global__ __launch_bounds__(kBlock) void k_sh_encode(
     {
       const int px = v % nxy, z = v / nxy;
       const long long wi = (long long)f * words + px / 64;
-      const int rank =
-        wpre[wi] + __popcll(bits[wi] & ((1ULL << (px % 64)) - 1ULL));
+      int rank = wpre[wi] + __popcll(bits[wi] & ((1ULL << (px % 64)) - 1ULL));
+      if (pix_rank) rank = pix_rank[fam_off[f] + rank];
       out = (unsigned)(rank * nz + z) |
         ((unsigned)((const unsigned short*)len)[e] << 16);
     }
@@ -1250,6 +1327,255 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_rows_sheet(
       if (io.bias[0]) line += io.bias[0][bin];
       if (line > kEpsF) out = io.y[0][bin] / line * fct;
       w[r] = out;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched line integrals over the sheet form, slab by slab
+// ---------------------------------------------------------------------------
+// The sheet of NB images (NB floats per sheet voxel) does not fit in LDS, so
+// a family's block stages it in slabs of consecutive pixels (all slices).
+// With the pixels ranked along the family's line, each lane meets its
+// elements in rising pixel rank, so a slab's elements are, for every lane
+// and row, a run that continues where the previous slab's stopped: a lane
+// keeps one quad per row in hand, sums the elements that fall in the slab
+// and moves on while the quad holds nothing beyond it. Each lane sums its
+// elements in the order sheet_sum does (the zero-length ones, which add
+// nothing there, are skipped), so every image's line integral is the
+// one-image sheet pass's bit for bit.
+constexpr int kSlabBlockMax = 1024;
+
+// Pixel-major copies of NB images for the slab staging:
+// X[(pixel * nz + z) * NB + b]. A block transposes a tile of kPzPx pixels
+// by kPzZ slices of every image through LDS: its loads read runs of kPzPx
+// pixels, its stores runs of kPzZ * NB floats per pixel.
+constexpr int kPzPx = 64, kPzZ = 4;
+template <int NB>
+__global__ __launch_bounds__(256) void k_pc_pack_pz(float* __restrict__ X,
+                                                    ImgIn in, int n_images,
+                                                    int nxy, int nz)
+{
+  constexpr int kPitch = kPzPx + 1;
+  constexpr int kN = kPzPx * kPzZ * NB; // floats of the tile
+  __shared__ float tile[kPzZ * NB * kPitch];
+  const int px0 = blockIdx.x * kPzPx, z0 = blockIdx.y * kPzZ;
+  float v[kN / 256];
+#pragma unroll
+  for (int u = 0; u < kN / 256; ++u)
+  {
+    const int i = (int)threadIdx.x + 256 * u;
+    const int p = i % kPzPx, r = i / kPzPx;
+    const int b = r % NB, z = z0 + r / NB;
+    v[u] = 0.0f;
+    if (px0 + p < nxy && z < nz && b < n_images)
+      v[u] = in.p[b][(long long)z * nxy + px0 + p];
+  }
+#pragma unroll
+  for (int u = 0; u < kN / 256; ++u)
+  {
+    const int i = (int)threadIdx.x + 256 * u;
+    tile[(i / kPzPx) * kPitch + i % kPzPx] = v[u];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kN / 256; ++u)
+  {
+    const int i = (int)threadIdx.x + 256 * u;
+    const int b = i % NB, r = i / NB;
+    const int zz = r % kPzZ, p = r / kPzZ;
+    if (px0 + p < nxy && z0 + zz < nz)
+      X[((long long)(px0 + p) * nz + z0 + zz) * NB + b] =
+        tile[(zz * NB + b) * kPitch + p];
+  }
+}
+
+template <int NB>
+__device__ __forceinline__ void slab_fma(const float* __restrict__ xp, float l,
+                                         float* acc)
+{
+  if constexpr (NB == 8)
+  {
+    const float4 a = reinterpret_cast<const float4*>(xp)[0];
+    const float4 c = reinterpret_cast<const float4*>(xp)[1];
+    acc[0] = fmaf(l, a.x, acc[0]);
+    acc[1] = fmaf(l, a.y, acc[1]);
+    acc[2] = fmaf(l, a.z, acc[2]);
+    acc[3] = fmaf(l, a.w, acc[3]);
+    acc[4] = fmaf(l, c.x, acc[4]);
+    acc[5] = fmaf(l, c.y, acc[5]);
+    acc[6] = fmaf(l, c.z, acc[6]);
+    acc[7] = fmaf(l, c.w, acc[7]);
+  }
+  else
+  {
+    const float4 a = reinterpret_cast<const float4*>(xp)[0];
+    acc[0] = fmaf(l, a.x, acc[0]);
+    acc[1] = fmaf(l, a.y, acc[1]);
+    acc[2] = fmaf(l, a.z, acc[2]);
+    acc[3] = fmaf(l, a.w, acc[3]);
+  }
+}
+
+// Stage slab `sb` (pixels [sb * slab_px, ...) of the family's sheet, all
+// slices, NB floats per voxel) into buf with direct global-to-LDS loads:
+// each wave instruction writes 64 consecutive float4 of the buffer, lane j
+// the j-th, fetched from that pixel's run in X. Returns nothing to wait on
+// in registers; the caller waits on the vector memory counter.
+__device__ __forceinline__ void slab_stage(const float4* __restrict__ X4,
+                                           const int* __restrict__ fam_xy,
+                                           int p0, int L, int slab_px, int n4,
+                                           int sb, float4* buf)
+{
+  const int pb = sb * slab_px;
+  const int ns = min(slab_px, L - pb) * n4;
+  const int wave = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
+  const int step = (int)(blockDim.x >> 6) * 64;
+  for (int c = wave * 64; c < ns; c += step)
+  {
+    const int i = c + ln;
+    if (i < ns)
+    {
+      const int p = i / n4, o = i - p * n4;
+      __builtin_amdgcn_global_load_lds(
+        X4 + (long long)fam_xy[p0 + pb + p] * n4 + o, buf + c, 16, 0, 0);
+    }
+  }
+}
+
+// k_pc_osem_rows for NB = 4 or 8 images over the sheet form: one block per
+// family of the subset (blockIdx.y: the subset's k-th view, blockIdx.x: the
+// tangential bin), blockDim.x = lanes * rows per round; thread t takes rows
+// r0 + t / lanes + k * (blockDim.x / lanes), k < R (the host checks that R
+// rounds cover the family), and keeps D quads of each in hand. X holds the
+// images pixel-major (k_pc_pack_pz); slabs of slab_px pixels (all slices)
+// are staged alternately into two LDS buffers of buf_f4 float4 each, the
+// next slab's loads in flight while the current one is summed.
+template <int F, int NB, int R, int D>
+__global__ __launch_bounds__(kSlabBlockMax) void k_pc_osem_rows_slab(
+  int n_subsets, int subset, int n_tang, int n_fam,
+  const int* __restrict__ fam_row, const int* __restrict__ fam_off,
+  const int* __restrict__ fam_xy, const unsigned* __restrict__ sel,
+  RowData rd, int lanes, const float* __restrict__ X, int nz, int slab_px,
+  int buf_f4, OsemRowsIO io, float* __restrict__ w)
+{
+  extern __shared__ float4 slab[];
+  const int view = (int)blockIdx.y * n_subsets + subset;
+  const int f = view * n_tang + (int)blockIdx.x;
+  if (f >= n_fam) return;
+  const int r0 = fam_row[f], r1 = fam_row[f + 1];
+  if (r0 == r1) return;
+  const int p0 = fam_off[f];
+  const int L = fam_off[f + 1] - p0;
+  const int n4 = nz * NB / 4; // float4 per staged pixel
+  const float4* X4 = reinterpret_cast<const float4*>(X);
+  slab_stage(X4, fam_xy, p0, L, slab_px, n4, 0, slab);
+  const int lane = (int)threadIdx.x & (lanes - 1);
+  const int per_round = (int)blockDim.x / lanes;
+  const long long step = 4LL * lanes;
+  long long e[R], e1[R];
+  uint4 q[R][D];
+  float acc[R][NB];
+#pragma unroll
+  for (int k = 0; k < R; ++k)
+  {
+    const int r = r0 + (int)threadIdx.x / lanes + k * per_round;
+    e[k] = 0;
+    e1[k] = 0;
+    if (r < r1)
+    {
+      e[k] = rd.off[r] + 4LL * lane;
+      e1[k] = rd.off[r + 1];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      q[k][d] = e[k] + d * step < e1[k]
+        ? *reinterpret_cast<const uint4*>(sel + e[k] + d * step)
+        : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[k][b] = 0.0f;
+  }
+  const int n_slabs = (L + slab_px - 1) / slab_px;
+  for (int sb = 0; sb < n_slabs; ++sb)
+  {
+    // this slab's loads have landed everywhere, and every thread is done
+    // with the previous slab (whose buffer the next slab's loads overwrite)
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (sb + 1 < n_slabs)
+      slab_stage(X4, fam_xy, p0, L, slab_px, n4, sb + 1,
+                 slab + ((sb + 1) & 1) * buf_f4);
+    const int pb = sb * slab_px;
+    const unsigned lo = (unsigned)(pb * nz);
+    const unsigned hi = (unsigned)((pb + min(slab_px, L - pb)) * nz);
+    const float* sl = reinterpret_cast<const float*>(slab + (sb & 1) * buf_f4);
+    // a row whose quad holds elements beyond the slab is parked (its
+    // elements in the slab are summed once) until the next slab
+    bool park[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) park[k] = false;
+    for (;;)
+    {
+      bool more = false;
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+      {
+        if (e[k] < e1[k] && !park[k])
+        {
+          const uint4 qq = q[k][0];
+          float l[4];
+          Quad<F>::dec(sheet_lengths(qq), l);
+          const unsigned u4[4] = {qq.x, qq.y, qq.z, qq.w};
+          bool done = true;
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+          {
+            const unsigned si = u4[u] & 0xffffu;
+            if ((u4[u] >> 16) != 0 && si >= lo)
+            {
+              if (si >= hi)
+                done = false;
+              else
+                slab_fma<NB>(sl + (si - lo) * NB, l[u], acc[k]);
+            }
+          }
+          park[k] = !done;
+          if (done)
+          {
+            e[k] += step;
+#pragma unroll
+            for (int d = 0; d + 1 < D; ++d) q[k][d] = q[k][d + 1];
+            const long long en = e[k] + (D - 1) * step;
+            if (en < e1[k])
+              q[k][D - 1] = *reinterpret_cast<const uint4*>(sel + en);
+            more = more || e[k] < e1[k];
+          }
+        }
+      }
+      if (!more) break;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k)
+  {
+    lane_reduce<NB>(acc[k], lanes);
+    const int r = r0 + (int)threadIdx.x / lanes + k * per_round;
+    if (r < r1 && lane == 0)
+    {
+      const float fc = rd.factor[r];
+      const int bin = rd.bin[r];
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+      {
+        float out = 0.0f;
+        if (b < io.n_images)
+        {
+          float line = acc[k][b] * fc;
+          if (io.bias[b]) line += io.bias[b][bin];
+          if (line > kEpsF) out = io.y[b][bin] / line * fc;
+        }
+        w[(long long)r * NB + b] = out;
+      }
     }
   }
 }
@@ -1815,6 +2141,9 @@ struct RowMatrix
   int* d_fam_off = nullptr;         // [n_fam + 1] first pixel of each sheet
   int* d_fam_xy = nullptr;          // sheet pixels (x + y nx)
   unsigned* d_sel = nullptr;        // [n_slots] sheet index | length << 16
+  // 1: sheet pixels ranked along each family's line, and every lane of the
+  // quad layout meets its elements in rising pixel rank (checked at build)
+  int line_order = 0;
   long long sheet_bytes = 0;
   long long n_sheet_px = 0;         // sheet pixels over all families
   // Per sheet entry (pixel, slice) of every family, the family's elements
@@ -1902,6 +2231,9 @@ struct fir_cache
   float* d_part = nullptr;
   long long fr_bytes = 0;
   int family_bp = 1;
+  // Batches of 4 or 8 images take their line integrals from slab-staged
+  // sheets (when the matrix's sheet pixels are in line order).
+  int slabs = 1;
 
   ~fir_cache()
   {
@@ -2189,11 +2521,44 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
   hipLaunchKernelGGL(k_sh_pixels, dim3(blocks_for(F)), dim3(kBlock), 0,
                      c->stream, bits, F, words, rm->d_fam_off, rm->d_fam_xy);
   FIR_CHECK_LAUNCH();
+  // Pixels ranked along each family's line (the batches' slab staging
+  // needs it; one-image passes give the same sums in either order).
+  unsigned short* pix_rank = nullptr;
+  const size_t order_lds = (size_t)max_px * (sizeof(double) + sizeof(int));
+  Grid g;
+  if ((rc = make_grid(c, &v, &g))) return rc;
+  const int n_tang = F / rm->n_views;
+  if (order_lds <= 65536 && F % rm->n_views == 0)
+  {
+    if ((rc = tmp.get(&pix_rank, (size_t)std::max(off[F], 1),
+                      "sheet pixel ranks")))
+      return rc;
+    hipLaunchKernelGGL(k_sh_order, dim3(F), dim3(kBlock), order_lds,
+                       c->stream, lor_geom_views(c), g, n_tang,
+                       rm->d_fam_off, rm->d_fam_xy, pix_rank);
+    FIR_CHECK_LAUNCH();
+  }
   hipLaunchKernelGGL(k_sh_encode, dim3(grid_rows), dim3(kBlock), 0, c->stream,
                      rm->n_rows, rm->d_fam_row, F, rm->d_off, rm->d_idx,
                      rm->d_len, lb, rm->d_yrow, xy, nz, words, bits, wpre,
-                     rm->d_sel);
+                     rm->d_fam_off, pix_rank, rm->d_sel);
   FIR_CHECK_LAUNCH();
+  rm->line_order = 0;
+  if (pix_rank)
+  {
+    int* ok;
+    if ((rc = tmp.get(&ok, 1, "sheet order flag"))) return rc;
+    const int one = 1;
+    FIR_HIP_CHECK(hipMemcpyAsync(ok, &one, sizeof(int), hipMemcpyHostToDevice,
+                                 c->stream));
+    hipLaunchKernelGGL(k_sh_check_order,
+                       dim3(blocks_for((long long)rm->n_rows * rm->lanes)),
+                       dim3(kBlock), 0, c->stream, rm->n_rows, rm->d_off,
+                       rm->d_sel, rm->lanes, nz, ok);
+    FIR_CHECK_LAUNCH();
+    FIR_HIP_CHECK(hipMemcpyAsync(&rm->line_order, ok, sizeof(int),
+                                 hipMemcpyDeviceToHost, c->stream));
+  }
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
   rm->sheet_bytes = 4LL * (F + 1) + 4LL * off[F] + 4LL * rm->n_slots;
   rm->n_sheet_px = off[F];
@@ -3048,6 +3413,134 @@ void launch_fam_bp(fir_cache* cache, int s, int nb, const float* w,
   }
 }
 
+// Rows per thread of the slab kernel's instances.
+constexpr int kSlabRounds[] = {1, 2, 3, 5};
+
+// Slab launch shape for a cache: rows per thread R (an instance) and the
+// block size (lanes * rows per round, whole waves), or R = 0 when a
+// family's rows do not fit in kSlabBlockMax threads times 5 rounds.
+struct SlabShape
+{
+  int R = 0;
+  int block = 0;
+  int slab_px = 0;
+  int buf_f4 = 0;  // float4 per LDS buffer (whole waves' worth)
+  size_t lds = 0;  // bytes of the two buffers
+};
+
+// Bytes of one of the two LDS slab buffers: FIR_SLAB_KB (8 to 80),
+// default 40 (two blocks of 4-lane rows fill a CU's 160 KB; 16 to 40 KB
+// measured within 10% of each other at 160 x 160 x 47 with 8 images).
+int slab_bytes()
+{
+  static const int kb = [] {
+    const char* e = std::getenv("FIR_SLAB_KB");
+    const int v = e ? std::atoi(e) : 0;
+    return v >= 8 && v <= 80 ? v : 40;
+  }();
+  return kb * 1024;
+}
+
+SlabShape slab_shape(const fir_cache* cache, int nb)
+{
+  SlabShape sh;
+  const RowMatrix* rm = cache->rm.get();
+  const int lanes = cache->fwd_lanes;
+  const int rows = rm->n_pairs;
+  for (int R : kSlabRounds)
+  {
+    const int per_round = (rows + R - 1) / R;
+    const int block = (per_round * lanes + 63) / 64 * 64;
+    if (block <= kSlabBlockMax)
+    {
+      sh.R = R;
+      sh.block = block;
+      break;
+    }
+  }
+  const int n4 = cache->vol.nz * nb / 4;
+  sh.slab_px = slab_bytes() / (n4 * 16);
+  sh.buf_f4 = (sh.slab_px * n4 + 63) / 64 * 64;
+  sh.lds = 2 * 16 * (size_t)sh.buf_f4;
+  if (sh.slab_px < 1) sh.R = 0;
+  return sh;
+}
+
+// Whether a batch of nb images (a launch width) takes the slab kernel.
+bool use_slabs(const fir_cache* cache, int nb)
+{
+  const RowMatrix* rm = cache->rm.get();
+  return (nb == 4 || nb == 8) && cache->slabs && use_family_bp(cache) &&
+    rm->line_order && cache->fwd_lanes == rm->lanes &&
+    slab_shape(cache, nb).R > 0;
+}
+
+// The pixel-major copies of a launch's images, in cache->d_x.
+void pack_images_pz(fir_cache* cache, int wd, const ImgIn& in, int nb)
+{
+  const int nxy = cache->vol.nx * cache->vol.ny, nz = cache->vol.nz;
+  const dim3 grid((nxy + kPzPx - 1) / kPzPx, (nz + kPzZ - 1) / kPzZ);
+  hipStream_t st = cache->ctx->stream;
+  if (wd == 8)
+    hipLaunchKernelGGL(k_pc_pack_pz<8>, grid, dim3(256), 0, st, cache->d_x,
+                       in, nb, nxy, nz);
+  else
+    hipLaunchKernelGGL(k_pc_pack_pz<4>, grid, dim3(256), 0, st, cache->d_x,
+                       in, nb, nxy, nz);
+}
+
+template <int F, int NB, int R, int D>
+void launch_rows_slab_r(fir_cache* cache, int s, const SlabShape& sh,
+                        const OsemRowsIO& io, float* w)
+{
+  const RowMatrix* rm = cache->rm.get();
+  const int S = cache->n_subsets;
+  const int vpr = fir::views_in_subset(rm->n_views, S, s);
+  const int n_tang = rm->n_fam / rm->n_views;
+  const int nz = cache->vol.nz;
+  auto kern = k_pc_osem_rows_slab<F, NB, R, D>;
+  if (sh.lds > 65536)
+  {
+    // (beyond the default dynamic LDS limit; set once per instance)
+    static size_t allowed = 0;
+    if (allowed < sh.lds &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)sh.lds) == hipSuccess)
+      allowed = sh.lds;
+  }
+  hipLaunchKernelGGL(kern, dim3(n_tang, vpr), dim3(sh.block), sh.lds,
+                     cache->ctx->stream, S, s, n_tang, rm->n_fam,
+                     rm->d_fam_row, rm->d_fam_off, rm->d_fam_xy, rm->d_sel,
+                     row_data(rm), cache->fwd_lanes, cache->d_x, nz,
+                     sh.slab_px, sh.buf_f4, io, w);
+}
+
+// (two quads of each row in hand: within 1-4% of one at 4 lanes)
+template <int F, int NB>
+void launch_rows_slab_nb(fir_cache* cache, int s, const SlabShape& sh,
+                         const OsemRowsIO& io, float* w)
+{
+  switch (sh.R)
+  {
+  case 1: launch_rows_slab_r<F, NB, 1, 2>(cache, s, sh, io, w); break;
+  case 2: launch_rows_slab_r<F, NB, 2, 2>(cache, s, sh, io, w); break;
+  case 3: launch_rows_slab_r<F, NB, 3, 2>(cache, s, sh, io, w); break;
+  default: launch_rows_slab_r<F, NB, 5, 2>(cache, s, sh, io, w); break;
+  }
+}
+
+// Batch line integrals of wd = 4 or 8 images from slab-staged sheets (the
+// images already packed pixel-major in cache->d_x).
+template <int F>
+void launch_rows_slab(fir_cache* cache, int s, int wd, const OsemRowsIO& io,
+                      float* w)
+{
+  const SlabShape sh = slab_shape(cache, wd);
+  if (wd == 8) launch_rows_slab_nb<F, 8>(cache, s, sh, io, w);
+  else launch_rows_slab_nb<F, 4>(cache, s, sh, io, w);
+}
+
 template <int F>
 void launch_osem_rows(fir_cache* cache, int s, int nb, const float* X,
                       const OsemRowsIO& io, float* w)
@@ -3285,7 +3778,8 @@ int fir_cache_forms(const fir_cache* cache, int* forms)
   if (!forms) return fail(FIR_E_ARG, "NULL output");
   const RowMatrix* rm = cache->rm.get();
   *forms = (rm->d_sel ? FIR_FORM_SHEETS : 0) |
-    (rm->d_sel && rm->d_fl && cache->d_part ? FIR_FORM_FAMILY_LISTS : 0);
+    (rm->d_sel && rm->d_fl && cache->d_part ? FIR_FORM_FAMILY_LISTS : 0) |
+    (use_slabs(cache, kMaxCacheBatch) ? FIR_FORM_SLABS : 0);
   return FIR_OK;
 }
 
@@ -3298,7 +3792,8 @@ int fir_cache_tune(fir_cache* cache, int fwd_lanes, int bp_lanes, int flags)
     return fail(FIR_E_ARG, "lanes per LOR / voxel must be a power of two "
                            "from 1 to 64");
   if (flags & ~(FIR_CACHE_XCD_MAP | FIR_CACHE_STORED_VOXELS |
-                FIR_CACHE_NO_SHEETS | FIR_CACHE_NO_FAMILY_BP))
+                FIR_CACHE_NO_SHEETS | FIR_CACHE_NO_FAMILY_BP |
+                FIR_CACHE_NO_SLABS))
     return fail(FIR_E_ARG, "unknown cache flags 0x%x", flags);
   const int sheets = cache->sheets, family_bp = cache->family_bp;
   cache->sheets = (flags & FIR_CACHE_NO_SHEETS) ? 0 : 1;
@@ -3324,6 +3819,7 @@ int fir_cache_tune(fir_cache* cache, int fwd_lanes, int bp_lanes, int flags)
     }
   }
   cache->zfast = (flags & FIR_CACHE_STORED_VOXELS) ? 0 : 1;
+  cache->slabs = (flags & FIR_CACHE_NO_SLABS) ? 0 : 1;
   cache->fwd_lanes = fwd_lanes;
   cache->bp_lanes = bp_lanes;
   cache->xcd_map = (flags & FIR_CACHE_XCD_MAP) ? 1 : 0;
@@ -3484,6 +3980,27 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
         launch_rows_sheet<FIR_LEN_FP16>(cache, subset, x[i0], io, cache->d_w);
         FIR_CHECK_LAUNCH();
         launch_backproject<FIR_LEN_FP16>(cache, subset, 1, cache->d_w, out);
+      }
+      FIR_CHECK_LAUNCH();
+      continue;
+    }
+    if (use_slabs(cache, wd))
+    {
+      // batch line integrals from slab-staged sheets, then the family
+      // backprojection of all its images from the weights
+      pack_images_pz(cache, wd, in, nb);
+      FIR_CHECK_LAUNCH();
+      if (rm->fmt == FIR_LEN_SNORM16)
+      {
+        launch_rows_slab<FIR_LEN_SNORM16>(cache, subset, wd, io, cache->d_w);
+        FIR_CHECK_LAUNCH();
+        launch_fam_bp<FIR_LEN_SNORM16>(cache, subset, wd, cache->d_w, out);
+      }
+      else
+      {
+        launch_rows_slab<FIR_LEN_FP16>(cache, subset, wd, io, cache->d_w);
+        FIR_CHECK_LAUNCH();
+        launch_fam_bp<FIR_LEN_FP16>(cache, subset, wd, cache->d_w, out);
       }
       FIR_CHECK_LAUNCH();
       continue;

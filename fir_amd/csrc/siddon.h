@@ -89,6 +89,28 @@ __host__ __device__ __forceinline__ int views_in_subset(int n_views,
                        : n_views;
 }
 
+// (view, tangential bin) -> the LOR's two angular crystals, the walk
+// running from the first to the second (ProjData.cc:301-340); C++ integer
+// division truncates toward zero.
+__device__ __forceinline__ int2 view_crystals(const LorGeom& g, int view,
+                                              int tt)
+{
+  const int tang = tt - g.tang_off;
+  const int shift = tang >= 0 ? tang / 2 : (tang - 1) / 2;
+  const int odd = (tang < 0 ? -tang : tang) & 1;
+  int c1 = view + shift + odd;
+  int c2 = view + g.n_views - shift;
+  if (c1 < 0)
+    c1 += g.n_cpr;
+  else if (c1 >= g.n_cpr)
+    c1 -= g.n_cpr;
+  if (c2 < 0)
+    c2 += g.n_cpr;
+  else if (c2 >= g.n_cpr)
+    c2 -= g.n_cpr;
+  return make_int2(c1, c2);
+}
+
 // Work index t -> LOR (t < 2^31, checked by the host). n_subsets == 0
 // selects every view. The bin index is the LOR's position in the
 // reference's segment-concatenated sinogram (ProjData.h:9-21).
@@ -147,23 +169,9 @@ __device__ __forceinline__ void lor_from_index(const LorGeom& g, long long t,
   const int view = n_subsets > 0 ? sv * n_subsets + subset : sv;
   L.bin = g.seg_base[P.x] + ((long long)view * na + P.y) * g.n_tang + tt;
 
-  // (view, tang) -> angular crystals (ProjData.cc:301-340); C++ integer
-  // division truncates toward zero.
-  const int tang = tt - g.tang_off;
-  const int shift = tang >= 0 ? tang / 2 : (tang - 1) / 2;
-  const int odd = (tang < 0 ? -tang : tang) & 1;
-  int c1 = view + shift + odd;
-  int c2 = view + g.n_views - shift;
-  if (c1 < 0)
-    c1 += g.n_cpr;
-  else if (c1 >= g.n_cpr)
-    c1 -= g.n_cpr;
-  if (c2 < 0)
-    c2 += g.n_cpr;
-  else if (c2 >= g.n_cpr)
-    c2 -= g.n_cpr;
-  const double2 p1 = g.xy[c1];
-  const double2 p2 = g.xy[c2];
+  const int2 c = view_crystals(g, view, tt);
+  const double2 p1 = g.xy[c.x];
+  const double2 p2 = g.xy[c.y];
   L.e1[0] = p1.x;
   L.e1[1] = p1.y;
   L.e1[2] = g.z[P.z];
@@ -199,21 +207,9 @@ __device__ __forceinline__ void lor_from_bin(const LorGeom& g,
   const int tt = (int)(r2 - (long long)axial * g.n_tang);
   const int4 P = g.pairs[ax_prefix[k] + axial];
   L.bin = b;
-  const int tang = tt - g.tang_off;
-  const int shift = tang >= 0 ? tang / 2 : (tang - 1) / 2;
-  const int odd = (tang < 0 ? -tang : tang) & 1;
-  int c1 = view + shift + odd;
-  int c2 = view + g.n_views - shift;
-  if (c1 < 0)
-    c1 += g.n_cpr;
-  else if (c1 >= g.n_cpr)
-    c1 -= g.n_cpr;
-  if (c2 < 0)
-    c2 += g.n_cpr;
-  else if (c2 >= g.n_cpr)
-    c2 -= g.n_cpr;
-  const double2 p1 = g.xy[c1];
-  const double2 p2 = g.xy[c2];
+  const int2 c = view_crystals(g, view, tt);
+  const double2 p1 = g.xy[c.x];
+  const double2 p2 = g.xy[c.y];
   L.e1[0] = p1.x;
   L.e1[1] = p1.y;
   L.e1[2] = g.z[P.z];

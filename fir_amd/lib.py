@@ -30,8 +30,10 @@ FIR_CACHE_XCD_MAP = 1
 FIR_CACHE_STORED_VOXELS = 2
 FIR_CACHE_NO_SHEETS = 4
 FIR_CACHE_NO_FAMILY_BP = 8
+FIR_CACHE_NO_SLABS = 16
 FIR_FORM_SHEETS = 1
 FIR_FORM_FAMILY_LISTS = 2
+FIR_FORM_SLABS = 4
 # Row order of path caches: per view, tangential bin then (segment, axial)
 # pair by centre slice -- the rows through one voxel are then close together.
 CACHE_ROW_ORDER = FIR_ORDER_VIEW_MAJOR | FIR_ORDER_TANG_MAJOR | FIR_ORDER_ZCENTER
@@ -462,7 +464,8 @@ class PathCache:
         f = C.c_int()
         _check(self.lib.fir_cache_forms(self.h, C.byref(f)))
         return dict(sheets=bool(f.value & FIR_FORM_SHEETS),
-                    family_lists=bool(f.value & FIR_FORM_FAMILY_LISTS))
+                    family_lists=bool(f.value & FIR_FORM_FAMILY_LISTS),
+                    slabs=bool(f.value & FIR_FORM_SLABS))
 
     def tune(self, fwd_lanes: int = 16, bp_lanes: int = 16, flags: int = 0):
         _check(self.lib.fir_cache_tune(self.h, fwd_lanes, bp_lanes, flags))

@@ -130,14 +130,17 @@ class BatchedOSEM:
     context and each group gets a path cache; caches of one image grid share
     the LOR-major path matrix, so only the per-subset-count transposes are
     extra. ``cache_zslices`` is their layout's z-slice interleave
-    (Context.set_cache_zslices).
+    (Context.set_cache_zslices) and ``cache_lanes`` the lanes per LOR and
+    per voxel their quads are laid out for (Context.set_cache_lanes): with
+    4 lanes per LOR, batches of 4 or 8 images take their line integrals
+    from slab-staged sheets (FIR_FORM_SLABS).
     """
 
     def __init__(self, workload, solves: List[dict], fwhm=(0.0, 0.0, 0.0),
                  conv_interval: int = 0, cut: float = 0.0, device: int = 0,
                  cache_format: Optional[int] = None,
                  cache_row_order: int = firlib.CACHE_ROW_ORDER,
-                 cache_zslices: int = 2):
+                 cache_zslices: int = 2, cache_lanes=(4, 16)):
         keys = []
         for s in solves:
             v = s["vol"]
@@ -153,11 +156,14 @@ class BatchedOSEM:
         if cache_format is not None:
             shared = firlib.context_for(workload, device)
             shared.set_uneven_subsets(True)
-            # groups project several images per launch, for which 2
-            # interleaved z-slices suit the gathers best; the sheet form's
-            # family lists serve their backprojections (and make the
-            # voxel-major transposes unnecessary)
+            # groups project several images per launch: 4 or 8 take their
+            # line integrals from slab-staged sheets (4 lanes per LOR suit
+            # them best), 2 gather from interleaved image copies (2
+            # interleaved z-slices suit that best); the sheet form's family
+            # lists serve the backprojections (and make the voxel-major
+            # transposes unnecessary)
             shared.set_cache_zslices(cache_zslices)
+            shared.set_cache_lanes(*cache_lanes)
             shared.set_cache_sheets(True)
         self.ctx = shared
         for k in keys:

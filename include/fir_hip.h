@@ -430,11 +430,15 @@ int fir_cache_info(const fir_cache* cache, int64_t* n_rows,
  * device has room for it (otherwise one-image passes fall back to the
  * image gathers and the transposed lists, with results within FP32
  * rounding); one-image OSEM passes use what is held unless fir_cache_tune
- * switches it off. No counterpart in the reference. */
+ * switches it off. FIR_FORM_SLABS: batches of 4 or 8 images take their
+ * line integrals from sheets staged slab by slab (sheet pixels ranked along
+ * each family's line, checked at build; the runtime forward lanes equal to
+ * the layout's). No counterpart in the reference. */
 enum
 {
   FIR_FORM_SHEETS = 1,
-  FIR_FORM_FAMILY_LISTS = 2
+  FIR_FORM_FAMILY_LISTS = 2,
+  FIR_FORM_SLABS = 4
 };
 int fir_cache_forms(const fir_cache* cache, int* forms);
 
@@ -463,13 +467,19 @@ int fir_cache_forms(const fir_cache* cache, int* forms);
  * subset's families that hold its pixel, in family order. The sums are the
  * same on every run. FIR_CACHE_NO_FAMILY_BP gathers the weights through
  * the transposed voxel lists instead (sums in another order: results
- * within FP32 rounding of each other). */
+ * within FP32 rounding of each other). Batches of 4 or 8 images with the
+ * family lists stage each family's sheet of all their images in LDS slab
+ * by slab (pixels along the family's line, all slices) and take their line
+ * integrals from it, each image's the same bit for bit as a one-image
+ * pass's; FIR_CACHE_NO_SLABS gathers them from interleaved image copies
+ * instead (also the same bits). */
 enum
 {
   FIR_CACHE_XCD_MAP = 1,
   FIR_CACHE_STORED_VOXELS = 2,
   FIR_CACHE_NO_SHEETS = 4,
-  FIR_CACHE_NO_FAMILY_BP = 8
+  FIR_CACHE_NO_FAMILY_BP = 8,
+  FIR_CACHE_NO_SLABS = 16
 };
 int fir_cache_tune(fir_cache* cache, int fwd_lanes, int bp_lanes, int flags);
 

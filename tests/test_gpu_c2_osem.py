@@ -17,7 +17,11 @@ Checks:
 * the plain path (FP64 walk per LOR) and the path cache in both 16-bit
   length formats (the bench's path: sheet-staged line integrals and the
   family backprojection, sensitivity from the same cache) land within
-  1e-4 relative L2 of the reference's image and of the FP64 oracle's.
+  1e-4 relative L2 of the reference's image and of the FP64 oracle's;
+* a batch of 8 images on a 4-lane cache (line integrals from the sheets
+  of all 8 images staged in LDS slab by slab, 576 rows and about a dozen
+  slabs per family) gives every image exactly its one-image pass, and so
+  does a batch of 4.
 """
 import hashlib
 import os
@@ -110,3 +114,28 @@ def test_c2_osem_path_cache_vs_reference(c2, fmt):
     print(f"path cache fmt {fmt}: rel L2 {e_ref:.3e} vs reference, "
           f"{e_64:.3e} vs FP64 oracle", flush=True)
     assert e_ref < BAR and e_64 < BAR
+
+
+def test_c2_slab_batches_equal_single_image_passes(c2):
+    ctx, v, w = c2["ctx"], c2["v"], c2["w"]
+    ctx.set_cache_lanes(4, 16)
+    try:
+        cache = ctx.path_cache(v, w.n_subsets, firlib.FIR_LEN_SNORM16,
+                               row_order=firlib.CACHE_ROW_ORDER)
+    finally:
+        ctx.set_cache_lanes(8, 16)
+    assert cache.forms()["slabs"]
+    rng = np.random.default_rng(12)
+    n = v.n_voxels
+    xs = [torch.from_numpy(rng.uniform(0.5, 2.0, n).astype(np.float32))
+          .cuda() for _ in range(8)]
+    ys = [c2["y"] * float(s) for s in rng.uniform(0.05, 2.0, 8)]
+    for nb in (8, 4):
+        bps = [torch.zeros(n, device="cuda") for _ in range(nb)]
+        cache.osem_project(xs[:nb], ys[:nb], None, bps, 5)
+        for x, y, bp in zip(xs[:nb], ys[:nb], bps):
+            one = torch.zeros(n, device="cuda")
+            cache.osem_project([x], [y], None, [one], 5)
+            torch.cuda.synchronize()
+            assert torch.equal(bp, one), nb
+    cache.close()

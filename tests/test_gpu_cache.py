@@ -315,7 +315,7 @@ def test_cache_lanes_keep_results(c1, c1_golden):
     for s, b in outs[1:]:
         assert rel_max(s, outs[0][0]) < 1e-6
         assert rel_max(b, outs[0][1]) < 1e-6
-    for bad in [(0, 16, 0), (16, 3, 0), (128, 16, 0), (16, 16, 16)]:
+    for bad in [(0, 16, 0), (16, 3, 0), (128, 16, 0), (16, 16, 32)]:
         with pytest.raises(firlib.FirError):
             cache.tune(*bad)
 
@@ -507,7 +507,8 @@ def test_family_backprojection(c1, c1_golden, fmt):
     projector."""
     w, ctx, v, caches = c1
     cache = ctx.path_cache(v, 4, fmt, row_order=firlib.CACHE_ROW_ORDER)
-    assert cache.forms() == {"sheets": True, "family_lists": True}
+    assert cache.forms() == {"sheets": True, "family_lists": True,
+                             "slabs": True}
     x = dev(c1_golden["act"].ravel() + 0.1)
     y = dev(c1_golden["y"])
     bias = dev(np.full_like(c1_golden["y"], 0.5))
@@ -532,18 +533,21 @@ def test_family_backprojection(c1, c1_golden, fmt):
 
 @pytest.mark.parametrize("n_images", [2, 3, 8, 11])
 def test_family_batch_backprojection(c1, c1_golden, n_images):
-    """Batches over a cache with family lists gather their line integrals
-    from the image copies and backproject all their images family by
-    family from the lists (one read of the lists per batch, weights of
-    every image in LDS): each image gets exactly what its one-image family
-    pass gives (the same weights, lists and summation order), within FP32
-    summation order of the transposed-list backprojection (which the cache
-    builds only when tuning asks for it), and the full backprojection and
-    sensitivity through the lists match the transposed ones the same way."""
+    """Batches over a cache with family lists take their line integrals
+    from slab-staged sheets (4 or 8 images; 2 gather from the image copies)
+    and backproject all their images family by family from the lists (one
+    read of the lists per batch, weights of every image in LDS): each image
+    gets exactly what its one-image family pass gives (the same weights,
+    lists and summation order), and exactly what the gathered line
+    integrals give; within FP32 summation order of the transposed-list
+    backprojection (which the cache builds only when tuning asks for it),
+    and the full backprojection and sensitivity through the lists match the
+    transposed ones the same way."""
     w, ctx, v, caches = c1
     cache = ctx.path_cache(v, 4, firlib.FIR_LEN_SNORM16,
                            row_order=firlib.CACHE_ROW_ORDER)
-    assert cache.forms()["family_lists"]
+    forms = cache.forms()
+    assert forms["family_lists"] and forms["slabs"], forms
     rng = np.random.default_rng(40 + n_images)
     xs = [dev(rng.uniform(0.5, 2.0, v.n_voxels).astype(np.float32))
           for _ in range(n_images)]
@@ -561,6 +565,11 @@ def test_family_batch_backprojection(c1, c1_golden, n_images):
         one = torch.zeros(v.n_voxels, device="cuda")
         cache.osem_project([x], [y], None, [one], 2)
         assert np.array_equal(host(bp), host(one))
+    cache.tune(8, 16, firlib.FIR_CACHE_NO_SLABS)
+    gathered = [torch.zeros(v.n_voxels, device="cuda") for _ in xs]
+    cache.osem_project(xs, ys, None, gathered, 2)
+    for bp, g in zip(bps, gathered):
+        assert np.array_equal(host(bp), host(g))
     cache.tune(8, 16, firlib.FIR_CACHE_NO_FAMILY_BP)
     assert cache.info()["device_bytes"] > bytes_before  # transposes built
     for x, y, bp in zip(xs, ys, bps):
