@@ -354,14 +354,37 @@ __global__ void k_pc_zero_subset_bins(float* __restrict__ sino, long long n_bins
   }
 }
 
-// Pass 2: the paths themselves.
+// Four lengths of a quad as one store.
+template <int F>
+__device__ __forceinline__ void store_len4(typename Len<F>::T* p,
+                                           const typename Len<F>::T* v)
+{
+  if constexpr (sizeof(typename Len<F>::T) == 2)
+    *reinterpret_cast<uint2*>(p) =
+      make_uint2(Len<F>::bits(v[0]) | (Len<F>::bits(v[1]) << 16),
+                 Len<F>::bits(v[2]) | (Len<F>::bits(v[3]) << 16));
+  else
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+// Pass 2: the paths themselves. With stage set (lanes <= 8), a thread
+// collects each round of its path's quad layout (4 * r elements, r the
+// round's quads) in its own LDS slots and writes the round out as whole
+// quads, padding included: a round's slots are contiguous, so the stores
+// fill whole cache lines at once instead of leaving lines of many rows
+// half written in L2 (the elements arrive in path order, which the
+// interleave scatters over the round). Dynamic LDS: kBlock * 4 * lanes
+// indices and as many lengths.
 template <int F>
 __global__ __launch_bounds__(kBlock) void k_pc_fill(
   Grid g, LorGeom lg, long long n, long long per_view,
   const int* __restrict__ count,
   const float* __restrict__ maxlen, const long long* __restrict__ eoff,
-  XY xy, int lanes, int* __restrict__ idx, typename Len<F>::T* __restrict__ len)
+  XY xy, int lanes, int stage, int* __restrict__ idx,
+  typename Len<F>::T* __restrict__ len)
 {
+  using T = typename Len<F>::T;
+  extern __shared__ int fill_lds[];
   const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
   if (t >= n || count[t] == 0) return;
   Lor L;
@@ -374,12 +397,56 @@ __global__ __launch_bounds__(kBlock) void k_pc_fill(
   const int cnt = count[t];
   const long long slots = (cnt + 3) & ~3;
   int p = 0;
+  if (!stage)
+  {
+    fir::walk_run(g, w, [&](int i, double l) {
+      if (p < cnt)
+      {
+        const long long e = base + quad_slot(p, slots, lanes);
+        idx[e] = to_layout(xy, i, yf);
+        len[e] = Len<F>::enc(l, m);
+      }
+      ++p;
+    });
+    return;
+  }
+  int* si = fill_lds + threadIdx.x;
+  T* sl = reinterpret_cast<T*>(fill_lds + 4 * lanes * kBlock) + threadIdx.x;
+  const int q = (int)(slots >> 2);
+  int rnd = 0, pl = 0;
+  int r = q < lanes ? q : lanes;
   fir::walk_run(g, w, [&](int i, double l) {
     if (p < cnt)
     {
-      const long long e = base + quad_slot(p, slots, lanes);
-      idx[e] = to_layout(xy, i, yf);
-      len[e] = Len<F>::enc(l, m);
+      // element pl of the round: element k * r + j of the round sits in
+      // slot 4 j + k (quad_slot)
+      const int k = pl / r, j = pl - k * r;
+      si[(4 * j + k) * kBlock] = to_layout(xy, i, yf);
+      sl[(4 * j + k) * kBlock] = Len<F>::enc(l, m);
+      ++pl;
+      if (pl == 4 * r || p + 1 == cnt)
+      {
+        const long long b0 = base + 4LL * rnd * lanes;
+        for (int jj = 0; jj < r; ++jj)
+        {
+          int v[4];
+          T lv[4];
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk)
+          {
+            const bool real = kk * r + jj < pl;
+            v[kk] = real ? si[(4 * jj + kk) * kBlock] : 0;
+            lv[kk] = real ? sl[(4 * jj + kk) * kBlock] : (T)0;
+          }
+          *reinterpret_cast<int4*>(idx + b0 + 4 * jj) =
+            make_int4(v[0], v[1], v[2], v[3]);
+          store_len4<F>(len + b0 + 4 * jj, lv);
+        }
+        ++rnd;
+        pl = 0;
+        const int left = q - rnd * lanes;
+        r = left < lanes ? left : lanes;
+      }
     }
     ++p;
   });
@@ -1002,127 +1069,150 @@ __global__ void k_sh_fam_rows(const int* __restrict__ rowid, int n_pairs,
   else if (f == n_fam) fam_row[f] = n_rows;
 }
 
-__device__ __forceinline__ int family_of_row(const int* __restrict__ fam_row,
-                                             int n_fam, int r)
+// Sheet pixels of every family, one block per family: a bitmap of the
+// pixels its rows' contributing elements cross, built in LDS (a wave per
+// row) and written out with the pixel count.
+__global__ __launch_bounds__(kBlock) void k_sh_mark_fam(
+  const int* __restrict__ fam_row, const long long* __restrict__ off,
+  const int* __restrict__ idx, const void* __restrict__ len, int lb,
+  const unsigned char* __restrict__ yrow, XY xy, int words,
+  unsigned long long* __restrict__ bits, int* __restrict__ n_px)
 {
-  int lo = 0, hi = n_fam - 1; // last family whose first row is <= r
-  while (lo < hi)
-  {
-    const int mid = (lo + hi + 1) >> 1;
-    if (fam_row[mid] <= r) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-// Pixel bitmap of every family: the pixels its rows' (contributing)
-// elements cross. kTrLanes lanes per row.
-__global__ __launch_bounds__(kBlock) void k_sh_mark(
-  int n_rows, const int* __restrict__ fam_row, int n_fam,
-  const long long* __restrict__ off, const int* __restrict__ idx,
-  const void* __restrict__ len, int lb, const unsigned char* __restrict__ yrow,
-  XY xy, int words, unsigned long long* __restrict__ bits)
-{
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = (int)(t / kTrLanes);
-  if (r >= n_rows) return;
-  const int j = (int)(t % kTrLanes);
-  const int f = family_of_row(fam_row, n_fam, r);
-  const bool yf = yrow[r] != 0;
+  extern __shared__ unsigned long long wb[]; // [words]
+  __shared__ int cnt;
+  const int f = blockIdx.x;
+  for (int i = threadIdx.x; i < words; i += kBlock) wb[i] = 0;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
   const int nxy = xy.nx * xy.ny;
-  const long long end = off[r + 1];
-  for (long long e = off[r] + j; e < end; e += kTrLanes)
+  const int wave = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
+  for (int r = fam_row[f] + wave; r < fam_row[f + 1]; r += kBlock / 64)
   {
-    const int v = tr_voxel(idx, len, lb, xy, yf, e, end);
-    if (v < 0) continue;
-    const int px = v % nxy;
-    // The rows of a family cross the same pixels: after the first few the
-    // bit is set already, and the atomic (contended) is skipped.
-    const long long wi = (long long)f * words + px / 64;
-    const unsigned long long m = 1ULL << (px % 64);
-    if (!(bits[wi] & m)) atomicOr(bits + wi, m);
+    const bool yf = yrow[r] != 0;
+    const long long end = off[r + 1];
+    for (long long e = off[r] + ln; e < end; e += 64)
+    {
+      const int v = tr_voxel(idx, len, lb, xy, yf, e, end);
+      if (v < 0) continue;
+      const int px = v % nxy;
+      atomicOr(wb + px / 64, 1ULL << (px % 64));
+    }
   }
-}
-
-// Pixels per family and, per 64-pixel word, the pixels before it.
-__global__ void k_sh_count(const unsigned long long* __restrict__ bits,
-                           int n_fam, int words,
-                           unsigned short* __restrict__ wpre,
-                           int* __restrict__ n_px)
-{
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= n_fam) return;
+  __syncthreads();
   int c = 0;
-  for (int k = 0; k < words; ++k)
+  for (int i = threadIdx.x; i < words; i += kBlock)
   {
-    wpre[(long long)f * words + k] = (unsigned short)c;
-    c += __popcll(bits[(long long)f * words + k]);
+    bits[(long long)f * words + i] = wb[i];
+    c += __popcll(wb[i]);
   }
-  n_px[f] = c;
+  atomicAdd(&cnt, c);
+  __syncthreads();
+  if (threadIdx.x == 0) n_px[f] = cnt;
 }
 
-// The pixels of every family's sheet, in pixel order.
-__global__ void k_sh_pixels(const unsigned long long* __restrict__ bits,
-                            int n_fam, int words,
-                            const int* __restrict__ fam_off,
-                            int* __restrict__ fam_xy)
+// Sheet pixels and sheet indices of every family, one block per family:
+// the family's pixels from its bitmap (pixel order), ranked along its line
+// when line_order is set (by the position of the pixel's centre along the
+// family's line, from the first crystal to the second: every row of the
+// family has the same two crystals in the xy-plane and walks from the
+// first, Siddon.cc:112-412; a walk steps to a neighbouring pixel in the
+// direction of the line, which moves the centre forward along it, so every
+// row visits its pixels in rising rank, and each lane's elements rise too),
+// written to fam_xy in rank order; then every slot of its rows gets its
+// sheet index, rank * nz + z (0 for padding and zero-length elements),
+// with the length's 16 bits above. Dynamic LDS: words * 12 bytes plus 14
+// bytes per sheet pixel.
+__global__ __launch_bounds__(kBlock) void k_sh_encode_fam(
+  LorGeom lg, Grid g, int n_tang, const int* __restrict__ fam_row,
+  const long long* __restrict__ off, const int* __restrict__ idx,
+  const void* __restrict__ len, int lb,
+  const unsigned char* __restrict__ yrow, XY xy, int nz, int words,
+  const unsigned long long* __restrict__ bits,
+  const int* __restrict__ fam_off, int line_order, int* __restrict__ fam_xy,
+  unsigned* __restrict__ sel)
 {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= n_fam) return;
-  int o = fam_off[f];
-  for (int k = 0; k < words; ++k)
+  extern __shared__ unsigned long long wb[]; // [words]
+  const int f = blockIdx.x;
+  const int p0 = fam_off[f], L = fam_off[f + 1] - p0;
+  int* wpre = reinterpret_cast<int*>(wb + words);  // [words]
+  double* key = reinterpret_cast<double*>(wpre + ((words + 1) & ~1)); // [L]
+  int* pxl = reinterpret_cast<int*>(key + L);       // [L]
+  unsigned short* rank = reinterpret_cast<unsigned short*>(pxl + L); // [L]
+  for (int i = threadIdx.x; i < words; i += kBlock)
+    wb[i] = bits[(long long)f * words + i];
+  __syncthreads();
+  if (threadIdx.x == 0)
   {
-    unsigned long long b = bits[(long long)f * words + k];
+    int c = 0;
+    for (int k = 0; k < words; ++k)
+    {
+      wpre[k] = c;
+      c += __popcll(wb[k]);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < words; k += kBlock)
+  {
+    unsigned long long b = wb[k];
+    int o = wpre[k];
     while (b)
     {
-      const int i = __ffsll((long long)b) - 1;
-      fam_xy[o++] = k * 64 + i;
+      pxl[o++] = k * 64 + __ffsll((long long)b) - 1;
       b &= b - 1;
     }
   }
-}
-
-// The sheet pixels of every family in the order its rows cross them: by
-// the position of the pixel's centre along the family's line, from the
-// first crystal to the second (every row of the family has the same two
-// crystals in the xy-plane and walks from the first, Siddon.cc:112-412).
-// A walk steps to a neighbouring pixel in the direction of the line, which
-// moves the centre forward along it, so every row visits its pixels in
-// rising rank, and each lane's elements (the quad interleave keeps a
-// lane's elements in path order) rise too. One block per family: pix_rank
-// gets, for the k-th pixel in pixel order, its rank along the line, and
-// fam_xy is rewritten in line order. A family holds at most max_px pixels
-// (the dynamic LDS: a double and an int each).
-__global__ __launch_bounds__(kBlock) void k_sh_order(
-  LorGeom lg, Grid g, int n_tang, const int* __restrict__ fam_off,
-  int* __restrict__ fam_xy, unsigned short* __restrict__ pix_rank)
-{
-  extern __shared__ double key[];
-  const int f = blockIdx.x;
-  const int p0 = fam_off[f], L = fam_off[f + 1] - p0;
-  int* px = reinterpret_cast<int*>(key + L);
-  const int view = f / n_tang, tt = f - view * n_tang;
-  const int2 c = fir::view_crystals(lg, view, tt);
-  const double2 a = lg.xy[c.x], b = lg.xy[c.y];
-  const double dx = b.x - a.x, dy = b.y - a.y;
-  for (int i = threadIdx.x; i < L; i += kBlock)
-  {
-    const int v = fam_xy[p0 + i];
-    const int x = v % g.row, y = v / g.row;
-    const double cx = g.low[0] + (x + 0.5) * g.step[0];
-    const double cy = g.low[1] + (y + 0.5) * g.step[1];
-    key[i] = (cx - a.x) * dx + (cy - a.y) * dy;
-    px[i] = v;
-  }
   __syncthreads();
-  for (int i = threadIdx.x; i < L; i += kBlock)
+  if (line_order)
   {
-    const double ki = key[i];
-    int k = 0;
-    for (int j = 0; j < L; ++j)
-      k += (key[j] < ki || (key[j] == ki && j < i)) ? 1 : 0;
-    pix_rank[p0 + i] = (unsigned short)k;
-    fam_xy[p0 + k] = px[i];
+    const int view = f / n_tang, tt = f - view * n_tang;
+    const int2 c = fir::view_crystals(lg, view, tt);
+    const double2 a = lg.xy[c.x], b = lg.xy[c.y];
+    const double dx = b.x - a.x, dy = b.y - a.y;
+    for (int i = threadIdx.x; i < L; i += kBlock)
+    {
+      const int x = pxl[i] % g.row, y = pxl[i] / g.row;
+      key[i] = (g.low[0] + (x + 0.5) * g.step[0] - a.x) * dx +
+               (g.low[1] + (y + 0.5) * g.step[1] - a.y) * dy;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < L; i += kBlock)
+    {
+      const double ki = key[i];
+      int k = 0;
+      for (int j = 0; j < L; ++j)
+        k += (key[j] < ki || (key[j] == ki && j < i)) ? 1 : 0;
+      rank[i] = (unsigned short)k;
+      fam_xy[p0 + k] = pxl[i];
+    }
+  }
+  else
+    for (int i = threadIdx.x; i < L; i += kBlock)
+    {
+      rank[i] = (unsigned short)i;
+      fam_xy[p0 + i] = pxl[i];
+    }
+  __syncthreads();
+  const int nxy = xy.nx * xy.ny;
+  const int wave = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
+  for (int r = fam_row[f] + wave; r < fam_row[f + 1]; r += kBlock / 64)
+  {
+    const bool yf = yrow[r] != 0;
+    const long long end = off[r + 1];
+    for (long long e = off[r] + ln; e < end; e += 64)
+    {
+      const int v = tr_voxel(idx, len, lb, xy, yf, e, end);
+      unsigned out = 0;
+      if (v >= 0)
+      {
+        const int px = v % nxy, z = v / nxy;
+        const int k = px / 64;
+        const int pr =
+          wpre[k] + __popcll(wb[k] & ((1ULL << (px % 64)) - 1ULL));
+        out = (unsigned)(rank[pr] * nz + z) |
+          ((unsigned)((const unsigned short*)len)[e] << 16);
+      }
+      sel[e] = out;
+    }
   }
 }
 
@@ -1154,42 +1244,6 @@ __global__ __launch_bounds__(kBlock) void k_sh_check_order(
       }
       prev = p;
     }
-  }
-}
-
-// Sheet index of every slot: pixel rank * nz + z (0 for the padding and
-// zero-length elements, which contribute nothing). The rank is the pixel's
-// position in pixel order, or, with pix_rank, its rank along the line.
-__global__ __launch_bounds__(kBlock) void k_sh_encode(
-  int n_rows, const int* __restrict__ fam_row, int n_fam,
-  const long long* __restrict__ off, const int* __restrict__ idx,
-  const void* __restrict__ len, int lb, const unsigned char* __restrict__ yrow,
-  XY xy, int nz, int words, const unsigned long long* __restrict__ bits,
-  const unsigned short* __restrict__ wpre, const int* __restrict__ fam_off,
-  const unsigned short* __restrict__ pix_rank, unsigned* __restrict__ sel)
-{
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = (int)(t / kTrLanes);
-  if (r >= n_rows) return;
-  const int j = (int)(t % kTrLanes);
-  const int f = family_of_row(fam_row, n_fam, r);
-  const bool yf = yrow[r] != 0;
-  const int nxy = xy.nx * xy.ny;
-  const long long end = off[r + 1];
-  for (long long e = off[r] + j; e < end; e += kTrLanes)
-  {
-    const int v = tr_voxel(idx, len, lb, xy, yf, e, end);
-    unsigned out = 0;
-    if (v >= 0)
-    {
-      const int px = v % nxy, z = v / nxy;
-      const long long wi = (long long)f * words + px / 64;
-      int rank = wpre[wi] + __popcll(bits[wi] & ((1ULL << (px % 64)) - 1ULL));
-      if (pix_rank) rank = pix_rank[fam_off[f] + rank];
-      out = (unsigned)(rank * nz + z) |
-        ((unsigned)((const unsigned short*)len)[e] << 16);
-    }
-    sel[e] = out;
   }
 }
 
@@ -1418,28 +1472,27 @@ __device__ __forceinline__ void slab_fma(const float* __restrict__ xp, float l,
 }
 
 // Stage slab `sb` (pixels [sb * slab_px, ...) of the family's sheet, all
-// slices, NB floats per voxel) into buf with direct global-to-LDS loads:
-// each wave instruction writes 64 consecutive float4 of the buffer, lane j
-// the j-th, fetched from that pixel's run in X. Returns nothing to wait on
-// in registers; the caller waits on the vector memory counter.
+// slices, NB floats per voxel: n4 float4 per pixel) into buf with direct
+// global-to-LDS loads: a wave copies one pixel's run at a time, each
+// instruction 64 consecutive float4 (lane j the j-th). Nothing lands in
+// registers; the caller waits on the vector memory counter.
 __device__ __forceinline__ void slab_stage(const float4* __restrict__ X4,
                                            const int* __restrict__ fam_xy,
                                            int p0, int L, int slab_px, int n4,
                                            int sb, float4* buf)
 {
   const int pb = sb * slab_px;
-  const int ns = min(slab_px, L - pb) * n4;
-  const int wave = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
-  const int step = (int)(blockDim.x >> 6) * 64;
-  for (int c = wave * 64; c < ns; c += step)
+  const int n = min(slab_px, L - pb);
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int ln = (int)threadIdx.x & 63;
+  const int n_waves = (int)(blockDim.x >> 6);
+  for (int p = wave; p < n; p += n_waves)
   {
-    const int i = c + ln;
-    if (i < ns)
-    {
-      const int p = i / n4, o = i - p * n4;
-      __builtin_amdgcn_global_load_lds(
-        X4 + (long long)fam_xy[p0 + pb + p] * n4 + o, buf + c, 16, 0, 0);
-    }
+    const float4* src = X4 + (long long)fam_xy[p0 + pb + p] * n4;
+    float4* dst = buf + p * n4;
+    for (int o = 0; o < n4; o += 64)
+      if (o + ln < n4)
+        __builtin_amdgcn_global_load_lds(src + o + ln, dst + o, 16, 0, 0);
   }
 }
 
@@ -1659,56 +1712,75 @@ __global__ __launch_bounds__(kFlBlock) void k_fl_count(
   if (threadIdx.x == 0) ftot[f] = total;
 }
 
-// Lists, fill pass: first quads of the family's entries (fl_off), the
-// elements (fl, cleared beforehand) and each list sorted by row.
+// Lists, fill pass: first quads of the family's entries (fl_off) and the
+// elements (fl, cleared beforehand), each list in row order.
 __global__ __launch_bounds__(kFlBlock) void k_fl_fill(
   const int* __restrict__ fam_row, const int* __restrict__ fam_off, int nz,
   const long long* __restrict__ off, const unsigned* __restrict__ sel,
   const long long* __restrict__ fbase, unsigned* __restrict__ fl_off,
   unsigned* __restrict__ fl)
 {
-  __shared__ unsigned pos[kSheetMaxFloats];
-  __shared__ unsigned cur[kSheetMaxFloats];
+  // cnt: the counts, then the list bases, then each group's row bits;
+  // cur: elements placed so far per entry
+  __shared__ unsigned cnt[kSheetMaxFloats];
+  __shared__ unsigned short cur[kSheetMaxFloats];
   __shared__ unsigned part[kFlBlock];
   const int f = blockIdx.x;
   const int n_st = (fam_off[f + 1] - fam_off[f]) * nz;
   const long long g0 = (long long)fam_off[f] * nz;
   const long long base = fbase[f];
-  fl_count(fam_row, off, sel, f, n_st, pos);
-  fl_scan(pos, n_st, part);
+  fl_count(fam_row, off, sel, f, n_st, cnt);
+  fl_scan(cnt, n_st, part);
   for (int i = threadIdx.x; i < n_st; i += kFlBlock)
   {
-    fl_off[g0 + i] = (unsigned)(base + pos[i]);
+    fl_off[g0 + i] = (unsigned)(base + cnt[i]);
+    cnt[i] = 0;
     cur[i] = 0;
   }
   __syncthreads();
+  // Rows in groups of 32, each element placed at its entry's count so far
+  // plus the group's earlier rows through that entry (a row crosses a voxel
+  // once): every list in row order, no sorting. A wave per row.
   const int r0 = fam_row[f], r1 = fam_row[f + 1];
-  const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
-  for (int r = r0 + wave; r < r1; r += kFlBlock / 64)
-    for (long long e = off[r] + lane; e < off[r + 1]; e += 64)
-    {
-      const unsigned u = sel[e];
-      if (!(u >> 16)) continue;
-      const unsigned k = atomicAdd(cur + (u & 0xffffu), 1u);
-      fl[(base + pos[u & 0xffffu]) * 4 + k] =
-        (unsigned)(r - r0) | (u & 0xffff0000u);
-    }
-  __syncthreads();
-  for (int i = threadIdx.x; i < n_st; i += kFlBlock)
+  const int wave = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
+  for (int ra = r0; ra < r1; ra += 32)
   {
-    unsigned* l = fl + (base + pos[i]) * 4;
-    const int n = (int)cur[i];
-    for (int a = 1; a < n; ++a)
-    {
-      const unsigned v = l[a];
-      int b = a - 1;
-      while (b >= 0 && (l[b] & 0xffffu) > (v & 0xffffu))
+    const int rb = min(ra + 32, r1);
+    for (int r = ra + wave; r < rb; r += kFlBlock / 64)
+      for (long long e = off[r] + ln; e < off[r + 1]; e += 64)
       {
-        l[b + 1] = l[b];
-        --b;
+        const unsigned u = sel[e];
+        if (u >> 16) atomicOr(cnt + (u & 0xffffu), 1u << (r - ra));
       }
-      l[b + 1] = v;
-    }
+    __syncthreads();
+    for (int r = ra + wave; r < rb; r += kFlBlock / 64)
+      for (long long e = off[r] + ln; e < off[r + 1]; e += 64)
+      {
+        const unsigned u = sel[e];
+        if (!(u >> 16)) continue;
+        const unsigned en = u & 0xffffu;
+        const unsigned k =
+          cur[en] + (unsigned)__popc(cnt[en] & ((1u << (r - ra)) - 1u));
+        fl[(unsigned long long)fl_off[g0 + en] * 4 + k] =
+          (unsigned)(r - r0) | (u & 0xffff0000u);
+      }
+    __syncthreads();
+    // the entry's first row in the group advances its count and clears
+    // the bits
+    for (int r = ra + wave; r < rb; r += kFlBlock / 64)
+      for (long long e = off[r] + ln; e < off[r + 1]; e += 64)
+      {
+        const unsigned u = sel[e];
+        if (!(u >> 16)) continue;
+        const unsigned en = u & 0xffffu;
+        const unsigned b = cnt[en];
+        if ((b & ((1u << (r - ra)) - 1u)) == 0)
+        {
+          cur[en] = (unsigned short)(cur[en] + __popc(b));
+          cnt[en] = 0;
+        }
+      }
+    __syncthreads();
   }
 }
 
@@ -2373,11 +2445,14 @@ int fill_rows(fir_ctx* c, const Grid& g, const LorGeom& lg, long long n,
               long long per_view, const int* count, const float* maxlen, const long long* eoff,
               RowMatrix* rm)
 {
-  hipLaunchKernelGGL(k_pc_fill<F>, dim3(blocks_for(n)), dim3(kBlock), 0,
+  const int lanes = c->cache_lanes[0];
+  const int stage = lanes <= 8 ? 1 : 0;
+  const size_t lds = stage ? (size_t)kBlock * 4 * lanes *
+      (sizeof(int) + sizeof(typename Len<F>::T)) : 0;
+  hipLaunchKernelGGL(k_pc_fill<F>, dim3(blocks_for(n)), dim3(kBlock), lds,
                      c->stream, g, lg, n, per_view, count, maxlen, eoff,
-                     XY{g.row, g.slice / g.row, rm->zq}, c->cache_lanes[0],
-                     rm->d_idx,
-                     (typename Len<F>::T*)rm->d_len);
+                     XY{g.row, g.slice / g.row, rm->zq}, lanes, stage,
+                     rm->d_idx, (typename Len<F>::T*)rm->d_len);
   FIR_CHECK_LAUNCH();
   return FIR_OK;
 }
@@ -2471,28 +2546,22 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
     return FIR_OK;
   size_t free_b = 0, total_b = 0;
   FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-  const long long temp = (long long)F * words * 10 + 8LL * F;
+  const long long temp = (long long)F * words * 8 + 8LL * F;
   if ((long long)free_b < temp + rm->n_slots * 4 + (2LL << 30)) return FIR_OK;
   DevBufs tmp;
   unsigned long long* bits;
-  unsigned short* wpre;
   int* n_px;
   int rc;
   if ((rc = tmp.get(&bits, (size_t)F * words, "sheet bitmaps")) ||
-      (rc = tmp.get(&wpre, (size_t)F * words, "sheet word ranks")) ||
       (rc = tmp.get(&n_px, (size_t)F, "sheet sizes")))
     return rc;
-  FIR_HIP_CHECK(hipMemsetAsync(bits, 0, sizeof(unsigned long long) * F * words,
-                               c->stream));
   const XY xy{v.nx, v.ny, rm->zq};
   const int lb = (int)len_bytes(rm->fmt);
-  const unsigned grid_rows = blocks_for((long long)rm->n_rows * kTrLanes);
-  hipLaunchKernelGGL(k_sh_mark, dim3(grid_rows), dim3(kBlock), 0, c->stream,
-                     rm->n_rows, rm->d_fam_row, F, rm->d_off, rm->d_idx,
-                     rm->d_len, lb, rm->d_yrow, xy, words, bits);
-  FIR_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_sh_count, dim3(blocks_for(F)), dim3(kBlock), 0,
-                     c->stream, bits, F, words, wpre, n_px);
+  const size_t mark_lds = sizeof(unsigned long long) * words;
+  if (mark_lds > 65536) return FIR_OK;
+  hipLaunchKernelGGL(k_sh_mark_fam, dim3(F), dim3(kBlock), mark_lds,
+                     c->stream, rm->d_fam_row, rm->d_off, rm->d_idx,
+                     rm->d_len, lb, rm->d_yrow, xy, words, bits, n_px);
   FIR_CHECK_LAUNCH();
   std::vector<int> npx(F), off(F + 1, 0);
   FIR_HIP_CHECK(hipMemcpyAsync(npx.data(), n_px, sizeof(int) * F,
@@ -2504,7 +2573,10 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
     off[f + 1] = off[f] + npx[f];
     max_px = std::max(max_px, npx[f]);
   }
-  if ((long long)max_px * nz > kSheetMaxFloats) return FIR_OK;
+  const size_t enc_lds =
+    8 * (size_t)words + 4 * (size_t)((words + 1) & ~1) + 14 * (size_t)max_px;
+  if ((long long)max_px * nz > kSheetMaxFloats || enc_lds > 65536)
+    return FIR_OK;
   if ((rc = dev_alloc(&rm->d_fam_off, (size_t)F + 1, "sheet offsets")) ||
       (rc = dev_alloc(&rm->d_fam_xy, (size_t)off[F], "sheet pixels")) ||
       (rc = dev_alloc(&rm->d_sel, (size_t)rm->n_slots, "sheet elements")))
@@ -2518,33 +2590,20 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
   }
   FIR_HIP_CHECK(hipMemcpyAsync(rm->d_fam_off, off.data(), sizeof(int) * (F + 1),
                                hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_sh_pixels, dim3(blocks_for(F)), dim3(kBlock), 0,
-                     c->stream, bits, F, words, rm->d_fam_off, rm->d_fam_xy);
-  FIR_CHECK_LAUNCH();
   // Pixels ranked along each family's line (the batches' slab staging
   // needs it; one-image passes give the same sums in either order).
-  unsigned short* pix_rank = nullptr;
-  const size_t order_lds = (size_t)max_px * (sizeof(double) + sizeof(int));
   Grid g;
   if ((rc = make_grid(c, &v, &g))) return rc;
   const int n_tang = F / rm->n_views;
-  if (order_lds <= 65536 && F % rm->n_views == 0)
-  {
-    if ((rc = tmp.get(&pix_rank, (size_t)std::max(off[F], 1),
-                      "sheet pixel ranks")))
-      return rc;
-    hipLaunchKernelGGL(k_sh_order, dim3(F), dim3(kBlock), order_lds,
-                       c->stream, lor_geom_views(c), g, n_tang,
-                       rm->d_fam_off, rm->d_fam_xy, pix_rank);
-    FIR_CHECK_LAUNCH();
-  }
-  hipLaunchKernelGGL(k_sh_encode, dim3(grid_rows), dim3(kBlock), 0, c->stream,
-                     rm->n_rows, rm->d_fam_row, F, rm->d_off, rm->d_idx,
-                     rm->d_len, lb, rm->d_yrow, xy, nz, words, bits, wpre,
-                     rm->d_fam_off, pix_rank, rm->d_sel);
+  const bool line = F % rm->n_views == 0;
+  hipLaunchKernelGGL(k_sh_encode_fam, dim3(F), dim3(kBlock), enc_lds,
+                     c->stream, lor_geom_views(c), g, n_tang, rm->d_fam_row,
+                     rm->d_off, rm->d_idx, rm->d_len, lb, rm->d_yrow, xy, nz,
+                     words, bits, rm->d_fam_off, line ? 1 : 0, rm->d_fam_xy,
+                     rm->d_sel);
   FIR_CHECK_LAUNCH();
   rm->line_order = 0;
-  if (pix_rank)
+  if (line)
   {
     int* ok;
     if ((rc = tmp.get(&ok, 1, "sheet order flag"))) return rc;
