@@ -12,8 +12,11 @@
 //   recomputeAttn: recompute the attenuation correction factors from the HU
 //     volume (falls back to reading the ACF file when no HU volume is
 //     given, and to recomputing when the ACF file does not exist).
+// With FIR_PATH_CACHE the path cache is built on a second host thread
+// while the sinogram and maps are read (it depends on the geometry only).
 #include <KeyParser.h>
 #include <ProjData.h>
+#include <ProjInterfileReader.h>
 #include <ScannerData.h>
 #include <VolData.h>
 #include <console.h>
@@ -22,9 +25,12 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <string>
+#include <thread>
 
 namespace
 {
@@ -74,6 +80,43 @@ bool flag(int argc, char** argv, int i)
 }
 }
 
+// The path cache built on a second host thread (reconAlgos::buildPathCache
+// on the shared device context, whose stream orders its work with the
+// reads' uploads); the thread is joined on every exit path and its error,
+// if any, is raised by finish().
+class CacheBuilder
+{
+public:
+  void start(const fir_vol_desc& v, int nSubsets)
+  {
+    mThread = std::thread([this, v, nSubsets] {
+      try
+      {
+        mCache = reconAlgos::buildPathCache(v, nSubsets);
+      }
+      catch (...)
+      {
+        mError = std::current_exception();
+      }
+    });
+  }
+  fir_cache* finish()
+  {
+    if (mThread.joinable()) mThread.join();
+    if (mError) std::rethrow_exception(mError);
+    return mCache ? mCache->cache : nullptr;
+  }
+  ~CacheBuilder()
+  {
+    if (mThread.joinable()) mThread.join();
+  }
+
+private:
+  std::thread mThread;
+  std::unique_ptr<reconAlgos::PathCacheHandle> mCache;
+  std::exception_ptr mError;
+};
+
 int main(int argc, char** argv)
 {
   try
@@ -95,22 +138,33 @@ int main(int argc, char** argv)
 
     PhaseTimer total("total");
     PhaseTimer tRead("read inputs");
+    // geometry first (scanner, sinogram header, image grid): a path cache
+    // depends on nothing else, so it is built on a second host thread
+    // while the data are read
+    ScannerData scanner(params.scannerFile);
+    const ProjHeader projHeader =
+      ProjInterfileReader(params.inputProjFile).getHeader();
+    // initial estimate: the template's data, or ones
+    VolData outputVol(params.outputVolHeader,
+                      VolData::ConstructionMode::READ_DATA_IF_PROVIDED, 1.0);
+    device::useGeometry(scanner, projHeader);
+    fir_ctx* ctx = device::context();
+    const int nSub = params.algo.nSubsets;
+    CacheBuilder builder;
+    if (reconAlgos::pathCacheRequested())
+    {
+      const fir_vol_desc v = device::describe(outputVol.getHeader());
+      builder.start(v, nSub);
+    }
+
     // the sinogram goes to the device chunk by chunk as it is read; every
     // step below keeps its data there
     ProjData inputProj;
     device::Buffer y = device::readProj(params.inputProjFile, inputProj);
-    inputProj.checkNSubsets(params.algo.nSubsets);
-    ScannerData scanner(params.scannerFile);
+    inputProj.checkNSubsets(nSub);
     scanner.checkProjData(inputProj);
-
-    // initial estimate: the template's data, or ones
-    VolData outputVol(params.outputVolHeader,
-                      VolData::ConstructionMode::READ_DATA_IF_PROVIDED, 1.0);
     printEmptyLine();
     tRead.stop();
-    device::useGeometry(scanner, inputProj.getHeader());
-    fir_ctx* ctx = device::context();
-    const int nSub = params.algo.nSubsets;
     const size_t vb =
       static_cast<size_t>(outputVol.getNVoxelsPerFrame()) * sizeof(float);
     const size_t nb = static_cast<size_t>(inputProj.getGeometry().nBins);
@@ -159,6 +213,9 @@ int main(int argc, char** argv)
         error("Bias projection doesn't fit with input projection");
       in.bias = bias.f();
     }
+
+    // the cache build is done before the context is used for projections
+    in.cache = builder.finish();
 
     PhaseTimer tAcf("attenuation");
     if (huNamed || acfExists)

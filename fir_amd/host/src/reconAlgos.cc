@@ -40,14 +40,49 @@ long long envNumber(const char* name, double scale)
   return static_cast<long long>(d * scale);
 }
 
-struct CacheHolder
+} // namespace
+
+PathCacheHandle::~PathCacheHandle()
 {
-  fir_cache* cache = nullptr;
-  ~CacheHolder()
-  {
-    if (cache) fir_cache_destroy(cache);
-  }
-};
+  if (cache) fir_cache_destroy(cache);
+}
+
+bool pathCacheRequested() { return pathCacheFormat() >= 0; }
+
+std::unique_ptr<PathCacheHandle> buildPathCache(const fir_vol_desc& v,
+                                                int nSubsets)
+{
+  const int cacheFormat = pathCacheFormat();
+  if (cacheFormat < 0) return nullptr;
+  fir_ctx* ctx = device::context();
+  auto h = std::make_unique<PathCacheHandle>();
+  print("Building the path cache");
+  PhaseTimer tBuild("path cache build");
+  // Cache rows per view: tangential bin, then pair by centre slice.
+  device::check(fir_set_lor_order(
+    ctx, FIR_ORDER_VIEW_MAJOR | FIR_ORDER_TANG_MAJOR | FIR_ORDER_ZCENTER));
+  const long long maxSubsets = envNumber("FIR_PATH_CACHE_SUBSETS", 1.0);
+  const long long maxBytes = envNumber("FIR_PATH_CACHE_GB", 1e9);
+  int rc = FIR_E_NOMEM;
+  if (maxSubsets == 0 && maxBytes == 0)
+    rc = fir_cache_create(ctx, &v, nSubsets, cacheFormat, &h->cache);
+  if (rc == FIR_E_NOMEM)
+    rc = fir_cache_create_partial(ctx, &v, nSubsets, cacheFormat,
+                                  static_cast<int>(maxSubsets), maxBytes,
+                                  &h->cache);
+  device::check(fir_set_lor_order(ctx, FIR_ORDER_SEGMENT));
+  device::check(rc);
+  device::check(fir_synchronize(ctx));
+  tBuild.stop();
+  device::check(fir_cache_subsets(h->cache, nullptr, &h->nCached));
+  if (h->nCached < nSubsets)
+    print("The path cache holds subsets 1 to ", h->nCached, " of ", nSubsets,
+          "; the others are walked on the fly");
+  return h;
+}
+
+namespace
+{
 
 // Shared body of OSEM and OSEM_ResoReco on device-resident data. With reso
 // (reconAlgos.cc:203-367): every sensitivity frame is convolved once (in
@@ -89,33 +124,22 @@ void osemCore(const ProjData& inputProj, const ScannerData& scanner,
   device::check(fir_fill(ctx, bp.f(), static_cast<int64_t>(nv), 0.0f));
   device::check(fir_cut_circle(ctx, &v, x.f(), 1, p.cutRadius));
 
-  CacheHolder cached;
-  int nCached = 0;
-  const int cacheFormat = pathCacheFormat();
-  if (cacheFormat >= 0)
+  // the path cache handed in, or one built here when FIR_PATH_CACHE asks
+  std::unique_ptr<PathCacheHandle> own;
+  struct
   {
-    print("Building the path cache");
-    PhaseTimer tBuild("path cache build");
-    // Cache rows per view: tangential bin, then pair by centre slice.
-    device::check(fir_set_lor_order(
-      ctx, FIR_ORDER_VIEW_MAJOR | FIR_ORDER_TANG_MAJOR | FIR_ORDER_ZCENTER));
-    const long long maxSubsets = envNumber("FIR_PATH_CACHE_SUBSETS", 1.0);
-    const long long maxBytes = envNumber("FIR_PATH_CACHE_GB", 1e9);
-    int rc = FIR_E_NOMEM;
-    if (maxSubsets == 0 && maxBytes == 0)
-      rc = fir_cache_create(ctx, &v, p.nSubsets, cacheFormat, &cached.cache);
-    if (rc == FIR_E_NOMEM)
-      rc = fir_cache_create_partial(ctx, &v, p.nSubsets, cacheFormat,
-                                    static_cast<int>(maxSubsets), maxBytes,
-                                    &cached.cache);
-    device::check(fir_set_lor_order(ctx, FIR_ORDER_SEGMENT));
-    device::check(rc);
-    device::check(fir_synchronize(ctx));
-    tBuild.stop();
-    device::check(fir_cache_subsets(cached.cache, nullptr, &nCached));
-    if (nCached < p.nSubsets)
-      print("The path cache holds subsets 1 to ", nCached, " of ", p.nSubsets,
-            "; the others are walked on the fly");
+    fir_cache* cache = nullptr;
+  } cached;
+  int nCached = 0;
+  if (in.cache)
+  {
+    cached.cache = in.cache;
+    device::check(fir_cache_subsets(in.cache, nullptr, &nCached));
+  }
+  else if ((own = buildPathCache(v, p.nSubsets)))
+  {
+    cached.cache = own->cache;
+    nCached = own->nCached;
   }
   if (in.computeSens)
   {
