@@ -1069,6 +1069,47 @@ __global__ void k_sh_fam_rows(const int* __restrict__ rowid, int n_pairs,
   else if (f == n_fam) fam_row[f] = n_rows;
 }
 
+// The voxels of up to kShK * 64 slots of each of two rows (lane ln takes
+// slots ln, ln + 64, ...), all loads in flight together: v[h][k] is slot
+// off[r_h] + ln + 64 k's voxel, or -1 (no row, past the row, or a
+// zero-length element). Slots past kShK * 64 are left to the caller.
+constexpr int kShK = 6;
+__device__ __forceinline__ void sh_load_pair(
+  const long long* __restrict__ off, const int* __restrict__ idx,
+  const void* __restrict__ len, int lb,
+  const unsigned char* __restrict__ yrow, const XY& xy, const int* r,
+  int r_end, int ln, int (*v)[kShK], long long* lo, long long* hi,
+  bool* yf)
+{
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+  {
+    const bool ok = r[h] < r_end;
+    lo[h] = ok ? off[r[h]] : 0;
+    hi[h] = ok ? off[r[h] + 1] : 0;
+    yf[h] = ok && yrow[r[h]] != 0;
+  }
+  int ix[2][kShK];
+  unsigned lv[2][kShK];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int k = 0; k < kShK; ++k)
+    {
+      const long long e = lo[h] + ln + 64 * k;
+      const bool in = e < hi[h];
+      ix[h][k] = in ? idx[e] : 0;
+      lv[h][k] = in ? (lb == 2 ? ((const unsigned short*)len)[e]
+                               : ((const unsigned*)len)[e])
+                    : 0u;
+    }
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int k = 0; k < kShK; ++k)
+      v[h][k] = lv[h][k] ? from_layout(xy, ix[h][k], yf[h]) : -1;
+}
+
 // Sheet pixels of every family, one block per family: a bitmap of the
 // pixels its rows' contributing elements cross, built in LDS (a wave per
 // row) and written out with the pixel count.
@@ -1086,16 +1127,30 @@ __global__ __launch_bounds__(kBlock) void k_sh_mark_fam(
   __syncthreads();
   const int nxy = xy.nx * xy.ny;
   const int wave = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
-  for (int r = fam_row[f] + wave; r < fam_row[f + 1]; r += kBlock / 64)
+  const int n_waves = kBlock / 64;
+  const int rf = fam_row[f], rl = fam_row[f + 1];
+  auto mark = [&](int v) {
+    const int px = v % nxy;
+    atomicOr(wb + px / 64, 1ULL << (px % 64));
+  };
+  for (int r0 = rf + wave; r0 < rl; r0 += 2 * n_waves)
   {
-    const bool yf = yrow[r] != 0;
-    const long long end = off[r + 1];
-    for (long long e = off[r] + ln; e < end; e += 64)
+    const int r[2] = {r0, r0 + n_waves};
+    int v[2][kShK];
+    long long lo[2], hi[2];
+    bool yf[2];
+    sh_load_pair(off, idx, len, lb, yrow, xy, r, rl, ln, v, lo, hi, yf);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
     {
-      const int v = tr_voxel(idx, len, lb, xy, yf, e, end);
-      if (v < 0) continue;
-      const int px = v % nxy;
-      atomicOr(wb + px / 64, 1ULL << (px % 64));
+#pragma unroll
+      for (int k = 0; k < kShK; ++k)
+        if (v[h][k] >= 0) mark(v[h][k]);
+      for (long long e = lo[h] + ln + 64 * kShK; e < hi[h]; e += 64)
+      {
+        const int w = tr_voxel(idx, len, lb, xy, yf[h], e, hi[h]);
+        if (w >= 0) mark(w);
+      }
     }
   }
   __syncthreads();
@@ -1120,8 +1175,9 @@ __global__ __launch_bounds__(kBlock) void k_sh_mark_fam(
 // row visits its pixels in rising rank, and each lane's elements rise too),
 // written to fam_xy in rank order; then every slot of its rows gets its
 // sheet index, rank * nz + z (0 for padding and zero-length elements),
-// with the length's 16 bits above. Dynamic LDS: words * 12 bytes plus 14
-// bytes per sheet pixel.
+// with the length's 16 bits above; list_quads[f] (cleared beforehand)
+// gets the quads of the family's per-entry element lists. Dynamic LDS:
+// words * 12 bytes, 14 bytes per sheet pixel and 2 per sheet entry.
 __global__ __launch_bounds__(kBlock) void k_sh_encode_fam(
   LorGeom lg, Grid g, int n_tang, const int* __restrict__ fam_row,
   const long long* __restrict__ off, const int* __restrict__ idx,
@@ -1129,7 +1185,7 @@ __global__ __launch_bounds__(kBlock) void k_sh_encode_fam(
   const unsigned char* __restrict__ yrow, XY xy, int nz, int words,
   const unsigned long long* __restrict__ bits,
   const int* __restrict__ fam_off, int line_order, int* __restrict__ fam_xy,
-  unsigned* __restrict__ sel)
+  unsigned* __restrict__ sel, unsigned* __restrict__ list_quads)
 {
   extern __shared__ unsigned long long wb[]; // [words]
   const int f = blockIdx.x;
@@ -1192,28 +1248,57 @@ __global__ __launch_bounds__(kBlock) void k_sh_encode_fam(
       fam_xy[p0 + i] = pxl[i];
     }
   __syncthreads();
+  // elements per sheet entry, two 16-bit counts to a word (a family has
+  // at most kFamRowsMax rows, each through an entry at most once)
+  unsigned* ecnt = reinterpret_cast<unsigned*>(rank + ((L + 1) & ~1)); // [n_st / 2]
+  const int n_st = L * nz;
+  for (int i = threadIdx.x; i < (n_st + 1) / 2; i += kBlock) ecnt[i] = 0;
+  __syncthreads();
   const int nxy = xy.nx * xy.ny;
   const int wave = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
-  for (int r = fam_row[f] + wave; r < fam_row[f + 1]; r += kBlock / 64)
-  {
-    const bool yf = yrow[r] != 0;
-    const long long end = off[r + 1];
-    for (long long e = off[r] + ln; e < end; e += 64)
+  const int n_waves = kBlock / 64;
+  const int rf = fam_row[f], rl = fam_row[f + 1];
+  const unsigned short* len16 = (const unsigned short*)len;
+  auto encode = [&](int v, long long e) {
+    unsigned out = 0;
+    if (v >= 0)
     {
-      const int v = tr_voxel(idx, len, lb, xy, yf, e, end);
-      unsigned out = 0;
-      if (v >= 0)
+      const int px = v % nxy, z = v / nxy;
+      const int k = px / 64;
+      const int pr = wpre[k] + __popcll(wb[k] & ((1ULL << (px % 64)) - 1ULL));
+      const unsigned en = (unsigned)(rank[pr] * nz + z);
+      out = en | ((unsigned)len16[e] << 16);
+      atomicAdd(ecnt + en / 2, 1u << (16 * (en & 1u)));
+    }
+    sel[e] = out;
+  };
+  for (int r0 = rf + wave; r0 < rl; r0 += 2 * n_waves)
+  {
+    const int r[2] = {r0, r0 + n_waves};
+    int v[2][kShK];
+    long long lo[2], hi[2];
+    bool yf[2];
+    sh_load_pair(off, idx, len, lb, yrow, xy, r, rl, ln, v, lo, hi, yf);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+#pragma unroll
+      for (int k = 0; k < kShK; ++k)
       {
-        const int px = v % nxy, z = v / nxy;
-        const int k = px / 64;
-        const int pr =
-          wpre[k] + __popcll(wb[k] & ((1ULL << (px % 64)) - 1ULL));
-        out = (unsigned)(rank[pr] * nz + z) |
-          ((unsigned)((const unsigned short*)len)[e] << 16);
+        const long long e = lo[h] + ln + 64 * k;
+        if (e < hi[h]) encode(v[h][k], e);
       }
-      sel[e] = out;
+      for (long long e = lo[h] + ln + 64 * kShK; e < hi[h]; e += 64)
+        encode(tr_voxel(idx, len, lb, xy, yf[h], e, hi[h]), e);
     }
   }
+  __syncthreads();
+  // the family's list quads (each entry's count rounded up to quads)
+  unsigned q = 0;
+  for (int i = threadIdx.x; i < n_st; i += kBlock)
+    q += (((ecnt[i / 2] >> (16 * (i & 1))) & 0xffffu) + 3u) / 4u;
+  for (int o = 32; o > 0; o >>= 1) q += __shfl_xor(q, o);
+  if (ln == 0) atomicAdd(list_quads + f, q);
 }
 
 // Whether every lane's contributing elements (lanes = the quad layout's)
@@ -1698,20 +1783,6 @@ __device__ __forceinline__ unsigned fl_scan(unsigned* cnt, int n_st,
   return total;
 }
 
-__global__ __launch_bounds__(kFlBlock) void k_fl_count(
-  const int* __restrict__ fam_row, const int* __restrict__ fam_off, int nz,
-  const long long* __restrict__ off, const unsigned* __restrict__ sel,
-  unsigned* __restrict__ ftot)
-{
-  __shared__ unsigned cnt[kSheetMaxFloats];
-  __shared__ unsigned part[kFlBlock];
-  const int f = blockIdx.x;
-  const int n_st = (fam_off[f + 1] - fam_off[f]) * nz;
-  fl_count(fam_row, off, sel, f, n_st, cnt);
-  const unsigned total = fl_scan(cnt, n_st, part);
-  if (threadIdx.x == 0) ftot[f] = total;
-}
-
 // Lists, fill pass: first quads of the family's entries (fl_off) and the
 // elements (fl, cleared beforehand), each list in row order.
 __global__ __launch_bounds__(kFlBlock) void k_fl_fill(
@@ -1938,6 +2009,245 @@ __global__ __launch_bounds__(kFlBlock) void k_fl_interleave(
           *reinterpret_cast<const uint4*>(cl + 4LL * (src + m));
       }
       roff += (unsigned)__popcll(act);
+    }
+  }
+}
+
+// The family lists in their final layout in one pass per family (the
+// interleave pass's layout, without the contiguous lists and the staging
+// buffer): the entries' list lengths, slots ordered by length (longest
+// first), each 64-slot group's round offsets (round m of a group holds the
+// m-th quad of its slots longer than m, which are a prefix of the group),
+// then the elements placed as k_fl_fill places them (rows in groups of 32,
+// each list in row order) straight at their final positions, and each
+// list's last quad padded with zeros. A family
+// whose round-offset table exceeds kFlRTable entries sets *overflow and
+// writes nothing (its lists then take the two-pass route).
+constexpr int kFlRTable = 4096;
+constexpr int kFlK = 6; // elements per lane of a row held in registers
+
+__global__ __launch_bounds__(kFlBlock) void k_fl_build(
+  const int* __restrict__ fam_row, const int* __restrict__ fam_off, int nz,
+  const long long* __restrict__ off, const unsigned* __restrict__ sel,
+  const long long* __restrict__ fbase, unsigned* __restrict__ fl_slot,
+  unsigned* __restrict__ fl_gb, unsigned* __restrict__ fl,
+  int* __restrict__ overflow)
+{
+  __shared__ unsigned cnt[kSheetMaxFloats];          // counts, then row bits
+  __shared__ unsigned short cur[kSheetMaxFloats];    // quads per slot, then
+                                                     // elements placed
+  __shared__ unsigned short slot[kSheetMaxFloats];   // entry -> slot
+  __shared__ unsigned rtab[kFlRTable];
+  __shared__ unsigned hist[kFlMaxQuads + 1];
+  __shared__ unsigned gbase[kSheetMaxFloats / 64 + 1];
+  __shared__ unsigned rofs[kSheetMaxFloats / 64 + 1];
+  __shared__ int too_big;
+  const int f = blockIdx.x;
+  const int n_st = (fam_off[f + 1] - fam_off[f]) * nz;
+  if (n_st == 0) return;
+  const long long g0 = (long long)fam_off[f] * nz;
+  const long long base = fbase[f];
+  const int n_groups = (n_st + 63) / 64;
+  const int r0 = fam_row[f], r1 = fam_row[f + 1];
+  for (int i = threadIdx.x; i < n_st; i += kFlBlock) cnt[i] = 0;
+  for (int i = threadIdx.x; i <= kFlMaxQuads; i += kFlBlock) hist[i] = 0;
+  if (threadIdx.x == 0) too_big = 0;
+  __syncthreads();
+  if (r0 < r1)
+  {
+    // eight loads in flight per thread
+    const long long ea = off[r0], eb = off[r1];
+    for (long long e0 = ea + threadIdx.x; e0 < eb; e0 += 8 * kFlBlock)
+    {
+      unsigned u[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+      {
+        const long long e = e0 + (long long)k * kFlBlock;
+        u[k] = e < eb ? sel[e] : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (u[k] >> 16) atomicAdd(cnt + (u[k] & 0xffffu), 1u);
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n_st; e += kFlBlock)
+    atomicAdd(hist + min((cnt[e] + 3u) / 4u, (unsigned)kFlMaxQuads), 1u);
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    unsigned run = 0;
+    for (int v = kFlMaxQuads; v >= 0; --v)
+    {
+      const unsigned c = hist[v];
+      hist[v] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n_st; e += kFlBlock)
+  {
+    const unsigned q = (cnt[e] + 3u) / 4u;
+    const unsigned sl = atomicAdd(hist + min(q, (unsigned)kFlMaxQuads), 1u);
+    slot[e] = (unsigned short)sl;
+    cur[sl] = (unsigned short)q;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    unsigned o = 0;
+    for (int g = 0; g < n_groups; ++g)
+    {
+      rofs[g] = o;
+      o += cur[64 * g] + 1u; // the group's longest list comes first
+    }
+    too_big = o > (unsigned)kFlRTable;
+  }
+  __syncthreads();
+  if (too_big)
+  {
+    if (threadIdx.x == 0) atomicOr(overflow, 1);
+    return;
+  }
+  for (int g = threadIdx.x; g < n_groups; g += kFlBlock)
+  {
+    const int s0 = 64 * g;
+    const int M = cur[s0];
+    int i = min(64, n_st - s0); // slots longer than m: a prefix of the group
+    unsigned R = 0;
+    for (int m = 0; m < M; ++m)
+    {
+      while (i > 0 && cur[s0 + i - 1] <= m) --i;
+      rtab[rofs[g] + m] = R;
+      R += (unsigned)i;
+    }
+    rtab[rofs[g] + M] = R;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    unsigned o = 0;
+    for (int g = 0; g < n_groups; ++g)
+    {
+      gbase[g] = o;
+      o += rtab[rofs[g] + cur[64 * g]];
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n_st; e += kFlBlock)
+  {
+    const unsigned sl = slot[e];
+    fl_slot[g0 + sl] = ((unsigned)e << 16) | cur[sl];
+    fl_gb[g0 + sl] = (unsigned)(base + gbase[sl >> 6]);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_st; i += kFlBlock)
+  {
+    cnt[i] = 0;
+    cur[i] = 0;
+  }
+  __syncthreads();
+  // Rows in groups of 32, a wave taking two rows of each: the rows' first
+  // 64 * kFlK elements are loaded into registers once (all in flight
+  // together) and serve the group's three passes; longer rows' remainders
+  // are read again in each pass.
+  const int wave = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
+  for (int ra = r0; ra < r1; ra += 32)
+  {
+    const int rb = min(ra + 32, r1);
+    unsigned v[2][kFlK];
+    long long lo[2], hi[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+      const int r = ra + wave + 16 * h;
+      lo[h] = r < rb ? off[r] : 0;
+      hi[h] = r < rb ? off[r + 1] : 0;
+#pragma unroll
+      for (int k = 0; k < kFlK; ++k)
+      {
+        const long long e = lo[h] + ln + 64 * k;
+        v[h][k] = e < hi[h] ? sel[e] : 0u;
+      }
+    }
+    // pass 1: the group's rows through each entry
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+      const unsigned bit = 1u << (wave + 16 * h);
+#pragma unroll
+      for (int k = 0; k < kFlK; ++k)
+        if (v[h][k] >> 16) atomicOr(cnt + (v[h][k] & 0xffffu), bit);
+      for (long long e = lo[h] + ln + 64 * kFlK; e < hi[h]; e += 64)
+      {
+        const unsigned u = sel[e];
+        if (u >> 16) atomicOr(cnt + (u & 0xffffu), bit);
+      }
+    }
+    __syncthreads();
+    // pass 2: each element at its entry's count plus the group's earlier
+    // rows through it
+    auto place = [&](unsigned u, int rl) {
+      const unsigned en = u & 0xffffu;
+      const unsigned c =
+        cur[en] + (unsigned)__popc(cnt[en] & ((1u << rl) - 1u));
+      const unsigned sl = slot[en], g = sl >> 6;
+      const unsigned long long quad =
+        (unsigned long long)(base + gbase[g]) + rtab[rofs[g] + (c >> 2)] +
+        (sl & 63u);
+      fl[quad * 4 + (c & 3u)] = (unsigned)(ra + rl - r0) | (u & 0xffff0000u);
+    };
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+      const int rl = wave + 16 * h;
+#pragma unroll
+      for (int k = 0; k < kFlK; ++k)
+        if (v[h][k] >> 16) place(v[h][k], rl);
+      for (long long e = lo[h] + ln + 64 * kFlK; e < hi[h]; e += 64)
+      {
+        const unsigned u = sel[e];
+        if (u >> 16) place(u, rl);
+      }
+    }
+    __syncthreads();
+    // pass 3: the entry's first row in the group advances its count and
+    // clears the bits
+    auto advance = [&](unsigned u, int rl) {
+      const unsigned en = u & 0xffffu;
+      const unsigned b = cnt[en];
+      if ((b & ((1u << rl) - 1u)) == 0)
+      {
+        cur[en] = (unsigned short)(cur[en] + __popc(b));
+        cnt[en] = 0;
+      }
+    };
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+      const int rl = wave + 16 * h;
+#pragma unroll
+      for (int k = 0; k < kFlK; ++k)
+        if (v[h][k] >> 16) advance(v[h][k], rl);
+      for (long long e = lo[h] + ln + 64 * kFlK; e < hi[h]; e += 64)
+      {
+        const unsigned u = sel[e];
+        if (u >> 16) advance(u, rl);
+      }
+    }
+    __syncthreads();
+  }
+  // the zero padding of each list's last quad (cur: the lists' lengths)
+  for (int en = threadIdx.x; en < n_st; en += kFlBlock)
+  {
+    const unsigned sl = slot[en], g = sl >> 6;
+    for (unsigned c = cur[en]; c & 3u; ++c)
+    {
+      const unsigned long long quad =
+        (unsigned long long)(base + gbase[g]) + rtab[rofs[g] + (c >> 2)] +
+        (sl & 63u);
+      fl[quad * 4 + (c & 3u)] = 0u;
     }
   }
 }
@@ -2225,6 +2535,7 @@ struct RowMatrix
   unsigned* d_fl_gb = nullptr;      // [n_sheet_px * nz] group's first quad
   unsigned* d_fl = nullptr;         // [4 * fl_quads]
   long long fl_quads = 0;
+  std::vector<unsigned> list_quads; // per family, from the sheet build
   int sheets_req = 0; // the context asked for the sheet form at build
   int n_views = 0;
   int n_rows = 0;
@@ -2573,8 +2884,9 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
     off[f + 1] = off[f] + npx[f];
     max_px = std::max(max_px, npx[f]);
   }
-  const size_t enc_lds =
-    8 * (size_t)words + 4 * (size_t)((words + 1) & ~1) + 14 * (size_t)max_px;
+  const size_t enc_lds = 8 * (size_t)words +
+    4 * (size_t)((words + 1) & ~1) + 12 * (size_t)max_px +
+    2 * (size_t)((max_px + 1) & ~1) + 2 * (size_t)(max_px * nz + 1);
   if ((long long)max_px * nz > kSheetMaxFloats || enc_lds > 65536)
     return FIR_OK;
   if ((rc = dev_alloc(&rm->d_fam_off, (size_t)F + 1, "sheet offsets")) ||
@@ -2596,12 +2908,20 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
   if ((rc = make_grid(c, &v, &g))) return rc;
   const int n_tang = F / rm->n_views;
   const bool line = F % rm->n_views == 0;
+  unsigned* list_quads;
+  if ((rc = tmp.get(&list_quads, (size_t)F, "family list sizes"))) return rc;
+  FIR_HIP_CHECK(hipMemsetAsync(list_quads, 0, sizeof(unsigned) * F,
+                               c->stream));
   hipLaunchKernelGGL(k_sh_encode_fam, dim3(F), dim3(kBlock), enc_lds,
                      c->stream, lor_geom_views(c), g, n_tang, rm->d_fam_row,
                      rm->d_off, rm->d_idx, rm->d_len, lb, rm->d_yrow, xy, nz,
                      words, bits, rm->d_fam_off, line ? 1 : 0, rm->d_fam_xy,
-                     rm->d_sel);
+                     rm->d_sel, list_quads);
   FIR_CHECK_LAUNCH();
+  rm->list_quads.resize(F);
+  FIR_HIP_CHECK(hipMemcpyAsync(rm->list_quads.data(), list_quads,
+                               sizeof(unsigned) * F, hipMemcpyDeviceToHost,
+                               c->stream));
   rm->line_order = 0;
   if (line)
   {
@@ -2634,21 +2954,13 @@ int build_family_lists(fir_ctx* c, RowMatrix* rm)
   if (!rm->d_sel || rm->n_pairs > kFamRowsMax || rm->n_sheet_px <= 0)
     return FIR_OK;
   DevBufs tmp;
-  unsigned* ftot;
   long long* fbase;
   int rc;
-  if ((rc = tmp.get(&ftot, (size_t)F, "family list sizes")) ||
-      (rc = tmp.get(&fbase, (size_t)F, "family list bases")))
-    return rc;
-  hipLaunchKernelGGL(k_fl_count, dim3(F), dim3(kFlBlock), 0, c->stream,
-                     rm->d_fam_row, rm->d_fam_off, nz, rm->d_off, rm->d_sel,
-                     ftot);
-  FIR_CHECK_LAUNCH();
-  std::vector<unsigned> tot(F);
+  if ((rc = tmp.get(&fbase, (size_t)F, "family list bases"))) return rc;
+  // the quads of every family's lists, counted by the sheet build
+  const std::vector<unsigned>& tot = rm->list_quads;
+  if ((int)tot.size() != F) return FIR_OK;
   std::vector<long long> base(F);
-  FIR_HIP_CHECK(hipMemcpyAsync(tot.data(), ftot, 4LL * F,
-                               hipMemcpyDeviceToHost, c->stream));
-  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
   long long quads = 0;
   for (int f = 0; f < F; ++f)
   {
@@ -2659,6 +2971,49 @@ int build_family_lists(fir_ctx* c, RowMatrix* rm)
   if (quads >= (1LL << 32)) return FIR_OK;
   size_t free_b = 0, total_b = 0;
   FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  if ((long long)free_b >= 16 * quads + 8 * entries + (3LL << 30))
+  {
+    // one pass per family into the final layout
+    int* overflow;
+    if ((rc = tmp.get(&overflow, 1, "family list overflow flag"))) return rc;
+    if ((rc = dev_alloc(&rm->d_fl_slot, (size_t)entries,
+                        "family list slots")) ||
+        (rc = dev_alloc(&rm->d_fl_gb, (size_t)entries, "family list groups")) ||
+        (rc = dev_alloc(&rm->d_fl, (size_t)(4 * std::max(quads, 1LL)),
+                        "family lists")))
+    {
+      (void)hipFree(rm->d_fl_slot);
+      (void)hipFree(rm->d_fl_gb);
+      rm->d_fl_slot = nullptr;
+      rm->d_fl_gb = nullptr;
+      rm->d_fl = nullptr;
+      return rc == FIR_E_NOMEM ? FIR_OK : rc;
+    }
+    FIR_HIP_CHECK(hipMemcpyAsync(fbase, base.data(), 8LL * F,
+                                 hipMemcpyHostToDevice, c->stream));
+    FIR_HIP_CHECK(hipMemsetAsync(overflow, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_fl_build, dim3(F), dim3(kFlBlock), 0, c->stream,
+                       rm->d_fam_row, rm->d_fam_off, nz, rm->d_off, rm->d_sel,
+                       fbase, rm->d_fl_slot, rm->d_fl_gb, rm->d_fl, overflow);
+    FIR_CHECK_LAUNCH();
+    int ovf = 0;
+    FIR_HIP_CHECK(hipMemcpyAsync(&ovf, overflow, sizeof(int),
+                                 hipMemcpyDeviceToHost, c->stream));
+    FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (!ovf)
+    {
+      rm->fl_quads = quads;
+      return FIR_OK;
+    }
+    // some family's round table is too large for LDS: the two-pass route
+    (void)hipFree(rm->d_fl_slot);
+    (void)hipFree(rm->d_fl_gb);
+    (void)hipFree(rm->d_fl);
+    rm->d_fl_slot = nullptr;
+    rm->d_fl_gb = nullptr;
+    rm->d_fl = nullptr;
+    FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  }
   // the contiguous sorted lists go into the final buffer first; runs of
   // families are then interleaved through a staging buffer and copied back
   const long long stage_q = std::min(quads, kFlStageQuads);
@@ -2939,11 +3294,15 @@ int build_rows(fir_ctx* c, const fir_vol_desc* v, int fmt,
   if ((rc = dev_alloc((unsigned char**)&rm->d_len,
                       rm->n_slots * len_bytes(fmt), "element lengths")))
     return rc;
-  // padding elements: index 0, length 0
-  FIR_HIP_CHECK(hipMemsetAsync(rm->d_idx, 0, sizeof(int) * rm->n_slots,
-                               c->stream));
-  FIR_HIP_CHECK(hipMemsetAsync(rm->d_len, 0, rm->n_slots * len_bytes(fmt),
-                               c->stream));
+  // padding elements: index 0, length 0 (the fill writes them itself when
+  // it stages whole rounds, lanes <= 8)
+  if (c->cache_lanes[0] > 8)
+  {
+    FIR_HIP_CHECK(hipMemsetAsync(rm->d_idx, 0, sizeof(int) * rm->n_slots,
+                                 c->stream));
+    FIR_HIP_CHECK(hipMemsetAsync(rm->d_len, 0, rm->n_slots * len_bytes(fmt),
+                                 c->stream));
+  }
 
   hipLaunchKernelGGL(k_pc_views, dim3(blocks_for(c->n_views + 1)),
                      dim3(kBlock), 0, c->stream, count, rowid, eoff, n,
