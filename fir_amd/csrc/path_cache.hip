@@ -1176,8 +1176,11 @@ __global__ __launch_bounds__(kBlock) void k_sh_mark_fam(
 // written to fam_xy in rank order; then every slot of its rows gets its
 // sheet index, rank * nz + z (0 for padding and zero-length elements),
 // with the length's 16 bits above; list_quads[f] (cleared beforehand)
-// gets the quads of the family's per-entry element lists. Dynamic LDS:
-// words * 12 bytes, 14 bytes per sheet pixel and 2 per sheet entry.
+// gets the quads of the family's per-entry element lists, and fam_zr each
+// sheet pixel's slices [lo, hi) that its elements cross (lo | hi << 16, in
+// rank order; the slices outside hold no element of the family: the
+// projectors neither stage nor sum them). Dynamic LDS: words * 12 bytes,
+// 14 bytes per sheet pixel and 2 per sheet entry.
 __global__ __launch_bounds__(kBlock) void k_sh_encode_fam(
   LorGeom lg, Grid g, int n_tang, const int* __restrict__ fam_row,
   const long long* __restrict__ off, const int* __restrict__ idx,
@@ -1185,7 +1188,8 @@ __global__ __launch_bounds__(kBlock) void k_sh_encode_fam(
   const unsigned char* __restrict__ yrow, XY xy, int nz, int words,
   const unsigned long long* __restrict__ bits,
   const int* __restrict__ fam_off, int line_order, int* __restrict__ fam_xy,
-  unsigned* __restrict__ sel, unsigned* __restrict__ list_quads)
+  unsigned* __restrict__ fam_zr, unsigned* __restrict__ sel,
+  unsigned* __restrict__ list_quads)
 {
   extern __shared__ unsigned long long wb[]; // [words]
   const int f = blockIdx.x;
@@ -1293,6 +1297,18 @@ __global__ __launch_bounds__(kBlock) void k_sh_encode_fam(
     }
   }
   __syncthreads();
+  auto count = [&](int en) { return (ecnt[en / 2] >> (16 * (en & 1))) & 0xffffu; };
+  for (int p = threadIdx.x; p < L; p += kBlock)
+  {
+    int lo = nz, hi = 0;
+    for (int z = 0; z < nz; ++z)
+      if (count(p * nz + z))
+      {
+        lo = min(lo, z);
+        hi = z + 1;
+      }
+    fam_zr[p0 + p] = lo < hi ? ((unsigned)lo | ((unsigned)hi << 16)) : 0u;
+  }
   // the family's list quads (each entry's count rounded up to quads)
   unsigned q = 0;
   for (int i = threadIdx.x; i < n_st; i += kBlock)
@@ -1401,29 +1417,25 @@ __device__ __forceinline__ void sheet_sum(const unsigned* __restrict__ sel,
   }
 }
 
-// k_pc_osem_rows for one image over the sheet form: one block per family
-// of the subset (blockIdx.y: the subset's k-th view, blockIdx.x: the
-// tangential bin) stages the family's sheet of `img` (x fastest) in LDS,
-// then reduces its rows as k_pc_osem_rows does.
-template <int F>
-__global__ __launch_bounds__(kSheetBlock) void k_pc_osem_rows_sheet(
-  int n_subsets, int subset, int n_tang, int n_fam,
-  const int* __restrict__ fam_row, const int* __restrict__ fam_off,
-  const int* __restrict__ fam_xy, const unsigned* __restrict__ sel,
-  RowData rd, int lanes, const float* __restrict__ img, int nxy, int nz,
-  OsemRowsIO io, float* __restrict__ w)
+// A family's sheet of one image staged into LDS from the image's
+// pixel-major copy xpz[pixel * nz + z]: sh[p * nz + z] for the slices
+// [lo, hi) of each sheet pixel p that the family's elements cross
+// (fam_zr), consecutive threads reading consecutive slices of one pixel,
+// eight loads in flight per thread. The entries outside are never read,
+// except entry 0 by padding and zero-length slots (length 0), which is set
+// to 0 when it lies outside.
+__device__ __forceinline__ void stage_sheet(float* sh,
+                                            const float* __restrict__ xpz,
+                                            const int* __restrict__ fam_xy,
+                                            const unsigned* __restrict__ fam_zr,
+                                            int p0, int L, int nz)
 {
-  __shared__ float sh[kSheetMaxFloats];
-  const int view = (int)blockIdx.y * n_subsets + subset;
-  const int f = view * n_tang + (int)blockIdx.x;
-  if (f >= n_fam) return;
-  const int r0 = fam_row[f], r1 = fam_row[f + 1];
-  if (r0 == r1) return;
-  const int p0 = fam_off[f];
-  const int L = fam_off[f + 1] - p0;
-  // Staging: eight loads in flight per thread before their LDS stores.
   const int n_st = L * nz;
-  if (n_st == 0 && threadIdx.x == 0) sh[0] = 0.0f; // padding slots read it
+  if (threadIdx.x == 0)
+  {
+    const unsigned r = n_st ? fam_zr[p0] : 0u;
+    if ((r & 0xffffu) != 0u || (r >> 16) == 0u) sh[0] = 0.0f;
+  }
   for (int i0 = threadIdx.x; i0 < n_st; i0 += 8 * kSheetBlock)
   {
     float val[8];
@@ -1436,15 +1448,41 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_rows_sheet(
       val[u] = 0.0f;
       if (i < n_st)
       {
-        const int z = i / L, p = i - z * L;
-        at[u] = p * nz + z;
-        val[u] = img[(long long)z * nxy + fam_xy[p0 + p]];
+        const int p = i / nz, z = i - p * nz;
+        const unsigned r = fam_zr[p0 + p];
+        if ((unsigned)z >= (r & 0xffffu) && (unsigned)z < (r >> 16))
+        {
+          at[u] = i;
+          val[u] = xpz[(long long)fam_xy[p0 + p] * nz + z];
+        }
       }
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u)
       if (at[u] >= 0) sh[at[u]] = val[u];
   }
+}
+
+// k_pc_osem_rows for one image over the sheet form: one block per family
+// of the subset (blockIdx.y: the subset's k-th view, blockIdx.x: the
+// tangential bin) stages the family's sheet of `img` (x fastest) in LDS,
+// then reduces its rows as k_pc_osem_rows does.
+template <int F>
+__global__ __launch_bounds__(kSheetBlock) void k_pc_osem_rows_sheet(
+  const int* __restrict__ ford,
+  const int* __restrict__ fam_row, const int* __restrict__ fam_off,
+  const int* __restrict__ fam_xy, const unsigned* __restrict__ fam_zr,
+  const unsigned* __restrict__ sel, RowData rd, int lanes,
+  const float* __restrict__ img, int nxy, int nz, OsemRowsIO io,
+  float* __restrict__ w)
+{
+  __shared__ float sh[kSheetMaxFloats];
+  const int f = ford[blockIdx.x];
+  const int r0 = fam_row[f], r1 = fam_row[f + 1];
+  if (r0 == r1) return;
+  const int p0 = fam_off[f];
+  const int L = fam_off[f + 1] - p0;
+  stage_sheet(sh, img, fam_xy, fam_zr, p0, L, nz);
   const int lane = (int)threadIdx.x & (lanes - 1);
   const int per_round = kSheetBlock / lanes;
   __syncthreads();
@@ -1591,16 +1629,14 @@ __device__ __forceinline__ void slab_stage(const float4* __restrict__ X4,
 // next slab's loads in flight while the current one is summed.
 template <int F, int NB, int R, int D>
 __global__ __launch_bounds__(kSlabBlockMax) void k_pc_osem_rows_slab(
-  int n_subsets, int subset, int n_tang, int n_fam,
+  const int* __restrict__ ford,
   const int* __restrict__ fam_row, const int* __restrict__ fam_off,
   const int* __restrict__ fam_xy, const unsigned* __restrict__ sel,
   RowData rd, int lanes, const float* __restrict__ X, int nz, int slab_px,
   int buf_f4, OsemRowsIO io, float* __restrict__ w)
 {
   extern __shared__ float4 slab[];
-  const int view = (int)blockIdx.y * n_subsets + subset;
-  const int f = view * n_tang + (int)blockIdx.x;
-  if (f >= n_fam) return;
+  const int f = ford[blockIdx.x];
   const int r0 = fam_row[f], r1 = fam_row[f + 1];
   if (r0 == r1) return;
   const int p0 = fam_off[f];
@@ -1876,13 +1912,15 @@ __global__ void k_fr_count(const int* __restrict__ fam_off,
 
 // ... and the entries themselves, as positions among the sheet pixels of
 // their subset's families (fam_loc: a family's first position), which is
-// where the launches over that subset keep the family's partial sheet.
+// where the launches over that subset keep the family's partial sheet,
+// with the slice range of each (the partials outside are never written).
 __global__ void k_fr_fill(const int* __restrict__ fam_off,
                           const int* __restrict__ fam_xy,
+                          const unsigned* __restrict__ fam_zr,
                           const int* __restrict__ fam_loc, int n_fam,
                           int total, int n_tang, int S, int nxy,
                           const long long* __restrict__ off,
-                          unsigned* __restrict__ cursor, int* __restrict__ ent)
+                          unsigned* __restrict__ cursor, int2* __restrict__ ent)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
@@ -1895,21 +1933,23 @@ __global__ void k_fr_fill(const int* __restrict__ fam_off,
   }
   const int s = (lo / n_tang) % S;
   const long long k = (long long)s * (nxy + 1) + fam_xy[i];
-  ent[off[k] + atomicAdd(cursor + k, 1u)] = fam_loc[lo] + (i - fam_off[lo]);
+  const long long at = off[k] + atomicAdd(cursor + k, 1u);
+  ent[at] = make_int2(fam_loc[lo] + (i - fam_off[lo]), (int)fam_zr[i]);
 }
 
-// Each pixel's entries in increasing order (family order).
+// Each pixel's entries in increasing order (family order), their slice
+// ranges alongside.
 __global__ void k_fr_sort(const long long* __restrict__ off, long long n,
-                          int* __restrict__ ent)
+                          int2* __restrict__ ent)
 {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const long long a = off[k], b = off[k + 1];
   for (long long i = a + 1; i < b; ++i)
   {
-    const int e = ent[i];
+    const int2 e = ent[i];
     long long j = i - 1;
-    while (j >= a && ent[j] > e)
+    while (j >= a && ent[j].x > e.x)
     {
       ent[j + 1] = ent[j];
       --j;
@@ -1927,15 +1967,32 @@ __global__ void k_fr_sort(const long long* __restrict__ off, long long n,
 // fl[4 * (quad - dst_base)] (a staging buffer for a run of families).
 constexpr int kFlMaxQuads = kFamRowsMax / 4; // a voxel is on a row at most once
 
+// Sort key of a family's entry e (pixel rank * nz + z) holding q list
+// quads, slots taken in decreasing key: entries with a list by length, then
+// the empty ones inside their pixel's slice range (whose partials are
+// written as 0, as the per-voxel sums read them), then the empty ones
+// outside it (never read or written: the slots past the family's count of
+// active slots).
+__device__ __forceinline__ unsigned fl_sort_key(unsigned q,
+                                                const unsigned* zr, int e,
+                                                int nz)
+{
+  if (q) return min(q, (unsigned)kFlMaxQuads) + 1u;
+  const int p = e / nz, z = e - p * nz;
+  const unsigned r = zr[p];
+  return (unsigned)z >= (r & 0xffffu) && (unsigned)z < (r >> 16) ? 1u : 0u;
+}
+
 __global__ __launch_bounds__(kFlBlock) void k_fl_interleave(
   int f_first, const int* __restrict__ fam_off, int nz,
-  const long long* __restrict__ fbase, const unsigned* __restrict__ cl_off,
-  const unsigned* __restrict__ cl, unsigned* __restrict__ fl_slot,
-  unsigned* __restrict__ fl_gb, unsigned* __restrict__ fl, long long dst_base)
+  const unsigned* __restrict__ fam_zr, const long long* __restrict__ fbase,
+  const unsigned* __restrict__ cl_off, const unsigned* __restrict__ cl,
+  unsigned* __restrict__ fl_slot, unsigned* __restrict__ fl_gb,
+  unsigned* __restrict__ fl, int* __restrict__ fl_nact, long long dst_base)
 {
   __shared__ unsigned short lq[kSheetMaxFloats];
   __shared__ unsigned short ord[kSheetMaxFloats];
-  __shared__ unsigned hist[kFlMaxQuads + 1];
+  __shared__ unsigned hist[kFlMaxQuads + 2];
   __shared__ unsigned gpre[kSheetMaxFloats / 64 + 1];
   const int f = f_first + (int)blockIdx.x;
   const int n_st = (fam_off[f + 1] - fam_off[f]) * nz;
@@ -1943,29 +2000,31 @@ __global__ __launch_bounds__(kFlBlock) void k_fl_interleave(
   const long long g0 = (long long)fam_off[f] * nz;
   const long long base = fbase[f];
   const int n_groups = (n_st + 63) / 64;
-  for (int i = threadIdx.x; i <= kFlMaxQuads; i += kFlBlock) hist[i] = 0;
+  const unsigned* zr = fam_zr + fam_off[f];
+  for (int i = threadIdx.x; i <= kFlMaxQuads + 1; i += kFlBlock) hist[i] = 0;
   for (int i = threadIdx.x; i <= n_groups; i += kFlBlock) gpre[i] = 0;
   __syncthreads();
   for (int e = threadIdx.x; e < n_st; e += kFlBlock)
   {
     const unsigned q = cl_off[g0 + e + 1] - cl_off[g0 + e];
     lq[e] = (unsigned short)q;
-    atomicAdd(hist + min(q, (unsigned)kFlMaxQuads), 1u);
+    atomicAdd(hist + fl_sort_key(q, zr, e, nz), 1u);
   }
   __syncthreads();
   if (threadIdx.x == 0)
   {
     unsigned run = 0;
-    for (int v = kFlMaxQuads; v >= 0; --v)
+    for (int v = kFlMaxQuads + 1; v >= 0; --v)
     {
       const unsigned c = hist[v];
       hist[v] = run;
       run += c;
     }
+    fl_nact[f] = (int)hist[0];
   }
   __syncthreads();
   for (int e = threadIdx.x; e < n_st; e += kFlBlock)
-    ord[atomicAdd(hist + min((unsigned)lq[e], (unsigned)kFlMaxQuads), 1u)] =
+    ord[atomicAdd(hist + fl_sort_key(lq[e], zr, e, nz), 1u)] =
       (unsigned short)e;
   __syncthreads();
   for (int j = threadIdx.x; j < n_st; j += kFlBlock)
@@ -2028,9 +2087,10 @@ constexpr int kFlK = 6; // elements per lane of a row held in registers
 
 __global__ __launch_bounds__(kFlBlock) void k_fl_build(
   const int* __restrict__ fam_row, const int* __restrict__ fam_off, int nz,
-  const long long* __restrict__ off, const unsigned* __restrict__ sel,
-  const long long* __restrict__ fbase, unsigned* __restrict__ fl_slot,
-  unsigned* __restrict__ fl_gb, unsigned* __restrict__ fl,
+  const unsigned* __restrict__ fam_zr, const long long* __restrict__ off,
+  const unsigned* __restrict__ sel, const long long* __restrict__ fbase,
+  unsigned* __restrict__ fl_slot, unsigned* __restrict__ fl_gb,
+  unsigned* __restrict__ fl, int* __restrict__ fl_nact,
   int* __restrict__ overflow)
 {
   __shared__ unsigned cnt[kSheetMaxFloats];          // counts, then row bits
@@ -2038,7 +2098,7 @@ __global__ __launch_bounds__(kFlBlock) void k_fl_build(
                                                      // elements placed
   __shared__ unsigned short slot[kSheetMaxFloats];   // entry -> slot
   __shared__ unsigned rtab[kFlRTable];
-  __shared__ unsigned hist[kFlMaxQuads + 1];
+  __shared__ unsigned hist[kFlMaxQuads + 2];
   __shared__ unsigned gbase[kSheetMaxFloats / 64 + 1];
   __shared__ unsigned rofs[kSheetMaxFloats / 64 + 1];
   __shared__ int too_big;
@@ -2050,7 +2110,7 @@ __global__ __launch_bounds__(kFlBlock) void k_fl_build(
   const int n_groups = (n_st + 63) / 64;
   const int r0 = fam_row[f], r1 = fam_row[f + 1];
   for (int i = threadIdx.x; i < n_st; i += kFlBlock) cnt[i] = 0;
-  for (int i = threadIdx.x; i <= kFlMaxQuads; i += kFlBlock) hist[i] = 0;
+  for (int i = threadIdx.x; i <= kFlMaxQuads + 1; i += kFlBlock) hist[i] = 0;
   if (threadIdx.x == 0) too_big = 0;
   __syncthreads();
   if (r0 < r1)
@@ -2072,24 +2132,26 @@ __global__ __launch_bounds__(kFlBlock) void k_fl_build(
     }
   }
   __syncthreads();
+  const unsigned* zr = fam_zr + fam_off[f];
   for (int e = threadIdx.x; e < n_st; e += kFlBlock)
-    atomicAdd(hist + min((cnt[e] + 3u) / 4u, (unsigned)kFlMaxQuads), 1u);
+    atomicAdd(hist + fl_sort_key((cnt[e] + 3u) / 4u, zr, e, nz), 1u);
   __syncthreads();
   if (threadIdx.x == 0)
   {
     unsigned run = 0;
-    for (int v = kFlMaxQuads; v >= 0; --v)
+    for (int v = kFlMaxQuads + 1; v >= 0; --v)
     {
       const unsigned c = hist[v];
       hist[v] = run;
       run += c;
     }
+    fl_nact[f] = (int)hist[0];
   }
   __syncthreads();
   for (int e = threadIdx.x; e < n_st; e += kFlBlock)
   {
     const unsigned q = (cnt[e] + 3u) / 4u;
-    const unsigned sl = atomicAdd(hist + min(q, (unsigned)kFlMaxQuads), 1u);
+    const unsigned sl = atomicAdd(hist + fl_sort_key(q, zr, e, nz), 1u);
     slot[e] = (unsigned short)sl;
     cur[sl] = (unsigned short)q;
   }
@@ -2257,19 +2319,19 @@ __global__ __launch_bounds__(kFlBlock) void k_fl_build(
 // go to the family's partial sheet in `part`.
 template <int F>
 __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
-  int n_subsets, int subset, int n_tang, int n_fam,
+  const int* __restrict__ ford,
   const int* __restrict__ fam_row, const int* __restrict__ fam_off,
-  const int* __restrict__ fam_xy, const unsigned* __restrict__ sel,
-  RowData rd, int lanes, const float* __restrict__ img, int nxy, int nz,
-  OsemRowsIO io, float* __restrict__ w, const unsigned* __restrict__ fl_slot,
+  const int* __restrict__ fam_xy, const unsigned* __restrict__ fam_zr,
+  const unsigned* __restrict__ sel, RowData rd, int lanes,
+  const float* __restrict__ img, int nxy, int nz, OsemRowsIO io,
+  float* __restrict__ w, const unsigned* __restrict__ fl_slot,
   const unsigned* __restrict__ fl_gb, const unsigned* __restrict__ fl,
-  const int* __restrict__ fam_loc, float* __restrict__ part)
+  const int* __restrict__ fl_nact, const int* __restrict__ fam_loc,
+  float* __restrict__ part)
 {
   __shared__ float sh[kSheetMaxFloats];
   __shared__ float wr[kFamRowsMax];
-  const int view = (int)blockIdx.y * n_subsets + subset;
-  const int f = view * n_tang + (int)blockIdx.x;
-  if (f >= n_fam) return;
+  const int f = ford[blockIdx.x];
   const int r0 = fam_row[f], r1 = fam_row[f + 1];
   const int p0 = fam_off[f];
   const int L = fam_off[f + 1] - p0;
@@ -2282,28 +2344,7 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
     for (int i = threadIdx.x; i < n_st; i += kSheetBlock) pf[i] = 0.0f;
     return;
   }
-  if (n_st == 0 && threadIdx.x == 0) sh[0] = 0.0f;
-  for (int i0 = threadIdx.x; i0 < n_st; i0 += 8 * kSheetBlock)
-  {
-    float val[8];
-    int at[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-    {
-      const int i = i0 + u * kSheetBlock;
-      at[u] = -1;
-      val[u] = 0.0f;
-      if (i < n_st)
-      {
-        const int z = i / L, p = i - z * L;
-        at[u] = p * nz + z;
-        val[u] = img[(long long)z * nxy + fam_xy[p0 + p]];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (at[u] >= 0) sh[at[u]] = val[u];
-  }
+  stage_sheet(sh, img, fam_xy, fam_zr, p0, L, nz);
   const int lane = (int)threadIdx.x & (lanes - 1);
   const int per_round = kSheetBlock / lanes;
   __syncthreads();
@@ -2334,11 +2375,16 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
   const long long g0 = (long long)p0 * nz;
   const int wave = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
   const unsigned long long below = (1ull << ln) - 1ull;
-  for (int g = wave; g * 64 < n_st; g += kSheetBlock / 64)
+  // the slots with a list or inside their pixel's slice range come first
+  const int n_act = fl_nact[f];
+  for (int g = wave; g * 64 < n_act; g += kSheetBlock / 64)
   {
     const int j = g * 64 + ln;
-    const unsigned sl = j < n_st ? fl_slot[g0 + j] : 0u;
-    const unsigned gb = j < n_st ? fl_gb[g0 + j] : 0u;
+    const unsigned sl = j < n_act ? fl_slot[g0 + j] : 0u;
+    // every slot of a group holds the group's first quad: one
+    // wave-uniform load instead of one per lane
+    const unsigned gb =
+      fl_gb[g0 + 64LL * __builtin_amdgcn_readfirstlane(g)];
     const int q = (int)(sl & 0xffffu);
     float acc = 0.0f;
     unsigned roff = 0;
@@ -2370,7 +2416,7 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
         acc = fmaf(l[3], wr[v[k].w & 0xffffu], acc);
       }
     }
-    if (j < n_st) pf[sl >> 16] = acc;
+    if (j < n_act) pf[sl >> 16] = acc;
   }
 }
 
@@ -2382,16 +2428,15 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
 // part[(entry * NB + b)], entry = fam_loc[f] * nz + sheet index.
 template <int F, int NB>
 __global__ __launch_bounds__(kSheetBlock) void k_pc_fam_bp(
-  int n_subsets, int subset, int n_tang, int n_fam,
+  const int* __restrict__ ford,
   const int* __restrict__ fam_row, const int* __restrict__ fam_off,
   const int* __restrict__ fam_loc, int nz, const float* __restrict__ w,
   const unsigned* __restrict__ fl_slot, const unsigned* __restrict__ fl_gb,
-  const unsigned* __restrict__ fl, float* __restrict__ part)
+  const unsigned* __restrict__ fl, const int* __restrict__ fl_nact,
+  float* __restrict__ part)
 {
   __shared__ float wr[kFamRowsMax * NB];
-  const int view = (int)blockIdx.y * n_subsets + subset;
-  const int f = view * n_tang + (int)blockIdx.x;
-  if (f >= n_fam) return;
+  const int f = ford[blockIdx.x];
   const int r0 = fam_row[f], r1 = fam_row[f + 1];
   const int p0 = fam_off[f];
   const int n_st = (fam_off[f + 1] - p0) * nz;
@@ -2408,11 +2453,16 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_fam_bp(
   const long long g0 = (long long)p0 * nz;
   const int wave = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
   const unsigned long long below = (1ull << ln) - 1ull;
-  for (int g = wave; g * 64 < n_st; g += kSheetBlock / 64)
+  // the slots with a list or inside their pixel's slice range come first
+  const int n_act = fl_nact[f];
+  for (int g = wave; g * 64 < n_act; g += kSheetBlock / 64)
   {
     const int j = g * 64 + ln;
-    const unsigned sl = j < n_st ? fl_slot[g0 + j] : 0u;
-    const unsigned gb = j < n_st ? fl_gb[g0 + j] : 0u;
+    const unsigned sl = j < n_act ? fl_slot[g0 + j] : 0u;
+    // every slot of a group holds the group's first quad: one
+    // wave-uniform load instead of one per lane
+    const unsigned gb =
+      fl_gb[g0 + 64LL * __builtin_amdgcn_readfirstlane(g)];
     const int q = (int)(sl & 0xffffu);
     float acc[NB];
 #pragma unroll
@@ -2451,7 +2501,7 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_fam_bp(
         }
       }
     }
-    if (j < n_st)
+    if (j < n_act)
     {
       float* o = pf + (long long)(sl >> 16) * NB;
 #pragma unroll
@@ -2461,41 +2511,63 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_fam_bp(
 }
 
 // out_b[v] = sum over the subset's families whose sheet holds v's pixel of
-// that family's partial of image b (family order). One thread per voxel,
-// slices fastest (consecutive threads read consecutive partials).
+// that family's partial of image b (family order). One wave per pixel, a
+// lane per slice (64 slices at a time): the pixel's entries (position and
+// slice range) are wave-uniform scalar loads, and a family whose elements
+// miss a lane's slice at that pixel adds +0 without a load (its partial
+// there, never written, is the empty sum).
+constexpr int kRedBlock = 256;
 template <int NB>
-__global__ __launch_bounds__(kBlock) void k_pc_fam_reduce(
-  const long long* __restrict__ off, const int* __restrict__ ent,
+__global__ __launch_bounds__(kRedBlock) void k_pc_fam_reduce(
+  const long long* __restrict__ off, const int2* __restrict__ ent,
   const float* __restrict__ part, int nxy, int nz, VoxOut out)
 {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)nxy * nz) return;
-  const int px = (int)(t / nz), z = (int)(t - (long long)px * nz);
+  const int px = __builtin_amdgcn_readfirstlane(
+    (int)blockIdx.x * (kRedBlock / 64) + ((int)threadIdx.x >> 6));
+  if (px >= nxy) return;
+  const int lane = (int)threadIdx.x & 63;
   const long long a = off[px], b = off[px + 1];
-  float acc[NB];
-#pragma unroll
-  for (int i = 0; i < NB; ++i) acc[i] = 0.0f;
-  long long k = a;
-  for (; k + 4 <= b; k += 4) // four loads in flight, added in order
+  for (int z0 = 0; z0 < nz; z0 += 64)
   {
-    Vec<NB> v[4];
+    const int z = z0 + lane;
+    float acc[NB];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      v[u] = Vec<NB>::load(part, (long long)ent[k + u] * nz + z);
+    for (int i = 0; i < NB; ++i) acc[i] = 0.0f;
+    auto get = [&](int2 e) {
+      Vec<NB> v;
+      const unsigned r = (unsigned)e.y;
+      if (z < nz && (unsigned)z >= (r & 0xffffu) && (unsigned)z < (r >> 16))
+        v = Vec<NB>::load(part, (long long)e.x * nz + z);
+      else
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+        for (int i = 0; i < NB; ++i) v.v[i] = 0.0f;
+      return v;
+    };
+    long long k = a;
+    for (; k + 4 <= b; k += 4) // four loads in flight, added in order
+    {
+      int2 e[4];
 #pragma unroll
-      for (int i = 0; i < NB; ++i) acc[i] += v[u].v[i];
+      for (int u = 0; u < 4; ++u) e[u] = ent[k + u];
+      Vec<NB> v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = get(e[u]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int i = 0; i < NB; ++i) acc[i] += v[u].v[i];
+    }
+    for (; k < b; ++k)
+    {
+      const Vec<NB> v = get(ent[k]);
+#pragma unroll
+      for (int i = 0; i < NB; ++i) acc[i] += v.v[i];
+    }
+    if (z < nz)
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        if (out.p[i]) out.p[i][(long long)z * nxy + px] = acc[i];
   }
-  for (; k < b; ++k)
-  {
-    const Vec<NB> v = Vec<NB>::load(part, (long long)ent[k] * nz + z);
-#pragma unroll
-    for (int i = 0; i < NB; ++i) acc[i] += v.v[i];
-  }
-#pragma unroll
-  for (int i = 0; i < NB; ++i)
-    if (out.p[i]) out.p[i][(long long)z * nxy + px] = acc[i];
 }
 
 } // namespace
@@ -2522,6 +2594,9 @@ struct RowMatrix
   int* d_fam_row = nullptr;         // [n_fam + 1] first row of each family
   int* d_fam_off = nullptr;         // [n_fam + 1] first pixel of each sheet
   int* d_fam_xy = nullptr;          // sheet pixels (x + y nx)
+  // per sheet pixel, the slices [lo, hi) of its entries that some element
+  // crosses, lo | hi << 16 (entries outside are never read or written)
+  unsigned* d_fam_zr = nullptr;
   unsigned* d_sel = nullptr;        // [n_slots] sheet index | length << 16
   // 1: sheet pixels ranked along each family's line, and every lane of the
   // quad layout meets its elements in rising pixel rank (checked at build)
@@ -2534,6 +2609,9 @@ struct RowMatrix
   unsigned* d_fl_slot = nullptr;    // [n_sheet_px * nz] entry << 16 | quads
   unsigned* d_fl_gb = nullptr;      // [n_sheet_px * nz] group's first quad
   unsigned* d_fl = nullptr;         // [4 * fl_quads]
+  // per family, its slots holding a list or inside their pixel's slice
+  // range (the slots before the ones never read)
+  int* d_fl_nact = nullptr;
   long long fl_quads = 0;
   std::vector<unsigned> list_quads; // per family, from the sheet build
   int sheets_req = 0; // the context asked for the sheet form at build
@@ -2556,7 +2634,7 @@ struct RowMatrix
   {
     return (long long)(n_views + 1) * (4 + 8) + (long long)n_rows * (4 + 4 + 1 + 8) +
       8 + n_slots * (4 + (long long)len_bytes(fmt)) + sheet_bytes +
-      (d_fl ? 8 * n_sheet_px * vol.nz + 16 * fl_quads : 0);
+      (d_fl ? 8 * n_sheet_px * vol.nz + 16 * fl_quads + 4LL * n_fam : 0);
   }
 
   ~RowMatrix()
@@ -2573,10 +2651,12 @@ struct RowMatrix
     (void)hipFree(d_fam_row);
     (void)hipFree(d_fam_off);
     (void)hipFree(d_fam_xy);
+    (void)hipFree(d_fam_zr);
     (void)hipFree(d_sel);
     (void)hipFree(d_fl_slot);
     (void)hipFree(d_fl_gb);
     (void)hipFree(d_fl);
+    (void)hipFree(d_fl_nact);
   }
 };
 
@@ -2608,8 +2688,13 @@ struct fir_cache
   // Family backprojection (with the sheet form): per subset and pixel the
   // sheet entries holding it, and the partial sheets of one launch.
   long long* d_fr_off = nullptr; // [n_subsets * (nxy + 1) + 1]
-  int* d_fr_ent = nullptr;       // [sheet entries], positions in the subset
+  // [sheet entries] positions in the subset and slice ranges (lo | hi << 16)
+  int2* d_fr_ent = nullptr;
   int* d_fam_loc = nullptr;      // [n_fam] first position of each family
+  // Per subset, the families the sheet kernels run over (those with rows
+  // or sheet pixels), most work first; subset s's at ford_first[s].
+  int* d_fam_ord = nullptr;
+  std::vector<int> ford_first;
   // partial sheets of one launch, up to 8 images: [subset entries * nz * 8]
   float* d_part = nullptr;
   long long fr_bytes = 0;
@@ -2630,6 +2715,7 @@ struct fir_cache
     (void)hipFree(d_fr_off);
     (void)hipFree(d_fr_ent);
     (void)hipFree(d_fam_loc);
+    (void)hipFree(d_fam_ord);
     (void)hipFree(d_part);
   }
 };
@@ -2891,12 +2977,16 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
     return FIR_OK;
   if ((rc = dev_alloc(&rm->d_fam_off, (size_t)F + 1, "sheet offsets")) ||
       (rc = dev_alloc(&rm->d_fam_xy, (size_t)off[F], "sheet pixels")) ||
+      (rc = dev_alloc(&rm->d_fam_zr, (size_t)std::max(off[F], 1),
+                      "sheet slice ranges")) ||
       (rc = dev_alloc(&rm->d_sel, (size_t)rm->n_slots, "sheet elements")))
   {
     (void)hipFree(rm->d_fam_off);
     (void)hipFree(rm->d_fam_xy);
+    (void)hipFree(rm->d_fam_zr);
     rm->d_fam_off = nullptr;
     rm->d_fam_xy = nullptr;
+    rm->d_fam_zr = nullptr;
     rm->d_sel = nullptr;
     return rc == FIR_E_NOMEM ? FIR_OK : rc;
   }
@@ -2916,7 +3006,7 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
                      c->stream, lor_geom_views(c), g, n_tang, rm->d_fam_row,
                      rm->d_off, rm->d_idx, rm->d_len, lb, rm->d_yrow, xy, nz,
                      words, bits, rm->d_fam_off, line ? 1 : 0, rm->d_fam_xy,
-                     rm->d_sel, list_quads);
+                     rm->d_fam_zr, rm->d_sel, list_quads);
   FIR_CHECK_LAUNCH();
   rm->list_quads.resize(F);
   FIR_HIP_CHECK(hipMemcpyAsync(rm->list_quads.data(), list_quads,
@@ -2939,7 +3029,7 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
                                  hipMemcpyDeviceToHost, c->stream));
   }
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
-  rm->sheet_bytes = 4LL * (F + 1) + 4LL * off[F] + 4LL * rm->n_slots;
+  rm->sheet_bytes = 4LL * (F + 1) + 8LL * off[F] + 4LL * rm->n_slots;
   rm->n_sheet_px = off[F];
   return FIR_OK;
 }
@@ -2979,22 +3069,27 @@ int build_family_lists(fir_ctx* c, RowMatrix* rm)
     if ((rc = dev_alloc(&rm->d_fl_slot, (size_t)entries,
                         "family list slots")) ||
         (rc = dev_alloc(&rm->d_fl_gb, (size_t)entries, "family list groups")) ||
+        (rc = dev_alloc(&rm->d_fl_nact, (size_t)F, "family list actives")) ||
         (rc = dev_alloc(&rm->d_fl, (size_t)(4 * std::max(quads, 1LL)),
                         "family lists")))
     {
       (void)hipFree(rm->d_fl_slot);
       (void)hipFree(rm->d_fl_gb);
+      (void)hipFree(rm->d_fl_nact);
       rm->d_fl_slot = nullptr;
       rm->d_fl_gb = nullptr;
+      rm->d_fl_nact = nullptr;
       rm->d_fl = nullptr;
       return rc == FIR_E_NOMEM ? FIR_OK : rc;
     }
     FIR_HIP_CHECK(hipMemcpyAsync(fbase, base.data(), 8LL * F,
                                  hipMemcpyHostToDevice, c->stream));
     FIR_HIP_CHECK(hipMemsetAsync(overflow, 0, sizeof(int), c->stream));
+    FIR_HIP_CHECK(hipMemsetAsync(rm->d_fl_nact, 0, 4LL * F, c->stream));
     hipLaunchKernelGGL(k_fl_build, dim3(F), dim3(kFlBlock), 0, c->stream,
-                       rm->d_fam_row, rm->d_fam_off, nz, rm->d_off, rm->d_sel,
-                       fbase, rm->d_fl_slot, rm->d_fl_gb, rm->d_fl, overflow);
+                       rm->d_fam_row, rm->d_fam_off, nz, rm->d_fam_zr,
+                       rm->d_off, rm->d_sel, fbase, rm->d_fl_slot,
+                       rm->d_fl_gb, rm->d_fl, rm->d_fl_nact, overflow);
     FIR_CHECK_LAUNCH();
     int ovf = 0;
     FIR_HIP_CHECK(hipMemcpyAsync(&ovf, overflow, sizeof(int),
@@ -3008,9 +3103,11 @@ int build_family_lists(fir_ctx* c, RowMatrix* rm)
     // some family's round table is too large for LDS: the two-pass route
     (void)hipFree(rm->d_fl_slot);
     (void)hipFree(rm->d_fl_gb);
+    (void)hipFree(rm->d_fl_nact);
     (void)hipFree(rm->d_fl);
     rm->d_fl_slot = nullptr;
     rm->d_fl_gb = nullptr;
+    rm->d_fl_nact = nullptr;
     rm->d_fl = nullptr;
     FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
   }
@@ -3028,16 +3125,20 @@ int build_family_lists(fir_ctx* c, RowMatrix* rm)
     return rc == FIR_E_NOMEM ? FIR_OK : rc;
   if ((rc = dev_alloc(&rm->d_fl_slot, (size_t)entries, "family list slots")) ||
       (rc = dev_alloc(&rm->d_fl_gb, (size_t)entries, "family list groups")) ||
+      (rc = dev_alloc(&rm->d_fl_nact, (size_t)F, "family list actives")) ||
       (rc = dev_alloc(&rm->d_fl, (size_t)(4 * std::max(quads, 1LL)),
                       "family lists")))
   {
     (void)hipFree(rm->d_fl_slot);
     (void)hipFree(rm->d_fl_gb);
+    (void)hipFree(rm->d_fl_nact);
     rm->d_fl_slot = nullptr;
     rm->d_fl_gb = nullptr;
+    rm->d_fl_nact = nullptr;
     rm->d_fl = nullptr;
     return rc == FIR_E_NOMEM ? FIR_OK : rc;
   }
+  FIR_HIP_CHECK(hipMemsetAsync(rm->d_fl_nact, 0, 4LL * F, c->stream));
   const unsigned end = (unsigned)quads;
   FIR_HIP_CHECK(hipMemcpyAsync(fbase, base.data(), 8LL * F,
                                hipMemcpyHostToDevice, c->stream));
@@ -3058,8 +3159,9 @@ int build_family_lists(fir_ctx* c, RowMatrix* rm)
     if (nq > stage_q)
       return fail(FIR_E_NOMEM, "family lists: a family exceeds the staging");
     hipLaunchKernelGGL(k_fl_interleave, dim3(f1 - f0), dim3(kFlBlock), 0,
-                       c->stream, f0, rm->d_fam_off, nz, fbase, cl_off,
-                       rm->d_fl, rm->d_fl_slot, rm->d_fl_gb, stage, q0);
+                       c->stream, f0, rm->d_fam_off, nz, rm->d_fam_zr, fbase,
+                       cl_off, rm->d_fl, rm->d_fl_slot, rm->d_fl_gb, stage,
+                       rm->d_fl_nact, q0);
     FIR_CHECK_LAUNCH();
     if (nq > 0)
       FIR_HIP_CHECK(hipMemcpyAsync(rm->d_fl + 4 * q0, stage, 16 * nq,
@@ -3068,6 +3170,74 @@ int build_family_lists(fir_ctx* c, RowMatrix* rm)
   }
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
   rm->fl_quads = quads;
+  return FIR_OK;
+}
+
+// Path slots of every family's rows.
+__global__ void k_fam_work(int n_fam, const int* __restrict__ fam_row,
+                           const long long* __restrict__ off,
+                           long long* __restrict__ work)
+{
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < n_fam) work[f] = off[fam_row[f + 1]] - off[fam_row[f]];
+}
+
+// The order in which the sheet kernels (one block per family) take each
+// subset's families (views k * S + s): only families with rows or sheet
+// pixels, the most work (path slots plus sheet entries) first, so that a
+// launch ends on its smallest families instead of on whichever large ones
+// come last in view order, and no block is spent on an empty family.
+int build_family_order(fir_ctx* c, fir_cache* cache)
+{
+  const RowMatrix* rm = cache->rm.get();
+  if (!rm->d_sel || cache->d_fam_ord) return FIR_OK;
+  const int F = rm->n_fam, S = cache->n_subsets, nz = cache->vol.nz;
+  const int n_tang = F / rm->n_views;
+  DevBufs tmp;
+  long long* work;
+  int rc;
+  if ((rc = tmp.get(&work, (size_t)F, "family work"))) return rc;
+  hipLaunchKernelGGL(k_fam_work, dim3((F + kBlock - 1) / kBlock),
+                     dim3(kBlock), 0, c->stream, F, rm->d_fam_row, rm->d_off,
+                     work);
+  FIR_CHECK_LAUNCH();
+  std::vector<long long> hw(F);
+  std::vector<int> frow(F + 1), foff(F + 1);
+  FIR_HIP_CHECK(hipMemcpyAsync(hw.data(), work, 8LL * F,
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipMemcpyAsync(frow.data(), rm->d_fam_row, 4LL * (F + 1),
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipMemcpyAsync(foff.data(), rm->d_fam_off, 4LL * (F + 1),
+                               hipMemcpyDeviceToHost, c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  std::vector<int> ord, first(S + 1);
+  ord.reserve(F);
+  for (int s0 = 0; s0 < S; ++s0)
+  {
+    first[s0] = (int)ord.size();
+    const int vpr = fir::views_in_subset(rm->n_views, S, s0);
+    for (int k = 0; k < vpr; ++k)
+      for (int t = 0; t < n_tang; ++t)
+      {
+        const int f = (k * S + s0) * n_tang + t;
+        if (frow[f + 1] > frow[f] || foff[f + 1] > foff[f]) ord.push_back(f);
+      }
+    auto cost = [&](int f) {
+      return hw[f] + (long long)(foff[f + 1] - foff[f]) * nz;
+    };
+    std::stable_sort(ord.begin() + first[s0], ord.end(),
+                     [&](int a, int b) { return cost(a) > cost(b); });
+  }
+  first[S] = (int)ord.size();
+  if ((rc = dev_alloc(&cache->d_fam_ord, std::max(ord.size(), (size_t)1),
+                      "family order")))
+    return rc;
+  if (!ord.empty())
+    FIR_HIP_CHECK(hipMemcpyAsync(cache->d_fam_ord, ord.data(),
+                                 4 * ord.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  cache->ford_first = first;
   return FIR_OK;
 }
 
@@ -3108,7 +3278,7 @@ int build_family_bp(fir_ctx* c, fir_cache* cache)
   }
   const long long part_floats = max_loc * nz * kMaxCacheBatch;
   const long long bytes =
-    8 * (n_keys + 1) + 4 * total + 4LL * rm->n_fam + 4 * part_floats;
+    8 * (n_keys + 1) + 8 * total + 4LL * rm->n_fam + 4 * part_floats;
   size_t free_b = 0, total_b = 0;
   FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
   if ((long long)free_b < bytes + 4 * n_keys + (2LL << 30)) return FIR_OK;
@@ -3152,9 +3322,9 @@ int build_family_bp(fir_ctx* c, fir_cache* cache)
                                hipMemcpyHostToDevice, c->stream));
   FIR_HIP_CHECK(hipMemsetAsync(cnt, 0, 4 * n_keys, c->stream));
   hipLaunchKernelGGL(k_fr_fill, dim3(blocks_for(total)), dim3(kBlock), 0,
-                     c->stream, rm->d_fam_off, rm->d_fam_xy, cache->d_fam_loc,
-                     rm->n_fam, (int)total, n_tang, S, nxy, cache->d_fr_off,
-                     cnt, cache->d_fr_ent);
+                     c->stream, rm->d_fam_off, rm->d_fam_xy, rm->d_fam_zr,
+                     cache->d_fam_loc, rm->n_fam, (int)total, n_tang, S, nxy,
+                     cache->d_fr_off, cnt, cache->d_fr_ent);
   FIR_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_fr_sort, dim3(blocks_for(n_keys)), dim3(kBlock), 0,
                      c->stream, cache->d_fr_off, n_keys, cache->d_fr_ent);
@@ -3745,19 +3915,26 @@ bool use_sheets(const fir_cache* cache)
   return cache->sheets && cache->rm->d_sel;
 }
 
+// Subset s's families for the sheet kernels: their order table and count.
+const int* family_order(const fir_cache* cache, int s, int* n)
+{
+  *n = cache->ford_first[s + 1] - cache->ford_first[s];
+  return cache->d_fam_ord + cache->ford_first[s];
+}
+
 // k_pc_osem_rows over the sheet form for one image (img x fastest).
 template <int F>
 void launch_rows_sheet(fir_cache* cache, int s, const float* img,
                        const OsemRowsIO& io, float* w)
 {
   const RowMatrix* rm = cache->rm.get();
-  const int S = cache->n_subsets;
-  const int vpr = fir::views_in_subset(rm->n_views, S, s);
-  const int n_tang = rm->n_fam / rm->n_views;
-  hipLaunchKernelGGL((k_pc_osem_rows_sheet<F>), dim3(n_tang, vpr),
-                     dim3(kSheetBlock), 0, cache->ctx->stream, S, s, n_tang,
-                     rm->n_fam, rm->d_fam_row, rm->d_fam_off, rm->d_fam_xy,
-                     rm->d_sel, row_data(rm), cache->fwd_lanes, img,
+  int nf;
+  const int* ford = family_order(cache, s, &nf);
+  if (nf == 0) return;
+  hipLaunchKernelGGL((k_pc_osem_rows_sheet<F>), dim3(nf), dim3(kSheetBlock),
+                     0, cache->ctx->stream, ford, rm->d_fam_row,
+                     rm->d_fam_off, rm->d_fam_xy, rm->d_fam_zr, rm->d_sel,
+                     row_data(rm), cache->fwd_lanes, img,
                      cache->vol.nx * cache->vol.ny, cache->vol.nz, io, w);
 }
 
@@ -3775,24 +3952,26 @@ void launch_osem_family(fir_cache* cache, int s, const float* img,
                         const OsemRowsIO& io, float* w, float* bp)
 {
   const RowMatrix* rm = cache->rm.get();
-  const int S = cache->n_subsets;
-  const int vpr = fir::views_in_subset(rm->n_views, S, s);
-  const int n_tang = rm->n_fam / rm->n_views;
   const int nxy = cache->vol.nx * cache->vol.ny, nz = cache->vol.nz;
-  hipLaunchKernelGGL((k_pc_osem_fam<F>), dim3(n_tang, vpr), dim3(kSheetBlock),
-                     0, cache->ctx->stream, S, s, n_tang, rm->n_fam,
-                     rm->d_fam_row, rm->d_fam_off, rm->d_fam_xy, rm->d_sel,
-                     row_data(rm), cache->fwd_lanes, img, nxy, nz, io, w,
-                     rm->d_fl_slot, rm->d_fl_gb, rm->d_fl, cache->d_fam_loc,
-                     cache->d_part);
+  int nf;
+  const int* ford = family_order(cache, s, &nf);
+  if (nf > 0)
+    hipLaunchKernelGGL((k_pc_osem_fam<F>), dim3(nf), dim3(kSheetBlock), 0,
+                       cache->ctx->stream, ford, rm->d_fam_row,
+                       rm->d_fam_off, rm->d_fam_xy, rm->d_fam_zr, rm->d_sel,
+                       row_data(rm), cache->fwd_lanes, img, nxy, nz, io, w,
+                       rm->d_fl_slot, rm->d_fl_gb, rm->d_fl, rm->d_fl_nact,
+                       cache->d_fam_loc, cache->d_part);
   VoxOut out;
   std::memset(&out, 0, sizeof(out));
   out.p[0] = bp;
   hipLaunchKernelGGL(k_pc_fam_reduce<1>,
-                     dim3(blocks_for((long long)nxy * nz)), dim3(kBlock), 0,
+                     dim3((nxy + kRedBlock / 64 - 1) / (kRedBlock / 64)),
+                     dim3(kRedBlock), 0,
                      cache->ctx->stream,
                      cache->d_fr_off + (long long)s * (nxy + 1),
-                     cache->d_fr_ent, cache->d_part, nxy, nz, out);
+                     cache->d_fr_ent, cache->d_part, nxy, nz,
+                     out);
 }
 
 // Family backprojection of nb images (nb = 1, 2, 4 or 8) of subset s with
@@ -3802,20 +3981,21 @@ void launch_fam_bp_nb(fir_cache* cache, int s, const float* w,
                       const VoxOut& out)
 {
   const RowMatrix* rm = cache->rm.get();
-  const int S = cache->n_subsets;
-  const int vpr = fir::views_in_subset(rm->n_views, S, s);
-  const int n_tang = rm->n_fam / rm->n_views;
   const int nxy = cache->vol.nx * cache->vol.ny, nz = cache->vol.nz;
-  hipLaunchKernelGGL((k_pc_fam_bp<F, NB>), dim3(n_tang, vpr),
-                     dim3(kSheetBlock), 0, cache->ctx->stream, S, s, n_tang,
-                     rm->n_fam, rm->d_fam_row, rm->d_fam_off,
-                     cache->d_fam_loc, nz, w, rm->d_fl_slot, rm->d_fl_gb,
-                     rm->d_fl, cache->d_part);
+  int nf;
+  const int* ford = family_order(cache, s, &nf);
+  if (nf > 0)
+    hipLaunchKernelGGL((k_pc_fam_bp<F, NB>), dim3(nf), dim3(kSheetBlock), 0,
+                       cache->ctx->stream, ford, rm->d_fam_row,
+                       rm->d_fam_off, cache->d_fam_loc, nz, w, rm->d_fl_slot,
+                       rm->d_fl_gb, rm->d_fl, rm->d_fl_nact, cache->d_part);
   hipLaunchKernelGGL(k_pc_fam_reduce<NB>,
-                     dim3(blocks_for((long long)nxy * nz)), dim3(kBlock), 0,
+                     dim3((nxy + kRedBlock / 64 - 1) / (kRedBlock / 64)),
+                     dim3(kRedBlock), 0,
                      cache->ctx->stream,
                      cache->d_fr_off + (long long)s * (nxy + 1),
-                     cache->d_fr_ent, cache->d_part, nxy, nz, out);
+                     cache->d_fr_ent, cache->d_part, nxy, nz,
+                     out);
 }
 
 template <int F>
@@ -3902,9 +4082,12 @@ void pack_images_pz(fir_cache* cache, int wd, const ImgIn& in, int nb)
   if (wd == 8)
     hipLaunchKernelGGL(k_pc_pack_pz<8>, grid, dim3(256), 0, st, cache->d_x,
                        in, nb, nxy, nz);
-  else
+  else if (wd == 4)
     hipLaunchKernelGGL(k_pc_pack_pz<4>, grid, dim3(256), 0, st, cache->d_x,
                        in, nb, nxy, nz);
+  else
+    hipLaunchKernelGGL(k_pc_pack_pz<1>, grid, dim3(256), 0, st, cache->d_x,
+                       in, 1, nxy, nz);
 }
 
 template <int F, int NB, int R, int D>
@@ -3912,10 +4095,10 @@ void launch_rows_slab_r(fir_cache* cache, int s, const SlabShape& sh,
                         const OsemRowsIO& io, float* w)
 {
   const RowMatrix* rm = cache->rm.get();
-  const int S = cache->n_subsets;
-  const int vpr = fir::views_in_subset(rm->n_views, S, s);
-  const int n_tang = rm->n_fam / rm->n_views;
   const int nz = cache->vol.nz;
+  int nf;
+  const int* ford = family_order(cache, s, &nf);
+  if (nf == 0) return;
   auto kern = k_pc_osem_rows_slab<F, NB, R, D>;
   if (sh.lds > 65536)
   {
@@ -3927,8 +4110,8 @@ void launch_rows_slab_r(fir_cache* cache, int s, const SlabShape& sh,
                             (int)sh.lds) == hipSuccess)
       allowed = sh.lds;
   }
-  hipLaunchKernelGGL(kern, dim3(n_tang, vpr), dim3(sh.block), sh.lds,
-                     cache->ctx->stream, S, s, n_tang, rm->n_fam,
+  hipLaunchKernelGGL(kern, dim3(nf), dim3(sh.block), sh.lds,
+                     cache->ctx->stream, ford,
                      rm->d_fam_row, rm->d_fam_off, rm->d_fam_xy, rm->d_sel,
                      row_data(rm), cache->fwd_lanes, cache->d_x, nz,
                      sh.slab_px, sh.buf_f4, io, w);
@@ -4107,7 +4290,8 @@ static int cache_create(fir_ctx* c, const fir_vol_desc* v, int n_subsets,
   cache->w_floats = (size_t)rm->n_rows * kMaxCacheBatch;
   cache->x_floats = (size_t)layout_voxels(cache_xy(cache), v->nz) *
     kMaxCacheBatch;
-  if ((rc = build_family_bp(c, cache)) ||
+  if ((rc = build_family_order(c, cache)) ||
+      (rc = build_family_bp(c, cache)) ||
       (!use_family_bp(cache) && (rc = build_transpose(c, cache))) ||
       (rc = dev_alloc(&cache->d_w, cache->w_floats, "row weights")) ||
       (rc = dev_alloc(&cache->d_x, cache->x_floats, "interleaved images")) ||
@@ -4373,29 +4557,35 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
     }
     io.n_images = nb;
     // (the sheet form exists for the 16-bit length formats only)
+    // one image: the sheets are staged from its pixel-major copy in d_x
     if (wd == 1 && use_family_bp(cache))
     {
+      pack_images_pz(cache, 1, in, 1);
+      FIR_CHECK_LAUNCH();
       if (rm->fmt == FIR_LEN_SNORM16)
-        launch_osem_family<FIR_LEN_SNORM16>(cache, subset, x[i0], io,
+        launch_osem_family<FIR_LEN_SNORM16>(cache, subset, cache->d_x, io,
                                             cache->d_w, bp[i0]);
       else
-        launch_osem_family<FIR_LEN_FP16>(cache, subset, x[i0], io, cache->d_w,
-                                         bp[i0]);
+        launch_osem_family<FIR_LEN_FP16>(cache, subset, cache->d_x, io,
+                                         cache->d_w, bp[i0]);
       FIR_CHECK_LAUNCH();
       continue;
     }
     if (wd == 1 && use_sheets(cache))
     {
+      pack_images_pz(cache, 1, in, 1);
+      FIR_CHECK_LAUNCH();
       if (rm->fmt == FIR_LEN_SNORM16)
       {
-        launch_rows_sheet<FIR_LEN_SNORM16>(cache, subset, x[i0], io,
+        launch_rows_sheet<FIR_LEN_SNORM16>(cache, subset, cache->d_x, io,
                                            cache->d_w);
         FIR_CHECK_LAUNCH();
         launch_backproject<FIR_LEN_SNORM16>(cache, subset, 1, cache->d_w, out);
       }
       else
       {
-        launch_rows_sheet<FIR_LEN_FP16>(cache, subset, x[i0], io, cache->d_w);
+        launch_rows_sheet<FIR_LEN_FP16>(cache, subset, cache->d_x, io,
+                                        cache->d_w);
         FIR_CHECK_LAUNCH();
         launch_backproject<FIR_LEN_FP16>(cache, subset, 1, cache->d_w, out);
       }
