@@ -942,6 +942,113 @@ __global__ __launch_bounds__(kBlock) void k_gauss(const float* __restrict__ in,
   }
 }
 
+// Multi-job OSEM steps: the element-wise phases of many images (different
+// grids) in one launch each. A block takes one tile (kGaussX x kGaussY
+// voxels of one slice) of one image; its image is found by a binary search
+// over the images' first tiles. The arithmetic per voxel is k_osem_update's
+// and k_gauss's.
+__device__ __forceinline__ int step_find_image(const StepImage* imgs, int n,
+                                               int tile)
+{
+  int lo = 0, hi = n - 1; // last image whose first tile is <= tile
+  while (lo < hi)
+  {
+    const int mid = (lo + hi + 1) >> 1;
+    if (imgs[mid].tile0 <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+struct StepVoxel
+{
+  int ii, j, k;
+  long long i;
+  bool in;
+};
+
+__device__ __forceinline__ StepVoxel step_voxel(const StepImage& m, int tile)
+{
+  const int t = tile - m.tile0;
+  const int bx = (m.nx + kGaussX - 1) / kGaussX;
+  const int by = (m.ny + kGaussY - 1) / kGaussY;
+  const int k = t / (bx * by);
+  const int r = t - k * bx * by;
+  const int tyy = r / bx, txx = r - tyy * bx;
+  StepVoxel v;
+  v.ii = txx * kGaussX + (int)threadIdx.x % kGaussX;
+  v.j = tyy * kGaussY + (int)threadIdx.x / kGaussX;
+  v.k = k;
+  v.in = v.ii < m.nx && v.j < m.ny;
+  v.i = ((long long)k * m.ny + v.j) * m.nx + v.ii;
+  return v;
+}
+
+__device__ __forceinline__ CutGeom step_cut_geom(const StepImage& m)
+{
+  CutGeom c;
+  c.dx = m.dx;
+  c.dy = m.dy;
+  c.half_w = (m.nx * m.dx) / 2.0;
+  c.half_h = (m.ny * m.dy) / 2.0;
+  c.nx = m.nx;
+  c.ny = m.ny;
+  return c;
+}
+
+// x = x * clamp(bp / sens), bp = 0 (k_osem_update with reset), the cut when
+// apply_cut.
+__global__ __launch_bounds__(kBlock) void k_step_update(
+  const StepImage* __restrict__ imgs, int n_imgs, float cut, int apply_cut)
+{
+  const int tile = blockIdx.x;
+  const StepImage& m = imgs[step_find_image(imgs, n_imgs, tile)];
+  const StepVoxel v = step_voxel(m, tile);
+  if (!v.in) return;
+  float b = m.bp[v.i];
+  const float s = m.sens[v.i];
+  b = (b > kEpsF && s > kEpsF) ? b / s : 0.0f;
+  float x = m.x[v.i];
+  x = (x > kEpsF && b > kEpsF) ? x * b : 0.0f;
+  if (apply_cut && radius_of(step_cut_geom(m), v.ii, v.j) > cut) x = 0.0f;
+  m.x[v.i] = x;
+  m.bp[v.i] = 0.0f;
+}
+
+// One axis of the Gaussian for every image (k_gauss): X: x -> t1,
+// Y: t1 -> t2, Z: t2 -> x with the edge restore and the cut.
+template <int AXIS>
+__global__ __launch_bounds__(kBlock) void k_step_gauss(
+  const StepImage* __restrict__ imgs, int n_imgs, int restore,
+  double restore_r, float cut)
+{
+  const int tile = blockIdx.x;
+  const StepImage& m = imgs[step_find_image(imgs, n_imgs, tile)];
+  const StepVoxel v = step_voxel(m, tile);
+  if (!v.in) return;
+  const float* in = AXIS == 0 ? m.x : (AXIS == 1 ? m.t1 : m.t2);
+  float* out = AXIS == 0 ? m.t1 : (AXIS == 1 ? m.t2 : m.x);
+  const long long plane = (long long)m.nx * m.ny;
+  const int coord = AXIS == 0 ? v.ii : (AXIS == 1 ? v.j : v.k);
+  const int lim = AXIS == 0 ? m.nx : (AXIS == 1 ? m.ny : m.nz);
+  const long long stride = AXIS == 0 ? 1 : (AXIS == 1 ? m.nx : plane);
+  float s, w;
+  gauss_sum(in, v.i, coord, lim, stride, m.taps + 64 * AXIS, m.h[AXIS], &s,
+            &w);
+  float o;
+  if (w > 0.0f)
+    o = s / w;
+  else
+    o = out[v.i];
+  if (AXIS == 2)
+  {
+    const double rad = radius_of(step_cut_geom(m), v.ii, v.j);
+    if (restore && rad >= restore_r) o = out[v.i];
+    if (cut > 0.0f && rad > cut) o = 0.0f;
+  }
+  out[v.i] = o;
+}
+
 // Taps passed by value (kernel arguments), so that no host-to-device copy
 // and no device allocation is needed to place them.
 struct GaussTaps
@@ -1680,6 +1787,48 @@ int convolve_ctx(fir_ctx* c, const fir_vol_desc* v, float* img, int n_frames,
   return FIR_OK;
 }
 } // namespace
+
+} // extern "C"
+
+namespace firi
+{
+int step_image_tiles(int nx, int ny, int nz)
+{
+  return ((nx + kGaussX - 1) / kGaussX) * ((ny + kGaussY - 1) / kGaussY) * nz;
+}
+
+int gauss_taps_for(fir_ctx* c, const fir_vol_desc* v, const float* fwhm,
+                   const float** taps, int h[3])
+{
+  const fir_ctx::TapSet* ts = nullptr;
+  const int rc = prepare_taps(c, v, fwhm, &ts);
+  if (rc) return rc;
+  *taps = ts->d;
+  for (int d = 0; d < 3; ++d) h[d] = ts->h[d];
+  return FIR_OK;
+}
+
+int launch_step_elementwise(hipStream_t st, const StepImage* imgs, int n_imgs,
+                            int n_tiles, bool gauss, float cut, int restore,
+                            double restore_r)
+{
+  if (n_tiles <= 0) return FIR_OK;
+  hipLaunchKernelGGL(k_step_update, dim3(n_tiles), dim3(kBlock), 0, st, imgs,
+                     n_imgs, cut, (!gauss && cut > 0.0f) ? 1 : 0);
+  FIR_CHECK_LAUNCH();
+  if (!gauss) return FIR_OK;
+  hipLaunchKernelGGL(k_step_gauss<0>, dim3(n_tiles), dim3(kBlock), 0, st,
+                     imgs, n_imgs, 0, 0.0, 0.0f);
+  hipLaunchKernelGGL(k_step_gauss<1>, dim3(n_tiles), dim3(kBlock), 0, st,
+                     imgs, n_imgs, 0, 0.0, 0.0f);
+  hipLaunchKernelGGL(k_step_gauss<2>, dim3(n_tiles), dim3(kBlock), 0, st,
+                     imgs, n_imgs, restore, restore_r, cut);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+} // namespace firi
+
+extern "C" {
 
 int fir_convolve(fir_ctx* c, const fir_vol_desc* v, float* img, int n_frames,
                  const float* fwhm, float cut)

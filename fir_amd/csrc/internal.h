@@ -114,6 +114,36 @@ fir::LorGeom lor_geom(const fir_ctx* ctx);
 fir::LorGeom lor_geom_bins(const fir_ctx* ctx);
 long long n_work(const fir_ctx* ctx, int n_subsets, int subset);
 int check_work(long long n);
+
+// One image of a multi-job OSEM step (fir_osem_step_run), for its
+// element-wise phases: the update and the Gaussian.
+struct StepImage
+{
+  float* x;
+  float* bp;
+  const float* sens;
+  float* t1;          // the Gaussian's intermediate images (nx*ny*nz each)
+  float* t2;
+  const float* taps;  // X, Y, Z taps, 64 floats each (NULL: no Gaussian)
+  int h[3];
+  int nx, ny, nz;
+  double dx, dy;
+  int tile0;          // the image's first tile (blocks of 32 x 8 voxels of
+                      // one slice) in the step's tile numbering
+};
+// Tiles of one image in that numbering.
+int step_image_tiles(int nx, int ny, int nz);
+// The context's taps of the Gaussian of fwhm on grid v (uploaded once and
+// kept until the context is destroyed).
+int gauss_taps_for(fir_ctx* c, const fir_vol_desc* v, const float* fwhm,
+                   const float** taps, int h[3]);
+// The element-wise phases of a step over n_imgs images (device array),
+// n_tiles tiles in all: the OSEM update, then (gauss) the three passes of
+// the Gaussian, the Z pass with the edge restore at radius restore_r (when
+// restore) and the circular cut; without the Gaussian the update cuts.
+int launch_step_elementwise(hipStream_t st, const StepImage* imgs, int n_imgs,
+                            int n_tiles, bool gauss, float cut, int restore,
+                            double restore_r);
 } // namespace firi
 
 #define FIR_HIP_CHECK(expr)                                                  \

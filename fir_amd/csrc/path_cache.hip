@@ -1528,15 +1528,16 @@ constexpr int kSlabBlockMax = 1024;
 // by kPzZ slices of every image through LDS: its loads read runs of kPzPx
 // pixels, its stores runs of kPzZ * NB floats per pixel.
 constexpr int kPzPx = 64, kPzZ = 4;
+constexpr int kPzPitch = kPzPx + 1;
 template <int NB>
-__global__ __launch_bounds__(256) void k_pc_pack_pz(float* __restrict__ X,
-                                                    ImgIn in, int n_images,
-                                                    int nxy, int nz)
+__device__ __forceinline__ void pack_pz_tile(float* __restrict__ X,
+                                             const ImgIn& in, int n_images,
+                                             int nxy, int nz, int bx, int bz,
+                                             float* tile)
 {
-  constexpr int kPitch = kPzPx + 1;
+  constexpr int kPitch = kPzPitch;
   constexpr int kN = kPzPx * kPzZ * NB; // floats of the tile
-  __shared__ float tile[kPzZ * NB * kPitch];
-  const int px0 = blockIdx.x * kPzPx, z0 = blockIdx.y * kPzZ;
+  const int px0 = bx * kPzPx, z0 = bz * kPzZ;
   float v[kN / 256];
 #pragma unroll
   for (int u = 0; u < kN / 256; ++u)
@@ -1565,6 +1566,15 @@ __global__ __launch_bounds__(256) void k_pc_pack_pz(float* __restrict__ X,
       X[((long long)(px0 + p) * nz + z0 + zz) * NB + b] =
         tile[(zz * NB + b) * kPitch + p];
   }
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_pc_pack_pz(float* __restrict__ X,
+                                                    ImgIn in, int n_images,
+                                                    int nxy, int nz)
+{
+  __shared__ float tile[kPzZ * NB * kPzPitch];
+  pack_pz_tile<NB>(X, in, n_images, nxy, nz, blockIdx.x, blockIdx.y, tile);
 }
 
 template <int NB>
@@ -1628,15 +1638,13 @@ __device__ __forceinline__ void slab_stage(const float4* __restrict__ X4,
 // are staged alternately into two LDS buffers of buf_f4 float4 each, the
 // next slab's loads in flight while the current one is summed.
 template <int F, int NB, int R, int D>
-__global__ __launch_bounds__(kSlabBlockMax) void k_pc_osem_rows_slab(
-  const int* __restrict__ ford,
-  const int* __restrict__ fam_row, const int* __restrict__ fam_off,
-  const int* __restrict__ fam_xy, const unsigned* __restrict__ sel,
-  RowData rd, int lanes, const float* __restrict__ X, int nz, int slab_px,
-  int buf_f4, OsemRowsIO io, float* __restrict__ w)
+__device__ __forceinline__ void slab_family(
+  int f, float4* slab, const int* __restrict__ fam_row,
+  const int* __restrict__ fam_off, const int* __restrict__ fam_xy,
+  const unsigned* __restrict__ sel, const RowData& rd, int lanes,
+  const float* __restrict__ X, int nz, int slab_px, int buf_f4,
+  const OsemRowsIO& io, float* __restrict__ w)
 {
-  extern __shared__ float4 slab[];
-  const int f = ford[blockIdx.x];
   const int r0 = fam_row[f], r1 = fam_row[f + 1];
   if (r0 == r1) return;
   const int p0 = fam_off[f];
@@ -1752,6 +1760,19 @@ __global__ __launch_bounds__(kSlabBlockMax) void k_pc_osem_rows_slab(
       }
     }
   }
+}
+
+template <int F, int NB, int R, int D>
+__global__ __launch_bounds__(kSlabBlockMax) void k_pc_osem_rows_slab(
+  const int* __restrict__ ford,
+  const int* __restrict__ fam_row, const int* __restrict__ fam_off,
+  const int* __restrict__ fam_xy, const unsigned* __restrict__ sel,
+  RowData rd, int lanes, const float* __restrict__ X, int nz, int slab_px,
+  int buf_f4, OsemRowsIO io, float* __restrict__ w)
+{
+  extern __shared__ float4 slab[];
+  slab_family<F, NB, R, D>(ford[blockIdx.x], slab, fam_row, fam_off, fam_xy,
+                           sel, rd, lanes, X, nz, slab_px, buf_f4, io, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -2427,16 +2448,13 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
 // half of k_pc_osem_fam does for one image, into the partial sheets
 // part[(entry * NB + b)], entry = fam_loc[f] * nz + sheet index.
 template <int F, int NB>
-__global__ __launch_bounds__(kSheetBlock) void k_pc_fam_bp(
-  const int* __restrict__ ford,
-  const int* __restrict__ fam_row, const int* __restrict__ fam_off,
-  const int* __restrict__ fam_loc, int nz, const float* __restrict__ w,
-  const unsigned* __restrict__ fl_slot, const unsigned* __restrict__ fl_gb,
-  const unsigned* __restrict__ fl, const int* __restrict__ fl_nact,
-  float* __restrict__ part)
+__device__ __forceinline__ void fam_bp_family(
+  int f, float* wr, const int* __restrict__ fam_row,
+  const int* __restrict__ fam_off, const int* __restrict__ fam_loc, int nz,
+  const float* __restrict__ w, const unsigned* __restrict__ fl_slot,
+  const unsigned* __restrict__ fl_gb, const unsigned* __restrict__ fl,
+  const int* __restrict__ fl_nact, float* __restrict__ part)
 {
-  __shared__ float wr[kFamRowsMax * NB];
-  const int f = ford[blockIdx.x];
   const int r0 = fam_row[f], r1 = fam_row[f + 1];
   const int p0 = fam_off[f];
   const int n_st = (fam_off[f + 1] - p0) * nz;
@@ -2510,6 +2528,20 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_fam_bp(
   }
 }
 
+template <int F, int NB>
+__global__ __launch_bounds__(kSheetBlock) void k_pc_fam_bp(
+  const int* __restrict__ ford,
+  const int* __restrict__ fam_row, const int* __restrict__ fam_off,
+  const int* __restrict__ fam_loc, int nz, const float* __restrict__ w,
+  const unsigned* __restrict__ fl_slot, const unsigned* __restrict__ fl_gb,
+  const unsigned* __restrict__ fl, const int* __restrict__ fl_nact,
+  float* __restrict__ part)
+{
+  __shared__ float wr[kFamRowsMax * NB];
+  fam_bp_family<F, NB>(ford[blockIdx.x], wr, fam_row, fam_off, fam_loc, nz, w,
+                       fl_slot, fl_gb, fl, fl_nact, part);
+}
+
 // out_b[v] = sum over the subset's families whose sheet holds v's pixel of
 // that family's partial of image b (family order). One wave per pixel, a
 // lane per slice (64 slices at a time): the pixel's entries (position and
@@ -2518,13 +2550,10 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_fam_bp(
 // there, never written, is the empty sum).
 constexpr int kRedBlock = 256;
 template <int NB>
-__global__ __launch_bounds__(kRedBlock) void k_pc_fam_reduce(
-  const long long* __restrict__ off, const int2* __restrict__ ent,
-  const float* __restrict__ part, int nxy, int nz, VoxOut out)
+__device__ __forceinline__ void fam_reduce_pixel(
+  int px, const long long* __restrict__ off, const int2* __restrict__ ent,
+  const float* __restrict__ part, int nxy, int nz, const VoxOut& out)
 {
-  const int px = __builtin_amdgcn_readfirstlane(
-    (int)blockIdx.x * (kRedBlock / 64) + ((int)threadIdx.x >> 6));
-  if (px >= nxy) return;
   const int lane = (int)threadIdx.x & 63;
   const long long a = off[px], b = off[px + 1];
   for (int z0 = 0; z0 < nz; z0 += 64)
@@ -2568,6 +2597,16 @@ __global__ __launch_bounds__(kRedBlock) void k_pc_fam_reduce(
       for (int i = 0; i < NB; ++i)
         if (out.p[i]) out.p[i][(long long)z * nxy + px] = acc[i];
   }
+}
+
+template <int NB>
+__global__ __launch_bounds__(kRedBlock) void k_pc_fam_reduce(
+  const long long* __restrict__ off, const int2* __restrict__ ent,
+  const float* __restrict__ part, int nxy, int nz, VoxOut out)
+{
+  const int px = __builtin_amdgcn_readfirstlane(
+    (int)blockIdx.x * (kRedBlock / 64) + ((int)threadIdx.x >> 6));
+  if (px < nxy) fam_reduce_pixel<NB>(px, off, ent, part, nxy, nz, out);
 }
 
 } // namespace
@@ -2695,8 +2734,11 @@ struct fir_cache
   // or sheet pixels), most work first; subset s's at ford_first[s].
   int* d_fam_ord = nullptr;
   std::vector<int> ford_first;
+  std::vector<int> h_ford;        // the same order on the host
+  std::vector<long long> h_fcost; // each family's work (the sort key)
   // partial sheets of one launch, up to 8 images: [subset entries * nz * 8]
   float* d_part = nullptr;
+  long long part_floats = 0;
   long long fr_bytes = 0;
   int family_bp = 1;
   // Batches of 4 or 8 images take their line integrals from slab-staged
@@ -3238,6 +3280,10 @@ int build_family_order(fir_ctx* c, fir_cache* cache)
                                  c->stream));
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
   cache->ford_first = first;
+  cache->h_ford = ord;
+  cache->h_fcost.resize(F);
+  for (int f = 0; f < F; ++f)
+    cache->h_fcost[f] = hw[f] + (long long)(foff[f + 1] - foff[f]) * nz;
   return FIR_OK;
 }
 
@@ -3331,6 +3377,7 @@ int build_family_bp(fir_ctx* c, fir_cache* cache)
   FIR_CHECK_LAUNCH();
   FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
   cache->fr_bytes = bytes;
+  cache->part_floats = part_floats;
   return FIR_OK;
 }
 
@@ -4656,6 +4703,572 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
     }
     FIR_CHECK_LAUNCH();
   }
+  return FIR_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Multi-job OSEM steps: many reconstructions, on several caches of one
+// context, advanced by one sub-iteration each in one launch per phase.
+// ---------------------------------------------------------------------------
+struct fir_osem_step
+{
+  fir_ctx* ctx = nullptr;
+  int fmt = 0;
+  int n_jobs = 0;
+  void* d_jobs = nullptr;      // StepJob[n_jobs]
+  int2* d_items = nullptr;     // (job, family), the most work first: the
+  int n_items = 0;             // jobs of 8-image slots, then of 4
+  int n_items8 = 0;
+  int* d_pack_pre = nullptr;   // [n_jobs + 1] first pack tile of each job
+  int pack_blocks = 0;
+  int* d_red_pre = nullptr;    // [n_jobs + 1] first reduction block
+  int red_blocks = 0;
+  firi::StepImage* d_imgs = nullptr;
+  int n_imgs = 0;
+  int img_tiles = 0;
+  int slab_R = 0, slab_block = 0;
+  size_t slab_lds = 0;
+  int grid_slab[2] = {0, 0}, grid_bp[2] = {0, 0};
+  int* d_ctr = nullptr;        // work-queue counters of the persistent
+                               // launches: next item, blocks done (each)
+  bool gauss = false;
+  float cut = 0.0f;
+  int restore = 0;
+  double restore_r = 0.0;
+  ~fir_osem_step()
+  {
+    if (ctx) (void)hipSetDevice(ctx->device);
+    (void)hipFree(d_jobs);
+    (void)hipFree(d_items);
+    (void)hipFree(d_pack_pre);
+    (void)hipFree(d_red_pre);
+    (void)hipFree(d_imgs);
+    (void)hipFree(d_ctr);
+  }
+};
+
+namespace
+{
+
+// One job of a step, as its kernels read it.
+struct StepJob
+{
+  const int* fam_row;
+  const int* fam_off;
+  const int* fam_xy;
+  const unsigned* sel;
+  RowData rd;
+  const unsigned* fl_slot;
+  const unsigned* fl_gb;
+  const unsigned* fl;
+  const int* fl_nact;
+  const int* fam_loc;
+  const long long* fr_off; // the subset's per-pixel table offsets
+  const int2* fr_ent;
+  float* w;                // workspace: per-row weights of 8 images
+  float* X;                // pixel-major copies of the images
+  float* part;             // partial sheets
+  OsemRowsIO io;
+  ImgIn in;
+  VoxOut out;
+  int nxy, nz, lanes, slab_px, buf_f4, pack_tiles_x;
+  int nb; // images per launch slot: 4 (jobs of up to 4 images) or 8
+};
+
+// The job of block b: the last job whose first block (pre) is <= b.
+__device__ __forceinline__ int step_job_of(const int* __restrict__ pre,
+                                           int n_jobs, int b)
+{
+  int lo = 0, hi = n_jobs - 1;
+  while (lo < hi)
+  {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Pixel-major copies of every job's images (k_pc_pack_pz, 8 images).
+__global__ __launch_bounds__(256) void k_step_pack(
+  const StepJob* __restrict__ jobs, const int* __restrict__ pre, int n_jobs)
+{
+  __shared__ float tile[kPzZ * kMaxCacheBatch * kPzPitch];
+  const int b = blockIdx.x;
+  const int j = step_job_of(pre, n_jobs, b);
+  const StepJob& J = jobs[j];
+  const int t = b - pre[j];
+  const int bz = t / J.pack_tiles_x, bx = t - bz * J.pack_tiles_x;
+  if (J.nb == 4)
+    pack_pz_tile<4>(J.X, J.in, J.io.n_images, J.nxy, J.nz, bx, bz, tile);
+  else
+    pack_pz_tile<8>(J.X, J.in, J.io.n_images, J.nxy, J.nz, bx, bz, tile);
+}
+
+// The next work item of a persistent kernel's block: a block-wide grab from
+// the launch's queue counter q[0]; past the last item the block counts
+// itself done in q[1], and the last block done resets both for the next
+// launch (every block's last grab precedes its count).
+__device__ __forceinline__ int step_next_item(int* q, int* s_it, int n_items)
+{
+  if (threadIdx.x == 0) *s_it = atomicAdd(q, 1);
+  __syncthreads();
+  // (wave-uniform: the job's pointers are then scalar loads)
+  const int it = __builtin_amdgcn_readfirstlane(*s_it);
+  if (it >= n_items && threadIdx.x == 0 &&
+      atomicAdd(q + 1, 1) == (int)gridDim.x - 1)
+  {
+    atomicExch(q, 0);
+    atomicExch(q + 1, 0);
+  }
+  return it;
+}
+
+// Persistent: the blocks of a grid that fills the device take the
+// (job, family) items of the jobs of NB-image slots from a queue, in the
+// table's order (the most work first, so the launch ends on small
+// families). Each item is one family of k_pc_osem_rows_slab. Registers
+// are held to two 768-thread blocks per CU, as that kernel's.
+template <int F, int NB, int R, int D>
+__global__ __launch_bounds__(kSlabBlockMax)
+__attribute__((amdgpu_waves_per_eu(6))) void k_step_slab(
+  const StepJob* __restrict__ jobs, const int2* __restrict__ items,
+  int n_items, int* __restrict__ q)
+{
+  extern __shared__ float4 slab[];
+  // the grabbed item is passed through the (then idle) slab buffer: a
+  // static LDS word beside two 40 KB buffers would leave room for one
+  // block per CU instead of two
+  int* s_it = reinterpret_cast<int*>(slab);
+  for (;;)
+  {
+    const int it = step_next_item(q, s_it, n_items);
+    __syncthreads(); // every wave has the item before the staging reuses it
+    if (it >= n_items) break;
+    const int2 jf = items[it];
+    const StepJob& J = jobs[jf.x];
+    slab_family<F, NB, R, D>(jf.y, slab, J.fam_row, J.fam_off, J.fam_xy,
+                             J.sel, J.rd, J.lanes, J.X, J.nz, J.slab_px,
+                             J.buf_f4, J.io, J.w);
+    __syncthreads(); // the next item's staging reuses the buffers
+  }
+}
+
+// Persistent as k_step_slab: each item one family of k_pc_fam_bp (the
+// weights the slab phase wrote); two 1024-thread blocks per CU.
+template <int F, int NB>
+__global__ __launch_bounds__(kSheetBlock)
+__attribute__((amdgpu_waves_per_eu(8))) void k_step_fam_bp(
+  const StepJob* __restrict__ jobs, const int2* __restrict__ items,
+  int n_items, int* __restrict__ q)
+{
+  __shared__ float wr[kFamRowsMax * NB];
+  __shared__ int s_it;
+  for (;;)
+  {
+    const int it = step_next_item(q, &s_it, n_items);
+    if (it >= n_items) break;
+    const int2 jf = items[it];
+    const StepJob& J = jobs[jf.x];
+    fam_bp_family<F, NB>(jf.y, wr, J.fam_row, J.fam_off, J.fam_loc, J.nz,
+                         J.w, J.fl_slot, J.fl_gb, J.fl, J.fl_nact, J.part);
+    __syncthreads(); // the next item overwrites the weights
+  }
+}
+
+// Per-voxel sums of every job's partial sheets into its bp images
+// (k_pc_fam_reduce, 8 images), one wave per pixel.
+__global__ __launch_bounds__(kRedBlock) void k_step_reduce(
+  const StepJob* __restrict__ jobs, const int* __restrict__ pre, int n_jobs)
+{
+  const int b = blockIdx.x;
+  const int j = step_job_of(pre, n_jobs, b);
+  const StepJob& J = jobs[j];
+  const int px = __builtin_amdgcn_readfirstlane(
+    (b - pre[j]) * (kRedBlock / 64) + ((int)threadIdx.x >> 6));
+  if (px >= J.nxy) return;
+  if (J.nb == 4)
+    fam_reduce_pixel<4>(px, J.fr_off, J.fr_ent, J.part, J.nxy, J.nz, J.out);
+  else
+    fam_reduce_pixel<8>(px, J.fr_off, J.fr_ent, J.part, J.nxy, J.nz, J.out);
+}
+
+constexpr size_t kWsAlign = 256;
+
+size_t ws_round(size_t bytes) { return (bytes + kWsAlign - 1) / kWsAlign * kWsAlign; }
+
+// Workspace layout of a job on a cache: weights, pixel-major images,
+// partial sheets, the Gaussian's two intermediate images per image.
+struct WsLayout
+{
+  size_t w = 0, x = 0, part = 0, t1 = 0, t2 = 0, total = 0;
+};
+
+WsLayout ws_layout(const fir_cache* cache)
+{
+  WsLayout L;
+  const size_t nvox = (size_t)cache->nvox;
+  size_t o = 0;
+  L.w = o;
+  o += ws_round(4 * (size_t)cache->rm->n_rows * kMaxCacheBatch);
+  L.x = o;
+  o += ws_round(4 * nvox * kMaxCacheBatch);
+  L.part = o;
+  o += ws_round(4 * (size_t)std::max(cache->part_floats, 1LL));
+  L.t1 = o;
+  o += ws_round(4 * nvox * kMaxCacheBatch);
+  L.t2 = o;
+  o += ws_round(4 * nvox * kMaxCacheBatch);
+  L.total = o;
+  return L;
+}
+
+// Blocks of a kernel that fit on the device at once (the persistent grid):
+// per CU as many as its 160 KB of LDS and 32 wave slots hold (the kernels
+// keep their registers within that); the runtime's occupancy figure when
+// it allows more. A larger grid is still correct, its extra blocks taking
+// their items once others finish.
+int resident_blocks(const void* kernel, int block, size_t lds, int device)
+{
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block,
+                                                   lds) != hipSuccess)
+    per_cu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                            device) != hipSuccess)
+    return 0;
+  const int waves = (block + 63) / 64;
+  const int by_lds = lds > 0 ? (int)((160 * 1024) / lds) : 32;
+  const int own = std::max(1, std::min(by_lds, 32 / std::max(waves, 1)));
+  return std::max(per_cu, own) * std::max(cus, 1);
+}
+
+// The slab and family-backprojection launches of the items of NB-image
+// slots (k = 0: NB 8, the table's first n8 items; k = 1: NB 4, the rest).
+template <int F, int NB, int R>
+int launch_step_nb(fir_osem_step* st, int k, const int2* items, int n)
+{
+  const StepJob* jobs = reinterpret_cast<const StepJob*>(st->d_jobs);
+  auto kern = k_step_slab<F, NB, R, 2>;
+  static size_t allowed = 0;
+  if (st->slab_lds > 65536 && allowed < st->slab_lds &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)st->slab_lds) == hipSuccess)
+    allowed = st->slab_lds;
+  if (st->grid_slab[k] == 0)
+  {
+    const int res = resident_blocks(reinterpret_cast<const void*>(kern),
+                                    st->slab_block, st->slab_lds,
+                                    st->ctx->device);
+    st->grid_slab[k] = std::max(1, std::min(n, res > 0 ? res : 512));
+  }
+  hipLaunchKernelGGL(kern, dim3(st->grid_slab[k]), dim3(st->slab_block),
+                     st->slab_lds, st->ctx->stream, jobs, items, n,
+                     st->d_ctr + 4 * k);
+  FIR_CHECK_LAUNCH();
+  auto bp = k_step_fam_bp<F, NB>;
+  if (st->grid_bp[k] == 0)
+  {
+    const int res = resident_blocks(reinterpret_cast<const void*>(bp),
+                                    kSheetBlock,
+                                    sizeof(float) * kFamRowsMax * NB,
+                                    st->ctx->device);
+    st->grid_bp[k] = std::max(1, std::min(n, res > 0 ? res : 512));
+  }
+  hipLaunchKernelGGL(bp, dim3(st->grid_bp[k]), dim3(kSheetBlock), 0,
+                     st->ctx->stream, jobs, items, n, st->d_ctr + 4 * k + 2);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+template <int F, int NB>
+int launch_step_r(fir_osem_step* st, int k, const int2* items, int n)
+{
+  switch (st->slab_R)
+  {
+  case 1: return launch_step_nb<F, NB, 1>(st, k, items, n);
+  case 2: return launch_step_nb<F, NB, 2>(st, k, items, n);
+  case 3: return launch_step_nb<F, NB, 3>(st, k, items, n);
+  default: return launch_step_nb<F, NB, 5>(st, k, items, n);
+  }
+}
+
+template <int F>
+int launch_step_projector(fir_osem_step* st)
+{
+  hipStream_t s = st->ctx->stream;
+  const StepJob* jobs = reinterpret_cast<const StepJob*>(st->d_jobs);
+  hipLaunchKernelGGL(k_step_pack, dim3(st->pack_blocks), dim3(256), 0, s,
+                     jobs, st->d_pack_pre, st->n_jobs);
+  FIR_CHECK_LAUNCH();
+  int rc;
+  const int n8 = st->n_items8, n4 = st->n_items - st->n_items8;
+  if (n8 > 0 && (rc = launch_step_r<F, 8>(st, 0, st->d_items, n8)))
+    return rc;
+  if (n4 > 0 && (rc = launch_step_r<F, 4>(st, 1, st->d_items + n8, n4)))
+    return rc;
+  hipLaunchKernelGGL(k_step_reduce, dim3(st->red_blocks), dim3(kRedBlock), 0,
+                     s, jobs, st->d_red_pre, st->n_jobs);
+  FIR_CHECK_LAUNCH();
+  return FIR_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int fir_cache_job_workspace(const fir_cache* cache, int64_t* bytes)
+{
+  int rc = check_cache(cache);
+  if (rc) return rc;
+  if (!bytes) return fail(FIR_E_ARG, "NULL output");
+  *bytes = (int64_t)ws_layout(cache).total;
+  return FIR_OK;
+}
+
+int fir_osem_step_create(int n_jobs, const fir_osem_job* jobs,
+                         const float* fwhm, float cut, fir_osem_step** out)
+{
+  if (!out) return fail(FIR_E_ARG, "NULL output");
+  *out = nullptr;
+  if (n_jobs <= 0 || !jobs) return fail(FIR_E_ARG, "no jobs");
+  int rc;
+  fir_ctx* c = nullptr;
+  int fmt = -1, lanes = 0, max_pairs = 0;
+  for (int j = 0; j < n_jobs; ++j)
+  {
+    const fir_osem_job& J = jobs[j];
+    if ((rc = check_cache(J.cache))) return rc;
+    const fir_cache* cache = J.cache;
+    const RowMatrix* rm = cache->rm.get();
+    if (c && cache->ctx != c)
+      return fail(FIR_E_ARG, "job %d: the caches of a step must share one "
+                             "context", j);
+    c = cache->ctx;
+    if (J.subset < 0 || J.subset >= cache->n_cached)
+      return fail(FIR_E_ARG, "job %d: subset %d is not held by its cache", j,
+                  J.subset);
+    if (J.n_images < 1 || J.n_images > kMaxCacheBatch)
+      return fail(FIR_E_ARG, "job %d: n_images %d outside 1..%d", j,
+                  J.n_images, kMaxCacheBatch);
+    for (int b = 0; b < J.n_images; ++b)
+      if (!J.x[b] || !J.y[b] || !J.bp[b])
+        return fail(FIR_E_ARG, "job %d: NULL image, sinogram or scratch", j);
+    if (!J.sens) return fail(FIR_E_ARG, "job %d: NULL sensitivity", j);
+    if (!J.workspace ||
+        (reinterpret_cast<uintptr_t>(J.workspace) & (kWsAlign - 1)))
+      return fail(FIR_E_ARG, "job %d: workspace NULL or not 256-byte aligned",
+                  j);
+    for (int k = 0; k < j; ++k)
+      if (jobs[k].workspace == J.workspace)
+        return fail(FIR_E_ARG, "jobs %d and %d share a workspace", k, j);
+    if (!use_slabs(cache, kMaxCacheBatch) || cache->h_ford.empty())
+      return fail(FIR_E_ARG, "job %d: its cache lacks the sheet form, family "
+                             "lists or slab form (fir_cache_forms)", j);
+    if (fmt >= 0 && rm->fmt != fmt)
+      return fail(FIR_E_ARG, "job %d: the caches' length formats differ", j);
+    fmt = rm->fmt;
+    if (lanes && cache->fwd_lanes != lanes)
+      return fail(FIR_E_ARG, "job %d: the caches' lanes per LOR differ", j);
+    lanes = cache->fwd_lanes;
+    max_pairs = std::max(max_pairs, rm->n_pairs);
+  }
+  const bool gauss =
+    fwhm && fwhm[0] > 0.0f && fwhm[1] > 0.0f && fwhm[2] > 0.0f;
+  FIR_HIP_CHECK(hipSetDevice(c->device));
+  std::unique_ptr<fir_osem_step> st(new fir_osem_step);
+  st->ctx = c;
+  st->fmt = fmt;
+  st->n_jobs = n_jobs;
+  st->gauss = gauss;
+  st->cut = cut;
+  if (gauss)
+  {
+    // as fir_convolve + fir_cut_circle: the edge restore beyond
+    // cut - 5 * max(FWHM x, y), then the cut (operations.cc:158-220)
+    const float fw = fwhm[0] > fwhm[1] ? fwhm[0] : fwhm[1];
+    st->restore = cut > 0.0f ? 1 : 0;
+    st->restore_r = (double)cut - 5.0 * fw;
+  }
+  // slab launch shape: one block size for every job (the same rows per
+  // family cover every job's families)
+  {
+    SlabShape best;
+    for (int R : kSlabRounds)
+    {
+      const int per_round = (max_pairs + R - 1) / R;
+      const int block = (per_round * lanes + 63) / 64 * 64;
+      if (block <= kSlabBlockMax)
+      {
+        best.R = R;
+        best.block = block;
+        break;
+      }
+    }
+    if (best.R == 0)
+      return fail(FIR_E_ARG, "families of %d rows exceed the slab kernel",
+                  max_pairs);
+    st->slab_R = best.R;
+    st->slab_block = best.block;
+  }
+  std::vector<StepJob> hj(n_jobs);
+  std::vector<std::pair<long long, int2>> items; // (cost, (job, family))
+  std::vector<int> pack_pre(n_jobs + 1), red_pre(n_jobs + 1);
+  std::vector<firi::StepImage> imgs;
+  int pack = 0, red = 0, tiles = 0;
+  for (int j = 0; j < n_jobs; ++j)
+  {
+    const fir_osem_job& J = jobs[j];
+    fir_cache* cache = J.cache;
+    const RowMatrix* rm = cache->rm.get();
+    const fir_vol_desc& v = cache->vol;
+    const int nxy = v.nx * v.ny, nz = v.nz;
+    const WsLayout L = ws_layout(cache);
+    char* ws = static_cast<char*>(J.workspace);
+    const int nb = J.n_images <= 4 ? 4 : 8;
+    const SlabShape sh = slab_shape(cache, nb);
+    StepJob& s = hj[j];
+    std::memset(&s, 0, sizeof(s));
+    s.fam_row = rm->d_fam_row;
+    s.fam_off = rm->d_fam_off;
+    s.fam_xy = rm->d_fam_xy;
+    s.sel = rm->d_sel;
+    s.rd = row_data(rm);
+    s.fl_slot = rm->d_fl_slot;
+    s.fl_gb = rm->d_fl_gb;
+    s.fl = rm->d_fl;
+    s.fl_nact = rm->d_fl_nact;
+    s.fam_loc = cache->d_fam_loc;
+    s.fr_off = cache->d_fr_off + (long long)J.subset * (nxy + 1);
+    s.fr_ent = cache->d_fr_ent;
+    s.w = reinterpret_cast<float*>(ws + L.w);
+    s.X = reinterpret_cast<float*>(ws + L.x);
+    s.part = reinterpret_cast<float*>(ws + L.part);
+    s.io.n_images = J.n_images;
+    for (int b = 0; b < J.n_images; ++b)
+    {
+      s.io.y[b] = J.y[b];
+      s.io.bias[b] = J.bias[b];
+      s.in.p[b] = J.x[b];
+      s.out.p[b] = J.bp[b];
+    }
+    s.nxy = nxy;
+    s.nz = nz;
+    s.lanes = lanes;
+    s.slab_px = sh.slab_px;
+    s.buf_f4 = sh.buf_f4;
+    s.pack_tiles_x = (nxy + kPzPx - 1) / kPzPx;
+    s.nb = nb;
+    st->slab_lds = std::max(st->slab_lds, sh.lds);
+    pack_pre[j] = pack;
+    pack += s.pack_tiles_x * ((nz + kPzZ - 1) / kPzZ);
+    red_pre[j] = red;
+    red += (nxy + kRedBlock / 64 - 1) / (kRedBlock / 64);
+    const int f0 = cache->ford_first[J.subset];
+    const int f1 = cache->ford_first[J.subset + 1];
+    for (int k = f0; k < f1; ++k)
+    {
+      const int f = cache->h_ford[k];
+      items.push_back({cache->h_fcost[f], make_int2(j, f)});
+    }
+    const float* taps = nullptr;
+    int h[3] = {0, 0, 0};
+    if (gauss && (rc = firi::gauss_taps_for(c, &v, fwhm, &taps, h)))
+      return rc;
+    for (int b = 0; b < J.n_images; ++b)
+    {
+      firi::StepImage m;
+      std::memset(&m, 0, sizeof(m));
+      m.x = J.x[b];
+      m.bp = J.bp[b];
+      m.sens = J.sens;
+      m.t1 = reinterpret_cast<float*>(ws + L.t1) + (size_t)b * cache->nvox;
+      m.t2 = reinterpret_cast<float*>(ws + L.t2) + (size_t)b * cache->nvox;
+      m.taps = taps;
+      for (int d = 0; d < 3; ++d) m.h[d] = h[d];
+      m.nx = v.nx;
+      m.ny = v.ny;
+      m.nz = v.nz;
+      m.dx = v.dx;
+      m.dy = v.dy;
+      m.tile0 = tiles;
+      tiles += firi::step_image_tiles(v.nx, v.ny, v.nz);
+      imgs.push_back(m);
+    }
+  }
+  pack_pre[n_jobs] = pack;
+  red_pre[n_jobs] = red;
+  // the jobs of 8-image slots first, each class the most work first; ties
+  // in job then family order
+  std::stable_sort(items.begin(), items.end(),
+                   [&](const std::pair<long long, int2>& a,
+                       const std::pair<long long, int2>& b) {
+                     const int ka = hj[a.second.x].nb == 4;
+                     const int kb = hj[b.second.x].nb == 4;
+                     return ka != kb ? ka < kb : a.first > b.first;
+                   });
+  for (const auto& x : items)
+    if (hj[x.second.x].nb == 8) ++st->n_items8;
+  std::vector<int2> it(items.size());
+  for (size_t k = 0; k < items.size(); ++k) it[k] = items[k].second;
+  st->n_items = (int)it.size();
+  st->pack_blocks = pack;
+  st->red_blocks = red;
+  st->n_imgs = (int)imgs.size();
+  st->img_tiles = tiles;
+  StepJob* d_jobs = nullptr;
+  if ((rc = dev_alloc(&d_jobs, (size_t)n_jobs, "step jobs")))
+    return rc;
+  st->d_jobs = d_jobs;
+  if (
+      (rc = dev_alloc(&st->d_items, std::max(it.size(), (size_t)1),
+                      "step items")) ||
+      (rc = dev_alloc(&st->d_pack_pre, (size_t)n_jobs + 1, "step pack")) ||
+      (rc = dev_alloc(&st->d_red_pre, (size_t)n_jobs + 1, "step reduce")) ||
+      (rc = dev_alloc(&st->d_imgs, imgs.size(), "step images")) ||
+      (rc = dev_alloc(&st->d_ctr, (size_t)8, "step queues")))
+    return rc;
+  hipStream_t s = c->stream;
+  FIR_HIP_CHECK(hipMemsetAsync(st->d_ctr, 0, 8 * sizeof(int), s));
+  FIR_HIP_CHECK(hipMemcpyAsync(st->d_jobs, hj.data(), sizeof(StepJob) * n_jobs,
+                               hipMemcpyHostToDevice, s));
+  if (!it.empty())
+    FIR_HIP_CHECK(hipMemcpyAsync(st->d_items, it.data(),
+                                 sizeof(int2) * it.size(),
+                                 hipMemcpyHostToDevice, s));
+  FIR_HIP_CHECK(hipMemcpyAsync(st->d_pack_pre, pack_pre.data(),
+                               sizeof(int) * (n_jobs + 1),
+                               hipMemcpyHostToDevice, s));
+  FIR_HIP_CHECK(hipMemcpyAsync(st->d_red_pre, red_pre.data(),
+                               sizeof(int) * (n_jobs + 1),
+                               hipMemcpyHostToDevice, s));
+  FIR_HIP_CHECK(hipMemcpyAsync(st->d_imgs, imgs.data(),
+                               sizeof(firi::StepImage) * imgs.size(),
+                               hipMemcpyHostToDevice, s));
+  FIR_HIP_CHECK(hipStreamSynchronize(s));
+  *out = st.release();
+  return FIR_OK;
+}
+
+int fir_osem_step_run(fir_osem_step* st)
+{
+  if (!st || !st->ctx) return fail(FIR_E_ARG, "step is NULL");
+  FIR_HIP_CHECK(hipSetDevice(st->ctx->device));
+  const int rc = st->fmt == FIR_LEN_SNORM16
+    ? launch_step_projector<FIR_LEN_SNORM16>(st)
+    : launch_step_projector<FIR_LEN_FP16>(st);
+  if (rc) return rc;
+  return firi::launch_step_elementwise(st->ctx->stream, st->d_imgs,
+                                       st->n_imgs, st->img_tiles, st->gauss,
+                                       st->cut, st->restore, st->restore_r);
+}
+
+int fir_osem_step_destroy(fir_osem_step* st)
+{
+  delete st;
   return FIR_OK;
 }
 

@@ -61,7 +61,11 @@ EXPORTS = (
     "fir_cache_tune", "fir_cache_forward", "fir_cache_backward",
     "fir_cache_sensitivity", "fir_cache_osem_project",
     "fir_histogram_events", "fir_rebin_weight",
+    "fir_cache_job_workspace", "fir_osem_step_create", "fir_osem_step_run",
+    "fir_osem_step_destroy",
 )
+
+JOB_MAX_IMAGES = 8
 
 
 class VolDesc(C.Structure):
@@ -80,6 +84,17 @@ class VolDesc(C.Structure):
     @property
     def n_voxels(self) -> int:
         return self.nx * self.ny * self.nz
+
+
+class OsemJob(C.Structure):
+    """fir_osem_job: up to 8 reconstructions on one path cache, one subset."""
+    _fields_ = [("cache", C.c_void_p), ("subset", C.c_int),
+                ("n_images", C.c_int),
+                ("x", C.c_void_p * JOB_MAX_IMAGES),
+                ("y", C.c_void_p * JOB_MAX_IMAGES),
+                ("bias", C.c_void_p * JOB_MAX_IMAGES),
+                ("bp", C.c_void_p * JOB_MAX_IMAGES),
+                ("sens", C.c_void_p), ("workspace", C.c_void_p)]
 
 
 class FirError(RuntimeError):
@@ -154,6 +169,10 @@ def load() -> C.CDLL:
         "fir_cache_backward": [vp, vp, vp],
         "fir_cache_sensitivity": [vp, vp],
         "fir_cache_osem_project": [vp, i, vp, vp, vp, vp, i],
+        "fir_cache_job_workspace": [vp, C.POINTER(C.c_int64)],
+        "fir_osem_step_create": [i, vp, vp, f, C.POINTER(C.c_void_p)],
+        "fir_osem_step_run": [vp],
+        "fir_osem_step_destroy": [vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -485,6 +504,61 @@ class PathCache:
         _check(self.lib.fir_cache_osem_project(
             self.h, n, _ptr_array(xs), _ptr_array(ys), pb, _ptr_array(bps),
             subset))
+
+    def job_workspace(self) -> int:
+        """Device bytes of one fir_osem_job workspace on this cache."""
+        out = C.c_int64()
+        _check(self.lib.fir_cache_job_workspace(self.h, C.byref(out)))
+        return int(out.value)
+
+
+class OsemStep:
+    """fir_osem_step: one sub-iteration of many jobs (dicts with keys
+    cache, subset, x, y, bp, sens, workspace and optionally bias; x, y, bp
+    and bias lists of up to 8 device tensors), every phase of all jobs in
+    one launch. The tensors must stay alive and in place while the step
+    exists (its tables hold their addresses)."""
+
+    def __init__(self, jobs, fwhm=(0.0, 0.0, 0.0), cut: float = 0.0):
+        self.lib = load()
+        arr = (OsemJob * len(jobs))()
+        self._keep = []
+        for k, jd in enumerate(jobs):
+            j = arr[k]
+            j.cache = jd["cache"].h
+            j.subset = int(jd["subset"])
+            n = len(jd["x"])
+            if not 1 <= n <= JOB_MAX_IMAGES:
+                raise ValueError(f"a job holds 1 to {JOB_MAX_IMAGES} images")
+            j.n_images = n
+            bias = jd.get("bias") or [None] * n
+            for b in range(n):
+                j.x[b] = _ptr(jd["x"][b])
+                j.y[b] = _ptr(jd["y"][b])
+                j.bp[b] = _ptr(jd["bp"][b])
+                j.bias[b] = _ptr(bias[b])
+            j.sens = _ptr(jd["sens"])
+            j.workspace = _ptr(jd["workspace"])
+            self._keep.append(jd)
+        fw = (C.c_float * 3)(*fwhm)
+        h = C.c_void_p()
+        _check(self.lib.fir_osem_step_create(len(jobs), arr, fw, float(cut),
+                                             C.byref(h)))
+        self.h = h
+
+    def run(self):
+        _check(self.lib.fir_osem_step_run(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fir_osem_step_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _libm = None

@@ -230,3 +230,145 @@ class BatchedOSEM:
         for c in self.caches:
             c.close()
         self.caches = []
+
+
+class MultiOSEM:
+    """Reconstructions of uneven size advanced together through
+    fir_osem_step: every phase of a sub-iteration -- the projector (pack,
+    line integrals, family backprojection, per-voxel sums), the update and
+    the Gaussian -- runs for all solves scheduled at that step in one
+    launch, the projector phases as persistent kernels over one table of
+    (job, family) work items.
+
+    Solves are grouped by (image grid, subset count), each group on its own
+    path cache (caches of one grid share the path matrix), its solves split
+    into jobs of up to 8 images. One iteration of every solve takes K =
+    max(subset counts) steps: a group with S subsets runs its sub-iteration
+    s at step floor(s * K / S), so each solve still visits its subsets in
+    order (reconAlgos.cc:101-199) and the steps' work stays spread out. The
+    iteration's K steps are captured in one HIP graph and replayed.
+
+    ``solves`` as for BatchedOSEM; conv_interval 0 (no Gaussian) or 1
+    (Gaussian every sub-iteration). Each image gets what BatchedOSEM gives
+    it."""
+
+    def __init__(self, workload, solves: List[dict], fwhm=(0.0, 0.0, 0.0),
+                 conv_interval: int = 0, cut: float = 0.0, device: int = 0,
+                 cache_format: int = firlib.FIR_LEN_SNORM16,
+                 cache_row_order: int = firlib.CACHE_ROW_ORDER,
+                 cache_zslices: int = 2):
+        if conv_interval not in (0, 1):
+            raise ValueError("MultiOSEM convolves every sub-iteration or "
+                             "never (conv_interval 1 or 0)")
+        self.fwhm = tuple(float(f) for f in fwhm)
+        conv = conv_interval == 1 and all(f > 0 for f in self.fwhm)
+        self.cut = float(cut)
+        ctx = firlib.context_for(workload, device)
+        ctx.set_uneven_subsets(True)
+        ctx.set_cache_zslices(cache_zslices)
+        ctx.set_cache_lanes(4, 16)  # the slab form's lanes per LOR
+        ctx.set_cache_sheets(True)
+        self.ctx = ctx
+        dev = solves[0]["y"].device
+        keys, self.members, self.caches, self.sens = [], [], [], []
+        for i, s in enumerate(solves):
+            v = s["vol"]
+            k = (v.nx, v.ny, v.nz, v.dx, v.dy, v.dz, v.ox, v.oy, v.oz,
+                 s["n_subsets"])
+            if k not in keys:
+                keys.append(k)
+                self.members.append([])
+            self.members[keys.index(k)].append(i)
+        self.x, self.bp, self.y = [], [], []
+        for s in solves:
+            nv = s["vol"].n_voxels
+            x = torch.ones(nv, device=dev)
+            ctx.cut_circle(s["vol"], x, self.cut)
+            self.x.append(x)
+            self.bp.append(torch.zeros(nv, device=dev))
+            self.y.append(s["y"].contiguous())
+        groups = []
+        for k, mem in zip(keys, self.members):
+            vol = solves[mem[0]]["vol"]
+            S = k[-1]
+            cache = ctx.path_cache(vol, S, cache_format,
+                                   row_order=cache_row_order)
+            self.caches.append(cache)
+            sens = torch.zeros(S * vol.n_voxels, device=dev)
+            cache.sensitivity(sens)
+            self.sens.append(sens)
+            # jobs of 8 images and one of the rest (up to 4 images take the
+            # cheaper 4-image slot)
+            chunks = [mem[a:a + firlib.JOB_MAX_IMAGES]
+                      for a in range(0, len(mem), firlib.JOB_MAX_IMAGES)]
+            groups.append((cache, sens, S, vol.n_voxels, chunks))
+        K = max(g[2] for g in groups)
+        plan = [[] for _ in range(K)]
+        for cache, sens, S, nv, chunks in groups:
+            for sub in range(S):
+                k = sub * K // S
+                for ch in chunks:
+                    plan[k].append((cache, sens, sub, nv, ch))
+        n_ws = max(len(p) for p in plan)
+        ws_bytes = max(c.job_workspace() for c in self.caches)
+        self._ws_raw = []
+        self.workspaces = []
+        for _ in range(n_ws):
+            raw = torch.empty(ws_bytes + 256, dtype=torch.uint8, device=dev)
+            off = (-raw.data_ptr()) % 256
+            self._ws_raw.append(raw)
+            self.workspaces.append(raw[off:off + ws_bytes])
+        self.steps = []
+        for p in plan:
+            jobs = []
+            for slot, (cache, sens, sub, nv, ch) in enumerate(p):
+                jobs.append(dict(
+                    cache=cache, subset=sub,
+                    x=[self.x[i] for i in ch], y=[self.y[i] for i in ch],
+                    bp=[self.bp[i] for i in ch],
+                    sens=sens[sub * nv:(sub + 1) * nv],
+                    workspace=self.workspaces[slot]))
+            self.steps.append(firlib.OsemStep(
+                jobs, self.fwhm if conv else (0.0, 0.0, 0.0), self.cut))
+        self.n_steps = K
+        self.graph = None
+        self.iterations_done = 0
+
+    def iteration_eager(self):
+        for st in self.steps:
+            st.run()
+        self.iterations_done += 1
+
+    def capture(self):
+        """One eager iteration, then the iteration's steps captured in one
+        HIP graph."""
+        if self.iterations_done == 0:
+            self.iteration_eager()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.ctx.set_stream(torch.cuda.current_stream())
+            for st in self.steps:
+                st.run()
+        self.ctx.set_stream(torch.cuda.current_stream())
+        self.graph = g
+
+    def iteration(self):
+        if self.graph is None:
+            self.iteration_eager()
+        else:
+            self.graph.replay()
+            self.iterations_done += 1
+
+    def images(self) -> List[torch.Tensor]:
+        return self.x
+
+    def close(self):
+        self.graph = None
+        for st in self.steps:
+            st.close()
+        self.steps = []
+        for c in self.caches:
+            c.close()
+        self.caches = []
+        self.ctx.close()

@@ -504,6 +504,53 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
                            const float* const* bias, float* const* bp,
                            int subset);
 
+/* ---- Many OSEM solves per launch (no counterpart in the reference) ----
+ * The reference runs one reconstruction at a time (reconAlgos.cc:63-201).
+ * An OSEM step here advances many reconstructions by one sub-iteration
+ * each: jobs of up to 8 images sharing a path cache, on any number of
+ * caches of one context (different image grids and subset counts), every
+ * phase of all jobs in one launch. The projector phases are persistent
+ * kernels: a grid that fills the device walks one table of (job, family)
+ * work items, the most work first, so the launch ends on the smallest
+ * families of whichever jobs hold them. */
+#define FIR_JOB_MAX_IMAGES 8
+typedef struct fir_osem_job
+{
+  fir_cache* cache;    /* path cache with the sheet form and family lists */
+  int subset;          /* the sub-iteration's subset (a held subset) */
+  int n_images;        /* 1 .. FIR_JOB_MAX_IMAGES */
+  float* x[FIR_JOB_MAX_IMAGES];           /* images, updated in place */
+  const float* y[FIR_JOB_MAX_IMAGES];     /* measured sinograms */
+  const float* bias[FIR_JOB_MAX_IMAGES];  /* additive terms, or NULL */
+  float* bp[FIR_JOB_MAX_IMAGES];          /* per-image scratch images */
+  const float* sens;   /* the subset's sensitivity frame (all images) */
+  void* workspace;     /* fir_cache_job_workspace bytes, 256-byte aligned;
+                          jobs of one step need distinct workspaces */
+} fir_osem_job;
+
+/* Device bytes of a job workspace on this cache: per-row weights, the
+ * images' pixel-major copies, the family partial sheets and the Gaussian's
+ * intermediate images of up to 8 images. */
+int fir_cache_job_workspace(const fir_cache* cache, int64_t* bytes);
+
+/* A prepared step: the jobs' tables are uploaded once, so that running it
+ * allocates and copies nothing (it can be captured in a HIP graph and
+ * replayed). fwhm (NULL or all 0: none) and cut as fir_osem_subiter: for
+ * every image, bp = A_s^T (y / (A_s x + bias)) (reconAlgos.cc:126-158),
+ * x = x * bp / sens with the clamps of reconAlgos.cc:160-166, then the
+ * Gaussian with its edge restore (operations.cc:11-185) and the circular
+ * cut (operations.cc:194-220); without a Gaussian, the cut only. bp is
+ * left at 0. Every image gets exactly what fir_cache_osem_project,
+ * fir_osem_update and fir_convolve + fir_cut_circle give it. All caches
+ * must belong to one context, hold the sheet form, family lists and slab
+ * form (fir_cache_forms) in one length format, with 4 lanes per LOR. */
+typedef struct fir_osem_step fir_osem_step;
+int fir_osem_step_create(int n_jobs, const fir_osem_job* jobs,
+                         const float* fwhm, float cut, fir_osem_step** out);
+/* Runs the step on the context's stream (eight launches for all jobs). */
+int fir_osem_step_run(fir_osem_step* step);
+int fir_osem_step_destroy(fir_osem_step* step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,3 +99,48 @@ def test_config4_uneven_batch_full_size():
         c.close()
     b.close()
     ctx.close()
+
+
+def test_config4_multi_step_equals_group_graphs():
+    """The 64 solves advanced by fir_osem_step launches (every phase of
+    every solve scheduled at a step in one launch, the projector phases
+    persistent over one (job, family) work table) give every solve exactly
+    the image the per-group graphs give it, over two iterations (the second
+    replayed from the captured graph)."""
+    w = synth.c2()
+    ctx = firlib.context_for(w)
+    v160 = firlib.VolDesc.from_spec(w.vol)
+    v128 = firlib.VolDesc.from_spec(io.VolSpec((128, 128, 47),
+                                               (2.0, 2.0, 3.27),
+                                               (-127.0, -127.0, 0.0)))
+    act, _ = synth.phantom(w)
+    y0 = torch.zeros(ctx.n_bins, device="cuda")
+    ctx.forward(v160, torch.from_numpy(act.ravel()).cuda(), y0)
+    y0 = y0 / y0.sum()
+    solves = []
+    for i in range(N_SOLVES):
+        seed = 100 + i
+        total = 10 ** np.random.default_rng(seed).uniform(5.0, 8.0)
+        gen = torch.Generator(device="cuda").manual_seed(seed)
+        solves.append(dict(vol=v128 if i % 2 else v160,
+                           n_subsets=(8, 16, 24)[i % 3],
+                           y=torch.poisson(y0 * float(total), generator=gen)))
+    del y0
+    kw = dict(fwhm=w.fwhm, conv_interval=1)
+    b = recon.BatchedOSEM(w, solves, cache_format=firlib.FIR_LEN_SNORM16,
+                          **kw)
+    b.capture()
+    b.iteration()
+    torch.cuda.synchronize()
+    ref = [x.cpu().numpy() for x in b.images()]
+    b.close()
+    ctx.close()
+    m = recon.MultiOSEM(w, solves, **kw)
+    assert m.n_steps == 24
+    m.capture()
+    m.iteration()
+    torch.cuda.synchronize()
+    out = [x.cpu().numpy() for x in m.images()]
+    m.close()
+    diff = [i for i in range(N_SOLVES) if not np.array_equal(out[i], ref[i])]
+    assert not diff, diff

@@ -174,3 +174,84 @@ def test_batched_uneven_groups_cached():
             c.close()
     b.close()
     ctx.close()
+
+
+def _c1_uneven_solves(n):
+    w = synth.c1()
+    ctx = firlib.context_for(w)
+    v64 = firlib.VolDesc.from_spec(w.vol)
+    v48 = firlib.VolDesc.from_spec(io.VolSpec((48, 48, 8), (2.0, 2.0, 2.0),
+                                              (-47.0, -47.0, 0.0)))
+    act, _ = synth.phantom(w)
+    y0 = torch.zeros(ctx.n_bins, device="cuda")
+    ctx.forward(v64, torch.from_numpy(act.ravel()).cuda(), y0)
+    y0 = y0 / y0.sum()
+    solves = []
+    for i in range(n):
+        g = torch.Generator(device="cuda").manual_seed(100 + i)
+        solves.append(dict(vol=v48 if i % 2 else v64,
+                           n_subsets=(2, 4, 8)[i % 3],
+                           y=torch.poisson(y0 * 1e5 * (1 + i), generator=g)))
+    ctx.close()
+    return w, solves
+
+
+@pytest.mark.parametrize("conv", [0, 1])
+def test_multi_step_equals_group_graphs_c1(conv):
+    """fir_osem_step (MultiOSEM): 21 solves on two grids with 2, 4 and 8
+    subsets -- jobs of 8 images and of the rest, 4-image slots for 4 or
+    fewer -- get exactly what the per-group graphs give, with and without
+    the Gaussian."""
+    w, solves = _c1_uneven_solves(21)
+    kw = dict(fwhm=w.fwhm, conv_interval=conv, cut=0.0 if conv else 60.0)
+    b = recon.BatchedOSEM(w, solves, cache_format=firlib.FIR_LEN_SNORM16,
+                          **kw)
+    b.capture()
+    b.iteration()
+    torch.cuda.synchronize()
+    ref = [x.cpu().numpy() for x in b.images()]
+    b.close()
+    m = recon.MultiOSEM(w, solves, **kw)
+    m.capture()
+    m.iteration()
+    torch.cuda.synchronize()
+    out = [x.cpu().numpy() for x in m.images()]
+    m.close()
+    assert all(np.isfinite(o).all() for o in out)
+    diff = [i for i in range(len(out)) if not np.array_equal(out[i], ref[i])]
+    assert not diff, diff
+
+
+def test_multi_step_rejects_bad_jobs():
+    """fir_osem_step_create refuses jobs sharing a workspace, and caches
+    without the sheet form (and so without the slab form)."""
+    w, solves = _c1_uneven_solves(2)
+    ctx = firlib.context_for(w)
+    ctx.set_cache_lanes(4, 16)
+    s0 = solves[0]
+    cache = ctx.path_cache(s0["vol"], s0["n_subsets"],
+                           firlib.FIR_LEN_SNORM16,
+                           row_order=firlib.CACHE_ROW_ORDER)
+    nv = s0["vol"].n_voxels
+    sens = torch.zeros(s0["n_subsets"] * nv, device="cuda")
+    cache.sensitivity(sens)
+    ws = torch.empty(cache.job_workspace() + 256, dtype=torch.uint8,
+                     device="cuda")
+    ws = ws[(-ws.data_ptr()) % 256:][:cache.job_workspace()]
+    job = dict(cache=cache, subset=0, x=[torch.ones(nv, device="cuda")],
+               y=[s0["y"]], bp=[torch.zeros(nv, device="cuda")],
+               sens=sens[:nv], workspace=ws)
+    st = firlib.OsemStep([job])
+    st.run()
+    st.close()
+    with pytest.raises(firlib.FirError, match="share a workspace"):
+        firlib.OsemStep([job, dict(job, subset=1)])
+    ctx.set_cache_sheets(False)
+    c2 = ctx.path_cache(s0["vol"], s0["n_subsets"], firlib.FIR_LEN_SNORM16,
+                        row_order=firlib.CACHE_ROW_ORDER)
+    assert not c2.forms()["sheets"]
+    with pytest.raises(firlib.FirError, match="slab form"):
+        firlib.OsemStep([dict(job, cache=c2)])
+    c2.close()
+    cache.close()
+    ctx.close()
