@@ -184,8 +184,11 @@ def run_uneven(args):
     8, 16 or 24 subsets (24 does not divide the 224 views: interleaved
     uneven subsets), 4 mm Gaussian every sub-iteration. Groups of equal
     (grid, subset count) share each read of their path cache (one shared
-    context; caches of one grid share the path matrix); each group's
-    iteration is one HIP graph. One step = one iteration of all 64 solves.
+    context; caches of one grid share the path matrix). One step = one
+    iteration of all 64 solves: 24 fir_osem_step launches sets (every phase
+    of every solve scheduled at a step in one launch, the projector phases
+    persistent over one work queue) captured in one HIP graph; the same
+    solves through per-group graphs (BatchedOSEM) are timed beside it.
     """
     import torch
     from fir_amd import interfile as io
@@ -211,28 +214,48 @@ def run_uneven(args):
         y = torch.poisson(y0 * float(total), generator=g)
         solves.append(dict(vol=v128 if i % 2 else v160,
                            n_subsets=(8, 16, 24)[i % 3], y=y))
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+
+    def timed(obj):
+        obj.capture()
+        for _ in range(args.warmup):
+            obj.iteration()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            obj.iteration()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
     torch.cuda.synchronize()
     free0 = torch.cuda.mem_get_info()[0]
     t0 = time.perf_counter()
     b = recon.BatchedOSEM(w, solves, fwhm=w.fwhm, conv_interval=1,
                           cache_format=firlib.FIR_LEN_SNORM16)
     torch.cuda.synchronize()
+    group_setup_s = time.perf_counter() - t0
+    log(f"{len(b.groups)} groups, caches built in {group_setup_s:.1f} s")
+    n_groups = len(b.groups)
+    group_ms = timed(b)
+    group_imgs = [x.cpu() for x in b.images()]
+    b.close()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m = recon.MultiOSEM(w, solves, fwhm=w.fwhm, conv_interval=1)
+    torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     used = free0 - torch.cuda.mem_get_info()[0]
-    log(f"{len(b.groups)} groups, caches built in {setup_s:.1f} s")
-    b.capture()
-    for _ in range(args.warmup):
-        b.iteration()
-    stream = torch.cuda.current_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(args.steps):
-        b.iteration()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
+    log(f"multi-solve steps: {m.n_steps} per iteration, caches and "
+        f"workspaces set up in {setup_s:.1f} s")
+    ms = timed(m)
+    # both paths ran capture + warmup + steps iterations from x = 1
+    same = all(torch.equal(a, x.cpu()) for a, x in zip(group_imgs,
+                                                        m.images()))
+    m.close()
+    del group_imgs
     # the plain path, one solve of the same data alone (160 grid, 16 subsets)
     single = recon.OSEMGroup(ctx, v160, 16, [solves[4]["y"]],
                              fwhm=w.fwhm, conv_interval=1)
@@ -256,17 +279,22 @@ def run_uneven(args):
         "config": {"workload": "uneven_64_solves_c2_data",
                    "grids": "128x128x47 and 160x160x47 @ 2x2x3.27 mm",
                    "subsets": [8, 16, 24], "n_solves": 64,
-                   "groups": len(b.groups),
-                   "path": "path cache (snorm16), batched, HIP graphs"},
+                   "groups": n_groups,
+                   "path": "path caches (snorm16); 24 multi-solve steps per "
+                           "iteration, one launch per phase for all solves "
+                           "(persistent projector kernels over one work "
+                           "queue), captured in one HIP graph"},
         "iteration_ms_all_solves": round(ms, 3),
         "iteration_ms_per_solve": round(ms / 64, 4),
+        "per_group_graphs_iteration_ms": round(group_ms, 3),
+        "images_identical_to_per_group_graphs": bool(same),
         "plain_single_solve_iteration_ms": round(plain_ms, 3),
         "speedup_per_solve_vs_plain": round(plain_ms * 64 / ms, 3),
         "cache_setup_s": round(setup_s, 2),
+        "per_group_cache_setup_s": round(group_setup_s, 2),
         "cache_device_gb": round(used / 1e9, 2),
     }
     print(json.dumps(out), flush=True)
-    b.close()
     ctx.close()
 
 
