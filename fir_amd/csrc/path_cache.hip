@@ -1792,6 +1792,34 @@ constexpr int kFlBlock = 1024;
 constexpr long long kFlStageQuads = 1LL << 26; // 1 GiB of list staging
 constexpr int kFlRounds = 8; // list rounds a lane has in flight (4: 1-2% slower)
 
+// A quad of list elements as stored: x = the first element's row in the
+// family (11 bits, a family has at most kFamRowsMax rows) and the three
+// row steps to the next elements (7 bits each; a list is in rising row
+// order, and a padding element, length 0, repeats the previous row),
+// y and z = the four 16-bit lengths as the length array holds them.
+// 12 bytes instead of four (row | length << 16) words.
+struct alignas(4) ListQuad
+{
+  unsigned x, y, z;
+};
+constexpr int kFlRowBits = 11;
+constexpr int kFlStepBits = 7;
+static_assert(kFamRowsMax <= (1 << kFlRowBits), "list rows exceed 11 bits");
+
+__device__ __forceinline__ ListQuad list_quad(const unsigned* fl, long long q)
+{
+  return *reinterpret_cast<const ListQuad*>(fl + 3 * q);
+}
+
+__device__ __forceinline__ void list_rows(unsigned x, unsigned* r)
+{
+  constexpr unsigned m = (1u << kFlStepBits) - 1u;
+  r[0] = x & ((1u << kFlRowBits) - 1u);
+  r[1] = r[0] + ((x >> kFlRowBits) & m);
+  r[2] = r[1] + ((x >> (kFlRowBits + kFlStepBits)) & m);
+  r[3] = r[2] + (x >> (kFlRowBits + 2 * kFlStepBits));
+}
+
 __device__ __forceinline__ void fl_count(const int* __restrict__ fam_row,
                                          const long long* __restrict__ off,
                                          const unsigned* __restrict__ sel,
@@ -2100,18 +2128,21 @@ __global__ __launch_bounds__(kFlBlock) void k_fl_interleave(
 // m-th quad of its slots longer than m, which are a prefix of the group),
 // then the elements placed as k_fl_fill places them (rows in groups of 32,
 // each list in row order) straight at their final positions, and each
-// list's last quad padded with zeros. A family
+// list's last quad padded with zeros. The families f_first + blockIdx.x
+// write their quads [q, q + n) as four words each at fl[4 * (q - dst_base)]
+// (a staging buffer that k_fl_pack then stores as ListQuads). A family
 // whose round-offset table exceeds kFlRTable entries sets *overflow and
 // writes nothing (its lists then take the two-pass route).
 constexpr int kFlRTable = 4096;
 constexpr int kFlK = 6; // elements per lane of a row held in registers
 
 __global__ __launch_bounds__(kFlBlock) void k_fl_build(
-  const int* __restrict__ fam_row, const int* __restrict__ fam_off, int nz,
+  int f_first, const int* __restrict__ fam_row,
+  const int* __restrict__ fam_off, int nz,
   const unsigned* __restrict__ fam_zr, const long long* __restrict__ off,
   const unsigned* __restrict__ sel, const long long* __restrict__ fbase,
   unsigned* __restrict__ fl_slot, unsigned* __restrict__ fl_gb,
-  unsigned* __restrict__ fl, int* __restrict__ fl_nact,
+  unsigned* __restrict__ fl, long long dst_base, int* __restrict__ fl_nact,
   int* __restrict__ overflow)
 {
   __shared__ unsigned cnt[kSheetMaxFloats];          // counts, then row bits
@@ -2123,7 +2154,7 @@ __global__ __launch_bounds__(kFlBlock) void k_fl_build(
   __shared__ unsigned gbase[kSheetMaxFloats / 64 + 1];
   __shared__ unsigned rofs[kSheetMaxFloats / 64 + 1];
   __shared__ int too_big;
-  const int f = blockIdx.x;
+  const int f = f_first + (int)blockIdx.x;
   const int n_st = (fam_off[f + 1] - fam_off[f]) * nz;
   if (n_st == 0) return;
   const long long g0 = (long long)fam_off[f] * nz;
@@ -2276,9 +2307,8 @@ __global__ __launch_bounds__(kFlBlock) void k_fl_build(
       const unsigned c =
         cur[en] + (unsigned)__popc(cnt[en] & ((1u << rl) - 1u));
       const unsigned sl = slot[en], g = sl >> 6;
-      const unsigned long long quad =
-        (unsigned long long)(base + gbase[g]) + rtab[rofs[g] + (c >> 2)] +
-        (sl & 63u);
+      const long long quad =
+        base - dst_base + gbase[g] + rtab[rofs[g] + (c >> 2)] + (sl & 63u);
       fl[quad * 4 + (c & 3u)] = (unsigned)(ra + rl - r0) | (u & 0xffff0000u);
     };
 #pragma unroll
@@ -2327,12 +2357,42 @@ __global__ __launch_bounds__(kFlBlock) void k_fl_build(
     const unsigned sl = slot[en], g = sl >> 6;
     for (unsigned c = cur[en]; c & 3u; ++c)
     {
-      const unsigned long long quad =
-        (unsigned long long)(base + gbase[g]) + rtab[rofs[g] + (c >> 2)] +
-        (sl & 63u);
+      const long long quad =
+        base - dst_base + gbase[g] + rtab[rofs[g] + (c >> 2)] + (sl & 63u);
       fl[quad * 4 + (c & 3u)] = 0u;
     }
   }
+}
+
+// The n list quads of a staging buffer (four row | length << 16 words
+// each) as ListQuads at dst. Sets *overflow when a row step of some quad
+// exceeds what the 7-bit fields hold, or the rows fall (the lists then
+// cannot be stored).
+__global__ __launch_bounds__(kBlock) void k_fl_pack(
+  const unsigned* __restrict__ src, long long n, unsigned* __restrict__ dst,
+  int* __restrict__ overflow)
+{
+  const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint4 v = reinterpret_cast<const uint4*>(src)[i];
+  const unsigned u[4] = {v.x, v.y, v.z, v.w};
+  // a quad starts with an element (padding only ends a list)
+  unsigned row = u[0] & 0xffffu, x = row;
+  bool ok = row < (1u << kFlRowBits);
+#pragma unroll
+  for (int k = 1; k < 4; ++k)
+  {
+    const unsigned r = (u[k] >> 16) ? (u[k] & 0xffffu) : row;
+    const unsigned d = r - row; // wraps above the 7 bits if the rows fall
+    ok = ok && d < (1u << kFlStepBits);
+    x |= d << (kFlRowBits + (k - 1) * kFlStepBits);
+    row = r;
+  }
+  if (!ok) atomicOr(overflow, 1);
+  unsigned* o = dst + 3 * i;
+  o[0] = x;
+  o[1] = (u[0] >> 16) | (u[1] & 0xffff0000u);
+  o[2] = (u[2] >> 16) | (u[3] & 0xffff0000u);
 }
 
 // k_pc_osem_rows_sheet plus the family's backprojection: the weights stay
@@ -2415,14 +2475,14 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
 #pragma unroll
       for (int k = 0; k < kFlRounds; ++k) act[k] = __ballot(m + k < q);
       if (act[0] == 0) break;
-      uint4 v[kFlRounds];
+      ListQuad v[kFlRounds];
       unsigned o = roff;
 #pragma unroll
       for (int k = 0; k < kFlRounds; ++k)
       {
-        v[k] = m + k < q ? *reinterpret_cast<const uint4*>(
-                             fl + 4LL * (gb + o + __popcll(act[k] & below)))
-                         : make_uint4(0u, 0u, 0u, 0u);
+        v[k] = m + k < q ? list_quad(fl, (long long)gb + o +
+                                            __popcll(act[k] & below))
+                         : ListQuad{0u, 0u, 0u};
         o += (unsigned)__popcll(act[k]);
       }
       roff = o;
@@ -2430,11 +2490,13 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
       for (int k = 0; k < kFlRounds; ++k)
       {
         float l[4];
-        Quad<F>::dec(sheet_lengths(v[k]), l);
-        acc = fmaf(l[0], wr[v[k].x & 0xffffu], acc);
-        acc = fmaf(l[1], wr[v[k].y & 0xffffu], acc);
-        acc = fmaf(l[2], wr[v[k].z & 0xffffu], acc);
-        acc = fmaf(l[3], wr[v[k].w & 0xffffu], acc);
+        unsigned rw[4];
+        Quad<F>::dec(make_uint2(v[k].y, v[k].z), l);
+        list_rows(v[k].x, rw);
+        acc = fmaf(l[0], wr[rw[0]], acc);
+        acc = fmaf(l[1], wr[rw[1]], acc);
+        acc = fmaf(l[2], wr[rw[2]], acc);
+        acc = fmaf(l[3], wr[rw[3]], acc);
       }
     }
     if (j < n_act) pf[sl >> 16] = acc;
@@ -2492,14 +2554,14 @@ __device__ __forceinline__ void fam_bp_family(
 #pragma unroll
       for (int k = 0; k < kFlRounds; ++k) act[k] = __ballot(m + k < q);
       if (act[0] == 0) break;
-      uint4 v[kFlRounds];
+      ListQuad v[kFlRounds];
       unsigned o = roff;
 #pragma unroll
       for (int k = 0; k < kFlRounds; ++k)
       {
-        v[k] = m + k < q ? *reinterpret_cast<const uint4*>(
-                             fl + 4LL * (gb + o + __popcll(act[k] & below)))
-                         : make_uint4(0u, 0u, 0u, 0u);
+        v[k] = m + k < q ? list_quad(fl, (long long)gb + o +
+                                            __popcll(act[k] & below))
+                         : ListQuad{0u, 0u, 0u};
         o += (unsigned)__popcll(act[k]);
       }
       roff = o;
@@ -2507,9 +2569,9 @@ __device__ __forceinline__ void fam_bp_family(
       for (int k = 0; k < kFlRounds; ++k)
       {
         float l[4];
-        Quad<F>::dec(sheet_lengths(v[k]), l);
-        const unsigned rw[4] = {v[k].x & 0xffffu, v[k].y & 0xffffu,
-                                v[k].z & 0xffffu, v[k].w & 0xffffu};
+        unsigned rw[4];
+        Quad<F>::dec(make_uint2(v[k].y, v[k].z), l);
+        list_rows(v[k].x, rw);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
         {
@@ -2643,11 +2705,12 @@ struct RowMatrix
   long long sheet_bytes = 0;
   long long n_sheet_px = 0;         // sheet pixels over all families
   // Per sheet entry (pixel, slice) of every family, the family's elements
-  // through that voxel (row in family | length << 16), whole quads, in the
-  // slot order and round-interleaved groups of k_fl_interleave.
+  // through that voxel (row in family and length), whole quads stored as
+  // ListQuads, in the slot order and round-interleaved groups of
+  // k_fl_interleave.
   unsigned* d_fl_slot = nullptr;    // [n_sheet_px * nz] entry << 16 | quads
   unsigned* d_fl_gb = nullptr;      // [n_sheet_px * nz] group's first quad
-  unsigned* d_fl = nullptr;         // [4 * fl_quads]
+  unsigned* d_fl = nullptr;         // [3 * fl_quads] ListQuads
   // per family, its slots holding a list or inside their pixel's slice
   // range (the slots before the ones never read)
   int* d_fl_nact = nullptr;
@@ -2673,7 +2736,7 @@ struct RowMatrix
   {
     return (long long)(n_views + 1) * (4 + 8) + (long long)n_rows * (4 + 4 + 1 + 8) +
       8 + n_slots * (4 + (long long)len_bytes(fmt)) + sheet_bytes +
-      (d_fl ? 8 * n_sheet_px * vol.nz + 16 * fl_quads + 4LL * n_fam : 0);
+      (d_fl ? 8 * n_sheet_px * vol.nz + 12 * fl_quads + 4LL * n_fam : 0);
   }
 
   ~RowMatrix()
@@ -2792,6 +2855,16 @@ struct DevBufs
     }
     p.push_back(*out);
     return FIR_OK;
+  }
+  void release(void* q)
+  {
+    for (size_t i = 0; i < p.size(); ++i)
+      if (p[i] == q)
+      {
+        (void)hipFree(q);
+        p.erase(p.begin() + (long)i);
+        return;
+      }
   }
 };
 
@@ -3078,8 +3151,9 @@ int build_sheets(fir_ctx* c, RowMatrix* rm)
 
 // The per-entry element lists of every family (the family backprojection's
 // input), for a matrix with the sheet form. Families with more rows than
-// the fused kernel holds, lists beyond 2^32 quads, or a device too full
-// leave the matrix without them; only HIP failures are errors.
+// the fused kernel holds, lists beyond 2^32 quads, row steps within a quad
+// beyond ListQuad's 7 bits, or a device too full leave the matrix without
+// them; only HIP failures are errors.
 int build_family_lists(fir_ctx* c, RowMatrix* rm)
 {
   const int F = rm->n_fam, nz = rm->vol.nz;
@@ -3087,8 +3161,11 @@ int build_family_lists(fir_ctx* c, RowMatrix* rm)
     return FIR_OK;
   DevBufs tmp;
   long long* fbase;
+  int* flags; // [0]: a round table overflowed, [1]: a quad does not pack
   int rc;
-  if ((rc = tmp.get(&fbase, (size_t)F, "family list bases"))) return rc;
+  if ((rc = tmp.get(&fbase, (size_t)F, "family list bases")) ||
+      (rc = tmp.get(&flags, 2, "family list flags")))
+    return rc;
   // the quads of every family's lists, counted by the sheet build
   const std::vector<unsigned>& tot = rm->list_quads;
   if ((int)tot.size() != F) return FIR_OK;
@@ -3101,48 +3178,19 @@ int build_family_lists(fir_ctx* c, RowMatrix* rm)
   }
   const long long entries = rm->n_sheet_px * nz;
   if (quads >= (1LL << 32)) return FIR_OK;
-  size_t free_b = 0, total_b = 0;
-  FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-  if ((long long)free_b >= 16 * quads + 8 * entries + (3LL << 30))
+  const long long stage_q = std::min(quads, kFlStageQuads);
+  // runs of families whose quads fit the staging buffer (a family holds far
+  // fewer: n_st * 512)
+  std::vector<int> runs{0};
+  for (int f0 = 0; f0 < F;)
   {
-    // one pass per family into the final layout
-    int* overflow;
-    if ((rc = tmp.get(&overflow, 1, "family list overflow flag"))) return rc;
-    if ((rc = dev_alloc(&rm->d_fl_slot, (size_t)entries,
-                        "family list slots")) ||
-        (rc = dev_alloc(&rm->d_fl_gb, (size_t)entries, "family list groups")) ||
-        (rc = dev_alloc(&rm->d_fl_nact, (size_t)F, "family list actives")) ||
-        (rc = dev_alloc(&rm->d_fl, (size_t)(4 * std::max(quads, 1LL)),
-                        "family lists")))
-    {
-      (void)hipFree(rm->d_fl_slot);
-      (void)hipFree(rm->d_fl_gb);
-      (void)hipFree(rm->d_fl_nact);
-      rm->d_fl_slot = nullptr;
-      rm->d_fl_gb = nullptr;
-      rm->d_fl_nact = nullptr;
-      rm->d_fl = nullptr;
-      return rc == FIR_E_NOMEM ? FIR_OK : rc;
-    }
-    FIR_HIP_CHECK(hipMemcpyAsync(fbase, base.data(), 8LL * F,
-                                 hipMemcpyHostToDevice, c->stream));
-    FIR_HIP_CHECK(hipMemsetAsync(overflow, 0, sizeof(int), c->stream));
-    FIR_HIP_CHECK(hipMemsetAsync(rm->d_fl_nact, 0, 4LL * F, c->stream));
-    hipLaunchKernelGGL(k_fl_build, dim3(F), dim3(kFlBlock), 0, c->stream,
-                       rm->d_fam_row, rm->d_fam_off, nz, rm->d_fam_zr,
-                       rm->d_off, rm->d_sel, fbase, rm->d_fl_slot,
-                       rm->d_fl_gb, rm->d_fl, rm->d_fl_nact, overflow);
-    FIR_CHECK_LAUNCH();
-    int ovf = 0;
-    FIR_HIP_CHECK(hipMemcpyAsync(&ovf, overflow, sizeof(int),
-                                 hipMemcpyDeviceToHost, c->stream));
-    FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
-    if (!ovf)
-    {
-      rm->fl_quads = quads;
-      return FIR_OK;
-    }
-    // some family's round table is too large for LDS: the two-pass route
+    int f1 = f0 + 1;
+    while (f1 < F && base[f1] + tot[f1] - base[f0] <= stage_q) ++f1;
+    if ((f1 < F ? base[f1] : quads) - base[f0] > stage_q)
+      return fail(FIR_E_NOMEM, "family lists: a family exceeds the staging");
+    runs.push_back(f0 = f1);
+  }
+  auto drop = [&]() {
     (void)hipFree(rm->d_fl_slot);
     (void)hipFree(rm->d_fl_gb);
     (void)hipFree(rm->d_fl_nact);
@@ -3151,66 +3199,118 @@ int build_family_lists(fir_ctx* c, RowMatrix* rm)
     rm->d_fl_gb = nullptr;
     rm->d_fl_nact = nullptr;
     rm->d_fl = nullptr;
+  };
+  auto alloc_lists = [&]() {
+    int r;
+    if ((r = dev_alloc(&rm->d_fl_slot, (size_t)entries,
+                       "family list slots")) ||
+        (r = dev_alloc(&rm->d_fl_gb, (size_t)entries, "family list groups")) ||
+        (r = dev_alloc(&rm->d_fl_nact, (size_t)F, "family list actives")) ||
+        (r = dev_alloc(&rm->d_fl, (size_t)(3 * std::max(quads, 1LL)),
+                       "family lists")))
+      drop();
+    return r;
+  };
+  // a run's quads from the staging buffer into their packed form
+  auto pack = [&](unsigned* stage, int i) {
+    const long long q0 = base[runs[i]];
+    const long long nq = (runs[i + 1] < F ? base[runs[i + 1]] : quads) - q0;
+    if (nq > 0)
+      hipLaunchKernelGGL(k_fl_pack, dim3((unsigned)((nq + kBlock - 1) / kBlock)),
+                         dim3(kBlock), 0, c->stream, stage, nq,
+                         rm->d_fl + 3 * q0, flags + 1);
+    return q0;
+  };
+  auto finish = [&](int* ovf) -> int {
+    FIR_HIP_CHECK(hipMemcpyAsync(ovf, flags, 2 * sizeof(int),
+                                 hipMemcpyDeviceToHost, c->stream));
+    FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return FIR_OK;
+  };
+  size_t free_b = 0, total_b = 0;
+  FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  unsigned* stage;
+  if ((long long)free_b >= 12 * quads + 16 * stage_q + 8 * entries + (3LL << 30))
+  {
+    // one pass per family into the final layout, run by run through the
+    // staging buffer
+    if ((rc = tmp.get(&stage, (size_t)(4 * std::max(stage_q, 1LL)),
+                      "family list staging")))
+      return rc == FIR_E_NOMEM ? FIR_OK : rc;
+    if ((rc = alloc_lists())) return rc == FIR_E_NOMEM ? FIR_OK : rc;
+    FIR_HIP_CHECK(hipMemcpyAsync(fbase, base.data(), 8LL * F,
+                                 hipMemcpyHostToDevice, c->stream));
+    FIR_HIP_CHECK(hipMemsetAsync(flags, 0, 2 * sizeof(int), c->stream));
+    FIR_HIP_CHECK(hipMemsetAsync(rm->d_fl_nact, 0, 4LL * F, c->stream));
+    for (size_t i = 0; i + 1 < runs.size(); ++i)
+    {
+      const long long q0 = base[runs[i]];
+      hipLaunchKernelGGL(k_fl_build, dim3(runs[i + 1] - runs[i]),
+                         dim3(kFlBlock), 0, c->stream, runs[i],
+                         rm->d_fam_row, rm->d_fam_off, nz, rm->d_fam_zr,
+                         rm->d_off, rm->d_sel, fbase, rm->d_fl_slot,
+                         rm->d_fl_gb, stage, q0, rm->d_fl_nact, flags);
+      FIR_CHECK_LAUNCH();
+      pack(stage, (int)i);
+      FIR_CHECK_LAUNCH();
+    }
+    int ovf[2] = {0, 0};
+    if ((rc = finish(ovf))) return rc;
+    if (!ovf[0] && !ovf[1])
+    {
+      rm->fl_quads = quads;
+      return FIR_OK;
+    }
+    drop();
+    if (!ovf[0]) return FIR_OK; // row steps beyond 7 bits
+    // some family's round table is too large for LDS: the two-pass route
+    tmp.release(stage);
     FIR_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
   }
-  // the contiguous sorted lists go into the final buffer first; runs of
-  // families are then interleaved through a staging buffer and copied back
-  const long long stage_q = std::min(quads, kFlStageQuads);
+  // the contiguous sorted lists go into a full-size buffer first; runs of
+  // families are then interleaved through the staging buffer and packed
   if ((long long)free_b <
-      16 * (quads + stage_q) + 12 * entries + (3LL << 30))
+      16 * (quads + stage_q) + 12 * quads + 12 * entries + (3LL << 30))
     return FIR_OK;
   unsigned* cl_off;
-  unsigned* stage;
+  unsigned* cl;
   if ((rc = tmp.get(&cl_off, (size_t)entries + 1, "family list offsets")) ||
+      (rc = tmp.get(&cl, (size_t)(4 * std::max(quads, 1LL)),
+                    "family contiguous lists")) ||
       (rc = tmp.get(&stage, (size_t)(4 * std::max(stage_q, 1LL)),
                     "family list staging")))
     return rc == FIR_E_NOMEM ? FIR_OK : rc;
-  if ((rc = dev_alloc(&rm->d_fl_slot, (size_t)entries, "family list slots")) ||
-      (rc = dev_alloc(&rm->d_fl_gb, (size_t)entries, "family list groups")) ||
-      (rc = dev_alloc(&rm->d_fl_nact, (size_t)F, "family list actives")) ||
-      (rc = dev_alloc(&rm->d_fl, (size_t)(4 * std::max(quads, 1LL)),
-                      "family lists")))
-  {
-    (void)hipFree(rm->d_fl_slot);
-    (void)hipFree(rm->d_fl_gb);
-    (void)hipFree(rm->d_fl_nact);
-    rm->d_fl_slot = nullptr;
-    rm->d_fl_gb = nullptr;
-    rm->d_fl_nact = nullptr;
-    rm->d_fl = nullptr;
-    return rc == FIR_E_NOMEM ? FIR_OK : rc;
-  }
+  if ((rc = alloc_lists())) return rc == FIR_E_NOMEM ? FIR_OK : rc;
+  FIR_HIP_CHECK(hipMemsetAsync(flags, 0, 2 * sizeof(int), c->stream));
   FIR_HIP_CHECK(hipMemsetAsync(rm->d_fl_nact, 0, 4LL * F, c->stream));
   const unsigned end = (unsigned)quads;
   FIR_HIP_CHECK(hipMemcpyAsync(fbase, base.data(), 8LL * F,
                                hipMemcpyHostToDevice, c->stream));
   FIR_HIP_CHECK(hipMemcpyAsync(cl_off + entries, &end, 4,
                                hipMemcpyHostToDevice, c->stream));
-  FIR_HIP_CHECK(hipMemsetAsync(rm->d_fl, 0, 16 * quads, c->stream));
+  FIR_HIP_CHECK(hipMemsetAsync(cl, 0, 16 * quads, c->stream));
   hipLaunchKernelGGL(k_fl_fill, dim3(F), dim3(kFlBlock), 0, c->stream,
                      rm->d_fam_row, rm->d_fam_off, nz, rm->d_off, rm->d_sel,
-                     fbase, cl_off, rm->d_fl);
+                     fbase, cl_off, cl);
   FIR_CHECK_LAUNCH();
-  // (a family holds far fewer quads than the staging buffer: n_st * 512)
-  for (int f0 = 0; f0 < F;)
+  for (size_t i = 0; i + 1 < runs.size(); ++i)
   {
-    int f1 = f0 + 1;
-    while (f1 < F && base[f1] + tot[f1] - base[f0] <= stage_q) ++f1;
-    const long long q0 = base[f0];
-    const long long nq = (f1 < F ? base[f1] : quads) - q0;
-    if (nq > stage_q)
-      return fail(FIR_E_NOMEM, "family lists: a family exceeds the staging");
-    hipLaunchKernelGGL(k_fl_interleave, dim3(f1 - f0), dim3(kFlBlock), 0,
-                       c->stream, f0, rm->d_fam_off, nz, rm->d_fam_zr, fbase,
-                       cl_off, rm->d_fl, rm->d_fl_slot, rm->d_fl_gb, stage,
-                       rm->d_fl_nact, q0);
+    const long long q0 = base[runs[i]];
+    hipLaunchKernelGGL(k_fl_interleave, dim3(runs[i + 1] - runs[i]),
+                       dim3(kFlBlock), 0, c->stream, runs[i], rm->d_fam_off,
+                       nz, rm->d_fam_zr, fbase, cl_off, cl, rm->d_fl_slot,
+                       rm->d_fl_gb, stage, rm->d_fl_nact, q0);
     FIR_CHECK_LAUNCH();
-    if (nq > 0)
-      FIR_HIP_CHECK(hipMemcpyAsync(rm->d_fl + 4 * q0, stage, 16 * nq,
-                                   hipMemcpyDeviceToDevice, c->stream));
-    f0 = f1;
+    pack(stage, (int)i);
+    FIR_CHECK_LAUNCH();
   }
-  FIR_HIP_CHECK(hipStreamSynchronize(c->stream));
+  int ovf[2] = {0, 0};
+  if ((rc = finish(ovf))) return rc;
+  if (ovf[1])
+  {
+    drop();
+    return FIR_OK;
+  }
   rm->fl_quads = quads;
   return FIR_OK;
 }
