@@ -1780,10 +1780,10 @@ __global__ __launch_bounds__(kSlabBlockMax) void k_pc_osem_rows_slab(
 // ---------------------------------------------------------------------------
 // The block that computed a family's line integrals also backprojects its
 // weights, which it keeps in LDS: every sheet entry (pixel rank * nz + z)
-// has a list of the family's elements through that voxel (row in family |
-// length << 16, rows ascending, padded to whole quads with zeros), and the
-// block sums each list against the LDS weights into a partial sheet
-// (global entry (fam_off[f] + pixel rank) * nz + z). A reduction then sums,
+// has a list of the family's elements through that voxel (row in family
+// and length, rows ascending, padded to whole quads, stored as ListQuads),
+// and the block sums each list against the LDS weights into a partial
+// sheet (global entry (fam_off[f] + pixel rank) * nz + z). A reduction then sums,
 // for every voxel, the partials of the subset's families whose sheet holds
 // its pixel, in family order. No atomics: the sums are the same every run.
 
