@@ -187,8 +187,10 @@ def run_uneven(args):
     context; caches of one grid share the path matrix). One step = one
     iteration of all 64 solves: 24 fir_osem_step launches sets (every phase
     of every solve scheduled at a step in one launch, the projector phases
-    persistent over one work queue) captured in one HIP graph; the same
-    solves through per-group graphs (BatchedOSEM) are timed beside it.
+    persistent over one work queue) captured in one HIP graph, on caches
+    whose LOR sort key and lanes per LOR the autotuner picks; the same
+    solves through per-group graphs (BatchedOSEM) on caches of the same
+    layout are timed beside it.
     """
     import torch
     from fir_amd import interfile as io
@@ -233,29 +235,34 @@ def run_uneven(args):
     torch.cuda.synchronize()
     free0 = torch.cuda.mem_get_info()[0]
     t0 = time.perf_counter()
+    m = recon.MultiOSEM(w, solves, fwhm=w.fwhm, conv_interval=1,
+                        autotune_layout=True)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    used = free0 - torch.cuda.mem_get_info()[0]
+    tuning, m_tuning_s = m.tuning, m.tuning_s
+    log(f"multi-solve steps: {m.n_steps} per iteration, layout tuned "
+        f"({tuning.best}), caches and workspaces set up in {setup_s:.1f} s")
+    ms = timed(m)
+    multi_imgs = [x.cpu() for x in m.images()]
+    layout = dict(cache_row_order=m.cache_row_order,
+                  cache_lanes=(m.cache_lanes, 16))
+    m.close()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     b = recon.BatchedOSEM(w, solves, fwhm=w.fwhm, conv_interval=1,
-                          cache_format=firlib.FIR_LEN_SNORM16)
+                          cache_format=firlib.FIR_LEN_SNORM16, **layout)
     torch.cuda.synchronize()
     group_setup_s = time.perf_counter() - t0
     log(f"{len(b.groups)} groups, caches built in {group_setup_s:.1f} s")
     n_groups = len(b.groups)
     group_ms = timed(b)
-    group_imgs = [x.cpu() for x in b.images()]
+    # both paths ran capture + warmup + steps iterations from x = 1 on
+    # caches of the same layout
+    same = all(torch.equal(a, x.cpu()) for a, x in zip(multi_imgs,
+                                                        b.images()))
     b.close()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    m = recon.MultiOSEM(w, solves, fwhm=w.fwhm, conv_interval=1)
-    torch.cuda.synchronize()
-    setup_s = time.perf_counter() - t0
-    used = free0 - torch.cuda.mem_get_info()[0]
-    log(f"multi-solve steps: {m.n_steps} per iteration, caches and "
-        f"workspaces set up in {setup_s:.1f} s")
-    ms = timed(m)
-    # both paths ran capture + warmup + steps iterations from x = 1
-    same = all(torch.equal(a, x.cpu()) for a, x in zip(group_imgs,
-                                                        m.images()))
-    m.close()
-    del group_imgs
+    del multi_imgs
     # the plain path, one solve of the same data alone (160 grid, 16 subsets)
     single = recon.OSEMGroup(ctx, v160, 16, [solves[4]["y"]],
                              fwhm=w.fwhm, conv_interval=1)
@@ -280,7 +287,8 @@ def run_uneven(args):
                    "grids": "128x128x47 and 160x160x47 @ 2x2x3.27 mm",
                    "subsets": [8, 16, 24], "n_solves": 64,
                    "groups": n_groups,
-                   "path": "path caches (snorm16); 24 multi-solve steps per "
+                   "path": "path caches (snorm16, LOR sort key and lanes "
+                           "per LOR autotuned); 24 multi-solve steps per "
                            "iteration, one launch per phase for all solves "
                            "(persistent projector kernels over one work "
                            "queue), captured in one HIP graph"},
@@ -290,7 +298,10 @@ def run_uneven(args):
         "images_identical_to_per_group_graphs": bool(same),
         "plain_single_solve_iteration_ms": round(plain_ms, 3),
         "speedup_per_solve_vs_plain": round(plain_ms * 64 / ms, 3),
-        "cache_setup_s": round(setup_s, 2),
+        "layout_autotune": {"best": tuning.best,
+                            "trials_ms": tuning.table(),
+                            "tuning_s": round(m_tuning_s, 2)},
+        "cache_setup_s": round(setup_s - m_tuning_s, 2),
         "per_group_cache_setup_s": round(group_setup_s, 2),
         "cache_device_gb": round(used / 1e9, 2),
     }

@@ -1,7 +1,8 @@
 """Autotuner of the projector kernels: picks, by timing the real workload on
 the device, the LOR sort key, the lanes per LOR / per voxel and the z-slice
-interleave of a path cache's layout, and the row order and per-XCD scatter
-copies of the plain projector.
+interleave of a path cache's layout, the LOR sort key and lanes per LOR of
+the multi-solve steps, and the row order and per-XCD scatter copies of the
+plain projector.
 
 Every candidate changes only the FP32 summation order of the results (the
 tests check that), so the choice is purely a matter of speed. Timings are
@@ -29,6 +30,16 @@ CACHE_LAYOUTS = ((firlib.CACHE_ROW_ORDER, 8, 16, 1),
                  (firlib.CACHE_ROW_ORDER, 16, 16, 4),
                  (firlib.CACHE_ROW_ORDER, 8, 8, 4),
                  (firlib.FIR_ORDER_VIEW_MAJOR, 8, 16, 4))
+# Layouts the multi-solve steps are tried with: (row order, lanes per LOR).
+# The steps take their line integrals from slab-staged sheets, which need a
+# family's rows together (tangential-major rows per view); the order of the
+# ring pairs within a family (by centre slice or by segment) and the lanes
+# per LOR are free.
+MULTI_LAYOUTS = ((firlib.CACHE_ROW_ORDER, 4),
+                 (firlib.CACHE_ROW_ORDER, 2),
+                 (firlib.CACHE_ROW_ORDER, 8),
+                 (firlib.FIR_ORDER_VIEW_MAJOR | firlib.FIR_ORDER_TANG_MAJOR, 4),
+                 (firlib.FIR_ORDER_VIEW_MAJOR | firlib.FIR_ORDER_TANG_MAJOR, 2))
 PLAIN_ORDERS = (firlib.FIR_ORDER_SEGMENT,
                 firlib.FIR_ORDER_SEGMENT | firlib.FIR_ORDER_TANG_MAJOR,
                 firlib.FIR_ORDER_VIEW_MAJOR | firlib.FIR_ORDER_ZCENTER)
@@ -128,4 +139,60 @@ def tune_plain(ctx: firlib.Context, vol: firlib.VolDesc, n_subsets: int,
     best = min(trials, key=lambda t: t.ms)
     ctx.set_lor_order(best.config["lor_order"])
     ctx.set_scatter_copies(best.config["scatter_copies"])
+    return Result(best.config, best.ms, trials)
+
+
+def tune_multi(ctx: firlib.Context, vol: firlib.VolDesc, n_subsets: int,
+               ys: Sequence[torch.Tensor], fwhm=(0.0, 0.0, 0.0),
+               len_format: int = firlib.FIR_LEN_SNORM16,
+               layouts: Sequence[Tuple[int, int]] = MULTI_LAYOUTS,
+               subset: int = 0, reps: int = 3) -> Result:
+    """Time one multi-solve step (fir_osem_step: the projector phases,
+    update and Gaussian) of one job of ``len(ys)`` images (up to 8, from
+    x = 1) on a cache of each layout (row order, lanes per LOR), and leave
+    the fastest lanes on the context. Layouts whose cache lacks the slab
+    form or the family lists, which the steps need, are skipped; every cache
+    is freed. The chosen row order is ``best["row_order"]``."""
+    dev = ys[0].device
+    nv = vol.n_voxels
+    bps = [torch.zeros(nv, device=dev) for _ in ys]
+    trials: List[Trial] = []
+    old_lanes = getattr(ctx, "_cache_lanes", (4, 16))
+    best = None
+    try:
+        for order, lanes in layouts:
+            ctx.set_cache_lanes(lanes, old_lanes[1])
+            cache = ctx.path_cache(vol, n_subsets, len_format, row_order=order)
+            try:
+                forms = cache.forms()
+                if not (forms["slabs"] and forms["family_lists"]):
+                    continue
+                sens = torch.zeros(n_subsets * nv, device=dev)
+                cache.sensitivity(sens)
+                xs = [torch.ones(nv, device=dev) for _ in ys]
+                nb = cache.job_workspace()
+                raw = torch.empty(nb + 256, dtype=torch.uint8, device=dev)
+                off = (-raw.data_ptr()) % 256
+                st = firlib.OsemStep([dict(
+                    cache=cache, subset=subset, x=xs, y=list(ys), bp=bps,
+                    sens=sens[subset * nv:(subset + 1) * nv],
+                    workspace=raw[off:off + nb])], fwhm)
+                try:
+                    ms = _time(st.run, reps)
+                finally:
+                    st.close()
+                torch.cuda.synchronize()
+            finally:
+                cache.close()
+            cfg = {"row_order": order, "fwd_lanes": lanes}
+            trials.append(Trial(cfg, ms))
+            if best is None or ms < best.ms:
+                best = trials[-1]
+    finally:
+        if best is None:
+            ctx.set_cache_lanes(*old_lanes)
+    if best is None:
+        raise RuntimeError("no layout gives the multi-solve steps' cache "
+                           "forms (slab form and family lists)")
+    ctx.set_cache_lanes(best.config["fwd_lanes"], old_lanes[1])
     return Result(best.config, best.ms, trials)

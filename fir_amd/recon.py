@@ -21,10 +21,12 @@ Per sub-iteration each image follows reconAlgos::OSEM
 """
 from __future__ import annotations
 
+import time
 from typing import List, Optional, Sequence
 
 import torch
 
+from . import autotune
 from . import lib as firlib
 
 
@@ -249,14 +251,19 @@ class MultiOSEM:
     iteration's K steps are captured in one HIP graph and replayed.
 
     ``solves`` as for BatchedOSEM; conv_interval 0 (no Gaussian) or 1
-    (Gaussian every sub-iteration). Each image gets what BatchedOSEM gives
-    it."""
+    (Gaussian every sub-iteration). With ``autotune_layout`` the caches'
+    LOR sort key (row order) and lanes per LOR are the fastest of
+    autotune.MULTI_LAYOUTS for one step of the largest group's first job
+    (``self.tuning``); otherwise ``cache_row_order`` and ``cache_lanes``.
+    Each image gets what BatchedOSEM gives it on caches of the same layout
+    (``self.cache_row_order``, ``self.cache_lanes``)."""
 
     def __init__(self, workload, solves: List[dict], fwhm=(0.0, 0.0, 0.0),
                  conv_interval: int = 0, cut: float = 0.0, device: int = 0,
                  cache_format: int = firlib.FIR_LEN_SNORM16,
                  cache_row_order: int = firlib.CACHE_ROW_ORDER,
-                 cache_zslices: int = 2):
+                 cache_zslices: int = 2, cache_lanes: int = 4,
+                 autotune_layout: bool = False):
         if conv_interval not in (0, 1):
             raise ValueError("MultiOSEM convolves every sub-iteration or "
                              "never (conv_interval 1 or 0)")
@@ -266,7 +273,7 @@ class MultiOSEM:
         ctx = firlib.context_for(workload, device)
         ctx.set_uneven_subsets(True)
         ctx.set_cache_zslices(cache_zslices)
-        ctx.set_cache_lanes(4, 16)  # the slab form's lanes per LOR
+        ctx.set_cache_lanes(cache_lanes, 16)  # the slab form's lanes per LOR
         ctx.set_cache_sheets(True)
         self.ctx = ctx
         dev = solves[0]["y"].device
@@ -287,6 +294,22 @@ class MultiOSEM:
             self.x.append(x)
             self.bp.append(torch.zeros(nv, device=dev))
             self.y.append(s["y"].contiguous())
+        self.tuning, self.tuning_s = None, 0.0
+        if autotune_layout:
+            t0 = time.perf_counter()
+            # the group with the most solves (then the largest grid) and the
+            # images of its first job
+            gi = max(range(len(keys)), key=lambda g: (
+                len(self.members[g]), solves[self.members[g][0]]["vol"].n_voxels))
+            mem = self.members[gi][:firlib.JOB_MAX_IMAGES]
+            self.tuning = autotune.tune_multi(
+                ctx, solves[mem[0]]["vol"], keys[gi][-1],
+                [self.y[i] for i in mem],
+                self.fwhm if conv else (0.0, 0.0, 0.0), cache_format)
+            cache_row_order = self.tuning.best["row_order"]
+            cache_lanes = self.tuning.best["fwd_lanes"]
+            self.tuning_s = time.perf_counter() - t0
+        self.cache_row_order, self.cache_lanes = cache_row_order, cache_lanes
         groups = []
         for k, mem in zip(keys, self.members):
             vol = solves[mem[0]]["vol"]

@@ -222,6 +222,39 @@ def test_multi_step_equals_group_graphs_c1(conv):
     assert not diff, diff
 
 
+def test_multi_step_autotuned_layout_c1():
+    """MultiOSEM with autotune_layout: the tuner times one step of the
+    largest group's first job on a cache of each candidate layout (LOR sort
+    key, lanes per LOR), keeps the fastest, and every solve then gets
+    exactly what the per-group graphs give on caches of that layout."""
+    w, solves = _c1_uneven_solves(21)
+    kw = dict(fwhm=w.fwhm, conv_interval=1)
+    m = recon.MultiOSEM(w, solves, autotune_layout=True, **kw)
+    t = m.tuning
+    assert len(t.trials) >= 3, t.table()
+    assert t.best_ms == min(tr.ms for tr in t.trials)
+    assert (m.cache_lanes, m.cache_row_order) == (
+        t.best["fwd_lanes"], t.best["row_order"])
+    lanes = {tr.config["fwd_lanes"] for tr in t.trials}
+    assert {2, 4, 8} <= lanes, t.table()
+    m.capture()
+    m.iteration()
+    torch.cuda.synchronize()
+    out = [x.cpu().numpy() for x in m.images()]
+    m.close()
+    b = recon.BatchedOSEM(w, solves, cache_format=firlib.FIR_LEN_SNORM16,
+                          cache_row_order=t.best["row_order"],
+                          cache_lanes=(t.best["fwd_lanes"], 16), **kw)
+    b.capture()
+    b.iteration()
+    torch.cuda.synchronize()
+    ref = [x.cpu().numpy() for x in b.images()]
+    b.close()
+    assert all(np.isfinite(o).all() for o in out)
+    diff = [i for i in range(len(out)) if not np.array_equal(out[i], ref[i])]
+    assert not diff, diff
+
+
 def test_multi_step_rejects_bad_jobs():
     """fir_osem_step_create refuses jobs sharing a workspace, and caches
     without the sheet form (and so without the slab form)."""
