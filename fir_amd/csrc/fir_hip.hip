@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -942,6 +943,232 @@ __global__ __launch_bounds__(kBlock) void k_gauss(const float* __restrict__ in,
   }
 }
 
+// The Gaussian's passes for frames whose rows are whole float4s (nx a
+// multiple of 4, 16-byte aligned frames) and half-widths up to kGaussHV,
+// H the pass's half-width: each thread computes four outputs from aligned
+// float4 loads kept in registers. X: 4 consecutive x of one row from 3
+// loads; Y (Z): 4 x by 4 consecutive y (z) from 4 + 2H loads; instead of
+// 2H + 1 scalar gathers per output. Every output is summed as gauss_sum
+// sums it (the same in-range taps in the same order, then s / w), so the
+// results are the same bit for bit. The window's centre tap (1) is always
+// in range, so w > 0 and no output falls back to its old value.
+constexpr int kGaussHV = 4;
+constexpr long long kGaussVecMinVoxels = 1LL << 19;
+
+__device__ __forceinline__ float4 ld4(const float* p)
+{
+  return *reinterpret_cast<const float4*>(p);
+}
+
+__device__ __forceinline__ float comp4(const float4& a, int c)
+{
+  return c == 0 ? a.x : (c == 1 ? a.y : (c == 2 ? a.z : a.w));
+}
+
+template <int H>
+__device__ __forceinline__ float gauss_out(const float* v, const float* tp,
+                                           int coord, int lim)
+{
+  float s = 0.0f, w = 0.0f;
+#pragma unroll
+  for (int q = 0; q <= 2 * H; ++q)
+  {
+    const int cq = coord + q - H;
+    if (cq >= 0 && cq < lim)
+    {
+      s += tp[q] * v[q];
+      w += tp[q];
+    }
+  }
+  return s / w;
+}
+
+// X pass of unit u (x-quad xq of row z * ny + y).
+template <int H>
+__device__ __forceinline__ void gauss_vx_unit(const float* __restrict__ in,
+                                              float* __restrict__ out,
+                                              const float* tp, int nx,
+                                              long long row, int xq)
+{
+  static_assert(H <= kGaussHV, "window of three float4");
+  const float* r = in + row * nx;
+  const int x0 = 4 * xq;
+  const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float4 a = x0 >= 4 ? ld4(r + x0 - 4) : z4;
+  const float4 b = ld4(r + x0);
+  const float4 c = x0 + 4 < nx ? ld4(r + x0 + 4) : z4;
+  const float v[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w,
+                       c.x, c.y, c.z, c.w};
+  float o[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+    o[g] = gauss_out<H>(v + 4 + g - H, tp, x0 + g, nx);
+  *reinterpret_cast<float4*>(out + row * nx + x0) =
+    make_float4(o[0], o[1], o[2], o[3]);
+}
+
+// Y or Z pass of the 4 x 4 patch at x-quad xq, axis positions a0..a0 + 3
+// (a = y or z), `base` the patch's first column (the other coordinate
+// applied), `stride` the axis stride. With EPI (the Z pass) the edge
+// restore and the cut of k_gauss's epilogue, per x of the patch's row y.
+template <int H, bool EPI>
+__device__ __forceinline__ void gauss_vyz_unit(
+  const float* __restrict__ in, float* __restrict__ out, const float* tp,
+  long long base, long long stride, int a0, int lim, int x0, int y,
+  const CutGeom& cg, int restore, double restore_r, float cut)
+{
+  const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 win[4 + 2 * H];
+#pragma unroll
+  for (int k = 0; k < 4 + 2 * H; ++k)
+  {
+    const int aa = a0 - H + k;
+    win[k] = aa >= 0 && aa < lim ? ld4(in + base + aa * stride) : z4;
+  }
+  bool keep[4] = {false, false, false, false};
+  bool zero[4] = {false, false, false, false};
+  if (EPI)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+    {
+      const double rad = radius_of(cg, x0 + c, y);
+      keep[c] = restore && rad >= restore_r;
+      zero[c] = cut > 0.0f && rad > cut;
+    }
+  const bool any_keep = keep[0] || keep[1] || keep[2] || keep[3];
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+  {
+    if (a0 + g >= lim) break;
+    float o[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+    {
+      float v[2 * H + 1];
+#pragma unroll
+      for (int q = 0; q <= 2 * H; ++q) v[q] = comp4(win[g + q], c);
+      o[c] = gauss_out<H>(v, tp, a0 + g, lim);
+    }
+    float* dst = out + base + (long long)(a0 + g) * stride;
+    if (EPI)
+    {
+      if (any_keep)
+      {
+        const float4 old = ld4(dst);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (keep[c]) o[c] = comp4(old, c);
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (zero[c]) o[c] = 0.0f;
+    }
+    *reinterpret_cast<float4*>(dst) = make_float4(o[0], o[1], o[2], o[3]);
+  }
+}
+
+template <int H>
+__device__ __forceinline__ void gauss_taps_h(const float* __restrict__ taps,
+                                             float* tp)
+{
+#pragma unroll
+  for (int q = 0; q <= 2 * H; ++q) tp[q] = taps[q];
+}
+
+// Units per frame of the vectorised passes: X (x-quad, row), Y (x-quad,
+// y-quad, z), Z (x-quad, y, z-quad).
+__host__ __device__ __forceinline__ long long gauss_units(int axis, int nx,
+                                                          int ny, int nz)
+{
+  const long long nxq = nx / 4;
+  return axis == 0 ? nxq * ny * nz
+                   : (axis == 1 ? nxq * ((ny + 3) / 4) * nz
+                                : nxq * ny * ((nz + 3) / 4));
+}
+
+// One unit of pass AXIS over frame in -> out.
+template <int AXIS, int H>
+__device__ __forceinline__ void gauss_v_unit(
+  const float* __restrict__ in, float* __restrict__ out, const float* tp,
+  int nx, int ny, int nz, long long u, const CutGeom& cg, int restore,
+  double restore_r, float cut)
+{
+  const int nxq = nx / 4;
+  const int xq = (int)(u % nxq);
+  const long long r = u / nxq;
+  const long long plane = (long long)nx * ny;
+  if (AXIS == 0)
+    gauss_vx_unit<H>(in, out, tp, nx, r, xq);
+  else if (AXIS == 1)
+  {
+    const int nyq = (ny + 3) / 4;
+    const int yq = (int)(r % nyq), z = (int)(r / nyq);
+    gauss_vyz_unit<H, false>(in, out, tp, z * plane + 4 * xq, nx, 4 * yq,
+                             ny, 4 * xq, 0, cg, 0, 0.0, 0.0f);
+  }
+  else
+  {
+    const int y = (int)(r % ny), zq = (int)(r / ny);
+    gauss_vyz_unit<H, true>(in, out, tp, (long long)y * nx + 4 * xq, plane,
+                            4 * zq, nz, 4 * xq, y, cg, restore, restore_r,
+                            cut);
+  }
+}
+
+template <int AXIS, int H>
+__global__ __launch_bounds__(kBlock) void k_gauss_v(
+  const float* __restrict__ in, float* __restrict__ out,
+  const float* __restrict__ taps, int nx, int ny, int nz, CutGeom cg,
+  int restore, double restore_r, float cut)
+{
+  const long long u = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (u >= gauss_units(AXIS, nx, ny, nz)) return;
+  float tp[2 * H + 1];
+  gauss_taps_h<H>(taps, tp);
+  gauss_v_unit<AXIS, H>(in, out, tp, nx, ny, nz, u, cg, restore, restore_r,
+                        cut);
+}
+
+// Whether a frame takes the vectorised Gaussian passes (k_gauss_v).
+bool gauss_vec_ok(int nx, const int* h)
+{
+  return nx % 4 == 0 && h[0] <= kGaussHV && h[1] <= kGaussHV &&
+    h[2] <= kGaussHV && h[0] >= 0 && h[1] >= 0 && h[2] >= 0;
+}
+
+template <int AXIS>
+void launch_gauss_v_axis(hipStream_t st, int h, const float* in, float* out,
+                         const float* taps, const fir_vol_desc* v,
+                         const CutGeom& cg, int restore, double rr, float cut)
+{
+  const long long n = gauss_units(AXIS, v->nx, v->ny, v->nz);
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+#define FIR_GAUSS_V(HH)                                                     \
+  hipLaunchKernelGGL((k_gauss_v<AXIS, HH>), grid, dim3(kBlock), 0, st, in, \
+                     out, taps, v->nx, v->ny, v->nz, cg, restore, rr, cut)
+  switch (h)
+  {
+  case 0: FIR_GAUSS_V(0); break;
+  case 1: FIR_GAUSS_V(1); break;
+  case 2: FIR_GAUSS_V(2); break;
+  case 3: FIR_GAUSS_V(3); break;
+  default: FIR_GAUSS_V(4); break;
+  }
+#undef FIR_GAUSS_V
+}
+
+void launch_gauss_v(hipStream_t st, int axis, int h, const float* in,
+                    float* out, const float* taps, const fir_vol_desc* v,
+                    const CutGeom& cg, int restore, double rr, float cut)
+{
+  if (axis == 0)
+    launch_gauss_v_axis<0>(st, h, in, out, taps, v, cg, restore, rr, cut);
+  else if (axis == 1)
+    launch_gauss_v_axis<1>(st, h, in, out, taps, v, cg, restore, rr, cut);
+  else
+    launch_gauss_v_axis<2>(st, h, in, out, taps, v, cg, restore, rr, cut);
+}
+
 // Multi-job OSEM steps: the element-wise phases of many images (different
 // grids) in one launch each. A block takes one tile (kGaussX x kGaussY
 // voxels of one slice) of one image; its image is found by a binary search
@@ -1047,6 +1274,34 @@ __global__ __launch_bounds__(kBlock) void k_step_gauss(
     if (cut > 0.0f && rad > cut) o = 0.0f;
   }
   out[v.i] = o;
+}
+
+// One vectorised pass of the Gaussian (k_gauss_v) for every image: a block
+// takes kBlock units of one image, found by a binary search over the
+// images' first blocks of this pass.
+template <int AXIS, int H>
+__global__ __launch_bounds__(kBlock) void k_step_gauss_v(
+  const StepImage* __restrict__ imgs, int n_imgs, int restore,
+  double restore_r, float cut)
+{
+  const int b = blockIdx.x;
+  int lo = 0, hi = n_imgs - 1; // last image whose first block is <= b
+  while (lo < hi)
+  {
+    const int mid = (lo + hi + 1) >> 1;
+    if (imgs[mid].vt0[AXIS] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const StepImage& m = imgs[lo];
+  const long long u = (long long)(b - m.vt0[AXIS]) * kBlock + threadIdx.x;
+  if (u >= gauss_units(AXIS, m.nx, m.ny, m.nz)) return;
+  const float* in = AXIS == 0 ? m.x : (AXIS == 1 ? m.t1 : m.t2);
+  float* out = AXIS == 0 ? m.t1 : (AXIS == 1 ? m.t2 : m.x);
+  float tp[2 * H + 1];
+  gauss_taps_h<H>(m.taps + 64 * AXIS, tp);
+  gauss_v_unit<AXIS, H>(in, out, tp, m.nx, m.ny, m.nz, u, step_cut_geom(m),
+                        AXIS == 2 ? restore : 0, restore_r,
+                        AXIS == 2 ? cut : 0.0f);
 }
 
 // Taps passed by value (kernel arguments), so that no host-to-device copy
@@ -1752,6 +2007,21 @@ int convolve_frame(fir_ctx* c, const fir_vol_desc* v, float* img,
   const float fw = fwhm[0] > fwhm[1] ? fwhm[0] : fwhm[1];
   const int restore = restore_cut > 0.0f ? 1 : 0;
   const double rr = (double)restore_cut - 5.0 * fw;
+  // (small frames keep the one-voxel-per-thread passes: four outputs per
+  // thread leave too few threads to hide the loads' latency)
+  if (gauss_vec_ok(v->nx, h) &&
+      (long long)v->nx * v->ny * v->nz >= kGaussVecMinVoxels &&
+      ((reinterpret_cast<uintptr_t>(img) | reinterpret_cast<uintptr_t>(t1) |
+        reinterpret_cast<uintptr_t>(t2)) & 15u) == 0)
+  {
+    launch_gauss_v(c->stream, 0, h[0], img, t1, taps, v, cg, 0, 0.0, 0.0f);
+    launch_gauss_v(c->stream, 1, h[1], t1, t2, taps + 64, v, cg, 0, 0.0,
+                   0.0f);
+    launch_gauss_v(c->stream, 2, h[2], t2, img, taps + 128, v, cg, restore,
+                   rr, final_cut);
+    FIR_CHECK_LAUNCH();
+    return FIR_OK;
+  }
   const dim3 nb((v->nx + kGaussX - 1) / kGaussX,
                 (v->ny + kGaussY - 1) / kGaussY, v->nz);
   hipLaunchKernelGGL((k_gauss<0, false>), nb, dim3(kBlock), 0,
@@ -1797,6 +2067,16 @@ int step_image_tiles(int nx, int ny, int nz)
   return ((nx + kGaussX - 1) / kGaussX) * ((ny + kGaussY - 1) / kGaussY) * nz;
 }
 
+int step_image_vtiles(int nx, int ny, int nz, int axis)
+{
+  return (int)((gauss_units(axis, nx, ny, nz) + kBlock - 1) / kBlock);
+}
+
+bool gauss_vec_shape(int nx, const int* h)
+{
+  return gauss_vec_ok(nx, h);
+}
+
 int gauss_taps_for(fir_ctx* c, const fir_vol_desc* v, const float* fwhm,
                    const float** taps, int h[3])
 {
@@ -1808,15 +2088,44 @@ int gauss_taps_for(fir_ctx* c, const fir_vol_desc* v, const float* fwhm,
   return FIR_OK;
 }
 
+template <int AXIS>
+void launch_step_gauss_v(hipStream_t st, const StepImage* imgs, int n_imgs,
+                         int blocks, int h, int restore, double restore_r,
+                         float cut)
+{
+#define FIR_STEP_GAUSS_V(HH)                                                \
+  hipLaunchKernelGGL((k_step_gauss_v<AXIS, HH>), dim3(blocks), dim3(kBlock), \
+                     0, st, imgs, n_imgs, restore, restore_r, cut)
+  switch (h)
+  {
+  case 0: FIR_STEP_GAUSS_V(0); break;
+  case 1: FIR_STEP_GAUSS_V(1); break;
+  case 2: FIR_STEP_GAUSS_V(2); break;
+  case 3: FIR_STEP_GAUSS_V(3); break;
+  default: FIR_STEP_GAUSS_V(4); break;
+  }
+#undef FIR_STEP_GAUSS_V
+}
+
 int launch_step_elementwise(hipStream_t st, const StepImage* imgs, int n_imgs,
                             int n_tiles, bool gauss, float cut, int restore,
-                            double restore_r)
+                            double restore_r, const int* vtiles,
+                            const int* vh)
 {
   if (n_tiles <= 0) return FIR_OK;
   hipLaunchKernelGGL(k_step_update, dim3(n_tiles), dim3(kBlock), 0, st, imgs,
                      n_imgs, cut, (!gauss && cut > 0.0f) ? 1 : 0);
   FIR_CHECK_LAUNCH();
   if (!gauss) return FIR_OK;
+  if (vtiles)
+  {
+    launch_step_gauss_v<0>(st, imgs, n_imgs, vtiles[0], vh[0], 0, 0.0, 0.0f);
+    launch_step_gauss_v<1>(st, imgs, n_imgs, vtiles[1], vh[1], 0, 0.0, 0.0f);
+    launch_step_gauss_v<2>(st, imgs, n_imgs, vtiles[2], vh[2], restore,
+                           restore_r, cut);
+    FIR_CHECK_LAUNCH();
+    return FIR_OK;
+  }
   hipLaunchKernelGGL(k_step_gauss<0>, dim3(n_tiles), dim3(kBlock), 0, st,
                      imgs, n_imgs, 0, 0.0, 0.0f);
   hipLaunchKernelGGL(k_step_gauss<1>, dim3(n_tiles), dim3(kBlock), 0, st,

@@ -130,9 +130,14 @@ struct StepImage
   double dx, dy;
   int tile0;          // the image's first tile (blocks of 32 x 8 voxels of
                       // one slice) in the step's tile numbering
+  int vt0[3];         // its first block in each vectorised Gaussian pass
 };
 // Tiles of one image in that numbering.
 int step_image_tiles(int nx, int ny, int nz);
+// Blocks of one image in the vectorised Gaussian pass of an axis, and
+// whether an image of that width and those half-widths can take them.
+int step_image_vtiles(int nx, int ny, int nz, int axis);
+bool gauss_vec_shape(int nx, const int* h);
 // The context's taps of the Gaussian of fwhm on grid v (uploaded once and
 // kept until the context is destroyed).
 int gauss_taps_for(fir_ctx* c, const fir_vol_desc* v, const float* fwhm,
@@ -141,9 +146,12 @@ int gauss_taps_for(fir_ctx* c, const fir_vol_desc* v, const float* fwhm,
 // n_tiles tiles in all: the OSEM update, then (gauss) the three passes of
 // the Gaussian, the Z pass with the edge restore at radius restore_r (when
 // restore) and the circular cut; without the Gaussian the update cuts.
+// With vtiles (the blocks of each vectorised pass over all images) the
+// passes are the vectorised ones, for half-widths vh shared by every image.
 int launch_step_elementwise(hipStream_t st, const StepImage* imgs, int n_imgs,
                             int n_tiles, bool gauss, float cut, int restore,
-                            double restore_r);
+                            double restore_r, const int* vtiles,
+                            const int* vh);
 } // namespace firi
 
 #define FIR_HIP_CHECK(expr)                                                  \

@@ -30,6 +30,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -4828,6 +4829,9 @@ struct fir_osem_step
   firi::StepImage* d_imgs = nullptr;
   int n_imgs = 0;
   int img_tiles = 0;
+  bool vec_gauss = false;      // the vectorised Gaussian passes
+  int vtiles[3] = {0, 0, 0};   // their blocks over all images
+  int vh[3] = {0, 0, 0};       // the half-widths every image shares
   int slab_R = 0, slab_block = 0;
   size_t slab_lds = 0;
   int grid_slab[2] = {0, 0}, grid_bp[2] = {0, 0};
@@ -5219,6 +5223,8 @@ int fir_osem_step_create(int n_jobs, const fir_osem_job* jobs,
   std::vector<int> pack_pre(n_jobs + 1), red_pre(n_jobs + 1);
   std::vector<firi::StepImage> imgs;
   int pack = 0, red = 0, tiles = 0;
+  int vtiles[3] = {0, 0, 0};
+  bool vec = true;
   for (int j = 0; j < n_jobs; ++j)
   {
     const fir_osem_job& J = jobs[j];
@@ -5296,6 +5302,19 @@ int fir_osem_step_create(int n_jobs, const fir_osem_job* jobs,
       m.dy = v.dy;
       m.tile0 = tiles;
       tiles += firi::step_image_tiles(v.nx, v.ny, v.nz);
+      for (int d = 0; d < 3; ++d)
+      {
+        m.vt0[d] = vtiles[d];
+        vtiles[d] += firi::step_image_vtiles(v.nx, v.ny, v.nz, d);
+      }
+      // the vectorised passes: rows of whole float4s, 16-byte aligned
+      // images, half-widths they handle and that every image shares
+      const uintptr_t al = reinterpret_cast<uintptr_t>(m.x) |
+        reinterpret_cast<uintptr_t>(m.t1) | reinterpret_cast<uintptr_t>(m.t2);
+      if (!gauss || !firi::gauss_vec_shape(v.nx, h) || (al & 15u) != 0 ||
+          (!imgs.empty() && (imgs[0].h[0] != h[0] || imgs[0].h[1] != h[1] ||
+                             imgs[0].h[2] != h[2])))
+        vec = false;
       imgs.push_back(m);
     }
   }
@@ -5319,6 +5338,12 @@ int fir_osem_step_create(int n_jobs, const fir_osem_job* jobs,
   st->red_blocks = red;
   st->n_imgs = (int)imgs.size();
   st->img_tiles = tiles;
+  st->vec_gauss = vec && !imgs.empty();
+  for (int d = 0; d < 3; ++d)
+  {
+    st->vtiles[d] = vtiles[d];
+    st->vh[d] = imgs.empty() ? 0 : imgs[0].h[d];
+  }
   StepJob* d_jobs = nullptr;
   if ((rc = dev_alloc(&d_jobs, (size_t)n_jobs, "step jobs")))
     return rc;
@@ -5361,9 +5386,10 @@ int fir_osem_step_run(fir_osem_step* st)
     ? launch_step_projector<FIR_LEN_SNORM16>(st)
     : launch_step_projector<FIR_LEN_FP16>(st);
   if (rc) return rc;
-  return firi::launch_step_elementwise(st->ctx->stream, st->d_imgs,
-                                       st->n_imgs, st->img_tiles, st->gauss,
-                                       st->cut, st->restore, st->restore_r);
+  return firi::launch_step_elementwise(
+    st->ctx->stream, st->d_imgs, st->n_imgs, st->img_tiles, st->gauss,
+    st->cut, st->restore, st->restore_r, st->vec_gauss ? st->vtiles : nullptr,
+    st->vh);
 }
 
 int fir_osem_step_destroy(fir_osem_step* st)
