@@ -427,9 +427,11 @@ int fir_cache_info(const fir_cache* cache, int64_t* n_rows,
  * FIR_FORM_SHEETS, the sheet form (16-bit sheet indices of tangential-major
  * rows), and FIR_FORM_FAMILY_LISTS, the family lists plus the partial
  * sheets of the family backprojection. The build adds each only when the
- * device has room for it (otherwise one-image passes fall back to the
+ * device has room for it, and the lists only when the rows of consecutive
+ * elements of each voxel's list lie at most 127 apart (their stored quads
+ * hold row steps in 7 bits); otherwise one-image passes fall back to the
  * image gathers and the transposed lists, with results within FP32
- * rounding); one-image OSEM passes use what is held unless fir_cache_tune
+ * rounding; one-image OSEM passes use what is held unless fir_cache_tune
  * switches it off. FIR_FORM_SLABS: batches of 4 or 8 images take their
  * line integrals from sheets staged slab by slab (sheet pixels ranked along
  * each family's line, checked at build; the runtime forward lanes equal to
