@@ -2500,7 +2500,18 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
         acc = fmaf(l[3], wr[rw[3]], acc);
       }
     }
-    if (j < n_act) pf[sl >> 16] = acc;
+    // into the sheet's LDS buffer (free once the line integrals are done),
+    // so that the partial sheet goes out in contiguous runs below
+    if (j < n_act) sh[sl >> 16] = acc;
+  }
+  __syncthreads();
+  // the entries inside their pixel's slice range, which the per-voxel sums
+  // read (the listed slots and the empty ones in range, written 0 above)
+  for (int i = threadIdx.x; i < n_st; i += kSheetBlock)
+  {
+    const int p = i / nz, z = i - p * nz;
+    const unsigned r = fam_zr[p0 + p];
+    if ((unsigned)z >= (r & 0xffffu) && (unsigned)z < (r >> 16)) pf[i] = sh[i];
   }
 }
 
