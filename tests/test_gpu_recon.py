@@ -222,6 +222,35 @@ def test_multi_step_equals_group_graphs_c1(conv):
     assert not diff, diff
 
 
+def test_multi_step_mixed_voxel_sizes_c1():
+    """A step whose images do not share the Gaussian's half-widths (a grid
+    of 4 mm pixels beside the 2 mm ones) runs the one-voxel-per-thread
+    passes for all of them instead of the vectorised ones, and still gives
+    exactly what the per-group graphs give."""
+    w, solves = _c1_uneven_solves(8)
+    v32 = firlib.VolDesc.from_spec(io.VolSpec((32, 32, 8), (4.0, 4.0, 2.0),
+                                              (-62.0, -62.0, 0.0)))
+    for i in (1, 3, 5):
+        solves[i] = dict(solves[i], vol=v32)
+    kw = dict(fwhm=w.fwhm, conv_interval=1)
+    b = recon.BatchedOSEM(w, solves, cache_format=firlib.FIR_LEN_SNORM16,
+                          **kw)
+    b.capture()
+    b.iteration()
+    torch.cuda.synchronize()
+    ref = [x.cpu().numpy() for x in b.images()]
+    b.close()
+    m = recon.MultiOSEM(w, solves, **kw)
+    m.capture()
+    m.iteration()
+    torch.cuda.synchronize()
+    out = [x.cpu().numpy() for x in m.images()]
+    m.close()
+    assert all(np.isfinite(o).all() for o in out)
+    diff = [i for i in range(len(out)) if not np.array_equal(out[i], ref[i])]
+    assert not diff, diff
+
+
 def test_multi_step_autotuned_layout_c1():
     """MultiOSEM with autotune_layout: the tuner times one step of the
     largest group's first job on a cache of each candidate layout (LOR sort
