@@ -2446,7 +2446,7 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_osem_fam(
       float line = acc[0] * fct;
       if (io.bias[0]) line += io.bias[0][bin];
       if (line > kEpsF) out = io.y[0][bin] / line * fct;
-      w[r] = out;
+      if (w) w[r] = out;
       wr[r - r0] = out;
     }
   }
@@ -4721,12 +4721,13 @@ int fir_cache_osem_project(fir_cache* cache, int n_images,
     {
       pack_images_pz(cache, 1, in, 1);
       FIR_CHECK_LAUNCH();
+      // (the weights stay in LDS: no per-row copy is stored)
       if (rm->fmt == FIR_LEN_SNORM16)
         launch_osem_family<FIR_LEN_SNORM16>(cache, subset, cache->d_x, io,
-                                            cache->d_w, bp[i0]);
+                                            nullptr, bp[i0]);
       else
         launch_osem_family<FIR_LEN_FP16>(cache, subset, cache->d_x, io,
-                                         cache->d_w, bp[i0]);
+                                         nullptr, bp[i0]);
       FIR_CHECK_LAUNCH();
       continue;
     }
