@@ -67,6 +67,28 @@ def traffic_from_profiles():
     return val, commit
 
 
+FP64_PEAK_TFLOPS = 78.6
+
+
+def fp64_flop_from_profiles():
+    """FP64 operations per walked projector launch (SQ_INSTS_VALU_*_F64
+    counters x 64 lanes, FMA counted twice), from the committed rocprofv3
+    summary profiles/plain_osem_project_c2_fp64_counters.md."""
+    p = os.path.join(ROOT, "profiles", "plain_osem_project_c2_fp64_counters.md")
+    if not os.path.exists(p):
+        return None, None
+    val, commit = None, None
+    for line in open(p):
+        if line.startswith("fp64_flop_per_launch:"):
+            try:
+                val = int(line.split(":", 1)[1])
+            except ValueError:
+                val = None
+        elif line.startswith("commit:"):
+            commit = line.split(":", 1)[1].strip()
+    return val, commit
+
+
 def rel_l2(a, b):
     a = np.asarray(a, np.float64).ravel()
     b = np.asarray(b, np.float64).ravel()
@@ -706,6 +728,22 @@ def main():
         eq = args.batch * bytes_launch / (per_sub_ms * 1e-3) / 1e9
         batched["single_image_bytes_rate_gb_s"] = round(eq, 1)
     traffic, traffic_commit = traffic_from_profiles()
+    # the walked projector against the FP64 vector peak (its FP64
+    # operations per launch counted by rocprofv3, the launch timed here)
+    flop, flop_commit = fp64_flop_from_profiles()
+    plain_fp64 = None
+    if flop and plain_proj_ms > 0:
+        tflops = flop / (plain_proj_ms * 1e-3) / 1e12
+        plain_fp64 = {
+            "kernel": "k_project_fam<true> (walked OSEM projector launch, "
+                      "FP64 Siddon walk per LOR)",
+            "bound": "latency of the FP64 walk",
+            "achieved": round(tflops, 3), "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
+            "flop_per_launch": flop,
+            "flop_source": ("rocprofv3 --pmc SQ_INSTS_VALU_*_F64, "
+                            "profiles/plain_osem_project_c2_fp64_counters.md"
+                            f" at commit {flop_commit}")}
     copy_gbs = copy_bandwidth()
     log(f"device copy: {copy_gbs:.0f} GB/s (1 GiB read + 1 GiB written)")
 
@@ -753,6 +791,7 @@ def main():
         "plain": {"osem_iter_ms": round(plain_ms, 4),
                   "autotune": {"best": pres.best, "trials_ms": pres.table()},
                   "projector_launch_ms": round(plain_proj_ms, 4),
+                  "fp64_roofline": plain_fp64,
                   "mlors_s": round(w.n_lors / (plain_ms * 1e-3) / 1e6, 3),
                   "fwd_ms": round(fwd_ms, 4),
                   "bp_ms": round(bp_ms, 4),
