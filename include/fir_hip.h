@@ -545,11 +545,15 @@ int fir_cache_job_workspace(const fir_cache* cache, int64_t* bytes);
  * left at 0. Every image gets exactly what fir_cache_osem_project,
  * fir_osem_update and fir_convolve + fir_cut_circle give it. All caches
  * must belong to one context, hold the sheet form, family lists and slab
- * form (fir_cache_forms) in one length format, with 4 lanes per LOR. */
+ * form (fir_cache_forms) in one length format, laid out for the same lanes
+ * per LOR (fir_set_cache_lanes; 4 suit the steps best). */
 typedef struct fir_osem_step fir_osem_step;
 int fir_osem_step_create(int n_jobs, const fir_osem_job* jobs,
                          const float* fwhm, float cut, fir_osem_step** out);
-/* Runs the step on the context's stream (eight launches for all jobs). */
+/* Runs the step on the context's stream: one launch per phase for all
+ * jobs (pack; line integrals and family backprojection, each once for the
+ * jobs of 8-image slots and once for those of 4; per-voxel sums; update;
+ * the Gaussian's three passes). */
 int fir_osem_step_run(fir_osem_step* step);
 int fir_osem_step_destroy(fir_osem_step* step);
 
