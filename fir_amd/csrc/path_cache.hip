@@ -2622,11 +2622,19 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_fam_bp(
 // slice range) are wave-uniform scalar loads, and a family whose elements
 // miss a lane's slice at that pixel adds +0 without a load (its partial
 // there, never written, is the empty sum).
-constexpr int kRedBlock = 256;
+constexpr int kRedBlock = 1024;
+constexpr int kRedPx = kRedBlock / 64; // pixels per block, one per wave
+constexpr size_t kRedTileMax = 65536;  // LDS bytes of the output tile
+// With `tile` (LDS, [NB][nz][kRedPx] floats) the pixel's sums go to column
+// px - px0 of the block's tile, which fam_reduce_store then writes out in
+// runs of kRedPx consecutive pixels (instead of 4-byte stores one plane
+// apart, which left most of each line to other blocks and moved up to 5x
+// the bytes); without, straight to the images.
 template <int NB>
 __device__ __forceinline__ void fam_reduce_pixel(
   int px, const long long* __restrict__ off, const int2* __restrict__ ent,
-  const float* __restrict__ part, int nxy, int nz, const VoxOut& out)
+  const float* __restrict__ part, int nxy, int nz, const VoxOut& out,
+  float* tile = nullptr, int col = 0)
 {
   const int lane = (int)threadIdx.x & 63;
   const long long a = off[px], b = off[px + 1];
@@ -2669,18 +2677,56 @@ __device__ __forceinline__ void fam_reduce_pixel(
     if (z < nz)
 #pragma unroll
       for (int i = 0; i < NB; ++i)
-        if (out.p[i]) out.p[i][(long long)z * nxy + px] = acc[i];
+      {
+        if (tile)
+          tile[(i * nz + z) * kRedPx + col] = acc[i];
+        else if (out.p[i])
+          out.p[i][(long long)z * nxy + px] = acc[i];
+      }
+  }
+}
+
+// The block's tile (pixels px0 .. px0 + kRedPx - 1) to the images.
+template <int NB>
+__device__ __forceinline__ void fam_reduce_store(const float* tile, int px0,
+                                                 int nxy, int nz,
+                                                 const VoxOut& out)
+{
+  const int n = NB * nz * kRedPx;
+  for (int t = threadIdx.x; t < n; t += kRedBlock)
+  {
+    const int c = t % kRedPx, r = t / kRedPx;
+    const int i = r / nz, z = r - i * nz;
+    if (px0 + c < nxy && out.p[i])
+      out.p[i][(long long)z * nxy + px0 + c] = tile[t];
   }
 }
 
 template <int NB>
 __global__ __launch_bounds__(kRedBlock) void k_pc_fam_reduce(
   const long long* __restrict__ off, const int2* __restrict__ ent,
-  const float* __restrict__ part, int nxy, int nz, VoxOut out)
+  const float* __restrict__ part, int nxy, int nz, VoxOut out, int stage)
 {
-  const int px = __builtin_amdgcn_readfirstlane(
-    (int)blockIdx.x * (kRedBlock / 64) + ((int)threadIdx.x >> 6));
-  if (px < nxy) fam_reduce_pixel<NB>(px, off, ent, part, nxy, nz, out);
+  extern __shared__ float red_tile[];
+  const int w = (int)threadIdx.x >> 6;
+  const int px0 = (int)blockIdx.x * kRedPx;
+  const int px = __builtin_amdgcn_readfirstlane(px0 + w);
+  if (px < nxy)
+    fam_reduce_pixel<NB>(px, off, ent, part, nxy, nz, out,
+                         stage ? red_tile : nullptr, w);
+  if (stage)
+  {
+    __syncthreads();
+    fam_reduce_store<NB>(red_tile, px0, nxy, nz, out);
+  }
+}
+
+// LDS bytes of the reduction's output tile for nb images of nz slices, or
+// 0 when it does not fit (the sums then go straight to the images).
+size_t red_tile_bytes(int nb, int nz)
+{
+  const size_t b = sizeof(float) * (size_t)nb * nz * kRedPx;
+  return b <= kRedTileMax ? b : 0;
 }
 
 } // namespace
@@ -4124,13 +4170,14 @@ void launch_osem_family(fir_cache* cache, int s, const float* img,
   VoxOut out;
   std::memset(&out, 0, sizeof(out));
   out.p[0] = bp;
+  const size_t tile = red_tile_bytes(1, nz);
   hipLaunchKernelGGL(k_pc_fam_reduce<1>,
                      dim3((nxy + kRedBlock / 64 - 1) / (kRedBlock / 64)),
-                     dim3(kRedBlock), 0,
+                     dim3(kRedBlock), tile,
                      cache->ctx->stream,
                      cache->d_fr_off + (long long)s * (nxy + 1),
                      cache->d_fr_ent, cache->d_part, nxy, nz,
-                     out);
+                     out, tile > 0 ? 1 : 0);
 }
 
 // Family backprojection of nb images (nb = 1, 2, 4 or 8) of subset s with
@@ -4148,13 +4195,14 @@ void launch_fam_bp_nb(fir_cache* cache, int s, const float* w,
                        cache->ctx->stream, ford, rm->d_fam_row,
                        rm->d_fam_off, cache->d_fam_loc, nz, w, rm->d_fl_slot,
                        rm->d_fl_gb, rm->d_fl, rm->d_fl_nact, cache->d_part);
+  const size_t tile = red_tile_bytes(NB, nz);
   hipLaunchKernelGGL(k_pc_fam_reduce<NB>,
                      dim3((nxy + kRedBlock / 64 - 1) / (kRedBlock / 64)),
-                     dim3(kRedBlock), 0,
+                     dim3(kRedBlock), tile,
                      cache->ctx->stream,
                      cache->d_fr_off + (long long)s * (nxy + 1),
                      cache->d_fr_ent, cache->d_part, nxy, nz,
-                     out);
+                     out, tile > 0 ? 1 : 0);
 }
 
 template <int F>
@@ -4841,6 +4889,7 @@ struct fir_osem_step
   firi::StepImage* d_imgs = nullptr;
   int n_imgs = 0;
   int img_tiles = 0;
+  size_t red_tile = 0;         // LDS of the reduction's output tile (0: none)
   bool vec_gauss = false;      // the vectorised Gaussian passes
   int vtiles[3] = {0, 0, 0};   // their blocks over all images
   int vh[3] = {0, 0, 0};       // the half-widths every image shares
@@ -4997,18 +5046,34 @@ __attribute__((amdgpu_waves_per_eu(8))) void k_step_fam_bp(
 // Per-voxel sums of every job's partial sheets into its bp images
 // (k_pc_fam_reduce, 8 images), one wave per pixel.
 __global__ __launch_bounds__(kRedBlock) void k_step_reduce(
-  const StepJob* __restrict__ jobs, const int* __restrict__ pre, int n_jobs)
+  const StepJob* __restrict__ jobs, const int* __restrict__ pre, int n_jobs,
+  int stage)
 {
+  extern __shared__ float red_tile[];
   const int b = blockIdx.x;
   const int j = step_job_of(pre, n_jobs, b);
   const StepJob& J = jobs[j];
-  const int px = __builtin_amdgcn_readfirstlane(
-    (b - pre[j]) * (kRedBlock / 64) + ((int)threadIdx.x >> 6));
-  if (px >= J.nxy) return;
-  if (J.nb == 4)
-    fam_reduce_pixel<4>(px, J.fr_off, J.fr_ent, J.part, J.nxy, J.nz, J.out);
-  else
-    fam_reduce_pixel<8>(px, J.fr_off, J.fr_ent, J.part, J.nxy, J.nz, J.out);
+  const int w = (int)threadIdx.x >> 6;
+  const int px0 = (b - pre[j]) * kRedPx;
+  const int px = __builtin_amdgcn_readfirstlane(px0 + w);
+  float* tile = stage ? red_tile : nullptr;
+  if (px < J.nxy)
+  {
+    if (J.nb == 4)
+      fam_reduce_pixel<4>(px, J.fr_off, J.fr_ent, J.part, J.nxy, J.nz, J.out,
+                          tile, w);
+    else
+      fam_reduce_pixel<8>(px, J.fr_off, J.fr_ent, J.part, J.nxy, J.nz, J.out,
+                          tile, w);
+  }
+  if (stage)
+  {
+    __syncthreads();
+    if (J.nb == 4)
+      fam_reduce_store<4>(red_tile, px0, J.nxy, J.nz, J.out);
+    else
+      fam_reduce_store<8>(red_tile, px0, J.nxy, J.nz, J.out);
+  }
 }
 
 constexpr size_t kWsAlign = 256;
@@ -5126,8 +5191,9 @@ int launch_step_projector(fir_osem_step* st)
     return rc;
   if (n4 > 0 && (rc = launch_step_r<F, 4>(st, 1, st->d_items + n8, n4)))
     return rc;
-  hipLaunchKernelGGL(k_step_reduce, dim3(st->red_blocks), dim3(kRedBlock), 0,
-                     s, jobs, st->d_red_pre, st->n_jobs);
+  hipLaunchKernelGGL(k_step_reduce, dim3(st->red_blocks), dim3(kRedBlock),
+                     st->red_tile, s, jobs, st->d_red_pre, st->n_jobs,
+                     st->red_tile > 0 ? 1 : 0);
   FIR_CHECK_LAUNCH();
   return FIR_OK;
 }
@@ -5235,6 +5301,8 @@ int fir_osem_step_create(int n_jobs, const fir_osem_job* jobs,
   std::vector<int> pack_pre(n_jobs + 1), red_pre(n_jobs + 1);
   std::vector<firi::StepImage> imgs;
   int pack = 0, red = 0, tiles = 0;
+  size_t red_tile = 0;
+  bool red_fit = true;
   int vtiles[3] = {0, 0, 0};
   bool vec = true;
   for (int j = 0; j < n_jobs; ++j)
@@ -5281,6 +5349,12 @@ int fir_osem_step_create(int n_jobs, const fir_osem_job* jobs,
     s.pack_tiles_x = (nxy + kPzPx - 1) / kPzPx;
     s.nb = nb;
     st->slab_lds = std::max(st->slab_lds, sh.lds);
+    {
+      // the reduction's output tile: staged only if every job's fits
+      const size_t tb = red_tile_bytes(nb, nz);
+      red_fit = red_fit && tb > 0;
+      red_tile = std::max(red_tile, tb);
+    }
     pack_pre[j] = pack;
     pack += s.pack_tiles_x * ((nz + kPzZ - 1) / kPzZ);
     red_pre[j] = red;
@@ -5348,6 +5422,7 @@ int fir_osem_step_create(int n_jobs, const fir_osem_job* jobs,
   st->n_items = (int)it.size();
   st->pack_blocks = pack;
   st->red_blocks = red;
+  st->red_tile = red_fit ? red_tile : 0;
   st->n_imgs = (int)imgs.size();
   st->img_tiles = tiles;
   st->vec_gauss = vec && !imgs.empty();
