@@ -2527,7 +2527,8 @@ __device__ __forceinline__ void fam_bp_family(
   const int* __restrict__ fam_off, const int* __restrict__ fam_loc, int nz,
   const float* __restrict__ w, const unsigned* __restrict__ fl_slot,
   const unsigned* __restrict__ fl_gb, const unsigned* __restrict__ fl,
-  const int* __restrict__ fl_nact, float* __restrict__ part)
+  const int* __restrict__ fl_nact, float* __restrict__ part,
+  float* stage = nullptr, const unsigned* __restrict__ fam_zr = nullptr)
 {
   const int r0 = fam_row[f], r1 = fam_row[f + 1];
   const int p0 = fam_off[f];
@@ -2595,9 +2596,27 @@ __device__ __forceinline__ void fam_bp_family(
     }
     if (j < n_act)
     {
-      float* o = pf + (long long)(sl >> 16) * NB;
+      if (stage)
+        stage[sl >> 16] = acc[0];
+      else
+      {
+        float* o = pf + (long long)(sl >> 16) * NB;
 #pragma unroll
-      for (int b = 0; b < NB; ++b) o[b] = acc[b];
+        for (int b = 0; b < NB; ++b) o[b] = acc[b];
+      }
+    }
+  }
+  if (stage)
+  {
+    // one image: the partial sheet gathered in LDS goes out in contiguous
+    // runs of each pixel's slice range (as k_pc_osem_fam stores it)
+    __syncthreads();
+    for (int i = threadIdx.x; i < n_st; i += kSheetBlock)
+    {
+      const int p = i / nz, z = i - p * nz;
+      const unsigned r = fam_zr[p0 + p];
+      if ((unsigned)z >= (r & 0xffffu) && (unsigned)z < (r >> 16))
+        pf[i] = stage[i];
     }
   }
 }
@@ -2609,11 +2628,13 @@ __global__ __launch_bounds__(kSheetBlock) void k_pc_fam_bp(
   const int* __restrict__ fam_loc, int nz, const float* __restrict__ w,
   const unsigned* __restrict__ fl_slot, const unsigned* __restrict__ fl_gb,
   const unsigned* __restrict__ fl, const int* __restrict__ fl_nact,
-  float* __restrict__ part)
+  float* __restrict__ part, const unsigned* __restrict__ fam_zr)
 {
-  __shared__ float wr[kFamRowsMax * NB];
+  // one image: room for the partial sheet as well
+  __shared__ float wr[kFamRowsMax * NB + (NB == 1 ? kSheetMaxFloats : 0)];
   fam_bp_family<F, NB>(ford[blockIdx.x], wr, fam_row, fam_off, fam_loc, nz, w,
-                       fl_slot, fl_gb, fl, fl_nact, part);
+                       fl_slot, fl_gb, fl, fl_nact, part,
+                       NB == 1 ? wr + kFamRowsMax : nullptr, fam_zr);
 }
 
 // out_b[v] = sum over the subset's families whose sheet holds v's pixel of
@@ -4194,7 +4215,8 @@ void launch_fam_bp_nb(fir_cache* cache, int s, const float* w,
     hipLaunchKernelGGL((k_pc_fam_bp<F, NB>), dim3(nf), dim3(kSheetBlock), 0,
                        cache->ctx->stream, ford, rm->d_fam_row,
                        rm->d_fam_off, cache->d_fam_loc, nz, w, rm->d_fl_slot,
-                       rm->d_fl_gb, rm->d_fl, rm->d_fl_nact, cache->d_part);
+                       rm->d_fl_gb, rm->d_fl, rm->d_fl_nact, cache->d_part,
+                       rm->d_fam_zr);
   const size_t tile = red_tile_bytes(NB, nz);
   hipLaunchKernelGGL(k_pc_fam_reduce<NB>,
                      dim3((nxy + kRedBlock / 64 - 1) / (kRedBlock / 64)),
